@@ -1,0 +1,157 @@
+/* snisr.h — C ABI of the MI355X-native tiled sparse-voxel ray-marching renderer.
+ *
+ * Plain C: every argument is a pointer, a size or a POD descriptor.  Device pointers are
+ * HIP device memory owned by the caller; `stream` is a hipStream_t passed as void*
+ * (NULL = the null stream).  Every entry point returns 0 on success or a negative
+ * snisr_status; snisr_last_error() returns a thread-local message for the last failure.
+ * Nothing is kept between calls (no __constant__ or global state), so calls on different
+ * streams are independent.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference checkout).  The reference is a pybind11/ATen extension whose ops write into
+ * caller-allocated tensors and return void (rendering/src/binding.cpp:13-27); the Python
+ * layer of this package (snisr.fastrendering) re-exposes the same names on torch tensors.
+ */
+#ifndef SNISR_H
+#define SNISR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum snisr_status {
+  SNISR_OK = 0,
+  SNISR_ERR_ARG = -1,        /* null pointer, bad size or inconsistent descriptor */
+  SNISR_ERR_SHAPE = -2,      /* a shape the kernels do not support */
+  SNISR_ERR_ALIGN = -3,      /* a pointer without the alignment the kernel needs */
+  SNISR_ERR_DEVICE = -4,     /* a HIP runtime error (launch or query) */
+  SNISR_ERR_CAPACITY = -5,   /* the workspace was too small for this call */
+  SNISR_ERR_UNSUPPORTED = -6
+} snisr_status;
+
+enum { SNISR_F32 = 0, SNISR_F16 = 1, SNISR_BF16 = 2 };
+
+/* Scene in the tiled sparse-brick layout (device pointers).
+ *   tile_index_map  i32[Tx*Ty*Tz]  dense tile cell (x*Ty*Tz + y*Tz + z) -> tile id or -1
+ *                   (the reference's tile_IndexMap, rendering/src/__init__.py:117)
+ *   tile_centers    f32[T,3]       (centers, __init__.py:115)
+ *   tile_res        i32[T]         voxels per tile side, a multiple of 8
+ *   tile_brick_base i32[T]         offset of the tile's (res/8)^3 brick table in brick_map
+ *   brick_map       i32[..]        brick cell ((bx*nb + by)*nb + bz) -> brick id or -1
+ *                                  (role of block_IndexMap, __init__.py:118)
+ *   sigma           f32[NB,512]    density per voxel of each brick, voxel (x*8+y)*8+z
+ *   feats           [NB,512,F]     features, F contiguous per voxel (f32 or f16)
+ *   mlp_params      f32[T,P]       per-tile MLP, per layer W[out][in] then b[out];
+ *                                  layers in->W, (L-2)x W->W, W->3; input = F features
+ *                                  + 15-d view encoding
+ *   mlp_f32t        f32 copy of mlp_params from snisr_pack_mlp(SNISR_F32): per layer Wt[in][out4]
+ *                   then b[out4], out4 = out rounded up to 4 (required by the FP32 path)
+ *   mlp_packed      device buffer from snisr_pack_mlp(SNISR_F16 / SNISR_BF16) (MFMA path)
+ */
+typedef struct snisr_scene {
+  int32_t tile_shape[3];
+  float scene_min[3];
+  float tile_size;
+  int32_t n_tiles;
+  int32_t n_bricks;
+  int32_t feat_dim;
+  int32_t feat_dtype;
+  int32_t mlp_in;
+  int32_t mlp_width;
+  int32_t mlp_layers;
+  int32_t mlp_packed_dtype;   /* SNISR_F16 or SNISR_BF16 for mlp_packed */
+  int64_t mlp_stride;
+  int64_t brick_map_len;
+  const int32_t* tile_index_map;
+  const float* tile_centers;
+  const int32_t* tile_res;
+  const int32_t* tile_brick_base;
+  const int32_t* brick_map;
+  const float* sigma;
+  const void* feats;
+  const float* mlp_params;
+  const float* mlp_f32t;
+  const void* mlp_packed;
+} snisr_scene;
+
+/* March parameters (rendering/src/__init__.py:37-41; rendering/config/base.yaml:8). */
+typedef struct snisr_march_params {
+  int32_t max_tiles;     /* K: non-empty tiles per ray */
+  float sample_step;     /* world units along the unit direction */
+  float early_stop;      /* a ray stops once T < early_stop */
+  float trans_th;        /* depth records the first sample reached with T < trans_th */
+} snisr_march_params;
+
+/* Rays: origins f32[3] (origin_stride 0) or f32[N,3] (origin_stride 3); dirs f32[N,3]
+ * (any length; marching normalises them); t_hit f32[N] or NULL (march bound, e.g. from
+ * snisr_mesh_first_hit); visited i32[N,max_tiles] or NULL (NULL: the tile DDA runs inline). */
+typedef struct snisr_rays {
+  const float* origins;
+  int32_t origin_stride;
+  int32_t image_width;   /* >0: rays are an image of this width (enables 8x8 packets) */
+  int64_t n;
+  const float* dirs;
+  const float* t_hit;
+  const int32_t* visited;
+} snisr_rays;
+
+/* Outputs (NULL = not written): rgb f32[N,3]; T f32[N] final transmittance; n_samples i32[N];
+ * depth f32[N] (t of the first sample reached with T < trans_th, -1 if none);
+ * bricks i32[N,brick_cap] occupied bricks marched in order, -1 padded (debug). */
+typedef struct snisr_render_out {
+  float* rgb;
+  float* T;
+  int32_t* n_samples;
+  float* depth;
+  int32_t* bricks;
+  int32_t brick_cap;
+} snisr_render_out;
+
+const char* snisr_last_error(void);
+int snisr_version(void);
+/* sizeof of the descriptors (0 scene, 1 march params, 2 rays, 3 render out) so that foreign
+ * bindings can check their struct layouts; no reference counterpart. */
+int64_t snisr_abi_sizeof(int32_t which);
+
+/* A1 — compute_rays (rendering/src/compute_rays.cu:46-68, include/compute_rays.h:7-11).
+ * intrinsic f32[3] = (cx, cy, 1/f), c2w f32[9] row-major: device pointers.
+ * rays_d f32[H,W,3] = R [(px-cx)/f, (py-cy)/f, 1] (not normalised). */
+int snisr_compute_rays(const float* intrinsic, const float* c2w, int32_t H, int32_t W, float* rays_d,
+                       void* stream);
+
+/* A2 — tracing_tiles (rendering/src/raytile_tracing.cu:63-95, include/raytile_tracing.h:8-13).
+ * origin f32[3] device; rays_d f32[n,3]; visited i32[n,K], -1 padded. */
+int snisr_trace_tiles(const snisr_scene* scene, const float* origin, const float* rays_d, int64_t n, int32_t K,
+                      int32_t* visited, void* stream);
+
+/* A3+A4+A5+A7 — one-thread-per-ray FP32 render: tile DDA, brick DDA, uniform march,
+ * trilinear sigma + features, per-tile FP32 MLP, early-terminating compositing.  Replaces
+ * rendering_diffuse_octree_fp16 (rendering/src/raydiffuse_render_octree_fp16.cu:238-287,
+ * include/raydiffuse_render_octree.h:7-21) and the FP32 marcher render_diffuse_kernel
+ * (training/src/cuda_func.cu:1687-1802) for the synthetic feature scenes. */
+int snisr_render_fp32(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
+                      const snisr_render_out* out, void* stream);
+
+/* Workspace bytes for snisr_render_mfma on n rays with room for `sample_capacity` samples
+ * (0 = a default of 48 samples per ray). */
+int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t sample_capacity);
+
+/* Same render with the per-tile MLP on MFMA (FP16 or BF16 operands, FP32 accumulation,
+ * FP32 transmittance).  Three stages: march (sigma only) emitting per-sample records bucketed
+ * by tile, a per-tile MFMA MLP over the buckets, and an ordered per-ray reduction.  Sample
+ * counts, T, depth and the brick lists are bit-identical to snisr_render_fp32. */
+int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
+                      const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Bytes of a packed weight buffer of the scene's MLP shape for `dtype`, and the packing itself
+ * (device to device, from scene->mlp_params into `packed`): SNISR_F32 gives the mlp_f32t layout,
+ * SNISR_F16 / SNISR_BF16 the MFMA fragment layout of mlp_packed. */
+int64_t snisr_pack_mlp_size(const snisr_scene* scene, int32_t dtype);
+int snisr_pack_mlp(const snisr_scene* scene, int32_t dtype, void* packed, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SNISR_H */

@@ -1,0 +1,240 @@
+"""ORACLE — test infrastructure only (imported by tests/, the smoke check and bench.py's
+cpu_baseline leg).  ctypes front end of ``liboracle_snisr.so``, the host C++/OpenMP
+restatement of the reference renderer's per-ray logic (snisr_oracle.cpp cites the reference
+file:line of every function)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_snisr.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "liboracle_snisr.so"], check=True)
+    return _LIB_PATH
+
+
+class _SceneC(C.Structure):
+    _fields_ = [("tile_shape", C.c_int * 3), ("scene_min", C.c_float * 3), ("tile_size", C.c_float),
+                ("tile_index_map", C.c_void_p), ("n_tiles", C.c_int), ("tile_centers", C.c_void_p),
+                ("tile_res", C.c_void_p), ("tile_brick_base", C.c_void_p), ("brick_map", C.c_void_p),
+                ("sigma", C.c_void_p), ("feats", C.c_void_p), ("feat_dim", C.c_int), ("feat_half", C.c_int),
+                ("mlp_in", C.c_int), ("mlp_width", C.c_int), ("mlp_layers", C.c_int),
+                ("mlp_params", C.c_void_p), ("mlp_stride", C.c_longlong)]
+
+
+class _ParamsC(C.Structure):
+    _fields_ = [("max_tiles", C.c_int), ("sample_step", C.c_float), ("early_stop", C.c_float),
+                ("trans_th", C.c_float), ("brick_debug", C.c_int)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _lib.orc_expf_c.restype = C.c_float
+        _lib.orc_expf_c.argtypes = [C.c_float]
+        _lib.orc_num_threads.restype = C.c_int
+        _lib.orc_half_to_float_c.restype = C.c_float
+        _lib.orc_half_to_float_c.argtypes = [C.c_uint16]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OracleScene:
+    """Holds contiguous copies of a ``snisr.scene.Scene``'s arrays and the C descriptor."""
+
+    def __init__(self, sc):
+        self.keep = dict(
+            tim=np.ascontiguousarray(sc.tile_index_map, np.int32),
+            tc=np.ascontiguousarray(sc.tile_centers, np.float32),
+            tr=np.ascontiguousarray(sc.tile_res, np.int32),
+            tb=np.ascontiguousarray(sc.tile_brick_base, np.int32),
+            bm=np.ascontiguousarray(sc.brick_map, np.int32),
+            sg=np.ascontiguousarray(sc.sigma, np.float32),
+            ft=np.ascontiguousarray(sc.feats),
+            mp=np.ascontiguousarray(sc.mlp_params, np.float32),
+        )
+        k = self.keep
+        c = _SceneC()
+        c.tile_shape[:] = [int(v) for v in sc.tile_shape]
+        c.scene_min[:] = [float(v) for v in sc.scene_min]
+        c.tile_size = sc.tile_size
+        c.tile_index_map = _p(k["tim"])
+        c.n_tiles = sc.n_tiles
+        c.tile_centers = _p(k["tc"])
+        c.tile_res = _p(k["tr"])
+        c.tile_brick_base = _p(k["tb"])
+        c.brick_map = _p(k["bm"])
+        c.sigma = _p(k["sg"])
+        c.feats = _p(k["ft"])
+        c.feat_dim = sc.feat_dim
+        c.feat_half = 1 if k["ft"].dtype == np.float16 else 0
+        c.mlp_in, c.mlp_width, c.mlp_layers = sc.mlp_in, sc.mlp_width, sc.mlp_layers
+        c.mlp_params = _p(k["mp"])
+        c.mlp_stride = k["mp"].shape[1]
+        self.c = c
+        self.sc = sc
+
+    def params(self, brick_debug=0, max_tiles=None, early_stop=None, sample_step=None):
+        sc = self.sc
+        p = _ParamsC()
+        p.max_tiles = sc.max_tiles if max_tiles is None else max_tiles
+        p.sample_step = sc.sample_step if sample_step is None else sample_step
+        p.early_stop = sc.early_stop if early_stop is None else early_stop
+        p.trans_th = sc.trans_th
+        p.brick_debug = brick_debug
+        return p
+
+
+def expf(x: float) -> float:
+    return lib().orc_expf_c(float(x))
+
+
+def half_to_float(bits: int) -> float:
+    return lib().orc_half_to_float_c(int(bits))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def compute_rays(H, W, intrinsic, c2w, dtype=np.float32):
+    out = np.empty((H * W, 3), dtype)
+    fn = lib().orc_compute_rays_f32 if dtype == np.float32 else lib().orc_compute_rays_f64
+    fn(C.c_int(H), C.c_int(W), _p(np.ascontiguousarray(intrinsic, np.float32)),
+       _p(np.ascontiguousarray(c2w, np.float32)), _p(out))
+    return out
+
+
+def ray_aabb(o, d, c, h):
+    out = np.empty(2, np.float32)
+    a = [np.ascontiguousarray(v, np.float32) for v in (o, d, c, h)]
+    lib().orc_ray_aabb_f32(*[_p(v) for v in a], _p(out))
+    return out
+
+
+def trace_tiles(osc: OracleScene, origin, dirs, K):
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.empty((dirs.shape[0], K), np.int32)
+    lib().orc_trace_tiles_f32(C.byref(osc.c), _p(np.ascontiguousarray(origin, np.float32)), _p(dirs),
+                              C.c_longlong(dirs.shape[0]), C.c_int(K), _p(out))
+    return out
+
+
+def render(osc: OracleScene, origins, dirs, t_hit=None, visited=None, ray_index=None, precision="f32",
+           with_mlp=True, brick_debug=0, **pkw):
+    """Per-ray render.  ``origins`` is f32[3] (shared) or f32[N,3].  Returns a dict of
+    rgb f32[M,3], T f32[M], n_samples i32[M], depth f32[M] (+ bricks i32[M,brick_debug])."""
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    N = dirs.shape[0]
+    origins = np.ascontiguousarray(origins, np.float32)
+    ostride = 0 if origins.size == 3 else 3
+    if ray_index is not None:
+        ray_index = np.ascontiguousarray(ray_index, np.int64)
+        M = ray_index.shape[0]
+    else:
+        M = N
+    p = osc.params(brick_debug=brick_debug, **pkw)
+    rgb = np.zeros((M, 3), np.float32)
+    T = np.zeros(M, np.float32)
+    ns = np.zeros(M, np.int32)
+    depth = np.zeros(M, np.float32)
+    bricks = np.full((M, brick_debug), -1, np.int32) if brick_debug else None
+    if t_hit is not None:
+        t_hit = np.ascontiguousarray(t_hit, np.float32)
+    if visited is not None:
+        visited = np.ascontiguousarray(visited, np.int32)
+        assert visited.shape == (N, p.max_tiles)
+    fn = lib().orc_render_f32 if precision == "f32" else lib().orc_render_f64
+    fn(C.byref(osc.c), C.byref(p), _p(origins), C.c_int(ostride), _p(dirs), _p(t_hit), _p(visited),
+       C.c_longlong(M), _p(ray_index), _p(rgb), _p(T), _p(ns), _p(depth), _p(bricks), C.c_int(1 if with_mlp else 0))
+    out = dict(rgb=rgb, T=T, n_samples=ns, depth=depth)
+    if bricks is not None:
+        out["bricks"] = bricks
+    return out
+
+
+def trilinear(osc: OracleScene, tile, f):
+    f = np.ascontiguousarray(f, np.float32).reshape(-1, 3)
+    M = f.shape[0]
+    sig = np.empty(M, np.float32)
+    feat = np.empty((M, osc.sc.feat_dim), np.float32)
+    lib().orc_trilinear_f32(C.byref(osc.c), C.c_int(tile), _p(f), C.c_longlong(M), _p(sig), _p(feat))
+    return sig, feat
+
+
+def mlp(osc: OracleScene, tile, x):
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, osc.sc.mlp_in)
+    out = np.empty((x.shape[0], 3), np.float64)
+    lib().orc_mlp_f64(C.byref(osc.c), C.c_int(tile), _p(x), C.c_longlong(x.shape[0]), _p(out))
+    return out
+
+
+def view_encoding(d):
+    d = np.ascontiguousarray(d, np.float64).reshape(-1, 3)
+    out = np.empty((d.shape[0], 15), np.float64)
+    lib().orc_view_encoding_f64(_p(d), C.c_longlong(d.shape[0]), _p(out))
+    return out
+
+
+def normalize(d):
+    d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+    out = np.empty_like(d)
+    lib().orc_normalize_f32(_p(d), C.c_longlong(d.shape[0]), _p(out))
+    return out
+
+
+def mesh_hit(vertices, faces, origins, dirs):
+    V = np.ascontiguousarray(vertices, np.float32)
+    Fc = np.ascontiguousarray(faces, np.int32)
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    N = dirs.shape[0]
+    o = np.ascontiguousarray(np.broadcast_to(np.asarray(origins, np.float32).reshape(-1, 3), (N, 3)))
+    t = np.empty(N, np.float32)
+    ids = np.empty(N, np.int32)
+    lib().orc_mesh_hit_f32(_p(V), _p(Fc), C.c_int(Fc.shape[0]), _p(o), _p(dirs), C.c_longlong(N), _p(t), _p(ids))
+    return t, ids
+
+
+def inverse_z_sampling(origin, dirs, inv_near, inv_bound, offset, spr):
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = dirs.shape[0]
+    pts = np.empty((n * spr, 3), np.float32)
+    st = np.empty(n * spr, np.float32)
+    lib().orc_inverse_z_sampling_f32(_p(np.ascontiguousarray(origin, np.float32)), _p(dirs),
+                                     _p(np.ascontiguousarray(inv_near, np.float32).ravel()),
+                                     _p(np.ascontiguousarray(inv_bound, np.float32).ravel()),
+                                     C.c_longlong(n), C.c_int(offset), C.c_int(spr), _p(pts), _p(st))
+    return pts, st
+
+
+def integrate_points(sample_results, steps, early, T, color):
+    """In-place on T (f32[n]) and color (f32[n,3])."""
+    res = np.ascontiguousarray(sample_results, np.float32)
+    st = np.ascontiguousarray(steps, np.float32)
+    n = T.size
+    lib().orc_integrate_points_f32(_p(res), _p(st), C.c_longlong(n), C.c_float(early), _p(T), _p(color))
+
+
+def assign_pixels(net, pixel_starts, block_starts):
+    net = np.ascontiguousarray(net, np.int16)
+    ps = np.ascontiguousarray(pixel_starts, np.int32)
+    bs = np.ascontiguousarray(block_starts, np.int32)
+    nb = int(bs[-1])
+    new_net = np.empty(nb, np.int16)
+    new_starts = np.empty(nb, np.int32)
+    lib().orc_assign_pixels(_p(net), _p(ps), _p(bs), C.c_int(net.shape[0]), _p(new_net), _p(new_starts))
+    return new_net, new_starts

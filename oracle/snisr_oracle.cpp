@@ -1,0 +1,686 @@
+// ORACLE — test infrastructure only.
+//
+// A host C++/OpenMP restatement of the reference renderer's per-ray logic, templated
+// on the arithmetic type (float = parity oracle, double = accuracy oracle).  It is
+// used by tests/, by the smoke check and by bench.py's cpu_baseline leg, never by
+// the product path.  Every function cites the reference file:line it restates
+// (paths relative to the reference checkout).
+//
+// Arithmetic contract shared with the GPU kernels (built with -ffp-contract=off):
+//   * no FMA contraction except where fma() is written explicitly;
+//   * float division and sqrt are IEEE correctly rounded;
+//   * exp() in float is the Cody-Waite / Taylor-Horner routine orc_expf below
+//     (the GPU path implements the same published method independently);
+//   * the DDA tie rules, floor/truncation and safe_divide's 1e8 follow the reference.
+//
+// Pinning (what each function is checked against; see tests/test_oracle_*.py):
+//   ray_aabb            slab closed form on random boxes / rays
+//   trace_tiles         brute-force enumeration of grid cells hit by the ray segment
+//   brick DDA + march   brute-force sample enumeration on tiny scenes; Python loop
+//   trilinear           torch.nn.functional.grid_sample(align_corners=True, zeros)
+//   composite           constant-sigma closed form T = exp(-sigma L); sum w + T = 1
+//   mlp                 numpy matmul
+//   mesh hit            analytic hits on axis-aligned boxes; numpy vectorised MT
+//   inverse_z / integrate / assign   closed forms and numpy restatements
+//   train adjoint       FP64 central finite differences
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+// ------------------------------------------------------------------ math helpers
+// exp in float: Cody-Waite range reduction x = k*ln2 + r (ln2 split in a 16-bit
+// head and a tail, Cody & Waite 1980), then the degree-7 Taylor polynomial of e^r
+// in Horner form with fused multiply-adds, scaled by 2^k with ldexp.  Arguments
+// below ln(FLT_MIN) return 0 so that no denormal is ever produced.
+static inline float orc_expf(float x) {
+  if (!(x == x)) return x;
+  if (x < -87.33654475f) return 0.0f;
+  if (x > 88.72283905f) return std::numeric_limits<float>::infinity();
+  const float k = rintf(x * 0x1.715476p+0f);
+  float r = fmaf(-k, 0x1.62e400p-1f, x);
+  r = fmaf(-k, 0x1.7f7d1cp-20f, r);
+  float p = 0x1.a01a02p-13f;          // 1/5040
+  p = fmaf(p, r, 0x1.6c16c2p-10f);    // 1/720
+  p = fmaf(p, r, 0x1.111112p-7f);     // 1/120
+  p = fmaf(p, r, 0x1.555556p-5f);     // 1/24
+  p = fmaf(p, r, 0x1.555556p-3f);     // 1/6
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  return ldexpf(p, (int)k);
+}
+template <class R> static inline R rexp(R x);
+template <> inline float rexp<float>(float x) { return orc_expf(x); }
+template <> inline double rexp<double>(double x) { return std::exp(x); }
+
+template <class R> struct V3 { R x, y, z; };
+template <class R> static inline V3<R> v3(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R> static inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> static inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> static inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> static inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> static inline V3<R> cross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> static inline R at(const V3<R>& a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+template <class R> static inline void set(V3<R>& a, int i, R v) { if (i == 0) a.x = v; else if (i == 1) a.y = v; else a.z = v; }
+
+// safe_divide (rendering/src/include/cutil_math.h:867-869): b != 0 ? a/b : 1e8
+template <class R> static inline R safe_div(R a, R b) { return b != R(0) ? a / b : R(100000000.0); }
+// signf (cutil_math.h:856-859): a >= 0 ? 1 : -1  (so -0.0 gives +1)
+template <class R> static inline int signi(R a) { return a >= R(0) ? 1 : -1; }
+// clamp(v, lo, hi) = max(lo, min(v, hi)) per component (cutil_math.h clamp)
+template <class R> static inline R clampr(R v, R lo, R hi) { return std::max(lo, std::min(v, hi)); }
+
+// normalised direction: d * (1 / sqrt(dot(d, d)))   (training path normalises,
+// training/src/cuda_func.cu:99; the render entry point applies it to every ray)
+template <class R> static inline V3<R> normalize3(V3<R> d) {
+  R inv = R(1) / std::sqrt(dot(d, d));
+  return inv * d;
+}
+
+// RayAABBIntersection (rendering/src/include/cuda_utils.h:553-598), per-axis half sizes.
+template <class R>
+static inline void ray_aabb(V3<R> o, V3<R> d, V3<R> c, V3<R> h, R& tlo, R& thi) {
+  R lo = 0, hi = R(100000.0);
+  for (int a = 0; a < 3; ++a) {
+    R inv = safe_div(R(1), at(d, a));
+    R l = (at(c, a) - at(h, a) - at(o, a)) * inv;
+    R u = (at(c, a) + at(h, a) - at(o, a)) * inv;
+    if (u < l) std::swap(l, u);
+    if (u < lo || l > hi) { tlo = thi = R(-1); return; }
+    lo = l > lo ? l : lo;
+    hi = u < hi ? u : hi;
+    if (lo > hi) { tlo = thi = R(-1); return; }
+  }
+  tlo = lo; thi = hi;
+}
+
+// ------------------------------------------------------------------ scene
+struct Scene {
+  int tile_shape[3];
+  float scene_min[3];
+  float tile_size;
+  const int* tile_index_map;   // dense tile cell -> tile id or -1   (x*Ty*Tz + y*Tz + z)
+  int n_tiles;
+  const float* tile_centers;   // [T,3]
+  const int* tile_res;         // [T] voxels per side (multiple of 8)
+  const int* tile_brick_base;  // [T] offset of the tile's brick table in brick_map
+  const int* brick_map;        // bricks of 8^3 voxels -> brick id or -1
+  const float* sigma;          // [NB, 512]
+  const void* feats;           // [NB, 512, F]
+  int feat_dim;
+  int feat_half;               // 1: feats are IEEE binary16, 0: float32
+  int mlp_in, mlp_width, mlp_layers;  // out = 3
+  const float* mlp_params;     // [T, P]  per layer: W[out][in] then b[out]
+  long long mlp_stride;        // P
+};
+
+struct Params {
+  int max_tiles;        // K, tiles recorded per ray (base.yaml:8)
+  float sample_step;    // world units along the unit direction
+  float early_stop;     // stop the ray once T < early_stop
+  float trans_th;       // first-hit threshold on T (__init__.py:37, trans_decay)
+  int brick_debug;      // capacity of the visited-brick debug list (0 = off)
+};
+
+static inline float half_to_float(uint16_t h) {
+  uint32_t s = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 0x1f, m = h & 0x3ff;
+  uint32_t bits;
+  if (e == 0) {
+    if (m == 0) bits = s;
+    else {  // subnormal half -> normal float
+      int ee = -1;
+      do { m <<= 1; ++ee; } while (!(m & 0x400));
+      bits = s | ((uint32_t)(127 - 15 - ee) << 23) | ((m & 0x3ff) << 13);
+    }
+  } else if (e == 31) bits = s | 0x7f800000u | (m << 13);
+  else bits = s | ((e + 112) << 23) | (m << 13);
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
+
+// one voxel corner lookup with zero padding outside the tile / in empty bricks
+static inline long long voxel_slot(const Scene& S, int tile, int ix, int iy, int iz) {
+  const int n = S.tile_res[tile];
+  if (ix < 0 || iy < 0 || iz < 0 || ix >= n || iy >= n || iz >= n) return -1;
+  const int nb = n >> 3;
+  const int b = S.brick_map[S.tile_brick_base[tile] + ((ix >> 3) * nb + (iy >> 3)) * nb + (iz >> 3)];
+  if (b < 0) return -1;
+  return (long long)b * 512 + (((ix & 7) * 8 + (iy & 7)) * 8 + (iz & 7));
+}
+
+template <class R> static inline R lerp(R a, R b, R t) { return a * (R(1) - t) + b * t; }
+
+// Trilinear interpolation at continuous voxel coordinate f (= p/voxel - 0.5):
+// corner order and the x, then y, then z lerp nesting follow interpolate3D
+// (rendering/src/include/interpolation.h:13-38); corner weights from floor(f),
+// which is the truncation of trilinear_base (interpolation.h:49-73) on the
+// non-negative coordinates it sees; empty voxels read as zero (tile.py:232
+// grid_sample padding_mode="zeros").  Writes sigma into *sig and F features.
+template <class R>
+static void trilinear(const Scene& S, int tile, V3<R> f, R* sig, R* feat) {
+  const R fx = std::floor(f.x), fy = std::floor(f.y), fz = std::floor(f.z);
+  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+  const R wx = f.x - fx, wy = f.y - fy, wz = f.z - fz;
+  long long slot[8];
+  // v1..v8 of interpolate3D: x0y0z0, x1y0z0, x0y1z0, x1y1z0, x0y0z1, x1y0z1, x0y1z1, x1y1z1
+  for (int c = 0; c < 8; ++c) slot[c] = voxel_slot(S, tile, ix + (c & 1), iy + ((c >> 1) & 1), iz + (c >> 2));
+  auto sv = [&](int c) -> R { return slot[c] < 0 ? R(0) : R(S.sigma[slot[c]]); };
+  {
+    R s = lerp(lerp(sv(0), sv(1), wx), lerp(sv(2), sv(3), wx), wy);
+    R t = lerp(lerp(sv(4), sv(5), wx), lerp(sv(6), sv(7), wx), wy);
+    *sig = lerp(s, t, wz);
+  }
+  if (!feat) return;
+  const int F = S.feat_dim;
+  for (int k = 0; k < F; ++k) {
+    auto fv = [&](int c) -> R {
+      if (slot[c] < 0) return R(0);
+      long long i = slot[c] * F + k;
+      return S.feat_half ? R(half_to_float(((const uint16_t*)S.feats)[i])) : R(((const float*)S.feats)[i]);
+    };
+    R s = lerp(lerp(fv(0), fv(1), wx), lerp(fv(2), fv(3), wx), wy);
+    R t = lerp(lerp(fv(4), fv(5), wx), lerp(fv(6), fv(7), wx), wy);
+    feat[k] = lerp(s, t, wz);
+  }
+}
+
+// Per-sample MLP: x = [features(F), PE_2(dir)(15)] -> Linear/ReLU ... -> 3 -> sigmoid.
+// Linear layout W[out][in] then b[out] per layer (networks.py:17-31 nn.Linear order);
+// the view encoding is Positional_Encoding(L=2) (networks.py:8-13): [d, sin d, cos d,
+// sin 2d, cos 2d]; ReLU hidden activations, sigmoid RGB (tile.py:194).
+template <class R>
+static void mlp_eval(const Scene& S, int tile, const R* x, R* rgb_out, std::vector<R>& bufa, std::vector<R>& bufb) {
+  const float* P = S.mlp_params + (long long)tile * S.mlp_stride;
+  const int L = S.mlp_layers, Wd = S.mlp_width;
+  int in = S.mlp_in;
+  bufa.assign(x, x + in);
+  for (int l = 0; l < L; ++l) {
+    const int out = (l == L - 1) ? 3 : Wd;
+    const float* W = P;
+    const float* b = P + (long long)out * in;
+    bufb.resize(out);
+    for (int j = 0; j < out; ++j) {
+      R acc = R(b[j]);
+      const float* w = W + (long long)j * in;
+      for (int i = 0; i < in; ++i) acc += R(w[i]) * bufa[i];
+      bufb[j] = (l == L - 1) ? acc : (acc > R(0) ? acc : R(0));
+    }
+    P = b + out;
+    in = out;
+    std::swap(bufa, bufb);
+  }
+  for (int c = 0; c < 3; ++c) rgb_out[c] = R(1) / (R(1) + rexp(-bufa[c]));
+}
+
+template <class R>
+static void view_encoding(V3<R> d, R* e) {
+  e[0] = d.x; e[1] = d.y; e[2] = d.z;
+  for (int l = 0; l < 2; ++l) {
+    const R s = R(1 << l);
+    e[3 + 6 * l + 0] = std::sin(s * d.x);
+    e[3 + 6 * l + 1] = std::sin(s * d.y);
+    e[3 + 6 * l + 2] = std::sin(s * d.z);
+    e[3 + 6 * l + 3] = std::cos(s * d.x);
+    e[3 + 6 * l + 4] = std::cos(s * d.y);
+    e[3 + 6 * l + 5] = std::cos(s * d.z);
+  }
+}
+
+// ------------------------------------------------------------------ A1 camera rays
+// compute_rays_kernel / get_direction (rendering/src/compute_rays.cu:7-44):
+// x = (px - cx) * invf, y = (py - cy) * invf, d = R [x, y, 1]  (not normalised).
+template <class R>
+static void compute_rays(int H, int W, const float* intr, const float* c2w, R* out) {
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)H * W; ++i) {
+    const int py = (int)(i / W), px = (int)(i - (long long)py * W);
+    const R x = (R(px) - R(intr[0])) * R(intr[2]);
+    const R y = (R(py) - R(intr[1])) * R(intr[2]);
+    for (int r = 0; r < 3; ++r) out[i * 3 + r] = R(c2w[3 * r]) * x + R(c2w[3 * r + 1]) * y + R(c2w[3 * r + 2]);
+  }
+}
+
+// ------------------------------------------------------------------ A2 tile DDA
+// voxel_traversal (rendering/src/include/cuda_utils.h:158-238) over the tile grid with
+// the entry clamp of tracing_tiles_kernel (rendering/src/raytile_tracing.cu:30-47).
+// Deviation: a start cell outside the grid (possible when scene_size - 1e-6 rounds
+// up to scene_size) records nothing instead of reading IndexMap out of bounds.
+template <class R>
+static int trace_tiles_one(const Scene& S, V3<R> origin, V3<R> d, int K, int* visited) {
+  for (int j = 0; j < K; ++j) visited[j] = -1;
+  if (K <= 0) return 0;
+  const V3<R> smin = v3<R>(S.scene_min[0], S.scene_min[1], S.scene_min[2]);
+  const R ts = R(S.tile_size);
+  const V3<R> size = v3<R>(R(S.tile_shape[0]) * ts, R(S.tile_shape[1]) * ts, R(S.tile_shape[2]) * ts);
+  V3<R> o = origin - smin;
+  if (o.x < 0 || o.x >= size.x || o.y < 0 || o.y >= size.y || o.z < 0 || o.z >= size.z) {
+    const V3<R> half = v3<R>(size.x / R(2), size.y / R(2), size.z / R(2));
+    R tl, th;
+    ray_aabb(origin, d, smin + half, half, tl, th);
+    o = o + tl * d;
+    const R eps = R(0.000001f);
+    o = v3<R>(clampr(o.x, R(0), size.x - eps), clampr(o.y, R(0), size.y - eps), clampr(o.z, R(0), size.z - eps));
+  }
+  const int M[3] = {S.tile_shape[0], S.tile_shape[1], S.tile_shape[2]};
+  int cur[3], step[3];
+  R tmax[3], tdel[3];
+  for (int a = 0; a < 3; ++a) {
+    cur[a] = (int)std::floor(at(o, a) / ts);
+    step[a] = at(d, a) >= 0 ? 1 : -1;
+    R nb = R(cur[a] + step[a]) * ts;
+    if (step[a] < 0) nb += ts;
+    tmax[a] = at(d, a) != 0 ? (nb - at(o, a)) / at(d, a) : R(100000000);
+    tdel[a] = at(d, a) != 0 ? (ts / at(d, a)) * R(step[a]) : R(100000000);
+  }
+  auto inside = [&]() {
+    return cur[0] >= 0 && cur[1] >= 0 && cur[2] >= 0 && cur[0] < M[0] && cur[1] < M[1] && cur[2] < M[2];
+  };
+  if (!inside()) return 0;
+  int idx = 0;
+  int id = S.tile_index_map[(cur[0] * M[1] + cur[1]) * M[2] + cur[2]];
+  if (id != -1) visited[idx++] = id;
+  while (idx < K) {
+    int a;
+    if (tmax[0] < tmax[1]) a = (tmax[0] < tmax[2]) ? 0 : 2;
+    else a = (tmax[1] < tmax[2]) ? 1 : 2;
+    cur[a] += step[a];
+    tmax[a] += tdel[a];
+    if (!inside()) break;
+    id = S.tile_index_map[(cur[0] * M[1] + cur[1]) * M[2] + cur[2]];
+    if (id != -1) visited[idx++] = id;
+  }
+  return idx;
+}
+
+// ------------------------------------------------------------------ A3/A4/A5/A7/A10 march
+struct RayOut {
+  float rgb[3];
+  float T;
+  int n_samples;
+  float depth;  // t of the first sample reached with T < trans_th, -1 if none
+  int n_bricks;
+};
+
+// One ray through the visited tiles.  Per tile: RayAABB entry (raydiffuse_render_octree_fp16.cu:202-213,
+// local origin clamped to [0, tile_size - 1e-5]), then the block DDA of block_traversal_rendering
+// (raydiffuse_render_octree_fp16.cu:39-138) over 8^3-voxel bricks; inside an occupied brick the
+// samples restart at the brick entry "near" and are z_k = fma(k, step, near) while z_k < far
+// (the reference accumulates zval += step, :71-105; the fma form is the stated deviation),
+// interval = min(step, far - z_k) (:101); compositing alpha = 1 - exp(-sigma*interval),
+// C += alpha*T*rgb, T *= 1 - alpha (rendering/src/include/cuda_utils.h:13-20); the ray stops once
+// T < early_stop after a sample (training/src/cuda_func.cu:1778-1791) or at the first sample whose
+// global t reaches the proxy-mesh hit t_hit.  sigma = max(trilinear sigma, 0); rgb = MLP(features).
+template <class R>
+static void march_ray(const Scene& S, const Params& P, V3<R> origin, V3<R> d_raw, R t_hit,
+                      const int* visited_in, RayOut& out, int* brick_dbg, bool with_mlp) {
+  const V3<R> d = normalize3(d_raw);
+  std::vector<int> vis(std::max(P.max_tiles, 1));
+  int nvis;
+  if (visited_in) {
+    nvis = 0;
+    while (nvis < P.max_tiles && visited_in[nvis] != -1) { vis[nvis] = visited_in[nvis]; ++nvis; }
+  } else {
+    nvis = trace_tiles_one(S, origin, d, P.max_tiles, vis.data());
+  }
+  R color[3] = {0, 0, 0};
+  R T = 1;
+  int nsamp = 0, nbr = 0;
+  R depth = -1;
+  bool first_hit = false, done = false;
+  const R ts = R(S.tile_size);
+  const R step = R(P.sample_step);
+  const int F = S.feat_dim;
+  std::vector<R> x(S.mlp_in), ba, bb;
+  R enc[15];
+  if (with_mlp) view_encoding(d, enc);
+  for (int vi = 0; vi < nvis && !done; ++vi) {
+    const int tile = vis[vi];
+    if (T < R(P.early_stop)) break;
+    const V3<R> tc = v3<R>(S.tile_centers[3 * tile], S.tile_centers[3 * tile + 1], S.tile_centers[3 * tile + 2]);
+    const R hs = ts / R(2);
+    R tl, th;
+    ray_aabb(origin, d, tc, v3<R>(hs, hs, hs), tl, th);
+    if (tl == R(-1)) continue;
+    const V3<R> corner = v3<R>(tc.x - hs, tc.y - hs, tc.z - hs);
+    V3<R> o = (origin + tl * d) - corner;
+    const R hi = ts - R(0.00001f);
+    o = v3<R>(clampr(o.x, R(0), hi), clampr(o.y, R(0), hi), clampr(o.z, R(0), hi));
+    const int n = S.tile_res[tile];
+    const int nb = n >> 3;
+    const R bs = ts / R(nb);
+    const R inv_vs = R(1) / (ts / R(n));
+    const int* bmap = S.brick_map + S.tile_brick_base[tile];
+    int cur[3], stp[3];
+    R tmax[3], tdel[3];
+    for (int a = 0; a < 3; ++a) {
+      cur[a] = (int)(at(o, a) / bs);  // make_int3(float3) truncates; o >= 0
+      stp[a] = signi(at(d, a));
+      R nbd = R(cur[a] + stp[a]) * bs;
+      if (stp[a] < 0) nbd += bs;
+      tmax[a] = safe_div(nbd - at(o, a), at(d, a));
+      tdel[a] = safe_div(bs * R(stp[a]), at(d, a));
+    }
+    R near_ = 0, far_ = std::min(std::min(tmax[0], tmax[1]), tmax[2]);
+    while (true) {
+      if (cur[0] < 0 || cur[1] < 0 || cur[2] < 0 || cur[0] >= nb || cur[1] >= nb || cur[2] >= nb) break;
+      const int b = bmap[(cur[0] * nb + cur[1]) * nb + cur[2]];
+      if (b != -1) {
+        if (brick_dbg && nbr < P.brick_debug) brick_dbg[nbr] = b;
+        ++nbr;
+        for (int k = 0;; ++k) {
+          const R z = std::fma(R(k), step, near_);
+          if (!(z < far_)) break;
+          const R tg = tl + z;
+          if (tg >= t_hit) { done = true; break; }
+          const V3<R> p = o + z * d;
+          const V3<R> f = v3<R>(p.x * inv_vs - R(0.5), p.y * inv_vs - R(0.5), p.z * inv_vs - R(0.5));
+          R sig;
+          trilinear(S, tile, f, &sig, with_mlp ? x.data() : nullptr);
+          sig = sig > R(0) ? sig : R(0);
+          if (!first_hit && T < R(P.trans_th)) { first_hit = true; depth = tg; }
+          const R interval = std::min(step, far_ - z);
+          const R alpha = R(1) - rexp(-(sig * interval));
+          const R w = alpha * T;
+          if (with_mlp) {
+            for (int i = 0; i < 15; ++i) x[F + i] = enc[i];
+            R rgb[3];
+            mlp_eval(S, tile, x.data(), rgb, ba, bb);
+            for (int c = 0; c < 3; ++c) color[c] = color[c] + w * rgb[c];
+          }
+          T = T * (R(1) - alpha);
+          ++nsamp;
+          if (T < R(P.early_stop)) { done = true; break; }
+        }
+        if (done) break;
+      }
+      int a;
+      if (tmax[0] < tmax[1]) a = (tmax[0] < tmax[2]) ? 0 : 2;
+      else a = (tmax[1] < tmax[2]) ? 1 : 2;
+      cur[a] += stp[a];
+      tmax[a] += tdel[a];
+      near_ = far_;
+      far_ = std::min(std::min(tmax[0], tmax[1]), tmax[2]);
+    }
+  }
+  for (int c = 0; c < 3; ++c) out.rgb[c] = float(color[c]);
+  out.T = float(T);
+  out.n_samples = nsamp;
+  out.depth = float(depth);
+  out.n_bricks = nbr;
+}
+
+// ------------------------------------------------------------------ A9 mesh hit
+// RayTriangleIntersection (rendering/src/include/cuda_utils.h:403-436, blur = 0) with the
+// two-sided vertex flip of ray_firstHit_inTile_kernel (training/src/cuda_func.cu:215-231).
+// Deviation: the flip tests the sign of dot(d, cross(v1-v0, v2-v1)) without normalising the
+// normal (same sign).  Returns -1e8 on a miss.
+template <class R>
+static inline R ray_tri(V3<R> o, V3<R> d, V3<R> a, V3<R> b, V3<R> c) {
+  const V3<R> e1 = b - a, e2 = c - b;
+  const R sim = dot(d, cross(e1, e2));
+  V3<R> v1 = b, v2 = c;
+  if (sim > 0) { v1 = c; v2 = b; }
+  const V3<R> v0v1 = v1 - a, v0v2 = v2 - a, v0o = o - a;
+  const V3<R> pv = cross(d, v0v2);
+  R det = dot(v0v1, pv);
+  if (det < R(0.00000001f)) return R(-100000000);
+  det = R(1) / det;
+  const R u = dot(v0o, pv) * det;
+  if (u < R(0) || u > R(1)) return R(-100000000);
+  const V3<R> qv = cross(v0o, v0v1);
+  const R v = dot(d, qv) * det;
+  if (v < R(0) || v > R(1)) return R(-100000000);
+  if ((u + v) < R(0) || (u + v) > R(1)) return R(-100000000);
+  return dot(v0v2, qv) * det;
+}
+
+// nearest hit: smallest t > 0, ties to the smaller triangle id (brute force over all triangles)
+template <class R>
+static void mesh_hit_one(const float* V, const int* Fc, int nf, V3<R> o, V3<R> d, R* t_out, int* id_out) {
+  R best = std::numeric_limits<R>::infinity();
+  int bid = -1;
+  for (int f = 0; f < nf; ++f) {
+    const int* ix = Fc + 3 * f;
+    const V3<R> a = v3<R>(V[3 * ix[0]], V[3 * ix[0] + 1], V[3 * ix[0] + 2]);
+    const V3<R> b = v3<R>(V[3 * ix[1]], V[3 * ix[1] + 1], V[3 * ix[1] + 2]);
+    const V3<R> c = v3<R>(V[3 * ix[2]], V[3 * ix[2] + 1], V[3 * ix[2] + 2]);
+    const R t = ray_tri(o, d, a, b, c);
+    if (t > R(0) && t < best) { best = t; bid = f; }
+  }
+  *t_out = best;
+  *id_out = bid;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// =================================================================== C entry points
+extern "C" {
+
+struct orc_scene_c {
+  int tile_shape[3];
+  float scene_min[3];
+  float tile_size;
+  const int* tile_index_map;
+  int n_tiles;
+  const float* tile_centers;
+  const int* tile_res;
+  const int* tile_brick_base;
+  const int* brick_map;
+  const float* sigma;
+  const void* feats;
+  int feat_dim, feat_half;
+  int mlp_in, mlp_width, mlp_layers;
+  const float* mlp_params;
+  long long mlp_stride;
+};
+
+struct orc_params_c {
+  int max_tiles;
+  float sample_step, early_stop, trans_th;
+  int brick_debug;
+};
+
+static Scene to_scene(const orc_scene_c* c) {
+  Scene S;
+  std::memcpy(S.tile_shape, c->tile_shape, sizeof S.tile_shape);
+  std::memcpy(S.scene_min, c->scene_min, sizeof S.scene_min);
+  S.tile_size = c->tile_size;
+  S.tile_index_map = c->tile_index_map;
+  S.n_tiles = c->n_tiles;
+  S.tile_centers = c->tile_centers;
+  S.tile_res = c->tile_res;
+  S.tile_brick_base = c->tile_brick_base;
+  S.brick_map = c->brick_map;
+  S.sigma = c->sigma;
+  S.feats = c->feats;
+  S.feat_dim = c->feat_dim;
+  S.feat_half = c->feat_half;
+  S.mlp_in = c->mlp_in;
+  S.mlp_width = c->mlp_width;
+  S.mlp_layers = c->mlp_layers;
+  S.mlp_params = c->mlp_params;
+  S.mlp_stride = c->mlp_stride;
+  return S;
+}
+static Params to_params(const orc_params_c* p) {
+  return Params{p->max_tiles, p->sample_step, p->early_stop, p->trans_th, p->brick_debug};
+}
+
+float orc_expf_c(float x) { return orc_expf(x); }
+float orc_half_to_float_c(uint16_t h) { return half_to_float(h); }
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_compute_rays_f32(int H, int W, const float* intr, const float* c2w, float* out) {
+  compute_rays<float>(H, W, intr, c2w, out);
+}
+void orc_compute_rays_f64(int H, int W, const float* intr, const float* c2w, double* out) {
+  compute_rays<double>(H, W, intr, c2w, out);
+}
+
+void orc_ray_aabb_f32(const float* o, const float* d, const float* c, const float* h, float* out) {
+  float tl, th;
+  ray_aabb<float>(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), v3(c[0], c[1], c[2]), v3(h[0], h[1], h[2]), tl, th);
+  out[0] = tl; out[1] = th;
+}
+
+// A2 over N rays sharing one origin (tracing_tiles_kernel, rendering/src/raytile_tracing.cu:9-60)
+void orc_trace_tiles_f32(const orc_scene_c* sc, const float* origin, const float* dirs, long long N, int K,
+                         int* visited) {
+  Scene S = to_scene(sc);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long i = 0; i < N; ++i)
+    trace_tiles_one<float>(S, v3(origin[0], origin[1], origin[2]),
+                           v3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), K, visited + i * K);
+}
+
+}  // extern "C"
+template <class R>
+static void render_impl(const orc_scene_c* sc, const orc_params_c* pc, const float* origin, int ostride,
+                        const float* dirs, const float* t_hit, const int* visited, long long N, const long long* ray_index,
+                        float* rgb, float* T, int* nsamp, float* depth, int* brick_dbg, int with_mlp) {
+  Scene S = to_scene(sc);
+  Params P = to_params(pc);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long j = 0; j < N; ++j) {
+    const long long i = ray_index ? ray_index[j] : j;
+    RayOut o;
+    const R th = t_hit ? R(t_hit[i]) : std::numeric_limits<R>::infinity();
+    const float* og = origin + i * ostride;
+    march_ray<R>(S, P, v3<R>(og[0], og[1], og[2]), v3<R>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]),
+                 th, visited ? visited + i * P.max_tiles : nullptr, o,
+                 brick_dbg ? brick_dbg + j * P.brick_debug : nullptr, with_mlp != 0);
+    if (rgb) for (int c = 0; c < 3; ++c) rgb[3 * j + c] = o.rgb[c];
+    if (T) T[j] = o.T;
+    if (nsamp) nsamp[j] = o.n_samples;
+    if (depth) depth[j] = o.depth;
+  }
+}
+
+extern "C" {
+// Full per-ray render of N rays (or of the subset ray_index[0..N) when ray_index != NULL;
+// outputs are then indexed by j).  brick_dbg, when non-NULL, receives per ray the first
+// brick_debug occupied brick ids marched, -1 padded (caller pre-fills with -1).
+void orc_render_f32(const orc_scene_c* sc, const orc_params_c* pc, const float* origin, int ostride,
+                    const float* dirs, const float* t_hit, const int* visited, long long N, const long long* ray_index, float* rgb,
+                    float* T, int* nsamp, float* depth, int* brick_dbg, int with_mlp) {
+  render_impl<float>(sc, pc, origin, ostride, dirs, t_hit, visited, N, ray_index, rgb, T, nsamp, depth, brick_dbg, with_mlp);
+}
+void orc_render_f64(const orc_scene_c* sc, const orc_params_c* pc, const float* origin, int ostride,
+                    const float* dirs, const float* t_hit, const int* visited, long long N, const long long* ray_index, float* rgb,
+                    float* T, int* nsamp, float* depth, int* brick_dbg, int with_mlp) {
+  render_impl<double>(sc, pc, origin, ostride, dirs, t_hit, visited, N, ray_index, rgb, T, nsamp, depth, brick_dbg, with_mlp);
+}
+
+// trilinear at M continuous voxel coordinates of one tile (sigma + features), for unit tests
+void orc_trilinear_f32(const orc_scene_c* sc, int tile, const float* f, long long M, float* sig, float* feat) {
+  Scene S = to_scene(sc);
+  for (long long i = 0; i < M; ++i)
+    trilinear<float>(S, tile, v3(f[3 * i], f[3 * i + 1], f[3 * i + 2]), sig + i, feat ? feat + i * S.feat_dim : nullptr);
+}
+
+// MLP on M input rows x[M, mlp_in] of one tile -> rgb[M,3]
+void orc_mlp_f64(const orc_scene_c* sc, int tile, const double* x, long long M, double* rgb) {
+  Scene S = to_scene(sc);
+  std::vector<double> a, b;
+  for (long long i = 0; i < M; ++i) mlp_eval<double>(S, tile, x + i * S.mlp_in, rgb + 3 * i, a, b);
+}
+
+void orc_view_encoding_f64(const double* d, long long M, double* e) {
+  for (long long i = 0; i < M; ++i) view_encoding<double>(v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]), e + 15 * i);
+}
+
+// A9: nearest triangle hit per ray (brute force), rays with per-ray origins
+void orc_mesh_hit_f32(const float* V, const int* Fc, int nf, const float* orig, const float* dirs, long long N,
+                      float* t, int* id) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long i = 0; i < N; ++i) {
+    float tt;
+    int ii;
+    mesh_hit_one<float>(V, Fc, nf, v3(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]),
+                        v3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), &tt, &ii);
+    t[i] = tt;
+    id[i] = ii;
+  }
+}
+
+void orc_normalize_f32(const float* d, long long N, float* out) {
+  for (long long i = 0; i < N; ++i) {
+    V3<float> n = normalize3(v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+    out[3 * i] = n.x; out[3 * i + 1] = n.y; out[3 * i + 2] = n.z;
+  }
+}
+
+// A6: inverse_z_sampling_kernel (rendering/src/sampling.cu:5-46).  For pixel p and point j:
+// z = 1/((offset+j)*ib[p] + in[p]), next = 1/((offset+j+1)*ib[p] + in[p]), step = max(next - z, 0),
+// point = origin + z * d[p].
+void orc_inverse_z_sampling_f32(const float* origin, const float* dirs, const float* inv_near,
+                                const float* inv_bound, long long npix, int offset, int spr, float* points,
+                                float* steps) {
+  for (long long t = 0; t < npix * spr; ++t) {
+    const long long p = t / spr;
+    const int j = (int)(t % spr);
+    const float z = 1.0f / ((float)(offset + j) * inv_bound[p] + inv_near[p]);
+    const float nz = 1.0f / ((float)(offset + j + 1) * inv_bound[p] + inv_near[p]);
+    steps[t] = std::max(nz - z, 0.0f);
+    for (int c = 0; c < 3; ++c) points[t * 3 + c] = origin[c] + z * dirs[p * 3 + c];
+  }
+}
+
+// A5: integrate_points_kernel (rendering/src/volume_rendering.cu:5-58), 16 samples per pixel,
+// stateful T and colour across calls; skips pixels with T < early_stop.
+void orc_integrate_points_f32(const float* res, const float* steps, long long npix, float early, float* T,
+                              float* color) {
+  for (long long p = 0; p < npix; ++p) {
+    float t = T[p];
+    if (t < early) continue;
+    float c[3] = {0, 0, 0};
+    for (int i = 0; i < 16; ++i) {
+      const float* v = res + (p * 16 + i) * 4;
+      const float alpha = 1.0f - orc_expf(-1.0f * v[3] * steps[p * 16 + i]);
+      for (int k = 0; k < 3; ++k) c[k] = c[k] + alpha * t * v[k];
+      t = t * (1 - alpha);
+    }
+    T[p] = t;
+    for (int k = 0; k < 3; ++k) color[p * 3 + k] += c[k];
+  }
+}
+
+// A8: assign_pixels_kernel (rendering/src/reorder.cu:25-67)
+void orc_assign_pixels(const int16_t* net, const int* pixel_starts, const int* block_starts, int nnet,
+                       int16_t* new_net, int* new_starts) {
+  for (int i = 0; i < nnet; ++i) {
+    const int start_loc = block_starts[i];
+    int ps = pixel_starts[i];
+    const int npx = pixel_starts[i + 1] - pixel_starts[i];
+    const int nb = block_starts[i + 1] - start_loc;
+    if (nb <= 0) continue;
+    const int base = npx / nb, add = npx - base * nb;
+    for (int b = 0; b < nb; ++b) {
+      new_net[start_loc + b] = net[i];
+      new_starts[start_loc + b] = ps;
+      ps += (b < add) ? base + 1 : base;
+    }
+  }
+}
+
+}  // extern "C"

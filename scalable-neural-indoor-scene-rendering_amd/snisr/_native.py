@@ -1,0 +1,86 @@
+"""ctypes binding of the C ABI in include/snisr.h (libsnisr_hip.so).
+
+Loading fails loudly when the library is missing: there is no CPU fallback behind this
+module.  Struct layouts are checked against ``snisr_abi_sizeof`` at load time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from .build import LIB_PATH
+
+SNISR_F32, SNISR_F16, SNISR_BF16 = 0, 1, 2
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("tile_shape", C.c_int32 * 3), ("scene_min", C.c_float * 3), ("tile_size", C.c_float),
+        ("n_tiles", C.c_int32), ("n_bricks", C.c_int32), ("feat_dim", C.c_int32), ("feat_dtype", C.c_int32),
+        ("mlp_in", C.c_int32), ("mlp_width", C.c_int32), ("mlp_layers", C.c_int32),
+        ("mlp_packed_dtype", C.c_int32), ("mlp_stride", C.c_int64), ("brick_map_len", C.c_int64),
+        ("tile_index_map", C.c_void_p), ("tile_centers", C.c_void_p), ("tile_res", C.c_void_p),
+        ("tile_brick_base", C.c_void_p), ("brick_map", C.c_void_p), ("sigma", C.c_void_p),
+        ("feats", C.c_void_p), ("mlp_params", C.c_void_p), ("mlp_f32t", C.c_void_p), ("mlp_packed", C.c_void_p),
+    ]
+
+
+class MarchParams(C.Structure):
+    _fields_ = [("max_tiles", C.c_int32), ("sample_step", C.c_float), ("early_stop", C.c_float),
+                ("trans_th", C.c_float)]
+
+
+class RaysDesc(C.Structure):
+    _fields_ = [("origins", C.c_void_p), ("origin_stride", C.c_int32), ("image_width", C.c_int32),
+                ("n", C.c_int64), ("dirs", C.c_void_p), ("t_hit", C.c_void_p), ("visited", C.c_void_p)]
+
+
+class RenderOut(C.Structure):
+    _fields_ = [("rgb", C.c_void_p), ("T", C.c_void_p), ("n_samples", C.c_void_p), ("depth", C.c_void_p),
+                ("bricks", C.c_void_p), ("brick_cap", C.c_int32)]
+
+
+_lib = None
+
+# (name, restype, argtypes) of every exported entry point declared in include/snisr.h
+SIGNATURES = [
+    ("snisr_last_error", C.c_char_p, []),
+    ("snisr_version", C.c_int, []),
+    ("snisr_abi_sizeof", C.c_int64, [C.c_int32]),
+    ("snisr_compute_rays", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("snisr_trace_tiles", C.c_int, [C.POINTER(SceneDesc), C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                    C.c_void_p, C.c_void_p]),
+    ("snisr_render_fp32", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
+                                    C.POINTER(RenderOut), C.c_void_p]),
+    ("snisr_render_mfma_workspace_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int64, C.c_int64]),
+    ("snisr_render_mfma", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
+                                    C.POINTER(RenderOut), C.c_void_p, C.c_int64, C.c_void_p]),
+    ("snisr_pack_mlp_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int32]),
+    ("snisr_pack_mlp", C.c_int, [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p]),
+]
+
+
+def lib():
+    """The loaded library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"snisr: HIP library not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    for which, st in enumerate((SceneDesc, MarchParams, RaysDesc, RenderOut)):
+        got = L.snisr_abi_sizeof(which)
+        if got != C.sizeof(st):
+            raise RuntimeError(f"snisr: ABI mismatch for {st.__name__}: C {got} vs ctypes {C.sizeof(st)}")
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().snisr_last_error().decode()
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")

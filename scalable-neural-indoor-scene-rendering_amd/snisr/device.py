@@ -1,0 +1,234 @@
+"""Torch-facing host layer: device-resident scenes and the render entry points.
+
+PyTorch provides device memory and the current stream; all compute runs in the HIP
+kernels of libsnisr_hip.so through the C ABI (include/snisr.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .scene import Scene
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _require_cuda(t: torch.Tensor, name: str, dtype=None):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{name} must be a CUDA (HIP) tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+class DeviceScene:
+    """A ``Scene`` uploaded to HBM with its C descriptor and packed MLP copies."""
+
+    def __init__(self, sc: Scene, device="cuda", mfma_dtype: int | None = None):
+        dev = torch.device(device)
+        self.host = sc
+        t = lambda a, dt=None: torch.from_numpy(np.ascontiguousarray(a)).to(dev) if dt is None else \
+            torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
+        self.tile_index_map = t(sc.tile_index_map)
+        self.tile_centers = t(sc.tile_centers)
+        self.tile_res = t(sc.tile_res)
+        self.tile_brick_base = t(sc.tile_brick_base)
+        self.brick_map = t(sc.brick_map)
+        self.sigma = t(sc.sigma) if sc.n_bricks else torch.zeros((1, 512), device=dev)
+        self.feats = t(sc.feats) if sc.n_bricks else torch.zeros((1, 512, sc.feat_dim), device=dev,
+                                                                 dtype=torch.float16 if sc.feat_half else torch.float32)
+        self.mlp_params = t(sc.mlp_params)
+        self.mlp_f32t = None
+        self.mlp_packed = None
+        self.desc = self._make_desc()
+        self.pack_f32t()
+        if mfma_dtype is not None:
+            self.pack_mfma(mfma_dtype)
+
+    def _make_desc(self) -> N.SceneDesc:
+        sc = self.host
+        d = N.SceneDesc()
+        d.tile_shape[:] = [int(v) for v in sc.tile_shape]
+        d.scene_min[:] = [float(v) for v in sc.scene_min]
+        d.tile_size = sc.tile_size
+        d.n_tiles = sc.n_tiles
+        d.n_bricks = sc.n_bricks
+        d.feat_dim = sc.feat_dim
+        d.feat_dtype = N.SNISR_F16 if self.feats.dtype == torch.float16 else N.SNISR_F32
+        d.mlp_in, d.mlp_width, d.mlp_layers = sc.mlp_in, sc.mlp_width, sc.mlp_layers
+        d.mlp_stride = self.mlp_params.shape[1]
+        d.brick_map_len = self.brick_map.numel()
+        d.tile_index_map = self.tile_index_map.data_ptr()
+        d.tile_centers = self.tile_centers.data_ptr()
+        d.tile_res = self.tile_res.data_ptr()
+        d.tile_brick_base = self.tile_brick_base.data_ptr()
+        d.brick_map = self.brick_map.data_ptr()
+        d.sigma = self.sigma.data_ptr()
+        d.feats = self.feats.data_ptr()
+        d.mlp_params = self.mlp_params.data_ptr()
+        return d
+
+    def set_params(self, mlp_params: torch.Tensor | None = None, feats: torch.Tensor | None = None,
+                   sigma: torch.Tensor | None = None):
+        """Point the descriptor at new device tensors of the same shapes (training / perturbed copies)."""
+        if mlp_params is not None:
+            _require_cuda(mlp_params, "mlp_params", torch.float32)
+            assert mlp_params.shape == self.mlp_params.shape
+            self.mlp_params = mlp_params
+            self.desc.mlp_params = mlp_params.data_ptr()
+            self.pack_f32t()
+            if self.mlp_packed is not None:
+                self.pack_mfma(self.desc.mlp_packed_dtype)
+        if feats is not None:
+            _require_cuda(feats, "feats")
+            assert feats.shape == self.feats.shape
+            self.feats = feats
+            self.desc.feats = feats.data_ptr()
+            self.desc.feat_dtype = N.SNISR_F16 if feats.dtype == torch.float16 else N.SNISR_F32
+        if sigma is not None:
+            _require_cuda(sigma, "sigma", torch.float32)
+            assert sigma.shape == self.sigma.shape
+            self.sigma = sigma
+            self.desc.sigma = sigma.data_ptr()
+
+    def pack_f32t(self):
+        L = N.lib()
+        nbytes = L.snisr_pack_mlp_size(C.byref(self.desc), N.SNISR_F32)
+        self.mlp_f32t = torch.empty(nbytes // 4, dtype=torch.float32, device=self.mlp_params.device)
+        N.check(L.snisr_pack_mlp(C.byref(self.desc), N.SNISR_F32, _ptr(self.mlp_f32t), _stream()), "pack_mlp(f32)")
+        self.desc.mlp_f32t = self.mlp_f32t.data_ptr()
+
+    def pack_mfma(self, dtype: int = N.SNISR_F16):
+        L = N.lib()
+        nbytes = L.snisr_pack_mlp_size(C.byref(self.desc), dtype)
+        if nbytes <= 0:
+            raise RuntimeError(f"MFMA packing unsupported for this MLP shape: {N.lib().snisr_last_error().decode()}")
+        self.mlp_packed = torch.empty(nbytes, dtype=torch.uint8, device=self.mlp_params.device)
+        N.check(L.snisr_pack_mlp(C.byref(self.desc), dtype, _ptr(self.mlp_packed), _stream()), "pack_mlp(mfma)")
+        self.desc.mlp_packed = self.mlp_packed.data_ptr()
+        self.desc.mlp_packed_dtype = dtype
+
+    def march_params(self, max_tiles=None, sample_step=None, early_stop=None, trans_th=None) -> N.MarchParams:
+        sc = self.host
+        p = N.MarchParams()
+        p.max_tiles = sc.max_tiles if max_tiles is None else max_tiles
+        p.sample_step = sc.sample_step if sample_step is None else sample_step
+        p.early_stop = sc.early_stop if early_stop is None else early_stop
+        p.trans_th = sc.trans_th if trans_th is None else trans_th
+        return p
+
+
+def _rays_desc(origins, dirs, t_hit=None, visited=None, image_width=0):
+    _require_cuda(origins, "origins", torch.float32)
+    _require_cuda(dirs, "dirs", torch.float32)
+    r = N.RaysDesc()
+    n = dirs.numel() // 3
+    r.origins = origins.data_ptr()
+    r.origin_stride = 0 if origins.numel() == 3 else 3
+    if r.origin_stride == 3 and origins.numel() != 3 * n:
+        raise ValueError("origins must be [3] or [N,3]")
+    r.image_width = image_width
+    r.n = n
+    r.dirs = dirs.data_ptr()
+    if t_hit is not None:
+        _require_cuda(t_hit, "t_hit", torch.float32)
+        r.t_hit = t_hit.data_ptr()
+    if visited is not None:
+        _require_cuda(visited, "visited", torch.int32)
+        r.visited = visited.data_ptr()
+    return r
+
+
+def _render_outputs(n, device, rgb=True, brick_cap=0):
+    out = dict(
+        rgb=torch.empty((n, 3), dtype=torch.float32, device=device) if rgb else None,
+        T=torch.empty(n, dtype=torch.float32, device=device),
+        n_samples=torch.empty(n, dtype=torch.int32, device=device),
+        depth=torch.empty(n, dtype=torch.float32, device=device),
+    )
+    if brick_cap:
+        out["bricks"] = torch.empty((n, brick_cap), dtype=torch.int32, device=device)
+    o = N.RenderOut()
+    o.rgb = out["rgb"].data_ptr() if rgb else None
+    o.T = out["T"].data_ptr()
+    o.n_samples = out["n_samples"].data_ptr()
+    o.depth = out["depth"].data_ptr()
+    if brick_cap:
+        o.bricks = out["bricks"].data_ptr()
+        o.brick_cap = brick_cap
+    return out, o
+
+
+def render_fp32(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_width=0, rgb=True, brick_cap=0,
+                **march_kw):
+    """One-thread-per-ray FP32 render (snisr_render_fp32)."""
+    r = _rays_desc(origins, dirs, t_hit, visited, image_width)
+    p = ds.march_params(**march_kw)
+    if visited is not None and visited.shape[-1] != p.max_tiles:
+        raise ValueError("visited must be [N, max_tiles]")
+    out, o = _render_outputs(r.n, dirs.device, rgb, brick_cap)
+    N.check(N.lib().snisr_render_fp32(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), _stream()),
+            "render_fp32")
+    return out
+
+
+class MfmaWorkspace:
+    """Caller-owned workspace of snisr_render_mfma, grown on demand."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, ds: DeviceScene, n: int, capacity: int = 0):
+        need = N.lib().snisr_render_mfma_workspace_size(C.byref(ds.desc), n, capacity)
+        if need < 0:
+            raise RuntimeError(N.lib().snisr_last_error().decode())
+        if self.buf is None or self.buf.numel() < need:
+            self.buf = torch.empty(need, dtype=torch.uint8, device=ds.sigma.device)
+        return self.buf, need
+
+
+def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_width=0, brick_cap=0,
+                workspace: MfmaWorkspace | None = None, capacity: int = 0, **march_kw):
+    """Render with the per-tile MLP on MFMA (snisr_render_mfma)."""
+    if ds.mlp_packed is None:
+        raise RuntimeError("scene has no MFMA-packed MLP; call DeviceScene.pack_mfma()")
+    r = _rays_desc(origins, dirs, t_hit, visited, image_width)
+    p = ds.march_params(**march_kw)
+    out, o = _render_outputs(r.n, dirs.device, True, brick_cap)
+    ws = workspace or MfmaWorkspace()
+    buf, need = ws.get(ds, r.n, capacity)
+    N.check(N.lib().snisr_render_mfma(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), _ptr(buf), need,
+                                      _stream()), "render_mfma")
+    return out
+
+
+def compute_rays(intrinsic: torch.Tensor, c2w: torch.Tensor, H: int, W: int, out: torch.Tensor | None = None):
+    _require_cuda(intrinsic, "intrinsic", torch.float32)
+    _require_cuda(c2w, "c2w", torch.float32)
+    if out is None:
+        out = torch.empty((H, W, 3), dtype=torch.float32, device=intrinsic.device)
+    _require_cuda(out, "rays_d", torch.float32)
+    N.check(N.lib().snisr_compute_rays(_ptr(intrinsic), _ptr(c2w), H, W, _ptr(out), _stream()), "compute_rays")
+    return out
+
+
+def trace_tiles(ds: DeviceScene, origin: torch.Tensor, rays_d: torch.Tensor, K: int, out=None):
+    _require_cuda(origin, "origin", torch.float32)
+    _require_cuda(rays_d, "rays_d", torch.float32)
+    n = rays_d.numel() // 3
+    if out is None:
+        out = torch.empty((n, K), dtype=torch.int32, device=rays_d.device)
+    _require_cuda(out, "visitedTiles", torch.int32)
+    N.check(N.lib().snisr_trace_tiles(C.byref(ds.desc), _ptr(origin), _ptr(rays_d), n, K, _ptr(out), _stream()),
+            "trace_tiles")
+    return out

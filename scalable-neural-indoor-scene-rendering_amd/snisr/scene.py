@@ -1,0 +1,308 @@
+"""Seeded synthetic scenes in the renderer's tiled sparse-brick format.
+
+The scene is a grid of cubic tiles (the reference's ``tile_IndexMap`` / ``centers``,
+rendering/src/__init__.py:111-117).  Each tile holds an n^3 voxel grid cut into 8^3-voxel
+bricks; a brick is either empty or fully occupied (coherent occupancy).  Per occupied
+brick the data is stored as
+
+* ``sigma``  float32 [n_bricks, 512]         density, voxel-major ((x*8+y)*8+z)
+* ``feats``  f16/f32 [n_bricks, 512, F]       features, F contiguous per voxel
+
+and ``brick_map`` maps (tile, brick cell) -> brick id or -1 (the role of the reference's
+``block_IndexMap``, __init__.py:118).  Empty voxels read as zero.
+
+Stated decisions for the synthetic inputs (the reference ships no data):
+
+* random streams: numpy ``Generator(PCG64)``, one child ``SeedSequence`` per component,
+  so the draws of one component never depend on another's size;
+* features ~ N(0, 0.5^2) (0.5 is the standard deviation);
+* density: ``LogNormal(0, 1)`` read as optical depth per voxel length, i.e. the stored
+  world-space sigma is ``exp(N(0,1)) / voxel_size``; with this unit the T<1e-3 early
+  termination is reached inside a few voxels, as on the reference's opaque scenes;
+* MLP weights Xavier-uniform, biases 0; FP16 scenes round the weights to FP16 values so
+  that FP16 storage is lossless and every precision sees the same network;
+* the view encoding is ``Positional_Encoding(L=2)`` of the unit direction (15 values).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+BRICK = 8
+BRICK_VOX = BRICK ** 3
+
+
+@dataclasses.dataclass
+class Camera:
+    H: int
+    W: int
+    origin: np.ndarray      # f32[3]
+    c2w: np.ndarray         # f32[9] row-major 3x3, columns = right, down, forward
+    intrinsic: np.ndarray   # f32[3] = (cx, cy, 1/focal)  (rendering/src/__init__.py:51)
+
+
+@dataclasses.dataclass
+class Mesh:
+    vertices: np.ndarray    # f32[V,3]
+    faces: np.ndarray       # i32[F,3]
+
+
+@dataclasses.dataclass
+class Scene:
+    tile_shape: np.ndarray      # i32[3]
+    scene_min: np.ndarray       # f32[3]
+    tile_size: float
+    tile_index_map: np.ndarray  # i32[prod(tile_shape)]
+    tile_centers: np.ndarray    # f32[T,3]
+    tile_res: np.ndarray        # i32[T]
+    tile_brick_base: np.ndarray # i32[T]
+    brick_map: np.ndarray       # i32[sum nb^3]
+    sigma: np.ndarray           # f32[NB,512]
+    feats: np.ndarray           # f16 or f32 [NB,512,F]
+    mlp_in: int
+    mlp_width: int
+    mlp_layers: int
+    mlp_params: np.ndarray      # f32[T,P]
+    sample_step: float
+    early_stop: float = 1e-3
+    trans_th: float = 0.8
+    max_tiles: int = 30
+
+    @property
+    def n_tiles(self) -> int:
+        return int(self.tile_centers.shape[0])
+
+    @property
+    def feat_dim(self) -> int:
+        return int(self.feats.shape[2])
+
+    @property
+    def feat_half(self) -> bool:
+        return self.feats.dtype == np.float16
+
+    @property
+    def n_bricks(self) -> int:
+        return int(self.sigma.shape[0])
+
+
+def mlp_layer_dims(mlp_in: int, width: int, layers: int):
+    """(in, out) of every Linear layer: in->width, (layers-2) x width->width, width->3."""
+    dims = []
+    i = mlp_in
+    for l in range(layers):
+        o = 3 if l == layers - 1 else width
+        dims.append((i, o))
+        i = o
+    return dims
+
+
+def mlp_param_count(mlp_in: int, width: int, layers: int) -> int:
+    return sum(o * i + o for i, o in mlp_layer_dims(mlp_in, width, layers))
+
+
+def mlp_unpack(params: np.ndarray, mlp_in: int, width: int, layers: int):
+    """Split one tile's packed vector into [(W[out,in], b[out]), ...]."""
+    out, off = [], 0
+    for i, o in mlp_layer_dims(mlp_in, width, layers):
+        W = params[off:off + o * i].reshape(o, i)
+        off += o * i
+        b = params[off:off + o]
+        off += o
+        out.append((W, b))
+    return out
+
+
+def _child_rngs(seed: int, n: int):
+    ss = np.random.SeedSequence(seed)
+    return [np.random.Generator(np.random.PCG64(s)) for s in ss.spawn(n)]
+
+
+def make_scene(seed: int, tile_shape=(1, 1, 1), tile_size=1.0, scene_min=(0.0, 0.0, 0.0),
+               res=64, occupancy=0.1, feat_dim=16, feat_half=False, mlp_width=64, mlp_layers=3,
+               sample_step=1.0 / 256, early_stop=1e-3, max_tiles=30, res_choices=None,
+               occupancy_range=None) -> Scene:
+    """Seeded tiled scene.  ``res_choices``/``occupancy_range`` draw per-tile values."""
+    r_occ, r_sig, r_feat, r_mlp, r_tile = _child_rngs(seed, 5)
+    tile_shape = np.asarray(tile_shape, np.int32)
+    Tx, Ty, Tz = (int(v) for v in tile_shape)
+    T = Tx * Ty * Tz
+    tile_index_map = np.arange(T, dtype=np.int32)
+    cells = np.stack(np.meshgrid(np.arange(Tx), np.arange(Ty), np.arange(Tz), indexing="ij"), -1).reshape(-1, 3)
+    smin = np.asarray(scene_min, np.float32)
+    ts = np.float32(tile_size)
+    tile_centers = (smin[None, :] + (cells.astype(np.float32) + np.float32(0.5)) * ts).astype(np.float32)
+    if res_choices is not None:
+        tile_res = r_tile.choice(np.asarray(res_choices, np.int32), size=T).astype(np.int32)
+    else:
+        tile_res = np.full(T, res, np.int32)
+    if occupancy_range is not None:
+        occ = r_tile.uniform(occupancy_range[0], occupancy_range[1], size=T)
+    else:
+        occ = np.full(T, occupancy)
+    assert np.all(tile_res % BRICK == 0)
+    nb = tile_res // BRICK
+    counts = nb.astype(np.int64) ** 3
+    base = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int32)
+    occupied = np.concatenate([r_occ.random(int(c)) < p for c, p in zip(counts, occ)])
+    brick_map = np.full(int(counts.sum()), -1, np.int32)
+    nbrk = int(occupied.sum())
+    brick_map[occupied] = np.arange(nbrk, dtype=np.int32)
+    vs = (ts / tile_res.astype(np.float32)).astype(np.float32)
+    # voxel size of every occupied brick's tile
+    brick_tile = np.repeat(np.arange(T), counts)[occupied]
+    sigma = np.empty((nbrk, BRICK_VOX), np.float32)
+    feats = np.empty((nbrk, BRICK_VOX, feat_dim), np.float16 if feat_half else np.float32)
+    chunk = 4096
+    for s in range(0, nbrk, chunk):
+        e = min(nbrk, s + chunk)
+        z = r_sig.standard_normal((e - s, BRICK_VOX), dtype=np.float32)
+        sigma[s:e] = (np.exp(z) / vs[brick_tile[s:e]][:, None]).astype(np.float32)
+        f = r_feat.standard_normal((e - s, BRICK_VOX, feat_dim), dtype=np.float32) * np.float32(0.5)
+        feats[s:e] = f.astype(feats.dtype)
+    mlp_in = feat_dim + 15
+    P = mlp_param_count(mlp_in, mlp_width, mlp_layers)
+    params = np.empty((T, P), np.float32)
+    for t in range(T):
+        chunks = []
+        for i, o in mlp_layer_dims(mlp_in, mlp_width, mlp_layers):
+            a = math.sqrt(6.0 / (i + o))
+            W = r_mlp.uniform(-a, a, size=(o, i)).astype(np.float32)
+            chunks += [W.ravel(), np.zeros(o, np.float32)]
+        params[t] = np.concatenate(chunks)
+    if feat_half:
+        params = params.astype(np.float16).astype(np.float32)
+    return Scene(tile_shape=tile_shape, scene_min=smin, tile_size=float(ts), tile_index_map=tile_index_map,
+                 tile_centers=tile_centers, tile_res=tile_res, tile_brick_base=base, brick_map=brick_map,
+                 sigma=sigma, feats=feats, mlp_in=mlp_in, mlp_width=mlp_width, mlp_layers=mlp_layers,
+                 mlp_params=params, sample_step=float(np.float32(sample_step)), early_stop=early_stop,
+                 max_tiles=max_tiles)
+
+
+def look_at_camera(eye, target, H, W, focal, world_up=(0.0, 0.0, 1.0)) -> Camera:
+    """Pinhole camera; rays are d = R [ (px-cx)/f, (py-cy)/f, 1 ] (compute_rays.cu:7-16),
+    so R's columns are right, down (image y grows downwards) and forward."""
+    eye = np.asarray(eye, np.float64)
+    fwd = np.asarray(target, np.float64) - eye
+    fwd /= np.linalg.norm(fwd)
+    up = np.asarray(world_up, np.float64)
+    if abs(np.dot(fwd, up)) > 0.999:
+        up = np.array([0.0, 1.0, 0.0])
+    right = np.cross(fwd, up)
+    right /= np.linalg.norm(right)
+    down = np.cross(fwd, right)
+    R = np.stack([right, down, fwd], axis=1)
+    return Camera(H=H, W=W, origin=eye.astype(np.float32), c2w=R.astype(np.float32).ravel(),
+                  intrinsic=np.array([W / 2.0, H / 2.0, 1.0 / focal], np.float32))
+
+
+def box_mesh(lo: np.ndarray, hi: np.ndarray) -> Mesh:
+    """12 triangles per axis-aligned box (lo, hi: [B,3])."""
+    lo = np.asarray(lo, np.float32)
+    hi = np.asarray(hi, np.float32)
+    B = lo.shape[0]
+    corners = np.empty((B, 8, 3), np.float32)
+    for c in range(8):
+        corners[:, c, 0] = np.where(c & 1, hi[:, 0], lo[:, 0])
+        corners[:, c, 1] = np.where(c & 2, hi[:, 1], lo[:, 1])
+        corners[:, c, 2] = np.where(c & 4, hi[:, 2], lo[:, 2])
+    quads = [(0, 2, 6, 4), (1, 5, 7, 3), (0, 4, 5, 1), (2, 3, 7, 6), (0, 1, 3, 2), (4, 6, 7, 5)]
+    tri = []
+    for a, b, c, d in quads:
+        tri += [(a, b, c), (a, c, d)]
+    tri = np.asarray(tri, np.int32)
+    faces = (tri[None, :, :] + 8 * np.arange(B, dtype=np.int32)[:, None, None]).reshape(-1, 3)
+    return Mesh(vertices=corners.reshape(-1, 3), faces=faces)
+
+
+def room_mesh(seed: int, room_lo, room_hi, n_triangles: int, avoid=None, size_range=(0.01, 0.05)) -> Mesh:
+    """Room walls (6 thin boxes enclosing the room) plus random furniture boxes up to
+    ``n_triangles`` (12 per box).  Boxes containing the point ``avoid`` are redrawn."""
+    rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence(seed).spawn(6)[5]))
+    lo = np.asarray(room_lo, np.float32)
+    hi = np.asarray(room_hi, np.float32)
+    th = np.float32(0.05)
+    walls_lo, walls_hi = [], []
+    for a in range(3):
+        for side in range(2):
+            wl = lo - th
+            wh = hi + th
+            if side == 0:
+                wh = wh.copy(); wh[a] = lo[a]
+            else:
+                wl = wl.copy(); wl[a] = hi[a]
+            walls_lo.append(wl); walls_hi.append(wh)
+    nbox = max(0, -(-n_triangles // 12) - 6)
+    size = rng.uniform(size_range[0], size_range[1], size=(nbox, 3)).astype(np.float32)
+    pos = rng.uniform(lo, hi, size=(nbox, 3)).astype(np.float32)
+    blo = np.minimum(np.maximum(pos - size / 2, lo), hi)
+    bhi = np.minimum(np.maximum(pos + size / 2, lo), hi)
+    if avoid is not None:
+        a = np.asarray(avoid, np.float32)
+        bad = np.all((blo - 0.05 <= a) & (a <= bhi + 0.05), axis=1)
+        blo, bhi = blo[~bad], bhi[~bad]
+    m = box_mesh(np.concatenate([np.asarray(walls_lo), blo]), np.concatenate([np.asarray(walls_hi), bhi]))
+    return m
+
+
+# --------------------------------------------------------------------------- configs
+def config_c1(seed: int = 1):
+    """Plain FP32 single tile: 64^3 over the unit cube, 8^3 bricks ~ Bernoulli(0.1), F=16 FP32,
+    MLP 31->64->64->3, 256 samples per unit length, T<1e-3; 320x180 pinhole, 60 deg hFOV,
+    on a seeded point of the radius-2 sphere around the cube centre."""
+    sc = make_scene(seed, tile_shape=(1, 1, 1), tile_size=1.0, res=64, occupancy=0.1, feat_dim=16,
+                    feat_half=False, mlp_width=64, mlp_layers=3, sample_step=1.0 / 256, early_stop=1e-3)
+    rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence(seed).spawn(7)[6]))
+    v = rng.standard_normal(3)
+    v /= np.linalg.norm(v)
+    centre = np.array([0.5, 0.5, 0.5])
+    W, H = 320, 180
+    focal = (W / 2) / math.tan(math.radians(60) / 2)
+    cam = look_at_camera(centre + 2.0 * v, centre, H, W, focal)
+    return sc, cam, None
+
+
+def room_camera(seed: int, room_lo, room_hi, H, W, focal=750.0) -> Camera:
+    rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence(seed).spawn(8)[7]))
+    lo = np.asarray(room_lo, np.float64)
+    hi = np.asarray(room_hi, np.float64)
+    eye = lo + (hi - lo) * np.array([rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.35, 0.65)])
+    yaw = rng.uniform(0, 2 * math.pi)
+    pitch = rng.uniform(-0.15, 0.15)
+    fwd = np.array([math.cos(yaw) * math.cos(pitch), math.sin(yaw) * math.cos(pitch), math.sin(pitch)])
+    return look_at_camera(eye, eye + fwd, H, W, focal)
+
+
+def config_c2(seed: int = 2, H: int = 720, W: int = 1280, n_triangles: int = 200_000, feat_half=True):
+    """16-tile (4x4x1) room of 128^3 tiles at 5% brick occupancy, F=32 FP16 features, per-tile
+    MLP 47->128->128->128->3; sample step 0.2 voxel (the reference's sample_step_scale,
+    rendering/src/__init__.py:41); proxy mesh of room walls + furniture boxes (~200k triangles);
+    camera at a seeded pose inside the room, focal 750 (rendering/config/base.yaml:7)."""
+    sc = make_scene(seed, tile_shape=(4, 4, 1), tile_size=1.0, res=128, occupancy=0.05, feat_dim=32,
+                    feat_half=feat_half, mlp_width=128, mlp_layers=4, sample_step=0.2 / 128, early_stop=1e-3)
+    lo, hi = (0.0, 0.0, 0.0), (4.0, 4.0, 1.0)
+    cam = room_camera(seed, lo, hi, H, W)
+    mesh = room_mesh(seed, lo, hi, n_triangles, avoid=cam.origin) if n_triangles else None
+    return sc, cam, mesh
+
+
+def config_c5(seed: int = 5, H: int = 1080, W: int = 1920, n_views: int = 4, n_triangles: int = 2_000_000):
+    """256 tiles (16x16 floor plan) of 128^3 at 4%, F=32 FP16, width-128x4 MLP, 2M-triangle mesh,
+    n_views seeded 1920x1080 views."""
+    sc = make_scene(seed, tile_shape=(16, 16, 1), tile_size=1.0, res=128, occupancy=0.04, feat_dim=32,
+                    feat_half=True, mlp_width=128, mlp_layers=4, sample_step=0.2 / 128, early_stop=1e-3)
+    lo, hi = (0.0, 0.0, 0.0), (16.0, 16.0, 1.0)
+    cams = [room_camera(seed * 100 + v, lo, hi, H, W) for v in range(n_views)]
+    mesh = room_mesh(seed, lo, hi, n_triangles, avoid=cams[0].origin)
+    return sc, cams, mesh
+
+
+def camera_rays_np(cam: Camera) -> np.ndarray:
+    """float32 restatement of compute_rays (compute_rays.cu:7-44) for host-side use."""
+    py, px = np.meshgrid(np.arange(cam.H, dtype=np.float32), np.arange(cam.W, dtype=np.float32), indexing="ij")
+    x = (px - cam.intrinsic[0]) * cam.intrinsic[2]
+    y = (py - cam.intrinsic[1]) * cam.intrinsic[2]
+    R = cam.c2w.reshape(3, 3)
+    d = np.stack([R[r, 0] * x + R[r, 1] * y + R[r, 2] for r in range(3)], -1)
+    return d.reshape(-1, 3).astype(np.float32)
