@@ -145,6 +145,10 @@ int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, in
 int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                       const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* Samples emitted by the last snisr_render_mfma on `workspace` and whether a ray overflowed the
+ * record capacity (its rgb is then NaN).  Synchronises `stream`.  No reference counterpart. */
+int snisr_render_mfma_stats(const void* workspace, int64_t* total, int32_t* overflow, void* stream);
+
 /* Bytes of a packed weight buffer of the scene's MLP shape for `dtype`, and the packing itself
  * (device to device, from scene->mlp_params into `packed`): SNISR_F32 gives the mlp_f32t layout,
  * SNISR_F16 / SNISR_BF16 the MFMA fragment layout of mlp_packed. */

@@ -173,7 +173,7 @@ __global__ __launch_bounds__(64) void k_render_fp32(snisr_scene S, snisr_march_p
   int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
   MarchResult res;
   march_ray(S, P, origin, d, t_hit, R.visited ? R.visited + i * P.max_tiles : nullptr, bricks, O.brick_cap,
-            [&](int tile, const Corners& c, float w) {
+            [&](int tile, const V3&, const Corners& c, float w) {
               gather_features<F, HALF>(S.feats, c, xs);
 #pragma unroll
               for (int k = 0; k < 15; ++k) xs[(F + k) * 64] = enc[k];
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(64) void k_march_sigma(snisr_scene S, snisr_march_p
   int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
   MarchResult res;
   march_ray(S, P, origin, d, t_hit, R.visited ? R.visited + i * P.max_tiles : nullptr, bricks, O.brick_cap,
-            [&](int, const Corners&, float) {}, res);
+            [&](int, const V3&, const Corners&, float) {}, res);
   if (bricks)
     for (int k = res.n_bricks; k < O.brick_cap; ++k) bricks[k] = -1;
   if (O.T) O.T[i] = res.T;

@@ -1,11 +1,617 @@
-// MFMA render path (in progress).
+// MFMA render path: sigma-only march that counts, scan, march that emits per-sample records
+// (contiguous per ray), per-tile MLP on v_mfma_f32_32x32x16_{f16,bf16} with FP32 accumulation,
+// and an ordered per-ray reduction (deterministic pixels).
+//
+// MLP data flow (one wavefront = 64 records = two 32-sample column blocks):
+//   D[32 out x 32 samples] = W[32 out x 16 k] * X[16 k x 32 samples]  per MFMA.
+//   A operand (weights): lane l holds W[row l%32][k slots 8h..8h+7], h = l/32, 16 bytes per lane,
+//   packed per (out-block, k-chunk) as 1 KB fragments so that a wavefront loads one fragment with
+//   one coalesced 16-byte load per lane, straight from L2 (the per-tile weights are reused by every
+//   wavefront of that tile).
+//   B operand (activations): lane l holds X[k slots 8h..8h+7][sample l%32].
+//   D: lane l holds rows 8(i/4) + 4h + (i%4), i = 0..15, of sample l%32.  After bias + ReLU these
+//   16 values become the next layer's B operand for the two k-chunks (2*ob, 2*ob+1) without any
+//   data movement; the next layer's weight columns are permuted to match (pack_input_index).
+#include <atomic>
+
 #include "snisr_common.hpp"
+#include "snisr_device.hpp"
+
 namespace snisr {
-int64_t mfma_pack_bytes(const snisr_scene*, int) { set_error("MFMA packing not built"); return -1; }
-int mfma_pack(const snisr_scene*, int, void*, void*) { set_error("MFMA packing not built"); return SNISR_ERR_UNSUPPORTED; }
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// ------------------------------------------------------------------------- packed layout
+__host__ __device__ constexpr int nkc_of(int in) { return (in + 15) / 16; }
+__host__ __device__ constexpr int nob_of(int out) { return (out + 31) / 32; }
+__host__ __device__ constexpr int layer_in(int in0, int wd, int l) { return l == 0 ? in0 : wd; }
+__host__ __device__ constexpr int layer_out(int wd, int L, int l) { return l == L - 1 ? 3 : wd; }
+__host__ __device__ constexpr int64_t layer_bytes(int in0, int wd, int L, int l) {
+  return (int64_t)nob_of(layer_out(wd, L, l)) * (nkc_of(layer_in(in0, wd, l)) * 1024 + 128);
 }
+__host__ __device__ constexpr int64_t layer_offset(int in0, int wd, int L, int l) {
+  int64_t o = 0;
+  for (int m = 0; m < l; ++m) o += layer_bytes(in0, wd, L, m);
+  return o;
+}
+__host__ __device__ constexpr int64_t tile_bytes(int in0, int wd, int L) { return layer_offset(in0, wd, L, L); }
+
+// input index held in k slot e (0..7) of lane half h for k-chunk kc of layer l
+__host__ __device__ constexpr int pack_input_index(int l, int kc, int h, int e) {
+  return l == 0 ? 16 * kc + 8 * h + e : 32 * (kc / 2) + 8 * (2 * (kc % 2) + e / 4) + 4 * h + (e % 4);
+}
+
+__device__ __forceinline__ uint16_t to_bits(float x, int dtype) {
+  if (dtype == SNISR_BF16) {
+    const __bf16 b = (__bf16)x;
+    return *reinterpret_cast<const uint16_t*>(&b);
+  }
+  const _Float16 h = (_Float16)x;
+  return *reinterpret_cast<const uint16_t*>(&h);
+}
+
+// one thread per packed 16-bit weight or per bias float of one layer of one tile
+__global__ void k_pack_mfma_layer(const float* __restrict__ src, int64_t src_stride, int64_t src_off,
+                                  uint8_t* __restrict__ dst, int64_t dst_stride, int64_t dst_off, int l, int in,
+                                  int out, int dtype, int n_tiles) {
+  const int nkc = nkc_of(in), nob = nob_of(out);
+  const int64_t nw = (int64_t)nob * nkc * 512, nbias = (int64_t)nob * 32;
+  const int64_t per_tile = nw + nbias;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= per_tile * n_tiles) return;
+  const int64_t tile = g / per_tile;
+  const int64_t r = g - tile * per_tile;
+  const float* W = src + tile * src_stride + src_off;
+  const float* b = W + (int64_t)out * in;
+  uint8_t* D = dst + tile * dst_stride + dst_off;
+  if (r < nw) {
+    const int frag = (int)(r / 512), within = (int)(r % 512);
+    const int ob = frag / nkc, kc = frag % nkc;
+    const int lane = within / 8, e = within % 8;
+    const int h = lane / 32, row = 32 * ob + lane % 32;
+    const int x = pack_input_index(l, kc, h, e);
+    const float v = (row < out && x < in) ? W[(int64_t)row * in + x] : 0.0f;
+    reinterpret_cast<uint16_t*>(D)[r] = to_bits(v, dtype);
+  } else {
+    const int q = (int)(r - nw);  // (ob*2 + h)*16 + i
+    const int ob = q / 32, h = (q / 16) % 2, i = q % 16;
+    const int row = 32 * ob + 8 * (i / 4) + 4 * h + (i % 4);
+    reinterpret_cast<float*>(D + (int64_t)nob * nkc * 1024)[q] = row < out ? b[row] : 0.0f;
+  }
+}
+
+int64_t mfma_pack_bytes(const snisr_scene* s, int dtype) {
+  if (!s || s->mlp_layers < 2 || s->mlp_width % 32 != 0 || s->mlp_in <= 0) {
+    set_error("MFMA packing needs >= 2 layers and a hidden width that is a multiple of 32");
+    return -1;
+  }
+  if (dtype != SNISR_F16 && dtype != SNISR_BF16) {
+    set_error("MFMA packing dtype must be SNISR_F16 or SNISR_BF16");
+    return -1;
+  }
+  return (int64_t)s->n_tiles * tile_bytes(s->mlp_in, s->mlp_width, s->mlp_layers);
+}
+
+int mfma_pack(const snisr_scene* s, int dtype, void* packed, void* stream) {
+  if (mfma_pack_bytes(s, dtype) < 0) return SNISR_ERR_SHAPE;
+  SNISR_REQUIRE(aligned(packed, 256), SNISR_ERR_ALIGN, "pack_mlp: packed buffer must be 256-byte aligned");
+  const int in0 = s->mlp_in, wd = s->mlp_width, L = s->mlp_layers;
+  const int64_t tb = tile_bytes(in0, wd, L);
+  int64_t src_off = 0;
+  for (int l = 0; l < L; ++l) {
+    const int in = layer_in(in0, wd, l), out = layer_out(wd, L, l);
+    const int64_t per_tile = (int64_t)nob_of(out) * nkc_of(in) * 512 + nob_of(out) * 32;
+    const int64_t total = per_tile * s->n_tiles;
+    hipLaunchKernelGGL(k_pack_mfma_layer, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, s->mlp_params, s->mlp_stride, src_off, (uint8_t*)packed, tb,
+                       layer_offset(in0, wd, L, l), l, in, out, dtype, s->n_tiles);
+    SNISR_LAUNCHED();
+    src_off += (int64_t)out * in + out;
+  }
+  return SNISR_OK;
+}
+
+// ------------------------------------------------------------------------- workspace
+struct Workspace {
+  int64_t* total;      // header: samples emitted in total
+  int32_t* overflow;   // header: 1 if a ray's records did not fit
+  int32_t* counts;     // [n]
+  int64_t* offsets;    // [n+1]
+  int64_t* block_sums; // [nblocks]
+  float4* rec_f;       // [cap] voxel coordinate + weight
+  int2* rec_ti;        // [cap] tile, ray
+  float4* rgbw;        // [cap] weighted colour of every record
+  int64_t cap;
+};
+
+constexpr int kScanItems = 8, kScanBlock = 256, kScanTile = kScanItems * kScanBlock;
+
+static inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+static int64_t layout(int64_t n, int64_t cap, uint8_t* base, Workspace* w) {
+  const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
+  int64_t off = 256;
+  const int64_t o_counts = off; off = align_up(off + 4 * n, 256);
+  const int64_t o_off = off; off = align_up(off + 8 * (n + 1), 256);
+  const int64_t o_bs = off; off = align_up(off + 8 * (nblocks + 1), 256);
+  const int64_t o_rf = off; off = align_up(off + 16 * cap, 256);
+  const int64_t o_ti = off; off = align_up(off + 8 * cap, 256);
+  const int64_t o_rgb = off; off = align_up(off + 16 * cap, 256);
+  if (w && base) {
+    w->total = reinterpret_cast<int64_t*>(base);
+    w->overflow = reinterpret_cast<int32_t*>(base + 8);
+    w->counts = reinterpret_cast<int32_t*>(base + o_counts);
+    w->offsets = reinterpret_cast<int64_t*>(base + o_off);
+    w->block_sums = reinterpret_cast<int64_t*>(base + o_bs);
+    w->rec_f = reinterpret_cast<float4*>(base + o_rf);
+    w->rec_ti = reinterpret_cast<int2*>(base + o_ti);
+    w->rgbw = reinterpret_cast<float4*>(base + o_rgb);
+    w->cap = cap;
+  }
+  return off;
+}
+
+static int64_t default_capacity(int64_t n) { return 48 * n; }
+
+// ------------------------------------------------------------------------- march passes
+template <bool EMIT>
+__global__ __launch_bounds__(64) void k_march_pass(snisr_scene S, snisr_march_params P, snisr_rays R,
+                                                   snisr_render_out O, Workspace W, int packet_w) {
+  const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (tid >= R.n) return;
+  const int64_t i = ray_of_thread(tid, R.n, packet_w);
+  const float* og = R.origins + i * R.origin_stride;
+  const V3 origin = v3(og[0], og[1], og[2]);
+  const V3 d = normalize_dir(v3(R.dirs[3 * i], R.dirs[3 * i + 1], R.dirs[3 * i + 2]));
+  const float t_hit = R.t_hit ? R.t_hit[i] : __builtin_inff();
+  const int32_t* vis = R.visited ? R.visited + i * P.max_tiles : nullptr;
+  MarchResult res;
+  if (EMIT) {
+    const int64_t base = W.offsets[i];
+    int64_t k = base;
+    bool over = false;
+    march_ray(S, P, origin, d, t_hit, vis, nullptr, 0,
+              [&](int tile, const V3& f, const Corners&, float w) {
+                if (k < W.cap) {
+                  W.rec_f[k] = make_float4(f.x, f.y, f.z, w);
+                  W.rec_ti[k] = make_int2(tile, (int)i);
+                } else {
+                  over = true;
+                }
+                ++k;
+              },
+              res);
+    if (over) *W.overflow = 1;
+  } else {
+    int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
+    march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, [&](int, const V3&, const Corners&, float) {}, res);
+    if (bricks)
+      for (int k = res.n_bricks; k < O.brick_cap; ++k) bricks[k] = -1;
+    W.counts[i] = res.n_samples;
+    if (O.T) O.T[i] = res.T;
+    if (O.n_samples) O.n_samples[i] = res.n_samples;
+    if (O.depth) O.depth[i] = res.depth;
+  }
+}
+
+// ------------------------------------------------------------------------- exclusive scan
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// block-level exclusive scan of one value per thread; returns the block total via *total
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t* total) {
+  __shared__ int64_t wsum[kScanBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t incl = wave_incl_scan(v);
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int64_t before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kScanBlock / 64; ++k) {
+    if (k < wid) before += wsum[k];
+    all += wsum[k];
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_local(const int32_t* __restrict__ counts, int64_t n,
+                                                           int64_t* __restrict__ offsets, int64_t* __restrict__ bsum) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int64_t v[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < n) ? counts[base + k] : 0;
+    s += v[k];
+  }
+  int64_t total;
+  int64_t run = block_excl_scan(s, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) offsets[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_blocks(int64_t* __restrict__ bsum, int64_t nblocks,
+                                                            int64_t* __restrict__ offsets, int64_t n,
+                                                            int64_t* __restrict__ total_out,
+                                                            int32_t* __restrict__ overflow) {
+  int64_t carry = 0;
+  for (int64_t start = 0; start < nblocks; start += kScanBlock) {
+    const int64_t j = start + threadIdx.x;
+    const int64_t v = j < nblocks ? bsum[j] : 0;
+    int64_t tot;
+    const int64_t ex = block_excl_scan(v, &tot);
+    if (j < nblocks) bsum[j] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    offsets[n] = carry;
+    *total_out = carry;
+    *overflow = 0;
+  }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_add(int64_t* __restrict__ offsets, int64_t n,
+                                                         const int64_t* __restrict__ bsum) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  const int64_t add = bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) offsets[base + k] += add;
+}
+
+// ------------------------------------------------------------------------- MFMA MLP
+template <int DT>
+struct Mfma;
+template <>
+struct Mfma<SNISR_F16> {
+  typedef f16x8 vec;
+  __device__ __forceinline__ static f32x16 mma(vec a, vec b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  }
+  __device__ __forceinline__ static vec cvt8(const float* v) {
+    vec r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = (_Float16)v[e];
+    return r;
+  }
+};
+template <>
+struct Mfma<SNISR_BF16> {
+  typedef bf16x8 vec;
+  __device__ __forceinline__ static f32x16 mma(vec a, vec b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+  __device__ __forceinline__ static vec cvt8(const float* v) {
+    vec r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = (__bf16)v[e];
+    return r;
+  }
+};
+
+template <class V>
+__device__ __forceinline__ V load_frag(const uint8_t* __restrict__ p) {
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  return *reinterpret_cast<const V*>(&q);
+}
+
+// 8 consecutive features of one corner voxel as floats
+template <bool HALF>
+__device__ __forceinline__ void load8(const void* __restrict__ feats, int64_t off, float* v) {
+  if (HALF) {
+    const uint4 q = *reinterpret_cast<const uint4*>(reinterpret_cast<const _Float16*>(feats) + off);
+    const _Float16* h = reinterpret_cast<const _Float16*>(&q);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (float)h[e];
+  } else {
+    const float4* p = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(feats) + off);
+    const float4 a = p[0], b = p[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  }
+}
+
+template <int F, int WD, int L, bool HALF, int DT>
+__global__ __launch_bounds__(256) void k_mlp_mfma(snisr_scene S, snisr_rays R, Workspace W) {
+  using M = Mfma<DT>;
+  using vec = typename M::vec;
+  constexpr int IN = F + 15;
+  constexpr int NKC0 = nkc_of(IN);     // input k-chunks (features then view encoding)
+  constexpr int NFC = F / 16;          // feature k-chunks
+  constexpr int NOB = WD / 32;         // hidden out-blocks
+  constexpr int NKH = WD / 16;         // hidden k-chunks
+  constexpr int64_t TB = tile_bytes(IN, WD, L);
+  static_assert(F % 16 == 0 && WD % 32 == 0 && NKC0 == NFC + 1, "shape");
+
+  const int lane = threadIdx.x & 63, h = lane >> 5, s = lane & 31;
+  const int64_t total = *W.total < W.cap ? *W.total : W.cap;
+  const int64_t nchunks = (total + 63) / 64;
+  const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const uint8_t* __restrict__ wts = reinterpret_cast<const uint8_t*>(S.mlp_packed);
+
+  for (int64_t c = wave0; c < nchunks; c += nwaves) {
+    // ---- gather: lane (h, s) builds its half of the input column for samples s of both blocks
+    vec xin[2][NKC0];
+    int tl[2];
+    float wgt[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int64_t r = c * 64 + cb * 32 + s;
+      tl[cb] = -1;
+      wgt[cb] = 0.0f;
+      float xv[NKC0][8];
+#pragma unroll
+      for (int kc = 0; kc < NKC0; ++kc)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xv[kc][e] = 0.0f;
+      if (r < total) {
+        const float4 rf = W.rec_f[r];
+        const int2 ti = W.rec_ti[r];
+        tl[cb] = ti.x;
+        wgt[cb] = rf.w;
+        Corners cn;
+        find_corners(S.brick_map + S.tile_brick_base[ti.x], S.tile_res[ti.x], v3(rf.x, rf.y, rf.z), cn);
+#pragma unroll
+        for (int kc = 0; kc < NFC; ++kc) {
+          float v[8][8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            if (cn.slot[k] >= 0) load8<HALF>(S.feats, (int64_t)cn.slot[k] * F + 16 * kc + 8 * h, v[k]);
+            else {
+#pragma unroll
+              for (int e = 0; e < 8; ++e) v[k][e] = 0.0f;
+            }
+          }
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const float q[8] = {v[0][e], v[1][e], v[2][e], v[3][e], v[4][e], v[5][e], v[6][e], v[7][e]};
+            xv[kc][e] = tri_combine(q, cn.wx, cn.wy, cn.wz);
+          }
+        }
+        const int64_t ray = ti.y;
+        const V3 d = normalize_dir(v3(R.dirs[3 * ray], R.dirs[3 * ray + 1], R.dirs[3 * ray + 2]));
+        float enc[15];
+        view_encoding(d, enc);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int x = 8 * h + e;
+          xv[NFC][e] = x < 15 ? enc[x] : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int kc = 0; kc < NKC0; ++kc) xin[cb][kc] = M::cvt8(xv[kc]);
+    }
+
+    // ---- one MLP pass per distinct tile among the 64 records (usually one)
+    int pend0 = tl[0], pend1 = tl[1];
+    while (true) {
+      const uint64_t m = __ballot(pend0 >= 0 || pend1 >= 0);
+      if (m == 0) break;
+      const int src = __builtin_ctzll(m);
+      const int cand = __shfl(pend0 >= 0 ? pend0 : pend1, src, 64);
+      const int t = __builtin_amdgcn_readfirstlane(cand);
+      const uint8_t* __restrict__ P = wts + (int64_t)t * TB;
+
+      f32x16 acc[NOB][2];
+      vec xb[2][NKH];
+      // layer 0
+      {
+        const uint8_t* Lp = P + layer_offset(IN, WD, L, 0);
+        const float* bias = reinterpret_cast<const float*>(Lp + NOB * NKC0 * 1024);
+#pragma unroll
+        for (int ob = 0; ob < NOB; ++ob) {
+          f32x16 b0;
+          const float4* bp = reinterpret_cast<const float4*>(bias + (ob * 2 + h) * 16);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float4 t4 = bp[q];
+            b0[4 * q] = t4.x; b0[4 * q + 1] = t4.y; b0[4 * q + 2] = t4.z; b0[4 * q + 3] = t4.w;
+          }
+          acc[ob][0] = b0;
+          acc[ob][1] = b0;
+#pragma unroll
+          for (int kc = 0; kc < NKC0; ++kc) {
+            const vec a = load_frag<vec>(Lp + (ob * NKC0 + kc) * 1024 + lane * 16);
+            acc[ob][0] = M::mma(a, xin[0][kc], acc[ob][0]);
+            acc[ob][1] = M::mma(a, xin[1][kc], acc[ob][1]);
+          }
+        }
+      }
+      // hidden layers
+#pragma unroll
+      for (int l = 1; l < L; ++l) {
+#pragma unroll
+        for (int ob = 0; ob < NOB; ++ob)
+#pragma unroll
+          for (int cb = 0; cb < 2; ++cb) {
+            float v[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] = acc[ob][cb][i] > 0.0f ? acc[ob][cb][i] : 0.0f;
+            xb[cb][2 * ob] = M::cvt8(v);
+            xb[cb][2 * ob + 1] = M::cvt8(v + 8);
+          }
+        const uint8_t* Lp = P + layer_offset(IN, WD, L, l);
+        const int nob = (l == L - 1) ? 1 : NOB;
+        const float* bias = reinterpret_cast<const float*>(Lp + nob * NKH * 1024);
+#pragma unroll
+        for (int ob = 0; ob < NOB; ++ob) {
+          if (ob >= nob) break;
+          f32x16 b0;
+          const float4* bp = reinterpret_cast<const float4*>(bias + (ob * 2 + h) * 16);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float4 t4 = bp[q];
+            b0[4 * q] = t4.x; b0[4 * q + 1] = t4.y; b0[4 * q + 2] = t4.z; b0[4 * q + 3] = t4.w;
+          }
+          acc[ob][0] = b0;
+          acc[ob][1] = b0;
+#pragma unroll
+          for (int kc = 0; kc < NKH; ++kc) {
+            const vec a = load_frag<vec>(Lp + (ob * NKH + kc) * 1024 + lane * 16);
+            acc[ob][0] = M::mma(a, xb[0][kc], acc[ob][0]);
+            acc[ob][1] = M::mma(a, xb[1][kc], acc[ob][1]);
+          }
+        }
+      }
+      // output rows 0..2 sit in acc[0][cb][0..2] of the lanes with h == 0
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        const int mine = (cb == 0 ? pend0 : pend1) == t;
+        if (h == 0 && mine) {
+          const int64_t r = c * 64 + cb * 32 + s;
+          const float w = wgt[cb];
+          W.rgbw[r] = make_float4(w * sigmoid_cw(acc[0][cb][0]), w * sigmoid_cw(acc[0][cb][1]),
+                                  w * sigmoid_cw(acc[0][cb][2]), 0.0f);
+        }
+      }
+      if (pend0 == t) pend0 = -1;
+      if (pend1 == t) pend1 = -1;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------- ordered reduction
+__global__ __launch_bounds__(256) void k_reduce(Workspace W, int64_t n, float* __restrict__ rgb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t b = W.offsets[i], e = W.offsets[i + 1];
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+  if (e > W.cap) {
+    c0 = c1 = c2 = __builtin_nanf("");
+  } else {
+    for (int64_t k = b; k < e; ++k) {
+      const float4 v = W.rgbw[k];
+      c0 = c0 + v.x;
+      c1 = c1 + v.y;
+      c2 = c2 + v.z;
+    }
+  }
+  rgb[3 * i] = c0;
+  rgb[3 * i + 1] = c1;
+  rgb[3 * i + 2] = c2;
+}
+
+int packet_width(const snisr_rays& R);  // render_fp32.hip
+
+// compute units of the current device (a per-device cache of a constant attribute)
+static int device_cus() {
+  static std::atomic<int> cus[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int v = cus[dev].load(std::memory_order_relaxed);
+  if (v <= 0) {
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cus[dev].store(v, std::memory_order_relaxed);
+  }
+  return v;
+}
+
+}  // namespace snisr
+
+using namespace snisr;
+
 extern "C" {
-int64_t snisr_render_mfma_workspace_size(const snisr_scene*, int64_t, int64_t) { snisr::set_error("MFMA path not built"); return -1; }
-int snisr_render_mfma(const snisr_scene*, const snisr_march_params*, const snisr_rays*, const snisr_render_out*, void*, int64_t, void*) {
-  snisr::set_error("MFMA path not built"); return SNISR_ERR_UNSUPPORTED; }
+
+int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t sample_capacity) {
+  if (!scene || n < 0 || sample_capacity < 0) {
+    set_error("render_mfma_workspace_size: bad arguments");
+    return -1;
+  }
+  const int64_t cap = sample_capacity ? sample_capacity : default_capacity(n);
+  return layout(n, cap, nullptr, nullptr);
 }
+
+int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
+                      const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream) {
+  SNISR_REQUIRE(out && out->rgb, SNISR_ERR_ARG, "render_mfma: out.rgb is required");
+  if (int e = check_scene(scene, true)) return e;
+  if (int e = check_params(params)) return e;
+  if (int e = check_rays(rays, params)) return e;
+  SNISR_REQUIRE(scene->mlp_packed, SNISR_ERR_ARG, "render_mfma: scene.mlp_packed is NULL (snisr_pack_mlp)");
+  SNISR_REQUIRE(aligned(scene->mlp_packed, 256), SNISR_ERR_ALIGN, "render_mfma: mlp_packed must be 256-byte aligned");
+  SNISR_REQUIRE(workspace && aligned(workspace, 256), SNISR_ERR_ARG, "render_mfma: workspace must be 256-byte aligned");
+  SNISR_REQUIRE(!out->bricks || out->brick_cap > 0, SNISR_ERR_ARG, "render_mfma: bricks needs brick_cap > 0");
+  const int64_t n = rays->n;
+  if (n == 0) return SNISR_OK;
+  // capacity: everything after the fixed part of the layout
+  const int64_t fixed = layout(n, 0, nullptr, nullptr);
+  SNISR_REQUIRE(workspace_bytes >= fixed, SNISR_ERR_CAPACITY, "render_mfma: workspace too small");
+  const int64_t cap = (workspace_bytes - fixed - 3 * 256) / 40;
+  Workspace W;
+  layout(n, cap, (uint8_t*)workspace, &W);
+  SNISR_REQUIRE(layout(n, cap, nullptr, nullptr) <= workspace_bytes, SNISR_ERR_CAPACITY, "render_mfma: layout");
+
+  const int F = scene->feat_dim, WD = scene->mlp_width, L = scene->mlp_layers;
+  const bool half = scene->feat_dtype == SNISR_F16;
+  const int dt = scene->mlp_packed_dtype;
+  hipStream_t st = (hipStream_t)stream;
+  const int pw = packet_width(*rays);
+  const dim3 mgrid((unsigned)((n + 63) / 64));
+  const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
+
+  hipLaunchKernelGGL(k_march_pass<false>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.counts, n, W.offsets,
+                     W.block_sums);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, st, W.block_sums, nblocks, W.offsets, n, W.total,
+                     W.overflow);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.offsets, n, W.block_sums);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_march_pass<true>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw);
+  SNISR_LAUNCHED();
+
+  const dim3 grid((unsigned)(device_cus() * 8)), block(256);
+  bool launched = false;
+#define SNISR_MFMA_CASE(F_, W_, L_, H_, D_)                                                  \
+  if (!launched && F == F_ && WD == W_ && L == L_ && half == H_ && dt == D_) {                \
+    hipLaunchKernelGGL((k_mlp_mfma<F_, W_, L_, H_, D_>), grid, block, 0, st, *scene, *rays, W); \
+    launched = true;                                                                         \
+  }
+  SNISR_MFMA_CASE(32, 128, 4, true, SNISR_F16)
+  SNISR_MFMA_CASE(32, 128, 4, true, SNISR_BF16)
+  SNISR_MFMA_CASE(16, 64, 3, false, SNISR_F16)
+  SNISR_MFMA_CASE(16, 64, 3, false, SNISR_BF16)
+  SNISR_MFMA_CASE(32, 128, 4, false, SNISR_F16)
+#undef SNISR_MFMA_CASE
+  if (!launched) {
+    set_error("render_mfma: unsupported shape F=%d width=%d layers=%d feat_dtype=%d packed=%d", F, WD, L,
+              scene->feat_dtype, dt);
+    return SNISR_ERR_SHAPE;
+  }
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, W, n, out->rgb);
+  SNISR_LAUNCHED();
+  return SNISR_OK;
+}
+
+// Total samples of the last snisr_render_mfma on this workspace and whether any ray overflowed
+// the record capacity (synchronises the stream).
+int snisr_render_mfma_stats(const void* workspace, int64_t* total, int32_t* overflow, void* stream) {
+  SNISR_REQUIRE(workspace && total && overflow, SNISR_ERR_ARG, "render_mfma_stats: NULL pointer");
+  int64_t hdr[2];
+  SNISR_HIP(hipMemcpyAsync(hdr, workspace, 16, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  SNISR_HIP(hipStreamSynchronize((hipStream_t)stream));
+  *total = hdr[0];
+  *overflow = (int32_t)(hdr[1] & 0xffffffff);
+  return SNISR_OK;
+}
+
+}  // extern "C"

@@ -206,7 +206,8 @@ struct MarchResult {
 // bricks; in an occupied brick samples z_k = fma(k, step, near) for z_k < far, interval
 // min(step, far - z_k); alpha = 1 - exp(-sigma*interval), T *= 1 - alpha
 // (rendering/src/include/cuda_utils.h:13-20); stop once T < early_stop or at t >= t_hit.
-// `on_sample(tile, f, w)` receives every composited sample: f = voxel coordinate, w = alpha*T.
+// `on_sample(tile, f, corners, w)` receives every composited sample: f = voxel coordinate
+// (p/voxel - 0.5 in the tile), its trilinear corners, w = alpha*T.
 template <class OnSample>
 __device__ __forceinline__ void march_ray(const snisr_scene& S, const snisr_march_params& P, V3 origin, V3 d,
                                           float t_hit, const int32_t* __restrict__ vis, int32_t* bricks,
@@ -271,7 +272,7 @@ __device__ __forceinline__ void march_ray(const snisr_scene& S, const snisr_marc
           const float interval = fminx(step, far_ - z);
           const float alpha = 1.0f - exp_cw(-(sig * interval));
           const float w = alpha * T;
-          on_sample(tile, c, w);
+          on_sample(tile, f, c, w);
           T = T * (1.0f - alpha);
           ++nsamp;
           if (T < P.early_stop) { done = true; break; }

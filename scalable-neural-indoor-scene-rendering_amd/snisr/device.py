@@ -197,9 +197,18 @@ class MfmaWorkspace:
         return self.buf, need
 
 
+def mfma_stats(buf) -> tuple[int, int]:
+    """(total samples, overflow flag) of the last render on this workspace (synchronises)."""
+    total, over = C.c_int64(0), C.c_int32(0)
+    N.check(N.lib().snisr_render_mfma_stats(_ptr(buf), C.byref(total), C.byref(over), _stream()),
+            "render_mfma_stats")
+    return total.value, over.value
+
+
 def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_width=0, brick_cap=0,
-                workspace: MfmaWorkspace | None = None, capacity: int = 0, **march_kw):
-    """Render with the per-tile MLP on MFMA (snisr_render_mfma)."""
+                workspace: MfmaWorkspace | None = None, capacity: int = 0, check: bool = True, **march_kw):
+    """Render with the per-tile MLP on MFMA (snisr_render_mfma).  With ``check`` the record
+    capacity is verified after the call and the render repeated with room for every sample."""
     if ds.mlp_packed is None:
         raise RuntimeError("scene has no MFMA-packed MLP; call DeviceScene.pack_mfma()")
     r = _rays_desc(origins, dirs, t_hit, visited, image_width)
@@ -209,6 +218,13 @@ def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_
     buf, need = ws.get(ds, r.n, capacity)
     N.check(N.lib().snisr_render_mfma(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), _ptr(buf), need,
                                       _stream()), "render_mfma")
+    if check:
+        total, over = mfma_stats(buf)
+        if over:
+            buf, need = ws.get(ds, r.n, int(total * 1.25) + 64)
+            N.check(N.lib().snisr_render_mfma(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), _ptr(buf),
+                                              need, _stream()), "render_mfma")
+        out["total_samples"] = total
     return out
 
 
