@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark: full-frame render of the 16-tile synthetic room (1280x720) on one MI355X.
+
+One step = one frame through the hot path with every input resident in HBM:
+camera rays -> BVH nearest hit on the 200k-triangle proxy mesh (bounds the march) -> tile DDA,
+brick DDA march and sigma compositing -> per-tile width-128x4 MLP on FP16 MFMA with FP32
+accumulation -> ordered per-ray reduction.  Prints one JSON line (see the field notes in main()).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "scalable-neural-indoor-scene-rendering_amd"))
+
+METRIC = "Mrays/s render (1280x720) & train fwd+bwd at PSNR>=50dB vs FP64, 1 MI355X"
+MFMA_PEAK_F16_TFLOPS = 2500.0   # MI355X dense FP16 MFMA
+HBM_PEAK_GBS = 8000.0
+
+
+def mlp_flops_per_sample(mlp_in, width, layers):
+    from snisr.scene import mlp_layer_dims
+    return 2 * sum(i * o for i, o in mlp_layer_dims(mlp_in, width, layers))
+
+
+def cpu_baseline(sc, cam, dirs_np, t_hit_np, frac, seed=0):
+    """The oracle (FP32, OpenMP over all host cores) on a seeded subset of the frame's rays,
+    given the GPU's mesh t_hit (the CPU mesh query is not part of the timed sample)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    O.build()
+    osc = O.OracleScene(sc)
+    n = dirs_np.shape[0]
+    idx = np.sort(np.random.default_rng(seed).choice(n, max(1, int(n * frac)), replace=False))
+    O.render(osc, cam.origin, dirs_np, t_hit=t_hit_np, ray_index=idx[:256])  # warm-up
+    t0 = time.perf_counter()
+    O.render(osc, cam.origin, dirs_np, t_hit=t_hit_np, ray_index=idx)
+    dt = time.perf_counter() - t0
+    return dict(value=round(len(idx) / dt / 1e6, 4), unit="Mrays/s", cores=O.num_threads(), kind="port",
+                sample=f"{len(idx)} seeded rays ({frac:.0%} of the 1280x720 frame), FP32 oracle march+MLP+"
+                       f"compositing with the GPU mesh t_hit, {dt:.2f} s wall")
+
+
+def psnr_vs_fp64(sc, cam, dirs_np, t_hit_np, rgb_gpu, n_rays=2000, seed=1):
+    import numpy as np
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    idx = np.sort(np.random.default_rng(seed).choice(dirs_np.shape[0], n_rays, replace=False))
+    ref = O.render(O.OracleScene(sc), cam.origin, dirs_np, t_hit=t_hit_np, ray_index=idx, precision="f64")
+    mse = float(np.mean((rgb_gpu[idx].astype(np.float64) - ref["rgb"].astype(np.float64)) ** 2))
+    return 10 * np.log10(1.0 / max(mse, 1e-30))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and PSNR legs")
+    ap.add_argument("--cpu-frac", type=float, default=0.05)
+    ap.add_argument("--fp32-frames", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: the render has no cross-GPU exchange in this build "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from snisr import _native as N
+    from snisr import device as D
+    from snisr import scene as S
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    sc, cam, mesh = S.config_c2()
+    ds = D.DeviceScene(sc, mfma_dtype=N.SNISR_F16)
+    dm = D.DeviceMesh(mesh.vertices, mesh.faces)
+    intr = torch.from_numpy(cam.intrinsic).to(dev)
+    c2w = torch.from_numpy(cam.c2w).to(dev)
+    origin = torch.from_numpy(cam.origin).to(dev)
+    H, W = cam.H, cam.W
+    n_rays = H * W
+    rays_d = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    t_hit = torch.empty(n_rays, dtype=torch.float32, device=dev)
+    tri_id = torch.empty(n_rays, dtype=torch.int32, device=dev)
+    ws = D.MfmaWorkspace()
+    flops_per_sample = mlp_flops_per_sample(sc.mlp_in, sc.mlp_width, sc.mlp_layers)
+
+    def step(events=None):
+        D.compute_rays(intr, c2w, H, W, out=rays_d)
+        dm.first_hit(origin, rays_d, image_width=W, t_out=t_hit, id_out=tri_id)
+        return D.render_mfma(ds, origin, rays_d, t_hit=t_hit, image_width=W, workspace=ws, check=False,
+                             events=events)
+
+    # size the workspace once (exact sample count of this view), then warm up
+    first = D.render_mfma(ds, origin, D.compute_rays(intr, c2w, H, W), image_width=W, workspace=ws, check=True)
+    del first
+    for _ in range(args.warmup):
+        out = step()
+    torch.cuda.synchronize()
+    total_samples, overflow = D.mfma_stats(ws.buf)
+    assert overflow == 0
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(args.steps)]
+    for row in ev:
+        for e in row:
+            e.record()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        out = step(ev[k])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    stage = np.array([[ev[k][j].elapsed_time(ev[k][j + 1]) for j in range(6)] for k in range(args.steps)])
+    stage_ms = stage.mean(0)
+    mlp_ms = float(stage_ms[4])
+    mrays = n_rays / (ms_per_step * 1e-3) / 1e6
+
+    flops = total_samples * flops_per_sample
+    achieved = flops / (mlp_ms * 1e-3) / 1e12
+    roofline = dict(bound="mfma", achieved=round(achieved, 2), peak=MFMA_PEAK_F16_TFLOPS, unit="TFLOP/s",
+                    frac=round(achieved / MFMA_PEAK_F16_TFLOPS, 4), traffic=None,
+                    kernel="k_mlp_mfma<32,128,4,f16 feats,f16 MFMA>",
+                    flop_per_launch=flops, samples_per_launch=total_samples)
+
+    rgb = out["rgb"].cpu().numpy()
+    extra = {}
+    # plain FP32 path (one thread per ray) on the same frame, for the MFMA/FP32 speed ratio
+    if args.fp32_frames > 0:
+        D.render_fp32(ds, origin, rays_d, t_hit=t_hit, image_width=W)
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(args.fp32_frames):
+            o32 = D.render_fp32(ds, origin, rays_d, t_hit=t_hit, image_width=W)
+        torch.cuda.synchronize()
+        fp32_ms = (time.perf_counter() - a) * 1e3 / args.fp32_frames
+        render_ms = float(stage_ms.sum())
+        extra["fp32_path_ms_per_frame"] = round(fp32_ms, 3)
+        extra["mfma_vs_fp32_path_render_speedup"] = round(fp32_ms / render_ms, 2)
+        extra["counts_equal_fp32_path"] = bool(torch.equal(o32["n_samples"], out["n_samples"]))
+
+    cpu = None
+    if not args.no_cpu:
+        dirs_np = rays_d.reshape(-1, 3).cpu().numpy()
+        th_np = t_hit.cpu().numpy()
+        extra["psnr_vs_fp64_db"] = round(psnr_vs_fp64(sc, cam, dirs_np, th_np, rgb), 2)
+        cpu = cpu_baseline(sc, cam, dirs_np, th_np, args.cpu_frac)
+
+    result = {
+        "metric": METRIC,
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16",
+        "data": "synthetic",
+        "config": {
+            "workload": "C2 full-frame render 1280x720: 4x4x1 tiles of 128^3 voxels, 5% 8^3-brick occupancy, "
+                        "F=32 FP16 features + FP32 sigma, per-tile MLP 47-128-128-128-3 (FP16 MFMA, FP32 acc), "
+                        "200k-triangle proxy mesh BVH first hit bounding the march, seed 2",
+            "rays": n_rays,
+            "samples_per_frame": int(total_samples),
+            "mesh_triangles": int(mesh.faces.shape[0]),
+        },
+        "stage_ms": {k: round(float(v), 4) for k, v in zip(
+            ["march_count", "scan", "march_emit", "tile_sort", "mlp_mfma", "reduce"], stage_ms)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -99,7 +99,10 @@ typedef struct snisr_rays {
 
 /* Outputs (NULL = not written): rgb f32[N,3]; T f32[N] final transmittance; n_samples i32[N];
  * depth f32[N] (t of the first sample reached with T < trans_th, -1 if none);
- * bricks i32[N,brick_cap] occupied bricks marched in order, -1 padded (debug). */
+ * bricks i32[N,brick_cap] occupied bricks marched in order, -1 padded (debug).
+ * stage_events: optional array of n_stage_events hipEvent_t recorded on `stream` at the stage
+ * boundaries (render_mfma: before the count march, after it, after the scan, after the emit march,
+ * after the tile sort, after the MLP, after the reduction; render_fp32: before and after its kernel). */
 typedef struct snisr_render_out {
   float* rgb;
   float* T;
@@ -107,6 +110,8 @@ typedef struct snisr_render_out {
   float* depth;
   int32_t* bricks;
   int32_t brick_cap;
+  int32_t n_stage_events;
+  void* const* stage_events;
 } snisr_render_out;
 
 const char* snisr_last_error(void);
@@ -139,15 +144,34 @@ int snisr_render_fp32(const snisr_scene* scene, const snisr_march_params* params
 int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t sample_capacity);
 
 /* Same render with the per-tile MLP on MFMA (FP16 or BF16 operands, FP32 accumulation,
- * FP32 transmittance).  Three stages: march (sigma only) emitting per-sample records bucketed
- * by tile, a per-tile MFMA MLP over the buckets, and an ordered per-ray reduction.  Sample
- * counts, T, depth and the brick lists are bit-identical to snisr_render_fp32. */
+ * FP32 transmittance).  Stages: a sigma-only march counting samples per ray, a scan, a second
+ * march emitting per-sample records contiguous per ray, a counting sort of the records by tile,
+ * the per-tile MFMA MLP (weights staged in LDS once per tile and workgroup), and an ordered
+ * per-ray reduction.  Sample counts, T, depth and the brick lists are bit-identical to
+ * snisr_render_fp32; the pixels are deterministic. */
 int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                       const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* Samples emitted by the last snisr_render_mfma on `workspace` and whether a ray overflowed the
  * record capacity (its rgb is then NaN).  Synchronises `stream`.  No reference counterpart. */
 int snisr_render_mfma_stats(const void* workspace, int64_t* total, int32_t* overflow, void* stream);
+
+/* A9 — proxy-mesh nearest hit.  Replaces ray_firstHit_inTile_kernel (training/src/cuda_func.cu:130-282,
+ * per-tile per-block face lists BConFaceIdx/BConFaceNum, training/scripts/tools.py:364-379) and its
+ * RayTriangleIntersection (rendering/src/include/cuda_utils.h:403-436) with a BVH over the whole mesh.
+ *
+ * snisr_bvh_build_bounds: host bytes to allocate for snisr_bvh_build's two outputs.
+ * snisr_bvh_build (host, synchronous): vertices f32[V,3], faces i32[F,3] -> nodes (64 B each) and
+ * leaf-ordered triangles (48 B each); n_nodes and the tree depth are returned.  Upload both arrays
+ * to device memory unchanged for snisr_mesh_first_hit, which must be given the same n_nodes, n_tris
+ * and depth.
+ * snisr_mesh_first_hit: per ray (directions normalised as in the marcher) the smallest t > 0 over
+ * all triangles, ties to the smaller triangle id; t = +inf and id = -1 without a hit. */
+int snisr_bvh_build_bounds(int64_t n_faces, int64_t* node_bytes, int64_t* tri_bytes);
+int snisr_bvh_build(const float* vertices, int64_t n_vertices, const int32_t* faces, int64_t n_faces, void* nodes_out,
+                    int64_t node_bytes, void* tris_out, int64_t tri_bytes, int32_t* n_nodes, int32_t* depth);
+int snisr_mesh_first_hit(const void* nodes, int32_t n_nodes, const void* tris, int64_t n_tris, int32_t depth,
+                         const snisr_rays* rays, float* t_out, int32_t* id_out, void* stream);
 
 /* Bytes of a packed weight buffer of the scene's MLP shape for `dtype`, and the packing itself
  * (device to device, from scene->mlp_params into `packed`): SNISR_F32 gives the mlp_f32t layout,

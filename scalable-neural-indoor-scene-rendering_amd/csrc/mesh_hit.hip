@@ -1,0 +1,143 @@
+// A9 — nearest proxy-mesh hit per ray over the BVH of bvh_build.cpp.
+//
+// Triangle test: RayTriangleIntersection (rendering/src/include/cuda_utils.h:403-436, blur 0)
+// with the two-sided vertex flip of ray_firstHit_inTile_kernel (training/src/cuda_func.cu:215-231);
+// the flip uses the sign of dot(d, cross(v1 - v0, v2 - v1)) (the reference normalises that normal
+// first, which does not change its sign).  A hit needs t > 0; the nearest t wins, ties go to the
+// smaller triangle id, so the result does not depend on the traversal order and equals a brute
+// force over all triangles.  Each 64-byte node holds both child boxes, so one node fetch (four
+// 16-byte loads of adjacent addresses) tests two children.
+#include "snisr_common.hpp"
+#include "snisr_device.hpp"
+
+namespace snisr {
+
+constexpr int kBvhStack = 64;
+
+__device__ __forceinline__ V3 sub3(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross3(V3 a, V3 b) {
+  return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// t of the hit, or -1e8 on a miss
+__device__ __forceinline__ float ray_triangle(V3 o, V3 d, V3 a, V3 b, V3 c) {
+  const float sim = dot3(d, cross3(sub3(b, a), sub3(c, b)));
+  V3 v1 = b, v2 = c;
+  if (sim > 0.0f) { v1 = c; v2 = b; }
+  const V3 v0v1 = sub3(v1, a), v0v2 = sub3(v2, a), v0o = sub3(o, a);
+  const V3 pv = cross3(d, v0v2);
+  float det = dot3(v0v1, pv);
+  if (det < 0.00000001f) return -100000000.0f;
+  det = 1.0f / det;
+  const float u = dot3(v0o, pv) * det;
+  if (u < 0.0f || u > 1.0f) return -100000000.0f;
+  const V3 qv = cross3(v0o, v0v1);
+  const float v = dot3(d, qv) * det;
+  if (v < 0.0f || v > 1.0f) return -100000000.0f;
+  if ((u + v) < 0.0f || (u + v) > 1.0f) return -100000000.0f;
+  return dot3(v0v2, qv) * det;
+}
+
+// slab test against a (widened) box: entry/exit t; a direction component of 0 uses +-1e30
+__device__ __forceinline__ void slab(const float* b, V3 o, V3 inv, float& tn, float& tf) {
+  const float x0 = (b[0] - o.x) * inv.x, x1 = (b[3] - o.x) * inv.x;
+  const float y0 = (b[1] - o.y) * inv.y, y1 = (b[4] - o.y) * inv.y;
+  const float z0 = (b[2] - o.z) * inv.z, z1 = (b[5] - o.z) * inv.z;
+  tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+  tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+}
+
+__global__ __launch_bounds__(64) void k_mesh_first_hit(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                                       snisr_rays R, float* __restrict__ t_out,
+                                                       int32_t* __restrict__ id_out, int packet_w) {
+  __shared__ int stack_mem[kBvhStack * 64];
+  int* stack = stack_mem + threadIdx.x;  // lane-strided: entry k at stack[k * 64]
+  const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (tid >= R.n) return;
+  const int64_t i = ray_of_thread(tid, R.n, packet_w);
+  const float* og = R.origins + i * R.origin_stride;
+  const V3 o = v3(og[0], og[1], og[2]);
+  const V3 d = normalize_dir(v3(R.dirs[3 * i], R.dirs[3 * i + 1], R.dirs[3 * i + 2]));
+  const V3 inv = v3(d.x != 0.0f ? 1.0f / d.x : __builtin_copysignf(1e30f, d.x),
+                    d.y != 0.0f ? 1.0f / d.y : __builtin_copysignf(1e30f, d.y),
+                    d.z != 0.0f ? 1.0f / d.z : __builtin_copysignf(1e30f, d.z));
+  float best = __builtin_inff();
+  int best_id = -1;
+  int sp = 0;
+  int node = 0;
+  while (true) {
+    const float4 q0 = nodes[4 * node], q1 = nodes[4 * node + 1], q2 = nodes[4 * node + 2], q3 = nodes[4 * node + 3];
+    const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+    const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+    const int c[2] = {__float_as_int(q3.x), __float_as_int(q3.y)};
+    float tn[2], tf[2];
+    slab(b0, o, inv, tn[0], tf[0]);
+    slab(b1, o, inv, tn[1], tf[1]);
+    bool hit[2];
+    int inner[2], ni = 0;
+    float inner_t[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      hit[k] = tn[k] <= tf[k] && tf[k] >= 0.0f && tn[k] <= best;
+      if (!hit[k]) continue;
+      if (c[k] >= 0) {
+        inner[ni] = c[k];
+        inner_t[ni] = tn[k];
+        ++ni;
+      } else {
+        const int code = ~c[k];
+        const int first = code >> 3, cnt = code & 7;
+        for (int j = 0; j < cnt; ++j) {
+          const float4 t0 = tris[3 * (first + j)], t1 = tris[3 * (first + j) + 1], t2 = tris[3 * (first + j) + 2];
+          const int id = __float_as_int(t0.w);
+          const float t = ray_triangle(o, d, v3(t0.x, t0.y, t0.z), v3(t1.x, t1.y, t1.z), v3(t2.x, t2.y, t2.z));
+          if (t > 0.0f && (t < best || (t == best && id < best_id))) {
+            best = t;
+            best_id = id;
+          }
+        }
+      }
+    }
+    if (ni == 2) {
+      const bool swap = inner_t[1] < inner_t[0];
+      const int near_n = swap ? inner[1] : inner[0], far_n = swap ? inner[0] : inner[1];
+      stack[sp * 64] = far_n;  // sp < kBvhStack: the tree depth is below the stack size
+      ++sp;
+      node = near_n;
+    } else if (ni == 1) {
+      node = inner[0];
+    } else {
+      if (sp == 0) break;
+      --sp;
+      node = stack[sp * 64];
+    }
+  }
+  t_out[i] = best;
+  id_out[i] = best_id;
+}
+
+int packet_width(const snisr_rays& R);  // render_fp32.hip
+
+}  // namespace snisr
+
+using namespace snisr;
+
+extern "C" {
+
+int snisr_mesh_first_hit(const void* nodes, int32_t n_nodes, const void* tris, int64_t n_tris, int32_t depth,
+                         const snisr_rays* rays, float* t_out, int32_t* id_out, void* stream) {
+  SNISR_REQUIRE(nodes && tris && rays && t_out && id_out, SNISR_ERR_ARG, "mesh_first_hit: NULL pointer");
+  SNISR_REQUIRE(n_nodes >= 1 && n_tris >= 0, SNISR_ERR_ARG, "mesh_first_hit: empty BVH");
+  SNISR_REQUIRE(depth >= 0 && depth < kBvhStack, SNISR_ERR_SHAPE, "mesh_first_hit: BVH depth %d >= %d", depth,
+                kBvhStack);
+  SNISR_REQUIRE(aligned(nodes, 16) && aligned(tris, 16), SNISR_ERR_ALIGN, "mesh_first_hit: 16-byte alignment");
+  if (int e = check_rays(rays, nullptr)) return e;
+  if (rays->n == 0) return SNISR_OK;
+  hipLaunchKernelGGL(k_mesh_first_hit, dim3((unsigned)((rays->n + 63) / 64)), dim3(64), 0, (hipStream_t)stream,
+                     (const float4*)nodes, (const float4*)tris, *rays, t_out, id_out, packet_width(*rays));
+  SNISR_LAUNCHED();
+  return SNISR_OK;
+}
+
+}  // extern "C"

@@ -268,8 +268,10 @@ int snisr_render_fp32(const snisr_scene* scene, const snisr_march_params* params
   const dim3 grid((unsigned)((rays->n + 63) / 64));
   hipStream_t st = (hipStream_t)stream;
   if (!want_rgb) {
+    SNISR_HIP(stage_event(out, 0, st));
     hipLaunchKernelGGL(k_march_sigma, grid, dim3(64), 0, st, *scene, *params, *rays, *out, pw);
     SNISR_LAUNCHED();
+    SNISR_HIP(stage_event(out, 1, st));
     return SNISR_OK;
   }
   SNISR_REQUIRE(scene->mlp_f32t, SNISR_ERR_ARG, "render_fp32: scene.mlp_f32t is NULL (snisr_pack_mlp(SNISR_F32))");
@@ -278,8 +280,10 @@ int snisr_render_fp32(const snisr_scene* scene, const snisr_march_params* params
   const bool half = scene->feat_dtype == SNISR_F16;
 #define SNISR_FP32_CASE(F_, W_, L_, H_)                                                                   \
   if (F == F_ && WD == W_ && L == L_ && half == H_) {                                                     \
+    SNISR_HIP(stage_event(out, 0, st));                                                                   \
     hipLaunchKernelGGL((k_render_fp32<F_, W_, L_, H_>), grid, dim3(64), 0, st, *scene, *params, *rays, *out, pw); \
     SNISR_LAUNCHED();                                                                                     \
+    SNISR_HIP(stage_event(out, 1, st));                                                                   \
     return SNISR_OK;                                                                                      \
   }
   SNISR_FP32_CASE(16, 64, 3, false)

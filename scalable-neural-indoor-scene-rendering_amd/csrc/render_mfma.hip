@@ -115,27 +115,37 @@ int mfma_pack(const snisr_scene* s, int dtype, void* packed, void* stream) {
 
 // ------------------------------------------------------------------------- workspace
 struct Workspace {
-  int64_t* total;      // header: samples emitted in total
-  int32_t* overflow;   // header: 1 if a ray's records did not fit
-  int32_t* counts;     // [n]
-  int64_t* offsets;    // [n+1]
-  int64_t* block_sums; // [nblocks]
-  float4* rec_f;       // [cap] voxel coordinate + weight
-  int2* rec_ti;        // [cap] tile, ray
-  float4* rgbw;        // [cap] weighted colour of every record
+  int64_t* total;       // header: samples emitted in total
+  int32_t* overflow;    // header: 1 if a ray's records did not fit
+  int32_t* counts;      // [n]
+  int64_t* offsets;     // [n+1]
+  int64_t* block_sums;  // [nblocks]
+  float4* rec_f;        // [cap] voxel coordinate + weight   (ray-major)
+  int2* rec_ti;         // [cap] tile, ray                    (ray-major)
+  float4* rgbw;         // [cap] weighted colour of every record (ray-major)
+  int32_t* perm;        // [cap + 64 T] record index in tile order, tile buckets 64-aligned, -1 = padding
+  int32_t* tile_count;  // [T]
+  int64_t* tile_start;  // [T+1] 64-aligned bucket starts
+  int64_t* tile_cursor; // [T]
   int64_t cap;
+  int32_t n_tiles;
 };
 
 constexpr int kScanItems = 8, kScanBlock = 256, kScanTile = kScanItems * kScanBlock;
+constexpr int kMaxTiles = 4096;  // tile histogram in LDS
 
 static inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
-static int64_t layout(int64_t n, int64_t cap, uint8_t* base, Workspace* w) {
+static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspace* w) {
   const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
   int64_t off = 256;
   const int64_t o_counts = off; off = align_up(off + 4 * n, 256);
   const int64_t o_off = off; off = align_up(off + 8 * (n + 1), 256);
   const int64_t o_bs = off; off = align_up(off + 8 * (nblocks + 1), 256);
+  const int64_t o_tc = off; off = align_up(off + 4 * (int64_t)T, 256);
+  const int64_t o_ts = off; off = align_up(off + 8 * ((int64_t)T + 1), 256);
+  const int64_t o_cu = off; off = align_up(off + 8 * (int64_t)T, 256);
+  const int64_t o_pm = off; off = align_up(off + 4 * (cap + 64 * (int64_t)T), 256);
   const int64_t o_rf = off; off = align_up(off + 16 * cap, 256);
   const int64_t o_ti = off; off = align_up(off + 8 * cap, 256);
   const int64_t o_rgb = off; off = align_up(off + 16 * cap, 256);
@@ -145,10 +155,15 @@ static int64_t layout(int64_t n, int64_t cap, uint8_t* base, Workspace* w) {
     w->counts = reinterpret_cast<int32_t*>(base + o_counts);
     w->offsets = reinterpret_cast<int64_t*>(base + o_off);
     w->block_sums = reinterpret_cast<int64_t*>(base + o_bs);
+    w->tile_count = reinterpret_cast<int32_t*>(base + o_tc);
+    w->tile_start = reinterpret_cast<int64_t*>(base + o_ts);
+    w->tile_cursor = reinterpret_cast<int64_t*>(base + o_cu);
+    w->perm = reinterpret_cast<int32_t*>(base + o_pm);
     w->rec_f = reinterpret_cast<float4*>(base + o_rf);
     w->rec_ti = reinterpret_cast<int2*>(base + o_ti);
     w->rgbw = reinterpret_cast<float4*>(base + o_rgb);
     w->cap = cap;
+    w->n_tiles = T;
   }
   return off;
 }
@@ -248,7 +263,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_local(const int32_t* __rest
 __global__ __launch_bounds__(kScanBlock) void k_scan_blocks(int64_t* __restrict__ bsum, int64_t nblocks,
                                                             int64_t* __restrict__ offsets, int64_t n,
                                                             int64_t* __restrict__ total_out,
-                                                            int32_t* __restrict__ overflow) {
+                                                            int32_t* __restrict__ overflow,
+                                                            int32_t* __restrict__ tile_count, int n_tiles) {
+  for (int t = threadIdx.x; t < n_tiles; t += blockDim.x) tile_count[t] = 0;
   int64_t carry = 0;
   for (int64_t start = 0; start < nblocks; start += kScanBlock) {
     const int64_t j = start + threadIdx.x;
@@ -326,163 +343,262 @@ __device__ __forceinline__ void load8(const void* __restrict__ feats, int64_t of
   }
 }
 
+// ------------------------------------------------------------------------- tile sort
+// Counting sort of the ray-major records by tile into 64-aligned buckets (perm holds record
+// indices; padding slots are -1).  The order inside a bucket is not deterministic, but every
+// record's result goes to its own ray-major slot and does not depend on its neighbours, so the
+// pixels are.
+constexpr int kSortBlock = 1024, kSortGrid = 512;
+
+__device__ __forceinline__ void record_range(int64_t total, int64_t& b, int64_t& e) {
+  const int64_t per = (total + gridDim.x - 1) / gridDim.x;
+  b = (int64_t)blockIdx.x * per;
+  e = b + per < total ? b + per : total;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_tile_hist(Workspace W) {
+  __shared__ int hist[kMaxTiles];
+  const int T = W.n_tiles;
+  for (int t = threadIdx.x; t < T; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
+  const int64_t total = *W.total < W.cap ? *W.total : W.cap;
+  int64_t b, e;
+  record_range(total, b, e);
+  for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) atomicAdd(&hist[W.rec_ti[k].x], 1);
+  __syncthreads();
+  for (int t = threadIdx.x; t < T; t += blockDim.x)
+    if (hist[t]) atomicAdd(&W.tile_count[t], hist[t]);
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_tile_scan(Workspace W) {
+  const int T = W.n_tiles;
+  int64_t carry = 0;
+  for (int start = 0; start < T; start += kScanBlock) {
+    const int t = start + threadIdx.x;
+    const int64_t v = t < T ? (int64_t)((W.tile_count[t] + 63) & ~63) : 0;
+    int64_t tot;
+    const int64_t ex = block_excl_scan(v, &tot);
+    if (t < T) {
+      W.tile_start[t] = carry + ex;
+      W.tile_cursor[t] = carry + ex;
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) W.tile_start[T] = carry;
+  __syncthreads();
+  // padding slots at the end of every bucket
+  for (int t = 0; t < T; ++t) {
+    const int64_t pb = W.tile_start[t] + W.tile_count[t], pe = W.tile_start[t + 1];
+    for (int64_t k = pb + threadIdx.x; k < pe; k += blockDim.x) W.perm[k] = -1;
+  }
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_tile_scatter(Workspace W) {
+  __shared__ int hist[kMaxTiles];
+  __shared__ int64_t base[kMaxTiles];
+  const int T = W.n_tiles;
+  for (int t = threadIdx.x; t < T; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
+  const int64_t total = *W.total < W.cap ? *W.total : W.cap;
+  int64_t b, e;
+  record_range(total, b, e);
+  for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) atomicAdd(&hist[W.rec_ti[k].x], 1);
+  __syncthreads();
+  for (int t = threadIdx.x; t < T; t += blockDim.x) {
+    base[t] = hist[t] ? atomicAdd((unsigned long long*)&W.tile_cursor[t], (unsigned long long)hist[t]) : 0;
+    hist[t] = 0;
+  }
+  __syncthreads();
+  for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+    const int t = W.rec_ti[k].x;
+    const int r = atomicAdd(&hist[t], 1);
+    W.perm[base[t] + r] = (int32_t)k;
+  }
+}
+
+
+// acc{0,1} += sum_kc A(ob, kc) * src{0,1}[kc] with A fragments read from LDS one step ahead;
+// the scheduling barrier keeps at most two fragments in registers.
+template <class M, int NK>
+__device__ __forceinline__ void mma_block(const uint8_t* Ab, int lane, const typename M::vec (&s0)[NK],
+                                          const typename M::vec (&s1)[NK], f32x16& acc0, f32x16& acc1) {
+  using vec = typename M::vec;
+  vec nxt = load_frag<vec>(Ab + lane * 16);
+#pragma unroll
+  for (int kc = 0; kc < NK; ++kc) {
+    const vec a = nxt;
+    if (kc + 1 < NK) nxt = load_frag<vec>(Ab + (kc + 1) * 1024 + lane * 16);
+    acc0 = M::mma(a, s0[kc], acc0);
+    acc1 = M::mma(a, s1[kc], acc1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+__device__ __forceinline__ f32x16 bias_block(const float* bias, int ob, int h) {
+  f32x16 acc;
+  const float4* bp = reinterpret_cast<const float4*>(bias + (ob * 2 + h) * 16);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 t4 = bp[q];
+    acc[4 * q] = t4.x; acc[4 * q + 1] = t4.y; acc[4 * q + 2] = t4.z; acc[4 * q + 3] = t4.w;
+  }
+  return acc;
+}
+
+template <class M>
+__device__ __forceinline__ void relu_cvt(const f32x16& acc, typename M::vec& lo, typename M::vec& hi) {
+  float v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = acc[i] > 0.0f ? acc[i] : 0.0f;
+  lo = M::cvt8(v);
+  hi = M::cvt8(v + 8);
+}
+
+// ------------------------------------------------------------------------- MFMA MLP
+// Persistent: one 512-thread workgroup per CU; workgroup g owns an equal contiguous span of the
+// 64-record chunks of the tile-sorted order.  For every tile inside its span it stages that tile's
+// packed weights (~86 KB for width 128 x 4) in LDS once, then its 8 wavefronts process the span's
+// chunks of that tile, each MFMA reading its A fragment from LDS (ds_read_b128, conflict-free).
+// Per layer the accumulators of one 32-row out-block live at a time (32 VGPRs); the layer's input
+// and output activations stay in registers in the B-operand layout.
 template <int F, int WD, int L, bool HALF, int DT>
-__global__ __launch_bounds__(256) void k_mlp_mfma(snisr_scene S, snisr_rays R, Workspace W) {
+__global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, Workspace W) {
   using M = Mfma<DT>;
   using vec = typename M::vec;
   constexpr int IN = F + 15;
-  constexpr int NKC0 = nkc_of(IN);     // input k-chunks (features then view encoding)
-  constexpr int NFC = F / 16;          // feature k-chunks
-  constexpr int NOB = WD / 32;         // hidden out-blocks
-  constexpr int NKH = WD / 16;         // hidden k-chunks
+  constexpr int NKC0 = nkc_of(IN);
+  constexpr int NFC = F / 16;
+  constexpr int NOB = WD / 32;
+  constexpr int NKH = WD / 16;
   constexpr int64_t TB = tile_bytes(IN, WD, L);
   static_assert(F % 16 == 0 && WD % 32 == 0 && NKC0 == NFC + 1, "shape");
+  extern __shared__ __align__(16) uint8_t lds_w[];
 
   const int lane = threadIdx.x & 63, h = lane >> 5, s = lane & 31;
-  const int64_t total = *W.total < W.cap ? *W.total : W.cap;
-  const int64_t nchunks = (total + 63) / 64;
-  const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  const uint8_t* __restrict__ wts = reinterpret_cast<const uint8_t*>(S.mlp_packed);
+  const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  const int T = W.n_tiles;
+  const int64_t nchunks = W.tile_start[T] >> 6;
+  const int64_t per = (nchunks + gridDim.x - 1) / gridDim.x;
+  int64_t cb0 = (int64_t)blockIdx.x * per;
+  const int64_t cend = cb0 + per < nchunks ? cb0 + per : nchunks;
+  const uint8_t* __restrict__ gw = reinterpret_cast<const uint8_t*>(S.mlp_packed);
 
-  for (int64_t c = wave0; c < nchunks; c += nwaves) {
-    // ---- gather: lane (h, s) builds its half of the input column for samples s of both blocks
-    vec xin[2][NKC0];
-    int tl[2];
-    float wgt[2];
+  while (cb0 < cend) {
+    // tile of chunk cb0 (buckets are sorted by tile id)
+    int lo = 0, hi = T;  // tile_start[lo]/64 <= cb0 < tile_start[hi]/64
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if ((W.tile_start[mid] >> 6) <= cb0) lo = mid; else hi = mid;
+    }
+    const int t = lo;
+    const int64_t seg_end = (W.tile_start[t + 1] >> 6) < cend ? (W.tile_start[t + 1] >> 6) : cend;
+    // stage the tile's weights
+    __syncthreads();
+    {
+      const uint4* src = reinterpret_cast<const uint4*>(gw + (int64_t)t * TB);
+      uint4* dst = reinterpret_cast<uint4*>(lds_w);
+      for (int k = threadIdx.x; k < (int)(TB / 16); k += blockDim.x) dst[k] = src[k];
+    }
+    __syncthreads();
+
+    for (int64_t c = cb0 + wv; c < seg_end; c += nwv) {
+      // ---- gather: lane (h, s) builds its half of the input column of sample s of both blocks
+      vec xin[2][NKC0];
+      float wgt[2];
+      int32_t rec[2];
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      const int64_t r = c * 64 + cb * 32 + s;
-      tl[cb] = -1;
-      wgt[cb] = 0.0f;
-      float xv[NKC0][8];
+      for (int cbk = 0; cbk < 2; ++cbk) {
+        const int32_t k = W.perm[c * 64 + cbk * 32 + s];
+        rec[cbk] = k;
+        wgt[cbk] = 0.0f;
+        float xv[NKC0][8];
 #pragma unroll
-      for (int kc = 0; kc < NKC0; ++kc)
+        for (int kc = 0; kc < NKC0; ++kc)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) xv[kc][e] = 0.0f;
-      if (r < total) {
-        const float4 rf = W.rec_f[r];
-        const int2 ti = W.rec_ti[r];
-        tl[cb] = ti.x;
-        wgt[cb] = rf.w;
-        Corners cn;
-        find_corners(S.brick_map + S.tile_brick_base[ti.x], S.tile_res[ti.x], v3(rf.x, rf.y, rf.z), cn);
+          for (int e = 0; e < 8; ++e) xv[kc][e] = 0.0f;
+        if (k >= 0) {
+          const float4 rf = W.rec_f[k];
+          const int2 ti = W.rec_ti[k];
+          wgt[cbk] = rf.w;
+          Corners cn;
+          find_corners(S.brick_map + S.tile_brick_base[ti.x], S.tile_res[ti.x], v3(rf.x, rf.y, rf.z), cn);
 #pragma unroll
-        for (int kc = 0; kc < NFC; ++kc) {
-          float v[8][8];
+          for (int kc = 0; kc < NFC; ++kc) {
+            float v[8][8];
 #pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            if (cn.slot[k] >= 0) load8<HALF>(S.feats, (int64_t)cn.slot[k] * F + 16 * kc + 8 * h, v[k]);
-            else {
+            for (int q = 0; q < 8; ++q) {
+              if (cn.slot[q] >= 0) load8<HALF>(S.feats, (int64_t)cn.slot[q] * F + 16 * kc + 8 * h, v[q]);
+              else {
 #pragma unroll
-              for (int e = 0; e < 8; ++e) v[k][e] = 0.0f;
+                for (int e = 0; e < 8; ++e) v[q][e] = 0.0f;
+              }
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              const float qv[8] = {v[0][e], v[1][e], v[2][e], v[3][e], v[4][e], v[5][e], v[6][e], v[7][e]};
+              xv[kc][e] = tri_combine(qv, cn.wx, cn.wy, cn.wz);
             }
           }
+          const int64_t ray = ti.y;
+          const V3 d = normalize_dir(v3(R.dirs[3 * ray], R.dirs[3 * ray + 1], R.dirs[3 * ray + 2]));
+          float enc[15];
+          view_encoding(d, enc);
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
-            const float q[8] = {v[0][e], v[1][e], v[2][e], v[3][e], v[4][e], v[5][e], v[6][e], v[7][e]};
-            xv[kc][e] = tri_combine(q, cn.wx, cn.wy, cn.wz);
+            const int x = 8 * h + e;
+            xv[NFC][e] = x < 15 ? enc[x] : 0.0f;
           }
         }
-        const int64_t ray = ti.y;
-        const V3 d = normalize_dir(v3(R.dirs[3 * ray], R.dirs[3 * ray + 1], R.dirs[3 * ray + 2]));
-        float enc[15];
-        view_encoding(d, enc);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int x = 8 * h + e;
-          xv[NFC][e] = x < 15 ? enc[x] : 0.0f;
-        }
+        for (int kc = 0; kc < NKC0; ++kc) xin[cbk][kc] = M::cvt8(xv[kc]);
+        __builtin_amdgcn_sched_barrier(0);
       }
-#pragma unroll
-      for (int kc = 0; kc < NKC0; ++kc) xin[cb][kc] = M::cvt8(xv[kc]);
-    }
 
-    // ---- one MLP pass per distinct tile among the 64 records (usually one)
-    int pend0 = tl[0], pend1 = tl[1];
-    while (true) {
-      const uint64_t m = __ballot(pend0 >= 0 || pend1 >= 0);
-      if (m == 0) break;
-      const int src = __builtin_ctzll(m);
-      const int cand = __shfl(pend0 >= 0 ? pend0 : pend1, src, 64);
-      const int t = __builtin_amdgcn_readfirstlane(cand);
-      const uint8_t* __restrict__ P = wts + (int64_t)t * TB;
-
-      f32x16 acc[NOB][2];
-      vec xb[2][NKH];
-      // layer 0
+      // ---- MLP, A fragments from LDS
+      vec xa[2][NKH], xb[2][NKH];
       {
-        const uint8_t* Lp = P + layer_offset(IN, WD, L, 0);
+        const uint8_t* Lp = lds_w + layer_offset(IN, WD, L, 0);
         const float* bias = reinterpret_cast<const float*>(Lp + NOB * NKC0 * 1024);
 #pragma unroll
         for (int ob = 0; ob < NOB; ++ob) {
-          f32x16 b0;
-          const float4* bp = reinterpret_cast<const float4*>(bias + (ob * 2 + h) * 16);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const float4 t4 = bp[q];
-            b0[4 * q] = t4.x; b0[4 * q + 1] = t4.y; b0[4 * q + 2] = t4.z; b0[4 * q + 3] = t4.w;
-          }
-          acc[ob][0] = b0;
-          acc[ob][1] = b0;
-#pragma unroll
-          for (int kc = 0; kc < NKC0; ++kc) {
-            const vec a = load_frag<vec>(Lp + (ob * NKC0 + kc) * 1024 + lane * 16);
-            acc[ob][0] = M::mma(a, xin[0][kc], acc[ob][0]);
-            acc[ob][1] = M::mma(a, xin[1][kc], acc[ob][1]);
-          }
+          f32x16 acc0 = bias_block(bias, ob, h), acc1 = acc0;
+          mma_block<M, NKC0>(Lp + ob * NKC0 * 1024, lane, xin[0], xin[1], acc0, acc1);
+          relu_cvt<M>(acc0, xa[0][2 * ob], xa[0][2 * ob + 1]);
+          relu_cvt<M>(acc1, xa[1][2 * ob], xa[1][2 * ob + 1]);
         }
       }
-      // hidden layers
 #pragma unroll
-      for (int l = 1; l < L; ++l) {
-#pragma unroll
-        for (int ob = 0; ob < NOB; ++ob)
-#pragma unroll
-          for (int cb = 0; cb < 2; ++cb) {
-            float v[16];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) v[i] = acc[ob][cb][i] > 0.0f ? acc[ob][cb][i] : 0.0f;
-            xb[cb][2 * ob] = M::cvt8(v);
-            xb[cb][2 * ob + 1] = M::cvt8(v + 8);
-          }
-        const uint8_t* Lp = P + layer_offset(IN, WD, L, l);
-        const int nob = (l == L - 1) ? 1 : NOB;
-        const float* bias = reinterpret_cast<const float*>(Lp + nob * NKH * 1024);
+      for (int l = 1; l < L - 1; ++l) {
+        auto& src = (l & 1) ? xa : xb;
+        auto& dst = (l & 1) ? xb : xa;
+        const uint8_t* Lp = lds_w + layer_offset(IN, WD, L, l);
+        const float* bias = reinterpret_cast<const float*>(Lp + NOB * NKH * 1024);
 #pragma unroll
         for (int ob = 0; ob < NOB; ++ob) {
-          if (ob >= nob) break;
-          f32x16 b0;
-          const float4* bp = reinterpret_cast<const float4*>(bias + (ob * 2 + h) * 16);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const float4 t4 = bp[q];
-            b0[4 * q] = t4.x; b0[4 * q + 1] = t4.y; b0[4 * q + 2] = t4.z; b0[4 * q + 3] = t4.w;
-          }
-          acc[ob][0] = b0;
-          acc[ob][1] = b0;
-#pragma unroll
-          for (int kc = 0; kc < NKH; ++kc) {
-            const vec a = load_frag<vec>(Lp + (ob * NKH + kc) * 1024 + lane * 16);
-            acc[ob][0] = M::mma(a, xb[0][kc], acc[ob][0]);
-            acc[ob][1] = M::mma(a, xb[1][kc], acc[ob][1]);
-          }
+          f32x16 acc0 = bias_block(bias, ob, h), acc1 = acc0;
+          mma_block<M, NKH>(Lp + ob * NKH * 1024, lane, src[0], src[1], acc0, acc1);
+          relu_cvt<M>(acc0, dst[0][2 * ob], dst[0][2 * ob + 1]);
+          relu_cvt<M>(acc1, dst[1][2 * ob], dst[1][2 * ob + 1]);
         }
       }
-      // output rows 0..2 sit in acc[0][cb][0..2] of the lanes with h == 0
-#pragma unroll
-      for (int cb = 0; cb < 2; ++cb) {
-        const int mine = (cb == 0 ? pend0 : pend1) == t;
-        if (h == 0 && mine) {
-          const int64_t r = c * 64 + cb * 32 + s;
-          const float w = wgt[cb];
-          W.rgbw[r] = make_float4(w * sigmoid_cw(acc[0][cb][0]), w * sigmoid_cw(acc[0][cb][1]),
-                                  w * sigmoid_cw(acc[0][cb][2]), 0.0f);
+      {
+        auto& src = ((L - 2) & 1) ? xb : xa;  // output of layer L-2
+        const uint8_t* Lp = lds_w + layer_offset(IN, WD, L, L - 1);
+        const float* bias = reinterpret_cast<const float*>(Lp + NKH * 1024);
+        f32x16 acc0 = bias_block(bias, 0, h), acc1 = acc0;
+        mma_block<M, NKH>(Lp, lane, src[0], src[1], acc0, acc1);
+        // rows 0..2 of the output block sit in elements 0..2 of the lanes with h == 0
+        if (h == 0) {
+          if (rec[0] >= 0)
+            W.rgbw[rec[0]] = make_float4(wgt[0] * sigmoid_cw(acc0[0]), wgt[0] * sigmoid_cw(acc0[1]),
+                                         wgt[0] * sigmoid_cw(acc0[2]), 0.0f);
+          if (rec[1] >= 0)
+            W.rgbw[rec[1]] = make_float4(wgt[1] * sigmoid_cw(acc1[0]), wgt[1] * sigmoid_cw(acc1[1]),
+                                         wgt[1] * sigmoid_cw(acc1[2]), 0.0f);
         }
       }
-      if (pend0 == t) pend0 = -1;
-      if (pend1 == t) pend1 = -1;
     }
+    cb0 = seg_end;
   }
 }
 
@@ -534,7 +650,7 @@ int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, in
     return -1;
   }
   const int64_t cap = sample_capacity ? sample_capacity : default_capacity(n);
-  return layout(n, cap, nullptr, nullptr);
+  return layout(n, cap, scene->n_tiles, nullptr, nullptr);
 }
 
 int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
@@ -550,12 +666,14 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
   const int64_t n = rays->n;
   if (n == 0) return SNISR_OK;
   // capacity: everything after the fixed part of the layout
-  const int64_t fixed = layout(n, 0, nullptr, nullptr);
+  SNISR_REQUIRE(scene->n_tiles <= kMaxTiles, SNISR_ERR_SHAPE, "render_mfma: at most %d tiles", kMaxTiles);
+  const int64_t fixed = layout(n, 0, scene->n_tiles, nullptr, nullptr);
   SNISR_REQUIRE(workspace_bytes >= fixed, SNISR_ERR_CAPACITY, "render_mfma: workspace too small");
-  const int64_t cap = (workspace_bytes - fixed - 3 * 256) / 40;
+  const int64_t cap = (workspace_bytes - fixed - 4 * 256) / 44;
   Workspace W;
-  layout(n, cap, (uint8_t*)workspace, &W);
-  SNISR_REQUIRE(layout(n, cap, nullptr, nullptr) <= workspace_bytes, SNISR_ERR_CAPACITY, "render_mfma: layout");
+  layout(n, cap, scene->n_tiles, (uint8_t*)workspace, &W);
+  SNISR_REQUIRE(cap > 0 && layout(n, cap, scene->n_tiles, nullptr, nullptr) <= workspace_bytes, SNISR_ERR_CAPACITY,
+                "render_mfma: workspace too small");
 
   const int F = scene->feat_dim, WD = scene->mlp_width, L = scene->mlp_layers;
   const bool half = scene->feat_dtype == SNISR_F16;
@@ -565,25 +683,40 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
   const dim3 mgrid((unsigned)((n + 63) / 64));
   const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
 
+  SNISR_HIP(stage_event(out, 0, st));
   hipLaunchKernelGGL(k_march_pass<false>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw);
   SNISR_LAUNCHED();
+  SNISR_HIP(stage_event(out, 1, st));
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.counts, n, W.offsets,
                      W.block_sums);
   SNISR_LAUNCHED();
   hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, st, W.block_sums, nblocks, W.offsets, n, W.total,
-                     W.overflow);
+                     W.overflow, W.tile_count, W.n_tiles);
   SNISR_LAUNCHED();
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.offsets, n, W.block_sums);
   SNISR_LAUNCHED();
+  SNISR_HIP(stage_event(out, 2, st));
   hipLaunchKernelGGL(k_march_pass<true>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw);
   SNISR_LAUNCHED();
+  SNISR_HIP(stage_event(out, 3, st));
 
-  const dim3 grid((unsigned)(device_cus() * 8)), block(256);
+  hipLaunchKernelGGL(k_tile_hist, dim3(kSortGrid), dim3(kSortBlock), 0, st, W);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kScanBlock), 0, st, W);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_tile_scatter, dim3(kSortGrid), dim3(kSortBlock), 0, st, W);
+  SNISR_LAUNCHED();
+  SNISR_HIP(stage_event(out, 4, st));
+
+  const dim3 grid((unsigned)device_cus()), block(512);
   bool launched = false;
-#define SNISR_MFMA_CASE(F_, W_, L_, H_, D_)                                                  \
-  if (!launched && F == F_ && WD == W_ && L == L_ && half == H_ && dt == D_) {                \
-    hipLaunchKernelGGL((k_mlp_mfma<F_, W_, L_, H_, D_>), grid, block, 0, st, *scene, *rays, W); \
-    launched = true;                                                                         \
+#define SNISR_MFMA_CASE(F_, W_, L_, H_, D_)                                                        \
+  if (!launched && F == F_ && WD == W_ && L == L_ && half == H_ && dt == D_) {                      \
+    auto kern = k_mlp_mfma<F_, W_, L_, H_, D_>;                                                     \
+    const size_t lds = (size_t)tile_bytes(F_ + 15, W_, L_);                                         \
+    SNISR_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+    hipLaunchKernelGGL(kern, grid, block, lds, st, *scene, *rays, W);                               \
+    launched = true;                                                                                \
   }
   SNISR_MFMA_CASE(32, 128, 4, true, SNISR_F16)
   SNISR_MFMA_CASE(32, 128, 4, true, SNISR_BF16)
@@ -597,8 +730,10 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
     return SNISR_ERR_SHAPE;
   }
   SNISR_LAUNCHED();
+  SNISR_HIP(stage_event(out, 5, st));
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, W, n, out->rgb);
   SNISR_LAUNCHED();
+  SNISR_HIP(stage_event(out, 6, st));
   return SNISR_OK;
 }
 

@@ -41,6 +41,12 @@ void set_error(const char* fmt, ...);
 
 inline bool aligned(const void* p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
 
+// Records stage event k of `out` (if the caller supplied that many) on `stream`.
+inline hipError_t stage_event(const snisr_render_out* out, int k, hipStream_t stream) {
+  if (!out || !out->stage_events || k >= out->n_stage_events || !out->stage_events[k]) return hipSuccess;
+  return hipEventRecord((hipEvent_t)out->stage_events[k], stream);
+}
+
 // Checks every field of a scene descriptor that the kernels rely on.
 int check_scene(const snisr_scene* s, bool need_feats);
 int check_params(const snisr_march_params* p);

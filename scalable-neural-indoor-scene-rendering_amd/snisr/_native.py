@@ -37,7 +37,8 @@ class RaysDesc(C.Structure):
 
 class RenderOut(C.Structure):
     _fields_ = [("rgb", C.c_void_p), ("T", C.c_void_p), ("n_samples", C.c_void_p), ("depth", C.c_void_p),
-                ("bricks", C.c_void_p), ("brick_cap", C.c_int32)]
+                ("bricks", C.c_void_p), ("brick_cap", C.c_int32), ("n_stage_events", C.c_int32),
+                ("stage_events", C.c_void_p)]
 
 
 _lib = None
@@ -56,6 +57,11 @@ SIGNATURES = [
     ("snisr_render_mfma", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
                                     C.POINTER(RenderOut), C.c_void_p, C.c_int64, C.c_void_p]),
     ("snisr_render_mfma_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_void_p]),
+    ("snisr_bvh_build_bounds", C.c_int, [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    ("snisr_bvh_build", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                  C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("snisr_mesh_first_hit", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int32,
+                                       C.POINTER(RaysDesc), C.c_void_p, C.c_void_p, C.c_void_p]),
     ("snisr_pack_mlp_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int32]),
     ("snisr_pack_mlp", C.c_int, [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p]),
 ]

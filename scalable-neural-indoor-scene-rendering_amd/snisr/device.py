@@ -149,6 +149,16 @@ def _rays_desc(origins, dirs, t_hit=None, visited=None, image_width=0):
     return r
 
 
+def _attach_events(o: N.RenderOut, events):
+    """Point the output descriptor at torch.cuda.Event handles (events must have been recorded once)."""
+    if not events:
+        return None
+    arr = (C.c_void_p * len(events))(*[C.c_void_p(e.cuda_event) for e in events])
+    o.n_stage_events = len(events)
+    o.stage_events = C.cast(arr, C.c_void_p)
+    return arr
+
+
 def _render_outputs(n, device, rgb=True, brick_cap=0):
     out = dict(
         rgb=torch.empty((n, 3), dtype=torch.float32, device=device) if rgb else None,
@@ -170,13 +180,14 @@ def _render_outputs(n, device, rgb=True, brick_cap=0):
 
 
 def render_fp32(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_width=0, rgb=True, brick_cap=0,
-                **march_kw):
+                events=None, **march_kw):
     """One-thread-per-ray FP32 render (snisr_render_fp32)."""
     r = _rays_desc(origins, dirs, t_hit, visited, image_width)
     p = ds.march_params(**march_kw)
     if visited is not None and visited.shape[-1] != p.max_tiles:
         raise ValueError("visited must be [N, max_tiles]")
     out, o = _render_outputs(r.n, dirs.device, rgb, brick_cap)
+    keep = _attach_events(o, events)  # noqa: F841
     N.check(N.lib().snisr_render_fp32(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), _stream()),
             "render_fp32")
     return out
@@ -206,7 +217,8 @@ def mfma_stats(buf) -> tuple[int, int]:
 
 
 def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_width=0, brick_cap=0,
-                workspace: MfmaWorkspace | None = None, capacity: int = 0, check: bool = True, **march_kw):
+                workspace: MfmaWorkspace | None = None, capacity: int = 0, check: bool = True, events=None,
+                **march_kw):
     """Render with the per-tile MLP on MFMA (snisr_render_mfma).  With ``check`` the record
     capacity is verified after the call and the render repeated with room for every sample."""
     if ds.mlp_packed is None:
@@ -214,6 +226,7 @@ def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_
     r = _rays_desc(origins, dirs, t_hit, visited, image_width)
     p = ds.march_params(**march_kw)
     out, o = _render_outputs(r.n, dirs.device, True, brick_cap)
+    keep = _attach_events(o, events)  # noqa: F841
     ws = workspace or MfmaWorkspace()
     buf, need = ws.get(ds, r.n, capacity)
     N.check(N.lib().snisr_render_mfma(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), _ptr(buf), need,
@@ -248,3 +261,31 @@ def trace_tiles(ds: DeviceScene, origin: torch.Tensor, rays_d: torch.Tensor, K: 
     N.check(N.lib().snisr_trace_tiles(C.byref(ds.desc), _ptr(origin), _ptr(rays_d), n, K, _ptr(out), _stream()),
             "trace_tiles")
     return out
+
+
+class DeviceMesh:
+    """Proxy mesh as a BVH in HBM (built on the host by snisr_bvh_build)."""
+
+    def __init__(self, vertices: np.ndarray, faces: np.ndarray, device="cuda"):
+        L = N.lib()
+        V = np.ascontiguousarray(vertices, np.float32)
+        F = np.ascontiguousarray(faces, np.int32)
+        nb, tb = C.c_int64(0), C.c_int64(0)
+        N.check(L.snisr_bvh_build_bounds(F.shape[0], C.byref(nb), C.byref(tb)), "bvh_build_bounds")
+        nodes = np.empty(nb.value // 4, np.float32)
+        tris = np.empty(max(tb.value, 48) // 4, np.float32)
+        nn, depth = C.c_int32(0), C.c_int32(0)
+        N.check(L.snisr_bvh_build(V.ctypes.data_as(C.c_void_p), V.shape[0], F.ctypes.data_as(C.c_void_p), F.shape[0],
+                                  nodes.ctypes.data_as(C.c_void_p), nb.value, tris.ctypes.data_as(C.c_void_p),
+                                  tris.nbytes, C.byref(nn), C.byref(depth)), "bvh_build")
+        self.n_nodes, self.depth, self.n_tris = nn.value, depth.value, F.shape[0]
+        self.nodes = torch.from_numpy(nodes[: 16 * self.n_nodes].copy()).to(device)
+        self.tris = torch.from_numpy(tris).to(device)
+
+    def first_hit(self, origins, dirs, image_width=0, t_out=None, id_out=None):
+        r = _rays_desc(origins, dirs, image_width=image_width)
+        t = t_out if t_out is not None else torch.empty(r.n, dtype=torch.float32, device=dirs.device)
+        ids = id_out if id_out is not None else torch.empty(r.n, dtype=torch.int32, device=dirs.device)
+        N.check(N.lib().snisr_mesh_first_hit(_ptr(self.nodes), self.n_nodes, _ptr(self.tris), self.n_tris, self.depth,
+                                             C.byref(r), _ptr(t), _ptr(ids), _stream()), "mesh_first_hit")
+        return t, ids

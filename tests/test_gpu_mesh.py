@@ -1,0 +1,63 @@
+"""GPU parity of the BVH nearest hit with the oracle's brute force over all triangles:
+triangle ids and t bit-exact."""
+import numpy as np
+import pytest
+import torch
+
+from snisr import device as D
+from snisr import scene as S
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def check(oracle_mod, mesh, origins, dirs):
+    dm = D.DeviceMesh(mesh.vertices, mesh.faces)
+    t, ids = dm.first_hit(to_dev(origins), to_dev(dirs))
+    dn = oracle_mod.normalize(dirs)
+    t_ref, id_ref = oracle_mod.mesh_hit(mesh.vertices, mesh.faces, origins, dn)
+    ids = ids.cpu().numpy()
+    t = t.cpu().numpy()
+    assert np.array_equal(ids, id_ref), np.flatnonzero(ids != id_ref)[:10]
+    assert np.array_equal(t.view(np.uint32), t_ref.view(np.uint32))
+    return ids
+
+
+def test_mesh_hit_c2_mesh_camera_rays(oracle_mod):
+    sc, cam, mesh = S.config_c2()
+    dirs = S.camera_rays_np(cam)
+    idx = np.random.default_rng(0).choice(dirs.shape[0], 1500, replace=False)
+    ids = check(oracle_mod, mesh, cam.origin, dirs[idx])
+    assert np.mean(ids >= 0) > 0.9
+
+
+def test_mesh_hit_random_rays_and_edge_cases(oracle_mod):
+    mesh = S.room_mesh(7, (0, 0, 0), (2, 2, 1), 24000)
+    rng = np.random.default_rng(1)
+    n = 6000
+    origins = rng.uniform([-0.5, -0.5, -0.5], [2.5, 2.5, 1.5], (n, 3)).astype(np.float32)
+    dirs = rng.standard_normal((n, 3)).astype(np.float32)
+    dirs[:300, 1:] = 0.0          # axis-parallel
+    dirs[300:600, 2] = 0.0
+    dirs[600:700, 0] = -0.0
+    origins[700:800] = np.array([5.0, 5.0, 5.0], np.float32)  # outside everything, pointing away
+    dirs[700:800] = np.abs(dirs[700:800])
+    check(oracle_mod, mesh, origins, dirs)
+
+
+def test_mesh_hit_tiny_and_empty_meshes(oracle_mod):
+    one = S.box_mesh(np.array([[0.2, 0.2, 0.2]]), np.array([[0.4, 0.5, 0.6]]))
+    rng = np.random.default_rng(2)
+    origins = rng.uniform(-1, 1.5, (500, 3)).astype(np.float32)
+    dirs = (np.array([0.3, 0.35, 0.4]) - origins + rng.normal(0, 0.1, (500, 3))).astype(np.float32)
+    check(oracle_mod, one, origins, dirs)
+    tri = S.Mesh(vertices=one.vertices[one.faces[0]].copy(), faces=np.array([[0, 1, 2]], np.int32))
+    check(oracle_mod, tri, origins, dirs)
+    empty = S.Mesh(vertices=np.zeros((3, 3), np.float32), faces=np.zeros((0, 3), np.int32))
+    dm = D.DeviceMesh(empty.vertices, empty.faces)
+    t, ids = dm.first_hit(to_dev(origins), to_dev(dirs))
+    assert torch.all(ids == -1) and torch.all(torch.isinf(t))
