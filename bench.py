@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and PSNR legs")
     ap.add_argument("--cpu-frac", type=float, default=0.05)
     ap.add_argument("--fp32-frames", type=int, default=3)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the render has no cross-GPU exchange in this build "
@@ -78,6 +80,14 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_probe:
+        a = torch.ones(args.copy_probe // 4, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "dispatches": 3}))
+        return
     sc, cam, mesh = S.config_c2()
     ds = D.DeviceScene(sc, mfma_dtype=N.SNISR_F16)
     dm = D.DeviceMesh(mesh.vertices, mesh.faces)

@@ -127,6 +127,7 @@ struct Workspace {
   int32_t* tile_count;  // [T]
   int64_t* tile_start;  // [T+1] 64-aligned bucket starts
   int64_t* tile_cursor; // [T]
+  uint16_t* enc;        // [n][16] per-ray view encoding in the MFMA operand type (15 values + 0)
   int64_t cap;
   int32_t n_tiles;
 };
@@ -145,6 +146,7 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
   const int64_t o_tc = off; off = align_up(off + 4 * (int64_t)T, 256);
   const int64_t o_ts = off; off = align_up(off + 8 * ((int64_t)T + 1), 256);
   const int64_t o_cu = off; off = align_up(off + 8 * (int64_t)T, 256);
+  const int64_t o_enc = off; off = align_up(off + 32 * n, 256);
   const int64_t o_pm = off; off = align_up(off + 4 * (cap + 64 * (int64_t)T), 256);
   const int64_t o_rf = off; off = align_up(off + 16 * cap, 256);
   const int64_t o_ti = off; off = align_up(off + 8 * cap, 256);
@@ -158,6 +160,7 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
     w->tile_count = reinterpret_cast<int32_t*>(base + o_tc);
     w->tile_start = reinterpret_cast<int64_t*>(base + o_ts);
     w->tile_cursor = reinterpret_cast<int64_t*>(base + o_cu);
+    w->enc = reinterpret_cast<uint16_t*>(base + o_enc);
     w->perm = reinterpret_cast<int32_t*>(base + o_pm);
     w->rec_f = reinterpret_cast<float4*>(base + o_rf);
     w->rec_ti = reinterpret_cast<int2*>(base + o_ti);
@@ -173,7 +176,7 @@ static int64_t default_capacity(int64_t n) { return 48 * n; }
 // ------------------------------------------------------------------------- march passes
 template <bool EMIT>
 __global__ __launch_bounds__(64) void k_march_pass(snisr_scene S, snisr_march_params P, snisr_rays R,
-                                                   snisr_render_out O, Workspace W, int packet_w) {
+                                                   snisr_render_out O, Workspace W, int packet_w, int enc_dtype) {
   const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
   if (tid >= R.n) return;
   const int64_t i = ray_of_thread(tid, R.n, packet_w);
@@ -205,6 +208,15 @@ __global__ __launch_bounds__(64) void k_march_pass(snisr_scene S, snisr_march_pa
     if (bricks)
       for (int k = res.n_bricks; k < O.brick_cap; ++k) bricks[k] = -1;
     W.counts[i] = res.n_samples;
+    float enc[15];
+    view_encoding(d, enc);
+    uint16_t e16[16];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) e16[k] = to_bits(enc[k], enc_dtype);
+    e16[15] = 0;
+    uint4* dst = reinterpret_cast<uint4*>(W.enc + i * 16);
+    dst[0] = *reinterpret_cast<const uint4*>(e16);
+    dst[1] = *reinterpret_cast<const uint4*>(e16 + 8);
     if (O.T) O.T[i] = res.T;
     if (O.n_samples) O.n_samples[i] = res.n_samples;
     if (O.depth) O.depth[i] = res.depth;
@@ -434,6 +446,63 @@ __device__ __forceinline__ void mma_block(const uint8_t* Ab, int lane, const typ
   }
 }
 
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+// Eight trilinear corner weights (corner order x0y0z0, x1y0z0, ... as in tri_combine) as half2
+// broadcasts, 0 for empty corners; the slots of empty corners are redirected to slot 0 so every
+// load stays in bounds (the scene always holds at least one brick).
+__device__ __forceinline__ void corner_weights_h2(Corners& c, h2 (&w2)[8]) {
+  const float wx[2] = {1.0f - c.wx, c.wx}, wy[2] = {1.0f - c.wy, c.wy}, wz[2] = {1.0f - c.wz, c.wz};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const float w = c.slot[q] >= 0 ? wx[q & 1] * wy[(q >> 1) & 1] * wz[q >> 2] : 0.0f;
+    const _Float16 hw = (_Float16)w;
+    w2[q] = h2{hw, hw};
+    c.slot[q] = c.slot[q] >= 0 ? c.slot[q] : 0;
+  }
+}
+
+// 8 consecutive FP16 features interpolated as sum_q w_q v_q with packed half FMAs
+template <class M>
+__device__ __forceinline__ typename M::vec interp8_h2(const _Float16* __restrict__ feats, const Corners& c,
+                                                      const h2 (&w2)[8], int F, int off) {
+  h2 acc[4];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const uint4 raw = *reinterpret_cast<const uint4*>(feats + (int64_t)c.slot[q] * F + off);
+    const h2* v = reinterpret_cast<const h2*>(&raw);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = q == 0 ? w2[0] * v[j] : __builtin_elementwise_fma(w2[q], v[j], acc[j]);
+  }
+  float f[8];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    f[2 * j] = (float)acc[j][0];
+    f[2 * j + 1] = (float)acc[j][1];
+  }
+  return M::cvt8(f);
+}
+
+template <>
+__device__ __forceinline__ f16x8 interp8_h2<Mfma<SNISR_F16>>(const _Float16* __restrict__ feats, const Corners& c,
+                                                             const h2 (&w2)[8], int F, int off) {
+  h2 acc[4];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const uint4 raw = *reinterpret_cast<const uint4*>(feats + (int64_t)c.slot[q] * F + off);
+    const h2* v = reinterpret_cast<const h2*>(&raw);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = q == 0 ? w2[0] * v[j] : __builtin_elementwise_fma(w2[q], v[j], acc[j]);
+  }
+  f16x8 r;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    r[2 * j] = acc[j][0];
+    r[2 * j + 1] = acc[j][1];
+  }
+  return r;
+}
+
 __device__ __forceinline__ f32x16 bias_block(const float* bias, int ob, int h) {
   f32x16 acc;
   const float4* bp = reinterpret_cast<const float4*>(bias + (ob * 2 + h) * 16);
@@ -452,6 +521,17 @@ __device__ __forceinline__ void relu_cvt(const f32x16& acc, typename M::vec& lo,
   for (int i = 0; i < 16; ++i) v[i] = acc[i] > 0.0f ? acc[i] : 0.0f;
   lo = M::cvt8(v);
   hi = M::cvt8(v + 8);
+}
+
+// FP16 operands: convert first, then ReLU on packed halves (max(x, 0) commutes with rounding)
+template <>
+__device__ __forceinline__ void relu_cvt<Mfma<SNISR_F16>>(const f32x16& acc, f16x8& lo, f16x8& hi) {
+  float v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = acc[i];
+  const f16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+  lo = __builtin_elementwise_max(Mfma<SNISR_F16>::cvt8(v), z);
+  hi = __builtin_elementwise_max(Mfma<SNISR_F16>::cvt8(v + 8), z);
 }
 
 // ------------------------------------------------------------------------- MFMA MLP
@@ -522,8 +602,15 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, W
           wgt[cbk] = rf.w;
           Corners cn;
           find_corners(S.brick_map + S.tile_brick_base[ti.x], S.tile_res[ti.x], v3(rf.x, rf.y, rf.z), cn);
+          if constexpr (HALF) {
+            h2 w2[8];
+            corner_weights_h2(cn, w2);
 #pragma unroll
-          for (int kc = 0; kc < NFC; ++kc) {
+            for (int kc = 0; kc < NFC; ++kc)
+              xin[cbk][kc] = interp8_h2<M>(reinterpret_cast<const _Float16*>(S.feats), cn, w2, F, 16 * kc + 8 * h);
+          }
+#pragma unroll
+          for (int kc = 0; kc < (HALF ? 0 : NFC); ++kc) {
             float v[8][8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
@@ -539,18 +626,19 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, W
               xv[kc][e] = tri_combine(qv, cn.wx, cn.wy, cn.wz);
             }
           }
-          const int64_t ray = ti.y;
-          const V3 d = normalize_dir(v3(R.dirs[3 * ray], R.dirs[3 * ray + 1], R.dirs[3 * ray + 2]));
-          float enc[15];
-          view_encoding(d, enc);
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int x = 8 * h + e;
-            xv[NFC][e] = x < 15 ? enc[x] : 0.0f;
-          }
         }
+        if (!HALF || k < 0) {
 #pragma unroll
-        for (int kc = 0; kc < NKC0; ++kc) xin[cbk][kc] = M::cvt8(xv[kc]);
+          for (int kc = 0; kc < NFC; ++kc) xin[cbk][kc] = M::cvt8(xv[kc]);
+        }
+        // view encoding: precomputed per ray in the operand type by the count pass
+        if (k >= 0) {
+          const int64_t ray = W.rec_ti[k].y;
+          xin[cbk][NFC] = load_frag<vec>(reinterpret_cast<const uint8_t*>(W.enc + ray * 16 + 8 * h));
+        } else {
+          float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          xin[cbk][NFC] = M::cvt8(z);
+        }
         __builtin_amdgcn_sched_barrier(0);
       }
 
@@ -684,7 +772,7 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
   const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
 
   SNISR_HIP(stage_event(out, 0, st));
-  hipLaunchKernelGGL(k_march_pass<false>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw);
+  hipLaunchKernelGGL(k_march_pass<false>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 1, st));
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.counts, n, W.offsets,
@@ -696,7 +784,7 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.offsets, n, W.block_sums);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 2, st));
-  hipLaunchKernelGGL(k_march_pass<true>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw);
+  hipLaunchKernelGGL(k_march_pass<true>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 3, st));
 
