@@ -20,6 +20,9 @@ sys.path.insert(0, os.path.join(REPO, "scalable-neural-indoor-scene-rendering_am
 METRIC = "Mrays/s render (1280x720) & train fwd+bwd at PSNR>=50dB vs FP64, 1 MI355X"
 MFMA_PEAK_F16_TFLOPS = 2500.0   # MI355X dense FP16 MFMA
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_mlp_mfma launch on this workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# (factors from a 256 MiB device copy), see profiles/c2_render_kernels.md
+MLP_HBM_BYTES_PER_LAUNCH = 262.8e6 + 126.8e6
 
 
 def mlp_flops_per_sample(mlp_in, width, layers):
@@ -136,7 +139,8 @@ def main():
     flops = total_samples * flops_per_sample
     achieved = flops / (mlp_ms * 1e-3) / 1e12
     roofline = dict(bound="mfma", achieved=round(achieved, 2), peak=MFMA_PEAK_F16_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / MFMA_PEAK_F16_TFLOPS, 4), traffic=None,
+                    frac=round(achieved / MFMA_PEAK_F16_TFLOPS, 4), traffic=MLP_HBM_BYTES_PER_LAUNCH,
+                    traffic_unit="bytes/launch",
                     kernel="k_mlp_mfma<32,128,4,f16 feats,f16 MFMA>",
                     flop_per_launch=flops, samples_per_launch=total_samples)
 
