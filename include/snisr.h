@@ -173,6 +173,24 @@ int snisr_bvh_build(const float* vertices, int64_t n_vertices, const int32_t* fa
 int snisr_mesh_first_hit(const void* nodes, int32_t n_nodes, const void* tris, int64_t n_tris, int32_t depth,
                          const snisr_rays* rays, float* t_out, int32_t* id_out, void* stream);
 
+/* A11 — training (FP32 features and MLP).  snisr_train_forward renders like snisr_render_fp32
+ * (identical traversal and samples) and keeps per-sample records in the workspace;
+ * snisr_train_backward then computes the gradients of L = sum_i <grad_rgb_i, rgb_i> with respect to
+ * sigma f32[NB,512], the features f32[NB,512,F] and the MLP parameters f32[T,P] (mlp_params layout),
+ * overwriting the three gradient buffers.  The MLP forward is recomputed in the backward rather than
+ * stored.  Replaces the forward/autograd backward of Tile.train_batch
+ * (training/scripts/tile/tile.py:631-820): compositing of utils.volume_rendering
+ * (training/scripts/utils.py:823-837), grid_sample trilinear gathers (tile.py:225-236) and the MLP
+ * (training/scripts/networks.py:17-41).  The workspace must not be reused between a forward and its
+ * backward.  train_forward's stage_events (4): start, after march+sort, after the MLP, after the
+ * reduction. */
+int64_t snisr_train_workspace_size(const snisr_scene* scene, int64_t n, int64_t sample_capacity);
+int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
+                        const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream);
+int snisr_train_backward(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
+                         const float* grad_rgb, float* dsigma, float* dfeats, float* dmlp, void* workspace,
+                         int64_t workspace_bytes, void* const* stage_events, int32_t n_stage_events, void* stream);
+
 /* Bytes of a packed weight buffer of the scene's MLP shape for `dtype`, and the packing itself
  * (device to device, from scene->mlp_params into `packed`): SNISR_F32 gives the mlp_f32t layout,
  * SNISR_F16 / SNISR_BF16 the MFMA fragment layout of mlp_packed. */

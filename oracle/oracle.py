@@ -5,6 +5,7 @@ file:line of every function)."""
 from __future__ import annotations
 
 import ctypes as C
+import ctypes as C_
 import os
 import subprocess
 
@@ -238,3 +239,27 @@ def assign_pixels(net, pixel_starts, block_starts):
     new_starts = np.empty(nb, np.int32)
     lib().orc_assign_pixels(_p(net), _p(ps), _p(bs), C.c_int(net.shape[0]), _p(new_net), _p(new_starts))
     return new_net, new_starts
+
+
+def train_grad(osc: OracleScene, origins, dirs, g, t_hit=None, precision="f64", **pkw):
+    """Colours C[N,3] and the gradients of L = sum_i <g_i, C_i> with respect to sigma, features
+    and the MLP parameters (the oracle's adjoint of the training render)."""
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    N = dirs.shape[0]
+    origins = np.ascontiguousarray(origins, np.float32)
+    ostride = 0 if origins.size == 3 else 3
+    dt = np.float64 if precision == "f64" else np.float32
+    sc = osc.sc
+    g = np.ascontiguousarray(g, dt).reshape(N, 3)
+    Cc = np.zeros((N, 3), dt)
+    dsig = np.zeros(sc.sigma.shape, dt)
+    dfeat = np.zeros(sc.feats.shape, dt)
+    dmlp = np.zeros(sc.mlp_params.shape, dt)
+    if t_hit is not None:
+        t_hit = np.ascontiguousarray(t_hit, np.float32)
+    p = osc.params(**pkw)
+    fn = lib().orc_train_grad_f64 if precision == "f64" else lib().orc_train_grad_f32
+    fn(C_.byref(osc.c), C_.byref(p), _p(origins),
+       C_.c_int(ostride), _p(dirs), _p(t_hit), C_.c_longlong(N), _p(g), _p(Cc),
+       _p(dsig), _p(dfeat), _p(dmlp))
+    return dict(rgb=Cc, dsigma=dsig, dfeats=dfeat, dmlp=dmlp)

@@ -24,6 +24,7 @@
 //   inverse_z / integrate / assign   closed forms and numpy restatements
 //   train adjoint       FP64 central finite differences
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -304,6 +305,16 @@ static int trace_tiles_one(const Scene& S, V3<R> origin, V3<R> d, int K, int* vi
 }
 
 // ------------------------------------------------------------------ A3/A4/A5/A7/A10 march
+template <class R>
+struct SampleRec {
+  int tile;
+  V3<R> f;       // voxel coordinate p/voxel - 0.5 in the tile
+  R sig;         // max(trilinear sigma, 0)
+  R interval;    // min(step, far - z)
+  R w;           // alpha * T
+  R T_after;     // T * (1 - alpha)
+};
+
 struct RayOut {
   float rgb[3];
   float T;
@@ -323,7 +334,8 @@ struct RayOut {
 // global t reaches the proxy-mesh hit t_hit.  sigma = max(trilinear sigma, 0); rgb = MLP(features).
 template <class R>
 static void march_ray(const Scene& S, const Params& P, V3<R> origin, V3<R> d_raw, R t_hit,
-                      const int* visited_in, RayOut& out, int* brick_dbg, bool with_mlp) {
+                      const int* visited_in, RayOut& out, int* brick_dbg, bool with_mlp,
+                      std::vector<SampleRec<R>>* recs = nullptr) {
   const V3<R> d = normalize3(d_raw);
   std::vector<int> vis(std::max(P.max_tiles, 1));
   int nvis;
@@ -399,6 +411,7 @@ static void march_ray(const Scene& S, const Params& P, V3<R> origin, V3<R> d_raw
             for (int c = 0; c < 3; ++c) color[c] = color[c] + w * rgb[c];
           }
           T = T * (R(1) - alpha);
+          if (recs) recs->push_back(SampleRec<R>{tile, f, sig, interval, w, T});
           ++nsamp;
           if (T < R(P.early_stop)) { done = true; break; }
         }
@@ -418,6 +431,122 @@ static void march_ray(const Scene& S, const Params& P, V3<R> origin, V3<R> d_raw
   out.n_samples = nsamp;
   out.depth = float(depth);
   out.n_bricks = nbr;
+}
+
+// ------------------------------------------------------------------ A11 training adjoint
+// Gradient of L = sum_rays <g_ray, C_ray> for the render of march_ray (features, density and the
+// per-tile MLP are the parameters; sample positions depend on geometry only).  Restates the
+// derivative of the compositing of utils.volume_rendering (training/scripts/utils.py:823-837,
+// here with the marcher's per-sample T) and of the MLP/trilinear chain that torch autograd
+// differentiates in Tile.train_batch (training/scripts/tile/tile.py:631-820):
+//   dL/dc_i     = w_i g
+//   dL/dsigma_i = interval_i (T_{i+1} <c_i, g> - sum_{k>i} w_k <c_k, g>)   (0 where sigma was clamped)
+//   MLP and trilinear by the chain rule; the trilinear weight of corner q is the product of the
+//   per-axis lerp weights.
+template <class R>
+static void accum(R* p, R v) {
+#pragma omp atomic
+  *p += v;
+}
+
+template <class R>
+static void train_ray(const Scene& S, const Params& P, V3<R> origin, V3<R> d_raw, R t_hit, const R* g, R* C_out,
+                      R* dsigma, R* dfeat, R* dmlp) {
+  std::vector<SampleRec<R>> recs;
+  RayOut ro;
+  march_ray<R>(S, P, origin, d_raw, t_hit, nullptr, ro, nullptr, false, &recs);
+  const V3<R> d = normalize3(d_raw);
+  R enc[15];
+  view_encoding(d, enc);
+  const int F = S.feat_dim, L = S.mlp_layers, Wd = S.mlp_width, IN = S.mlp_in;
+  const int n = (int)recs.size();
+  // forward per sample, keeping every layer's activations
+  std::vector<std::vector<std::vector<R>>> act(n);  // act[i][l] = input of layer l (l = 0..L-1)
+  std::vector<std::array<R, 3>> col(n);
+  R C[3] = {0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    std::vector<R> x(IN);
+    R sg;
+    trilinear(S, recs[i].tile, recs[i].f, &sg, x.data());
+    for (int k = 0; k < 15; ++k) x[F + k] = enc[k];
+    act[i].push_back(x);
+    const float* Pm = S.mlp_params + (long long)recs[i].tile * S.mlp_stride;
+    int in = IN;
+    for (int l = 0; l < L; ++l) {
+      const int out = (l == L - 1) ? 3 : Wd;
+      const float* Wt = Pm;
+      const float* b = Pm + (long long)out * in;
+      std::vector<R> y(out);
+      for (int j = 0; j < out; ++j) {
+        R acc = R(b[j]);
+        for (int q = 0; q < in; ++q) acc += R(Wt[(long long)j * in + q]) * act[i][l][q];
+        y[j] = (l == L - 1) ? acc : (acc > R(0) ? acc : R(0));
+      }
+      Pm = b + out;
+      in = out;
+      if (l < L - 1) act[i].push_back(y);
+      else for (int c = 0; c < 3; ++c) col[i][c] = R(1) / (R(1) + rexp(-y[c]));
+    }
+    for (int c = 0; c < 3; ++c) C[c] += recs[i].w * col[i][c];
+  }
+  for (int c = 0; c < 3; ++c) C_out[c] = C[c];
+  // reverse pass over the samples
+  R suffix = 0;
+  for (int i = n - 1; i >= 0; --i) {
+    const SampleRec<R>& r = recs[i];
+    const R cg = col[i][0] * g[0] + col[i][1] * g[1] + col[i][2] * g[2];
+    R dsig = r.interval * (r.T_after * cg - suffix);
+    suffix += r.w * cg;
+    if (!(r.sig > R(0))) dsig = 0;
+    // MLP backward
+    std::vector<R> delta(3);
+    for (int c = 0; c < 3; ++c) delta[c] = r.w * g[c] * col[i][c] * (R(1) - col[i][c]);
+    // parameter offsets of each layer
+    std::vector<long long> woff(L), boff(L);
+    std::vector<int> lin(L), lout(L);
+    long long off = 0;
+    int in = IN;
+    for (int l = 0; l < L; ++l) {
+      lin[l] = in;
+      lout[l] = (l == L - 1) ? 3 : Wd;
+      woff[l] = off;
+      boff[l] = off + (long long)lout[l] * in;
+      off = boff[l] + lout[l];
+      in = lout[l];
+    }
+    R* gm = dmlp + (long long)r.tile * S.mlp_stride;
+    const float* Pm = S.mlp_params + (long long)r.tile * S.mlp_stride;
+    std::vector<R> dx;
+    for (int l = L - 1; l >= 0; --l) {
+      const std::vector<R>& a = act[i][l];
+      for (int o = 0; o < lout[l]; ++o) {
+        if (delta[o] == R(0)) continue;
+        accum(&gm[boff[l] + o], delta[o]);
+        for (int q = 0; q < lin[l]; ++q) accum(&gm[woff[l] + (long long)o * lin[l] + q], delta[o] * a[q]);
+      }
+      std::vector<R> prev(lin[l], R(0));
+      for (int q = 0; q < lin[l]; ++q) {
+        R acc = 0;
+        for (int o = 0; o < lout[l]; ++o) acc += R(Pm[woff[l] + (long long)o * lin[l] + q]) * delta[o];
+        prev[q] = acc;
+      }
+      if (l > 0)
+        for (int q = 0; q < lin[l]; ++q) prev[q] = a[q] > R(0) ? prev[q] : R(0);
+      delta.swap(prev);
+      if (l == 0) dx = delta;
+    }
+    // trilinear adjoint: corner weights are products of the per-axis lerp weights
+    const R fx = std::floor(r.f.x), fy = std::floor(r.f.y), fz = std::floor(r.f.z);
+    const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    const R wx = r.f.x - fx, wy = r.f.y - fy, wz = r.f.z - fz;
+    for (int q = 0; q < 8; ++q) {
+      const long long slot = voxel_slot(S, r.tile, ix + (q & 1), iy + ((q >> 1) & 1), iz + (q >> 2));
+      if (slot < 0) continue;
+      const R wq = ((q & 1) ? wx : R(1) - wx) * (((q >> 1) & 1) ? wy : R(1) - wy) * ((q >> 2) ? wz : R(1) - wz);
+      if (dsig != R(0)) accum(&dsigma[slot], wq * dsig);
+      for (int e = 0; e < F; ++e) accum(&dfeat[slot * F + e], wq * dx[e]);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ A9 mesh hit
@@ -589,6 +718,35 @@ void orc_render_f64(const orc_scene_c* sc, const orc_params_c* pc, const float* 
                     const float* dirs, const float* t_hit, const int* visited, long long N, const long long* ray_index, float* rgb,
                     float* T, int* nsamp, float* depth, int* brick_dbg, int with_mlp) {
   render_impl<double>(sc, pc, origin, ostride, dirs, t_hit, visited, N, ray_index, rgb, T, nsamp, depth, brick_dbg, with_mlp);
+}
+
+}  // extern "C"
+template <class R>
+static void train_impl(const orc_scene_c* sc, const orc_params_c* pc, const float* origin, int ostride,
+                       const float* dirs, const float* t_hit, long long N, const R* g, R* C, R* dsigma, R* dfeat,
+                       R* dmlp) {
+  Scene S = to_scene(sc);
+  Params P = to_params(pc);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long i = 0; i < N; ++i) {
+    const float* og = origin + i * ostride;
+    const R th = t_hit ? R(t_hit[i]) : std::numeric_limits<R>::infinity();
+    train_ray<R>(S, P, v3<R>(og[0], og[1], og[2]), v3<R>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), th,
+                 g + 3 * i, C + 3 * i, dsigma, dfeat, dmlp);
+  }
+}
+extern "C" {
+// Training forward colours C[N,3] and the gradients of L = sum <g_i, C_i> with respect to sigma
+// [NB,512], features [NB,512,F] and the MLP parameters [T,P] (accumulated into zeroed buffers).
+void orc_train_grad_f64(const orc_scene_c* sc, const orc_params_c* pc, const float* origin, int ostride,
+                        const float* dirs, const float* t_hit, long long N, const double* g, double* C,
+                        double* dsigma, double* dfeat, double* dmlp) {
+  train_impl<double>(sc, pc, origin, ostride, dirs, t_hit, N, g, C, dsigma, dfeat, dmlp);
+}
+void orc_train_grad_f32(const orc_scene_c* sc, const orc_params_c* pc, const float* origin, int ostride,
+                        const float* dirs, const float* t_hit, long long N, const float* g, float* C, float* dsigma,
+                        float* dfeat, float* dmlp) {
+  train_impl<float>(sc, pc, origin, ostride, dirs, t_hit, N, g, C, dsigma, dfeat, dmlp);
 }
 
 // trilinear at M continuous voxel coordinates of one tile (sigma + features), for unit tests

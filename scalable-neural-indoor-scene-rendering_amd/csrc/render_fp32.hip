@@ -62,96 +62,6 @@ __global__ void k_pack_f32t(const float* __restrict__ src, int64_t src_stride, f
 }
 
 // ---------------------------------------------------------------------------- FP32 render
-// One dense layer for one lane: y = b + x Wt, x read from this lane's LDS column (stride 64),
-// Wt rows read as float4 (all lanes of a wave in one tile read the same address).
-template <int IN, int OUT>
-__device__ __forceinline__ void dense_f32(const float* __restrict__ Wt, const float* __restrict__ b,
-                                          const float* xs, float (&y)[OUT]) {
-  static_assert(OUT % 4 == 0, "padded width");
-#pragma unroll
-  for (int j = 0; j < OUT; ++j) y[j] = b[j];
-#pragma unroll 1
-  for (int i = 0; i < IN; ++i) {
-    const float xi = xs[i * 64];
-    const float4* w = reinterpret_cast<const float4*>(Wt + i * OUT);
-#pragma unroll
-    for (int j = 0; j < OUT / 4; ++j) {
-      const float4 q = w[j];
-      y[4 * j + 0] = __builtin_fmaf(xi, q.x, y[4 * j + 0]);
-      y[4 * j + 1] = __builtin_fmaf(xi, q.y, y[4 * j + 1]);
-      y[4 * j + 2] = __builtin_fmaf(xi, q.z, y[4 * j + 2]);
-      y[4 * j + 3] = __builtin_fmaf(xi, q.w, y[4 * j + 3]);
-    }
-  }
-}
-
-// Per-sample MLP on one lane: input in xs[0..IN), hidden layers ReLU, sigmoid RGB.
-template <int IN, int WD, int L>
-__device__ __forceinline__ void mlp_f32(const float* __restrict__ P, float* xs, float rgb[3]) {
-  float y[WD];
-  dense_f32<IN, WD>(P, P + IN * WD, xs, y);
-  P += IN * WD + WD;
-#pragma unroll
-  for (int j = 0; j < WD; ++j) xs[j * 64] = y[j] > 0.0f ? y[j] : 0.0f;
-#pragma unroll 1
-  for (int l = 1; l < L - 1; ++l) {
-    dense_f32<WD, WD>(P, P + WD * WD, xs, y);
-    P += WD * WD + WD;
-#pragma unroll
-    for (int j = 0; j < WD; ++j) xs[j * 64] = y[j] > 0.0f ? y[j] : 0.0f;
-  }
-  float z[4];
-  dense_f32<WD, 4>(P, P + WD * 4, xs, z);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) rgb[c] = sigmoid_cw(z[c]);
-}
-
-template <bool HALF>
-struct FeatLoad;
-template <>
-struct FeatLoad<true> {  // 8 halves per 16-byte load
-  static constexpr int kVec = 8;
-  __device__ __forceinline__ static void load(const void* base, int64_t off, float* v) {
-    const uint4 q = *reinterpret_cast<const uint4*>(reinterpret_cast<const _Float16*>(base) + off);
-    const _Float16* h = reinterpret_cast<const _Float16*>(&q);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = (float)h[k];
-  }
-};
-template <>
-struct FeatLoad<false> {  // 4 floats per 16-byte load
-  static constexpr int kVec = 4;
-  __device__ __forceinline__ static void load(const void* base, int64_t off, float* v) {
-    const float4 q = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(base) + off);
-    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-  }
-};
-
-// Trilinear features of one sample into this lane's LDS column xs[0..F).
-template <int F, bool HALF>
-__device__ __forceinline__ void gather_features(const void* __restrict__ feats, const Corners& c, float* xs) {
-  using LD = FeatLoad<HALF>;
-  constexpr int V = LD::kVec;
-  static_assert(F % V == 0, "feature dim must fill whole 16-byte vectors");
-#pragma unroll
-  for (int ch = 0; ch < F / V; ++ch) {
-    float v[8][V];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (c.slot[k] >= 0) LD::load(feats, (int64_t)c.slot[k] * F + ch * V, v[k]);
-      else {
-#pragma unroll
-        for (int e = 0; e < V; ++e) v[k][e] = 0.0f;
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-      const float q[8] = {v[0][e], v[1][e], v[2][e], v[3][e], v[4][e], v[5][e], v[6][e], v[7][e]};
-      xs[(ch * V + e) * 64] = tri_combine(q, c.wx, c.wy, c.wz);
-    }
-  }
-}
-
 template <int F, int WD, int L, bool HALF>
 __global__ __launch_bounds__(64) void k_render_fp32(snisr_scene S, snisr_march_params P, snisr_rays R,
                                                     snisr_render_out O, int packet_w) {
@@ -173,7 +83,7 @@ __global__ __launch_bounds__(64) void k_render_fp32(snisr_scene S, snisr_march_p
   int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
   MarchResult res;
   march_ray(S, P, origin, d, t_hit, R.visited ? R.visited + i * P.max_tiles : nullptr, bricks, O.brick_cap,
-            [&](int tile, const V3&, const Corners& c, float w) {
+            [&](int tile, const V3&, const Corners& c, float w, float, float, float) {
               gather_features<F, HALF>(S.feats, c, xs);
 #pragma unroll
               for (int k = 0; k < 15; ++k) xs[(F + k) * 64] = enc[k];
@@ -210,7 +120,7 @@ __global__ __launch_bounds__(64) void k_march_sigma(snisr_scene S, snisr_march_p
   int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
   MarchResult res;
   march_ray(S, P, origin, d, t_hit, R.visited ? R.visited + i * P.max_tiles : nullptr, bricks, O.brick_cap,
-            [&](int, const V3&, const Corners&, float) {}, res);
+            [&](int, const V3&, const Corners&, float, float, float, float) {}, res);
   if (bricks)
     for (int k = res.n_bricks; k < O.brick_cap; ++k) bricks[k] = -1;
   if (O.T) O.T[i] = res.T;

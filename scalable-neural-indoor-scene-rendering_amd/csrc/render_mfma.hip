@@ -128,6 +128,10 @@ struct Workspace {
   int64_t* tile_start;  // [T+1] 64-aligned bucket starts
   int64_t* tile_cursor; // [T]
   uint16_t* enc;        // [n][16] per-ray view encoding in the MFMA operand type (15 values + 0)
+  float4* rec_b;        // [cap] training: interval, T after the sample, clamped sigma, 0
+  float4* crec;         // [cap] training: unweighted colour of every record
+  float* dsig;          // [cap] training: dL/dsigma of every record
+  float* act;           // training: activations/deltas per sorted record (train_act_floats each)
   int64_t cap;
   int32_t n_tiles;
 };
@@ -137,7 +141,7 @@ constexpr int kMaxTiles = 4096;  // tile histogram in LDS
 
 static inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
-static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspace* w) {
+static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspace* w, int64_t train_floats = 0) {
   const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
   int64_t off = 256;
   const int64_t o_counts = off; off = align_up(off + 4 * n, 256);
@@ -151,6 +155,13 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
   const int64_t o_rf = off; off = align_up(off + 16 * cap, 256);
   const int64_t o_ti = off; off = align_up(off + 8 * cap, 256);
   const int64_t o_rgb = off; off = align_up(off + 16 * cap, 256);
+  int64_t o_rb = 0, o_cr = 0, o_ds = 0, o_act = 0;
+  if (train_floats > 0) {
+    o_rb = off; off = align_up(off + 16 * cap, 256);
+    o_cr = off; off = align_up(off + 16 * cap, 256);
+    o_ds = off; off = align_up(off + 4 * cap, 256);
+    o_act = off; off = align_up(off + 4 * train_floats * (cap + 64 * (int64_t)T), 256);
+  }
   if (w && base) {
     w->total = reinterpret_cast<int64_t*>(base);
     w->overflow = reinterpret_cast<int32_t*>(base + 8);
@@ -165,6 +176,10 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
     w->rec_f = reinterpret_cast<float4*>(base + o_rf);
     w->rec_ti = reinterpret_cast<int2*>(base + o_ti);
     w->rgbw = reinterpret_cast<float4*>(base + o_rgb);
+    w->rec_b = train_floats > 0 ? reinterpret_cast<float4*>(base + o_rb) : nullptr;
+    w->crec = train_floats > 0 ? reinterpret_cast<float4*>(base + o_cr) : nullptr;
+    w->dsig = train_floats > 0 ? reinterpret_cast<float*>(base + o_ds) : nullptr;
+    w->act = train_floats > 0 ? reinterpret_cast<float*>(base + o_act) : nullptr;
     w->cap = cap;
     w->n_tiles = T;
   }
@@ -174,7 +189,7 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
 static int64_t default_capacity(int64_t n) { return 48 * n; }
 
 // ------------------------------------------------------------------------- march passes
-template <bool EMIT>
+template <int MODE>  // 0: count, 1: emit records, 2: emit records + training extras
 __global__ __launch_bounds__(64) void k_march_pass(snisr_scene S, snisr_march_params P, snisr_rays R,
                                                    snisr_render_out O, Workspace W, int packet_w, int enc_dtype) {
   const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
@@ -186,15 +201,16 @@ __global__ __launch_bounds__(64) void k_march_pass(snisr_scene S, snisr_march_pa
   const float t_hit = R.t_hit ? R.t_hit[i] : __builtin_inff();
   const int32_t* vis = R.visited ? R.visited + i * P.max_tiles : nullptr;
   MarchResult res;
-  if (EMIT) {
+  if (MODE > 0) {
     const int64_t base = W.offsets[i];
     int64_t k = base;
     bool over = false;
     march_ray(S, P, origin, d, t_hit, vis, nullptr, 0,
-              [&](int tile, const V3& f, const Corners&, float w) {
+              [&](int tile, const V3& f, const Corners&, float w, float sig, float interval, float T_after) {
                 if (k < W.cap) {
                   W.rec_f[k] = make_float4(f.x, f.y, f.z, w);
                   W.rec_ti[k] = make_int2(tile, (int)i);
+                  if (MODE == 2) W.rec_b[k] = make_float4(interval, T_after, sig, 0.0f);
                 } else {
                   over = true;
                 }
@@ -204,7 +220,7 @@ __global__ __launch_bounds__(64) void k_march_pass(snisr_scene S, snisr_march_pa
     if (over) *W.overflow = 1;
   } else {
     int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
-    march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, [&](int, const V3&, const Corners&, float) {}, res);
+    march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, [&](int, const V3&, const Corners&, float, float, float, float) {}, res);
     if (bricks)
       for (int k = res.n_bricks; k < O.brick_cap; ++k) bricks[k] = -1;
     W.counts[i] = res.n_samples;
@@ -711,6 +727,262 @@ __global__ __launch_bounds__(256) void k_reduce(Workspace W, int64_t n, float* _
   rgb[3 * i + 2] = c2;
 }
 
+
+// ------------------------------------------------------------------------- training (FP32)
+// Forward: same count/scan/emit (+ interval, T after, sigma per record) and tile sort as the render,
+// then a per-record FP32 MLP.  Backward: a per-ray reverse pass gives dL/dc and dL/dsigma of
+// every record; a per-record kernel recomputes the MLP forward (segment recomputation), writes
+// each layer's input and output delta rows, backpropagates to the features and scatter-adds the
+// trilinear adjoint into the feature and density gradients; a segmented reduction over each tile's
+// rows gives dW = sum delta (x) input and db = sum delta.
+__host__ __device__ constexpr int in_pad_of(int in0, int wd, int l) { return l == 0 ? ((in0 + 3) & ~3) : wd; }
+__host__ __device__ constexpr int out_pad_of(int wd, int L, int l) { return l == L - 1 ? 4 : wd; }
+__host__ __device__ constexpr int64_t train_act_floats(int in0, int wd, int L) {
+  int64_t s = 0;
+  for (int l = 0; l < L; ++l) s += in_pad_of(in0, wd, l) + out_pad_of(wd, L, l);
+  return s;
+}
+__host__ __device__ constexpr int64_t act_a_off(int in0, int wd, int L, int l) {
+  int64_t s = 0;
+  for (int m = 0; m < l; ++m) s += in_pad_of(in0, wd, m);
+  return s;
+}
+__host__ __device__ constexpr int64_t act_d_off(int in0, int wd, int L, int l) {
+  int64_t s = 0;
+  for (int m = 0; m < L; ++m) s += in_pad_of(in0, wd, m);
+  for (int m = 0; m < l; ++m) s += out_pad_of(wd, L, m);
+  return s;
+}
+// float offset of layer l's W[out][in] block in the mlp_params layout
+__host__ __device__ constexpr int64_t param_w_off(int in0, int wd, int L, int l) {
+  int64_t s = 0;
+  for (int m = 0; m < l; ++m) {
+    const int i = m == 0 ? in0 : wd, o = m == L - 1 ? 3 : wd;
+    s += (int64_t)o * i + o;
+  }
+  return s;
+}
+// float offset of layer l's Wt block in the f32t layout
+__host__ __device__ constexpr int64_t f32t_w_off(int in0, int wd, int L, int l) {
+  int64_t s = 0;
+  for (int m = 0; m < l; ++m) {
+    const int i = m == 0 ? in0 : wd, o4 = m == L - 1 ? 4 : wd;
+    s += (int64_t)i * o4 + o4;
+  }
+  return s;
+}
+
+// delta of layer l's input: out[i] = sum_o Wt[i][o] dl[o] for i < IN_L, masked by a[i] > 0 when
+// `a` is given; written to the LDS column and, when given, to a global row
+template <int IN_L, int OUT4>
+__device__ __forceinline__ void backprop_layer(const float* __restrict__ Wt, const float (&dl)[OUT4],
+                                               const float* a, float* drow, float* xs) {
+#pragma unroll 1
+  for (int i = 0; i < IN_L; ++i) {
+    const float4* w = reinterpret_cast<const float4*>(Wt + i * OUT4);
+    float acc = 0.0f;
+#pragma unroll
+    for (int q = 0; q < OUT4 / 4; ++q) {
+      const float4 t = w[q];
+      acc = __builtin_fmaf(t.x, dl[4 * q], acc);
+      acc = __builtin_fmaf(t.y, dl[4 * q + 1], acc);
+      acc = __builtin_fmaf(t.z, dl[4 * q + 2], acc);
+      acc = __builtin_fmaf(t.w, dl[4 * q + 3], acc);
+    }
+    if (a) acc = a[i] > 0.0f ? acc : 0.0f;
+    if (drow) drow[i] = acc;
+    xs[i * 64] = acc;
+  }
+}
+
+template <int F, int WD, int L, bool HALF, bool BWD>
+__global__ __launch_bounds__(64) void k_train_mlp(snisr_scene S, snisr_rays R, Workspace W, float* __restrict__ dsigma,
+                                                  float* __restrict__ dfeats) {
+  constexpr int IN = F + 15;
+  constexpr int XD = IN > WD ? IN : WD;
+  constexpr int64_t WS = f32t_stride(IN, WD, L);
+  __shared__ float lds[XD * 64];
+  float* xs = lds + threadIdx.x;
+  const int T = W.n_tiles;
+  const int64_t NJ = W.cap + 64 * (int64_t)T;
+  const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (j >= W.tile_start[T]) return;
+  const int32_t k = W.perm[j];
+  if (k < 0) return;
+  const float4 rf = W.rec_f[k];
+  const int2 ti = W.rec_ti[k];
+  const int tile = ti.x;
+  Corners cn;
+  find_corners(S.brick_map + S.tile_brick_base[tile], S.tile_res[tile], v3(rf.x, rf.y, rf.z), cn);
+  gather_features<F, HALF>(S.feats, cn, xs);
+  const int64_t ray = ti.y;
+  const V3 d = normalize_dir(v3(R.dirs[3 * ray], R.dirs[3 * ray + 1], R.dirs[3 * ray + 2]));
+  float enc[15];
+  view_encoding(d, enc);
+#pragma unroll
+  for (int q = 0; q < 15; ++q) xs[(F + q) * 64] = enc[q];
+  const float* P = S.mlp_f32t + (int64_t)tile * WS;
+  if (!BWD) {
+    float rgb[3];
+    mlp_f32<IN, WD, L>(P, xs, rgb);
+    W.crec[k] = make_float4(rgb[0], rgb[1], rgb[2], 0.0f);
+    W.rgbw[k] = make_float4(rf.w * rgb[0], rf.w * rgb[1], rf.w * rgb[2], 0.0f);
+    return;
+  }
+  // ---- recompute the forward, storing every layer's input row
+  float* act = W.act;
+  constexpr int IP0 = in_pad_of(IN, WD, 0);
+  float* A0 = act + NJ * act_a_off(IN, WD, L, 0) + j * IP0;
+#pragma unroll 1
+  for (int i = 0; i < IP0; ++i) A0[i] = i < IN ? xs[i * 64] : 0.0f;
+  float y[WD];
+  dense_f32<IN, WD>(P, P + IN * WD, xs, y);
+  {
+    float* A1 = act + NJ * act_a_off(IN, WD, L, 1) + j * WD;
+#pragma unroll
+    for (int o = 0; o < WD; ++o) {
+      const float v = y[o] > 0.0f ? y[o] : 0.0f;
+      xs[o * 64] = v;
+      A1[o] = v;
+    }
+  }
+#pragma unroll 1
+  for (int l = 1; l < L - 1; ++l) {
+    const float* Pl = P + f32t_w_off(IN, WD, L, l);
+    dense_f32<WD, WD>(Pl, Pl + WD * WD, xs, y);
+    float* An = act + NJ * act_a_off(IN, WD, L, l + 1) + j * WD;
+#pragma unroll
+    for (int o = 0; o < WD; ++o) {
+      const float v = y[o] > 0.0f ? y[o] : 0.0f;
+      xs[o * 64] = v;
+      An[o] = v;
+    }
+  }
+  const float* PL = P + f32t_w_off(IN, WD, L, L - 1);
+  float z[4];
+  dense_f32<WD, 4>(PL, PL + WD * 4, xs, z);
+  // ---- output delta from dL/dc (written into rgbw by the compositing adjoint)
+  const float4 dc = W.rgbw[k];
+  float dl4[4];
+  {
+    const float c0 = sigmoid_cw(z[0]), c1 = sigmoid_cw(z[1]), c2 = sigmoid_cw(z[2]);
+    dl4[0] = dc.x * c0 * (1.0f - c0);
+    dl4[1] = dc.y * c1 * (1.0f - c1);
+    dl4[2] = dc.z * c2 * (1.0f - c2);
+    dl4[3] = 0.0f;
+    float* DL = act + NJ * act_d_off(IN, WD, L, L - 1) + j * 4;
+    DL[0] = dl4[0]; DL[1] = dl4[1]; DL[2] = dl4[2]; DL[3] = 0.0f;
+  }
+  // ---- backpropagate through the hidden layers
+  backprop_layer<WD, 4>(PL, dl4, act + NJ * act_a_off(IN, WD, L, L - 1) + j * WD,
+                        act + NJ * act_d_off(IN, WD, L, L - 2) + j * WD, xs);
+#pragma unroll 1
+  for (int l = L - 2; l >= 1; --l) {
+    float dl[WD];
+#pragma unroll
+    for (int o = 0; o < WD; ++o) dl[o] = xs[o * 64];
+    backprop_layer<WD, WD>(P + f32t_w_off(IN, WD, L, l), dl, act + NJ * act_a_off(IN, WD, L, l) + j * WD,
+                           act + NJ * act_d_off(IN, WD, L, l - 1) + j * WD, xs);
+  }
+  {
+    float dl[WD];
+#pragma unroll
+    for (int o = 0; o < WD; ++o) dl[o] = xs[o * 64];
+    backprop_layer<F, WD>(P, dl, nullptr, nullptr, xs);  // dL/d(features) only
+  }
+  // ---- trilinear adjoint
+  const float dsg = W.dsig[k];
+  const float wx[2] = {1.0f - cn.wx, cn.wx}, wy[2] = {1.0f - cn.wy, cn.wy}, wz[2] = {1.0f - cn.wz, cn.wz};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int slot = cn.slot[q];
+    if (slot < 0) continue;
+    const float wq = wx[q & 1] * wy[(q >> 1) & 1] * wz[q >> 2];
+    if (dsg != 0.0f) atomicAdd(&dsigma[slot], wq * dsg);
+#pragma unroll 4
+    for (int e = 0; e < F; ++e) atomicAdd(&dfeats[(int64_t)slot * F + e], wq * xs[e * 64]);
+  }
+}
+
+// dL/dsigma and dL/dc of every record of ray i, in reverse sample order
+__global__ __launch_bounds__(256) void k_train_comp_bwd(Workspace W, int64_t n, const float* __restrict__ g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t b = W.offsets[i], e = W.offsets[i + 1];
+  if (e > W.cap) return;  // overflowed in the forward: no records
+  const float g0 = g[3 * i], g1 = g[3 * i + 1], g2 = g[3 * i + 2];
+  float suffix = 0.0f;
+  for (int64_t k = e - 1; k >= b; --k) {
+    const float4 c = W.crec[k];
+    const float4 rf = W.rec_f[k];
+    const float4 rb = W.rec_b[k];
+    const float cg = c.x * g0 + c.y * g1 + c.z * g2;
+    float ds = rb.x * (rb.y * cg - suffix);
+    suffix = suffix + rf.w * cg;
+    if (!(rb.z > 0.0f)) ds = 0.0f;
+    W.dsig[k] = ds;
+    W.rgbw[k] = make_float4(rf.w * g0, rf.w * g1, rf.w * g2, 0.0f);
+  }
+}
+
+// dW_l[o][i] += sum_j D_l[j][o] A_l[j][i], db_l[o] += sum_j D_l[j][o] over one split of one tile's rows;
+// a 256-thread block owns a 32x32 output block (4 outputs per thread).
+template <int IN, int WD, int L>
+__global__ __launch_bounds__(256) void k_train_dw(Workspace W, int64_t mlp_stride, float* __restrict__ dmlp) {
+  __shared__ float Ds[32][33];
+  __shared__ float As[32][33];
+  const int T = W.n_tiles;
+  const int tile = blockIdx.z / L, l = blockIdx.z % L;
+  const int in_l = l == 0 ? IN : WD, out_l = l == L - 1 ? 3 : WD;
+  const int ip = l == 0 ? in_pad_of(IN, WD, 0) : WD, op = l == L - 1 ? 4 : WD;
+  const int nib = (in_l + 31) / 32, nob = (out_l + 31) / 32;
+  if ((int)blockIdx.y >= nib * nob) return;
+  const int ob = blockIdx.y / nib, ib = blockIdx.y % nib;
+  const int64_t NJ = W.cap + 64 * (int64_t)T;
+  const float* A = W.act + NJ * (l == 0 ? 0 : act_a_off(IN, WD, L, 1) + (int64_t)(l - 1) * WD);
+  const float* D = W.act + NJ * act_d_off(IN, WD, L, l);
+  const int64_t j0 = W.tile_start[tile], cnt = W.tile_count[tile];
+  const int64_t per = (cnt + gridDim.x - 1) / gridDim.x;
+  const int64_t jb = j0 + (int64_t)blockIdx.x * per;
+  const int64_t je = jb + per < j0 + cnt ? jb + per : j0 + cnt;
+  const int ty = threadIdx.x >> 3, tx = threadIdx.x & 7;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, accb = 0.0f;
+  for (int64_t r0 = jb; r0 < je; r0 += 32) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < 1024; q += 256) {
+      const int rr = q >> 5, cc = q & 31;
+      const int64_t jr = r0 + rr;
+      const int o = ob * 32 + cc, i = ib * 32 + cc;
+      Ds[rr][cc] = (jr < je && o < op) ? D[jr * op + o] : 0.0f;
+      As[rr][cc] = (jr < je && i < ip) ? A[jr * ip + i] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int rr = 0; rr < 32; ++rr) {
+      const float dv = Ds[rr][ty];
+      acc[0] = __builtin_fmaf(dv, As[rr][tx * 4 + 0], acc[0]);
+      acc[1] = __builtin_fmaf(dv, As[rr][tx * 4 + 1], acc[1]);
+      acc[2] = __builtin_fmaf(dv, As[rr][tx * 4 + 2], acc[2]);
+      acc[3] = __builtin_fmaf(dv, As[rr][tx * 4 + 3], acc[3]);
+      accb += dv;
+    }
+  }
+  const int o = ob * 32 + ty;
+  if (o >= out_l) return;
+  float* G = dmlp + (int64_t)tile * mlp_stride + param_w_off(IN, WD, L, l);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = ib * 32 + tx * 4 + q;
+    if (i < in_l && acc[q] != 0.0f) atomicAdd(&G[(int64_t)o * in_l + i], acc[q]);
+  }
+  if (ib == 0 && tx == 0 && accb != 0.0f) atomicAdd(&G[(int64_t)out_l * in_l + o], accb);
+}
+
+__global__ void k_fill_zero(float* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
 int packet_width(const snisr_rays& R);  // render_fp32.hip
 
 // compute units of the current device (a per-device cache of a constant attribute)
@@ -772,7 +1044,7 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
   const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
 
   SNISR_HIP(stage_event(out, 0, st));
-  hipLaunchKernelGGL(k_march_pass<false>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt);
+  hipLaunchKernelGGL(k_march_pass<0>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 1, st));
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.counts, n, W.offsets,
@@ -784,7 +1056,7 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.offsets, n, W.block_sums);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 2, st));
-  hipLaunchKernelGGL(k_march_pass<true>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt);
+  hipLaunchKernelGGL(k_march_pass<1>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 3, st));
 
@@ -822,6 +1094,153 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, W, n, out->rgb);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 6, st));
+  return SNISR_OK;
+}
+
+
+int64_t snisr_train_workspace_size(const snisr_scene* scene, int64_t n, int64_t sample_capacity) {
+  if (!scene || n < 0 || sample_capacity < 0) {
+    set_error("train_workspace_size: bad arguments");
+    return -1;
+  }
+  const int64_t cap = sample_capacity ? sample_capacity : default_capacity(n);
+  return layout(n, cap, scene->n_tiles, nullptr, nullptr,
+                train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers));
+}
+
+static int train_capacity(const snisr_scene* scene, int64_t n, int64_t bytes, int64_t* cap_out) {
+  const int64_t tf = train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers);
+  const int64_t fixed = layout(n, 0, scene->n_tiles, nullptr, nullptr, tf);
+  SNISR_REQUIRE(bytes >= fixed, SNISR_ERR_CAPACITY, "train: workspace too small");
+  const int64_t per = 44 + 36 + 4 * tf;  // render record arrays + training arrays + activation rows
+  int64_t cap = (bytes - fixed - 8 * 256) / per;
+  while (cap > 0 && layout(n, cap, scene->n_tiles, nullptr, nullptr, tf) > bytes) --cap;
+  SNISR_REQUIRE(cap > 0, SNISR_ERR_CAPACITY, "train: workspace too small");
+  *cap_out = cap;
+  return SNISR_OK;
+}
+
+#define SNISR_TRAIN_SHAPES(X) X(16, 32, 3, false) X(16, 64, 3, false) X(32, 128, 4, false) X(32, 128, 4, true)
+
+// Training forward: the render of snisr_render_fp32 (same traversal and samples) that keeps the
+// per-sample records in the workspace for snisr_train_backward.  Replaces the forward of the
+// autograd chain of Tile.train_batch (training/scripts/tile/tile.py:631-820).
+int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
+                        const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream) {
+  SNISR_REQUIRE(out && out->rgb, SNISR_ERR_ARG, "train_forward: out.rgb is required");
+  if (int e = check_scene(scene, true)) return e;
+  if (int e = check_params(params)) return e;
+  if (int e = check_rays(rays, params)) return e;
+  SNISR_REQUIRE(scene->mlp_f32t && aligned(scene->mlp_f32t, 16), SNISR_ERR_ARG,
+                "train_forward: scene.mlp_f32t (snisr_pack_mlp(SNISR_F32)) is required");
+  SNISR_REQUIRE(workspace && aligned(workspace, 256), SNISR_ERR_ARG, "train_forward: workspace must be 256-byte aligned");
+  SNISR_REQUIRE(scene->n_tiles <= kMaxTiles, SNISR_ERR_SHAPE, "train_forward: at most %d tiles", kMaxTiles);
+  const int64_t n = rays->n;
+  if (n == 0) return SNISR_OK;
+  int64_t cap = 0;
+  if (int e = train_capacity(scene, n, workspace_bytes, &cap)) return e;
+  Workspace W;
+  layout(n, cap, scene->n_tiles, (uint8_t*)workspace, &W,
+         train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers));
+  const int F = scene->feat_dim, WD = scene->mlp_width, L = scene->mlp_layers;
+  const bool half = scene->feat_dtype == SNISR_F16;
+  hipStream_t st = (hipStream_t)stream;
+  const int pw = packet_width(*rays);
+  const dim3 mgrid((unsigned)((n + 63) / 64));
+  const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
+  bool ok = false;
+#define SNISR_CHECK_SHAPE(F_, W_, L_, H_) ok = ok || (F == F_ && WD == W_ && L == L_ && half == H_);
+  SNISR_TRAIN_SHAPES(SNISR_CHECK_SHAPE)
+#undef SNISR_CHECK_SHAPE
+  SNISR_REQUIRE(ok, SNISR_ERR_SHAPE, "train_forward: unsupported shape F=%d width=%d layers=%d", F, WD, L);
+  SNISR_HIP(stage_event(out, 0, st));
+  hipLaunchKernelGGL(k_march_pass<0>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, (int)SNISR_F16);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.counts, n, W.offsets,
+                     W.block_sums);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, st, W.block_sums, nblocks, W.offsets, n, W.total,
+                     W.overflow, W.tile_count, W.n_tiles);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.offsets, n, W.block_sums);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_march_pass<2>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, (int)SNISR_F16);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_tile_hist, dim3(kSortGrid), dim3(kSortBlock), 0, st, W);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kScanBlock), 0, st, W);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_tile_scatter, dim3(kSortGrid), dim3(kSortBlock), 0, st, W);
+  SNISR_LAUNCHED();
+  SNISR_HIP(stage_event(out, 1, st));
+  const dim3 jgrid((unsigned)((cap + 64 * (int64_t)scene->n_tiles + 63) / 64));
+#define SNISR_FWD(F_, W_, L_, H_)                                                                     \
+  if (F == F_ && WD == W_ && L == L_ && half == H_)                                                     \
+    hipLaunchKernelGGL((k_train_mlp<F_, W_, L_, H_, false>), jgrid, dim3(64), 0, st, *scene, *rays, W, \
+                       (float*)nullptr, (float*)nullptr);
+  SNISR_TRAIN_SHAPES(SNISR_FWD)
+#undef SNISR_FWD
+  SNISR_LAUNCHED();
+  SNISR_HIP(stage_event(out, 2, st));
+  hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, W, n, out->rgb);
+  SNISR_LAUNCHED();
+  SNISR_HIP(stage_event(out, 3, st));
+  return SNISR_OK;
+}
+
+// Training backward: gradients of L = sum_i <grad_rgb_i, rgb_i> for the rays of the preceding
+// snisr_train_forward on the same workspace (which it reads).  dsigma f32[NB,512], dfeats
+// f32[NB,512,F] and dmlp f32[T,P] (mlp_params layout) are overwritten.  Hand-written adjoint of
+// the compositing, trilinear gather and MLP that torch autograd provides in the reference
+// (training/scripts/tile/tile.py:780-785).  stage_events (optional, 4): before the compositing
+// adjoint, after it, after the MLP/trilinear adjoint, after the weight-gradient reduction.
+int snisr_train_backward(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
+                         const float* grad_rgb, float* dsigma, float* dfeats, float* dmlp, void* workspace,
+                         int64_t workspace_bytes, void* const* stage_events, int32_t n_stage_events, void* stream) {
+  if (int e = check_scene(scene, true)) return e;
+  if (int e = check_params(params)) return e;
+  if (int e = check_rays(rays, params)) return e;
+  SNISR_REQUIRE(grad_rgb && dsigma && dfeats && dmlp, SNISR_ERR_ARG, "train_backward: NULL gradient pointer");
+  SNISR_REQUIRE(workspace && aligned(workspace, 256), SNISR_ERR_ARG, "train_backward: workspace");
+  const int64_t n = rays->n;
+  hipStream_t st = (hipStream_t)stream;
+  snisr_render_out ev{};
+  ev.stage_events = stage_events;
+  ev.n_stage_events = n_stage_events;
+  const int F = scene->feat_dim, WD = scene->mlp_width, L = scene->mlp_layers;
+  const bool half = scene->feat_dtype == SNISR_F16;
+  const int64_t nsig = (int64_t)scene->n_bricks * 512, nfeat = nsig * F, nmlp = (int64_t)scene->n_tiles * scene->mlp_stride;
+  SNISR_HIP(stage_event(&ev, 0, st));
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((nsig + 255) / 256)), dim3(256), 0, st, dsigma, nsig);
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((nfeat + 255) / 256)), dim3(256), 0, st, dfeats, nfeat);
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((nmlp + 255) / 256)), dim3(256), 0, st, dmlp, nmlp);
+  SNISR_LAUNCHED();
+  if (n == 0) return SNISR_OK;
+  int64_t cap = 0;
+  if (int e = train_capacity(scene, n, workspace_bytes, &cap)) return e;
+  Workspace W;
+  layout(n, cap, scene->n_tiles, (uint8_t*)workspace, &W,
+         train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers));
+  hipLaunchKernelGGL(k_train_comp_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, W, n, grad_rgb);
+  SNISR_LAUNCHED();
+  SNISR_HIP(stage_event(&ev, 1, st));
+  const dim3 jgrid((unsigned)((cap + 64 * (int64_t)scene->n_tiles + 63) / 64));
+  bool ok = false;
+#define SNISR_BWD(F_, W_, L_, H_)                                                                           \
+  if (F == F_ && WD == W_ && L == L_ && half == H_) {                                                       \
+    hipLaunchKernelGGL((k_train_mlp<F_, W_, L_, H_, true>), jgrid, dim3(64), 0, st, *scene, *rays, W, dsigma, \
+                       dfeats);                                                                             \
+    SNISR_LAUNCHED();                                                                                       \
+    SNISR_HIP(stage_event(&ev, 2, st));                                                                     \
+    const dim3 g(8, 16, (unsigned)(scene->n_tiles * L_));                                                   \
+    hipLaunchKernelGGL((k_train_dw<F_ + 15, W_, L_>), g, dim3(256), 0, st, W, scene->mlp_stride, dmlp);     \
+    SNISR_LAUNCHED();                                                                                       \
+    ok = true;                                                                                              \
+  }
+  SNISR_TRAIN_SHAPES(SNISR_BWD)
+#undef SNISR_BWD
+  SNISR_REQUIRE(ok, SNISR_ERR_SHAPE, "train_backward: unsupported shape F=%d width=%d layers=%d", F, WD, L);
+  SNISR_HIP(stage_event(&ev, 3, st));
   return SNISR_OK;
 }
 

@@ -206,8 +206,9 @@ struct MarchResult {
 // bricks; in an occupied brick samples z_k = fma(k, step, near) for z_k < far, interval
 // min(step, far - z_k); alpha = 1 - exp(-sigma*interval), T *= 1 - alpha
 // (rendering/src/include/cuda_utils.h:13-20); stop once T < early_stop or at t >= t_hit.
-// `on_sample(tile, f, corners, w)` receives every composited sample: f = voxel coordinate
-// (p/voxel - 0.5 in the tile), its trilinear corners, w = alpha*T.
+// `on_sample(tile, f, corners, w, sigma, interval, T_after)` receives every composited sample:
+// f = voxel coordinate (p/voxel - 0.5 in the tile), its trilinear corners, w = alpha*T, the
+// clamped density, the interval and the transmittance after the sample.
 template <class OnSample>
 __device__ __forceinline__ void march_ray(const snisr_scene& S, const snisr_march_params& P, V3 origin, V3 d,
                                           float t_hit, const int32_t* __restrict__ vis, int32_t* bricks,
@@ -272,8 +273,9 @@ __device__ __forceinline__ void march_ray(const snisr_scene& S, const snisr_marc
           const float interval = fminx(step, far_ - z);
           const float alpha = 1.0f - exp_cw(-(sig * interval));
           const float w = alpha * T;
-          on_sample(tile, f, c, w);
-          T = T * (1.0f - alpha);
+          const float T_next = T * (1.0f - alpha);
+          on_sample(tile, f, c, w, sig, interval, T_next);
+          T = T_next;
           ++nsamp;
           if (T < P.early_stop) { done = true; break; }
         }
@@ -307,6 +309,99 @@ __device__ __forceinline__ void view_encoding(V3 d, float e[15]) {
     e[3 + 6 * l + 3] = cosf(s * d.x);
     e[3 + 6 * l + 4] = cosf(s * d.y);
     e[3 + 6 * l + 5] = cosf(s * d.z);
+  }
+}
+
+// ---------------------------------------------------------------------------- per-lane FP32 MLP
+// Activations live in a lane-private LDS column (entry k of lane t at xs[k * 64], wavefront-sized
+// blocks), weights in the f32t layout Wt[in][out4] then b[out4].
+// One dense layer for one lane: y = b + x Wt, x read from this lane's LDS column (stride 64),
+// Wt rows read as float4 (all lanes of a wave in one tile read the same address).
+template <int IN, int OUT>
+__device__ __forceinline__ void dense_f32(const float* __restrict__ Wt, const float* __restrict__ b,
+                                          const float* xs, float (&y)[OUT]) {
+  static_assert(OUT % 4 == 0, "padded width");
+#pragma unroll
+  for (int j = 0; j < OUT; ++j) y[j] = b[j];
+#pragma unroll 1
+  for (int i = 0; i < IN; ++i) {
+    const float xi = xs[i * 64];
+    const float4* w = reinterpret_cast<const float4*>(Wt + i * OUT);
+#pragma unroll
+    for (int j = 0; j < OUT / 4; ++j) {
+      const float4 q = w[j];
+      y[4 * j + 0] = __builtin_fmaf(xi, q.x, y[4 * j + 0]);
+      y[4 * j + 1] = __builtin_fmaf(xi, q.y, y[4 * j + 1]);
+      y[4 * j + 2] = __builtin_fmaf(xi, q.z, y[4 * j + 2]);
+      y[4 * j + 3] = __builtin_fmaf(xi, q.w, y[4 * j + 3]);
+    }
+  }
+}
+
+// Per-sample MLP on one lane: input in xs[0..IN), hidden layers ReLU, sigmoid RGB.
+template <int IN, int WD, int L>
+__device__ __forceinline__ void mlp_f32(const float* __restrict__ P, float* xs, float rgb[3]) {
+  float y[WD];
+  dense_f32<IN, WD>(P, P + IN * WD, xs, y);
+  P += IN * WD + WD;
+#pragma unroll
+  for (int j = 0; j < WD; ++j) xs[j * 64] = y[j] > 0.0f ? y[j] : 0.0f;
+#pragma unroll 1
+  for (int l = 1; l < L - 1; ++l) {
+    dense_f32<WD, WD>(P, P + WD * WD, xs, y);
+    P += WD * WD + WD;
+#pragma unroll
+    for (int j = 0; j < WD; ++j) xs[j * 64] = y[j] > 0.0f ? y[j] : 0.0f;
+  }
+  float z[4];
+  dense_f32<WD, 4>(P, P + WD * 4, xs, z);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) rgb[c] = sigmoid_cw(z[c]);
+}
+
+template <bool HALF>
+struct FeatLoad;
+template <>
+struct FeatLoad<true> {  // 8 halves per 16-byte load
+  static constexpr int kVec = 8;
+  __device__ __forceinline__ static void load(const void* base, int64_t off, float* v) {
+    const uint4 q = *reinterpret_cast<const uint4*>(reinterpret_cast<const _Float16*>(base) + off);
+    const _Float16* h = reinterpret_cast<const _Float16*>(&q);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = (float)h[k];
+  }
+};
+template <>
+struct FeatLoad<false> {  // 4 floats per 16-byte load
+  static constexpr int kVec = 4;
+  __device__ __forceinline__ static void load(const void* base, int64_t off, float* v) {
+    const float4 q = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(base) + off);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  }
+};
+
+// Trilinear features of one sample into this lane's LDS column xs[0..F).
+template <int F, bool HALF>
+__device__ __forceinline__ void gather_features(const void* __restrict__ feats, const Corners& c, float* xs) {
+  using LD = FeatLoad<HALF>;
+  constexpr int V = LD::kVec;
+  static_assert(F % V == 0, "feature dim must fill whole 16-byte vectors");
+#pragma unroll
+  for (int ch = 0; ch < F / V; ++ch) {
+    float v[8][V];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (c.slot[k] >= 0) LD::load(feats, (int64_t)c.slot[k] * F + ch * V, v[k]);
+      else {
+#pragma unroll
+        for (int e = 0; e < V; ++e) v[k][e] = 0.0f;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const float q[8] = {v[0][e], v[1][e], v[2][e], v[3][e], v[4][e], v[5][e], v[6][e], v[7][e]};
+      xs[(ch * V + e) * 64] = tri_combine(q, c.wx, c.wy, c.wz);
+    }
   }
 }
 

@@ -62,6 +62,12 @@ SIGNATURES = [
                                   C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("snisr_mesh_first_hit", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int32,
                                        C.POINTER(RaysDesc), C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("snisr_train_workspace_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int64, C.c_int64]),
+    ("snisr_train_forward", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
+                                      C.POINTER(RenderOut), C.c_void_p, C.c_int64, C.c_void_p]),
+    ("snisr_train_backward", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                       C.c_void_p, C.c_int32, C.c_void_p]),
     ("snisr_pack_mlp_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int32]),
     ("snisr_pack_mlp", C.c_int, [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p]),
 ]
