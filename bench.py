@@ -59,6 +59,57 @@ def psnr_vs_fp64(sc, cam, dirs_np, t_hit_np, rgb_gpu, n_rays=2000, seed=1):
     return 10 * np.log10(1.0 / max(mse, 1e-30))
 
 
+def train_bench(args, dev):
+    """C3: one training step = forward + backward (hand-written adjoint) + Adam on the 16-tile
+    scene with FP32 features/MLP, 8192 rays per tile (131,072 rays), MSE against the render of a
+    perturbed copy."""
+    import numpy as np
+    import torch
+    from snisr import device as D
+    from snisr import scene as S
+    from snisr import train as TR
+
+    sc, _, _ = S.config_c2(n_triangles=0, H=8, W=8, feat_half=False)
+    tgt_sc = S.perturbed_copy(sc, 3)
+    origins_np, dirs_np = S.config_c3_rays(sc, 3, 8192)
+    o = torch.from_numpy(origins_np).to(dev)
+    d = torch.from_numpy(dirs_np).to(dev)
+    ds_t = D.DeviceScene(tgt_sc)
+    with torch.no_grad():
+        target, _ = TR.train_forward(ds_t, o, d)
+        target = target["rgb"].clone()
+    del ds_t
+    ds = D.DeviceScene(sc)
+    sigma = ds.sigma.clone().requires_grad_(True)
+    feats = ds.feats.clone().requires_grad_(True)
+    mlp = ds.mlp_params.clone().requires_grad_(True)
+    opt = torch.optim.Adam([sigma, feats, mlp], lr=1e-3)
+    ws = TR.TrainWorkspace()
+    losses = []
+
+    def step():
+        opt.zero_grad(set_to_none=False)
+        rgb = TR.render(sigma, feats, mlp, o, d, ds, workspace=ws)
+        loss = torch.mean((rgb - target) ** 2)
+        loss.backward()
+        opt.step()
+        return loss
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.train_steps):
+        losses.append(step())
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.train_steps
+    n = o.shape[0]
+    return {"train_mrays_s": round(n / (ms * 1e-3) / 1e6, 3), "train_ms_per_step": round(ms, 3),
+            "train_rays_per_step": int(n), "train_loss_first_last": [float(losses[0]), float(losses[-1])],
+            "train_config": "C3: C2 scene with FP32 features/MLP, 16 tiles x 8192 rays, MSE vs a perturbed "
+                            "copy, fwd + hand-written adjoint + Adam"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +118,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and PSNR legs")
     ap.add_argument("--cpu-frac", type=float, default=0.05)
     ap.add_argument("--fp32-frames", type=int, default=3)
+    ap.add_argument("--train-steps", type=int, default=10, help="timed C3 training steps (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -159,6 +211,9 @@ def main():
         extra["fp32_path_ms_per_frame"] = round(fp32_ms, 3)
         extra["mfma_vs_fp32_path_render_speedup"] = round(fp32_ms / render_ms, 2)
         extra["counts_equal_fp32_path"] = bool(torch.equal(o32["n_samples"], out["n_samples"]))
+
+    if args.train_steps > 0:
+        extra.update(train_bench(args, dev))
 
     cpu = None
     if not args.no_cpu:

@@ -127,7 +127,8 @@ int snisr_compute_rays(const float* intrinsic, const float* c2w, int32_t H, int3
                        void* stream);
 
 /* A2 — tracing_tiles (rendering/src/raytile_tracing.cu:63-95, include/raytile_tracing.h:8-13).
- * origin f32[3] device; rays_d f32[n,3]; visited i32[n,K], -1 padded. */
+ * origin f32[3] device; rays_d f32[n,3]; visited i32[n,K], -1 padded.  Of the scene only the tile
+ * grid is read (tile_shape, scene_min, tile_size, tile_index_map); scene_size = tile_shape * tile_size. */
 int snisr_trace_tiles(const snisr_scene* scene, const float* origin, const float* rays_d, int64_t n, int32_t K,
                       int32_t* visited, void* stream);
 
@@ -172,6 +173,34 @@ int snisr_bvh_build(const float* vertices, int64_t n_vertices, const int32_t* fa
                     int64_t node_bytes, void* tris_out, int64_t tri_bytes, int32_t* n_nodes, int32_t* depth);
 int snisr_mesh_first_hit(const void* nodes, int32_t n_nodes, const void* tris, int64_t n_tris, int32_t depth,
                          const snisr_rays* rays, float* t_out, int32_t* id_out, void* stream);
+
+/* A6 — inverse_z_sampling (rendering/src/sampling.cu:48-81, include/sampling.h:7-15).  For pixel p
+ * (row-major H x W) and point j < sample_per_ray: z = 1/((offset+j) ib[p] + in[p]), step =
+ * max(z_{j+1} - z_j, 0), point = origin + z d[p].  rays_d may be a strided view: element (y, x, c) at
+ * rays_d[y*d_row_stride + x*d_col_stride + c] (floats), as the reference's rays_d[::2, ::2]. */
+int snisr_inverse_z_sampling(const float* origin, const float* rays_d, int64_t d_row_stride, int64_t d_col_stride,
+                             int32_t H, int32_t W, const float* inverse_near, const float* inverse_bound, int32_t offset,
+                             int32_t sample_per_ray, float* sample_points, float* integrate_steps, void* stream);
+
+/* A5 — integrate_points (rendering/src/volume_rendering.cu:60-87, include/volume_rendering.h:7-12):
+ * 16 samples per pixel of sample_results f32[npix,16,4] (rgb, sigma) and steps f32[npix,16];
+ * transparency f32[npix] and colors f32[npix,3] are updated in place; pixels with T < early_stop
+ * are left untouched. */
+int snisr_integrate_points(const float* sample_results, const float* integrate_steps, int64_t npix, float early_stop,
+                           float* transparency, float* colors, void* stream);
+
+/* A8 — sort_by_key (rendering/src/reorder.cu:11-23, include/reorder.h:7): stable ascending sort of
+ * keys i16[n] carrying values i32[n], then the run heads (unique_by_key on keys with starts): the
+ * first m keys are the distinct keys and starts[0..m) the starts[] entries at the head of each run.
+ * m is returned through *n_unique (host pointer; the call synchronises `stream`, like the
+ * reference's int return).  Needs snisr_sort_by_key_workspace_size(n) bytes of device workspace. */
+int64_t snisr_sort_by_key_workspace_size(int64_t n);
+int snisr_sort_by_key(int16_t* keys, int32_t* values, int32_t* starts, int64_t n, void* workspace,
+                      int64_t workspace_bytes, int32_t* n_unique, void* stream);
+
+/* A8 — assign_pixels (rendering/src/reorder.cu:69-94, include/reorder.h:10-15). */
+int snisr_assign_pixels(const int16_t* net, const int32_t* pixel_starts, const int32_t* block_starts, int64_t n_net,
+                        int16_t* new_net, int32_t* new_pixel_starts, void* stream);
 
 /* A11 — training (FP32 features and MLP).  snisr_train_forward renders like snisr_render_fp32
  * (identical traversal and samples) and keeps per-sample records in the workspace;

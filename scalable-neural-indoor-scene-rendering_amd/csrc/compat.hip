@@ -1,0 +1,205 @@
+// Reference-format operators of the render path's second half: inverse-depth sampling (A6),
+// per-pixel compositing of 16-sample chunks (A5), and the pixel regrouping scheduler (A8).
+#include <hipcub/hipcub.hpp>
+
+#include "snisr_common.hpp"
+#include "snisr_device.hpp"
+
+namespace snisr {
+
+// A6: z_j = 1/((offset+j)*ib + in), step_j = max(z_{j+1} - z_j, 0), point = origin + z d
+// (rendering/src/sampling.cu:5-46; __fdiv_rn there is IEEE round-to-nearest division, as here).
+__global__ __launch_bounds__(256) void k_inverse_z(const float* __restrict__ origin, const float* __restrict__ rays_d,
+                                                   int64_t d_row_stride, int64_t d_col_stride, int W,
+                                                   const float* __restrict__ inv_near, const float* __restrict__ inv_bound,
+                                                   int64_t npix, int offset, int spr, float* __restrict__ points,
+                                                   float* __restrict__ steps) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npix * spr) return;
+  const int64_t p = t / spr;
+  const int j = (int)(t - p * spr);
+  const float ib = inv_bound[p], in = inv_near[p];
+  const float z = 1.0f / ((float)(offset + j) * ib + in);
+  const float nz = 1.0f / ((float)(offset + j + 1) * ib + in);
+  steps[t] = fmaxx(nz - z, 0.0f);
+  const int64_t py = p / W, px = p - py * W;
+  const float* d = rays_d + py * d_row_stride + px * d_col_stride;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) points[t * 3 + c] = origin[c] + z * d[c];
+}
+
+// A5: 16 samples per pixel; skip pixels with T < early_stop; alpha = 1 - exp(-sigma*step),
+// C += alpha*T*rgb, T *= 1 - alpha; T and colours updated in place
+// (rendering/src/volume_rendering.cu:5-58).
+__global__ __launch_bounds__(256) void k_integrate_points(const float4* __restrict__ res, const float* __restrict__ steps,
+                                                          int64_t npix, float early, float* __restrict__ T,
+                                                          float* __restrict__ color) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  float t = T[p];
+  if (t < early) return;
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const float4 v = res[p * 16 + i];
+    const float alpha = 1.0f - exp_cw(-1.0f * v.w * steps[p * 16 + i]);
+    const float at = alpha * t;
+    c0 = c0 + at * v.x;
+    c1 = c1 + at * v.y;
+    c2 = c2 + at * v.z;
+    t = t * (1.0f - alpha);
+  }
+  T[p] = t;
+  color[p * 3] += c0;
+  color[p * 3 + 1] += c1;
+  color[p * 3 + 2] += c2;
+}
+
+// A8: start of every run of equal keys in a sorted array: flag[i] = i == 0 || keys[i] != keys[i-1]
+__global__ __launch_bounds__(256) void k_run_flags(const int16_t* __restrict__ keys, int64_t n, int32_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+// compacts the run heads: keys_out[r] = keys[i], starts_out[r] = starts_in[i] for run head i with
+// exclusive rank r; the last thread also writes the run count
+__global__ __launch_bounds__(256) void k_run_compact(const int16_t* __restrict__ keys, const int32_t* __restrict__ starts_in,
+                                                     const int32_t* __restrict__ flag, const int32_t* __restrict__ rank,
+                                                     int64_t n, int16_t* __restrict__ keys_out,
+                                                     int32_t* __restrict__ starts_out, int32_t* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flag[i]) {
+    keys_out[rank[i]] = keys[i];
+    starts_out[rank[i]] = starts_in[i];
+  }
+  if (i == n - 1) *count = rank[i] + flag[i];
+}
+
+// A8: assign_pixels_kernel (rendering/src/reorder.cu:25-67): group i's pixel range is split into
+// block_starts[i+1]-block_starts[i] chunks whose sizes differ by at most one (larger first)
+__global__ __launch_bounds__(256) void k_assign_pixels(const int16_t* __restrict__ net, const int32_t* __restrict__ pixel_starts,
+                                                       const int32_t* __restrict__ block_starts, int64_t nnet,
+                                                       int16_t* __restrict__ new_net, int32_t* __restrict__ new_starts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnet) return;
+  const int start_loc = block_starts[i];
+  int ps = pixel_starts[i];
+  const int npx = pixel_starts[i + 1] - ps;
+  const int nb = block_starts[i + 1] - start_loc;
+  if (nb <= 0) return;
+  const int base = npx / nb, add = npx - base * nb;
+  for (int b = 0; b < nb; ++b) {
+    new_net[start_loc + b] = net[i];
+    new_starts[start_loc + b] = ps;
+    ps += (b < add) ? base + 1 : base;
+  }
+}
+
+}  // namespace snisr
+
+using namespace snisr;
+
+extern "C" {
+
+int snisr_inverse_z_sampling(const float* origin, const float* rays_d, int64_t d_row_stride, int64_t d_col_stride,
+                             int32_t H, int32_t W, const float* inverse_near, const float* inverse_bound, int32_t offset,
+                             int32_t sample_per_ray, float* sample_points, float* integrate_steps, void* stream) {
+  SNISR_REQUIRE(origin && rays_d && inverse_near && inverse_bound && sample_points && integrate_steps, SNISR_ERR_ARG,
+                "inverse_z_sampling: NULL pointer");
+  SNISR_REQUIRE(H >= 0 && W >= 0 && sample_per_ray > 0, SNISR_ERR_ARG, "inverse_z_sampling: bad size");
+  const int64_t total = (int64_t)H * W * sample_per_ray;
+  if (total == 0) return SNISR_OK;
+  hipLaunchKernelGGL(k_inverse_z, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, origin,
+                     rays_d, d_row_stride, d_col_stride, W, inverse_near, inverse_bound, (int64_t)H * W, offset,
+                     sample_per_ray, sample_points, integrate_steps);
+  SNISR_LAUNCHED();
+  return SNISR_OK;
+}
+
+int snisr_integrate_points(const float* sample_results, const float* integrate_steps, int64_t npix, float early_stop,
+                           float* transparency, float* colors, void* stream) {
+  SNISR_REQUIRE(sample_results && integrate_steps && transparency && colors, SNISR_ERR_ARG,
+                "integrate_points: NULL pointer");
+  SNISR_REQUIRE(aligned(sample_results, 16), SNISR_ERR_ALIGN, "integrate_points: sample_results must be 16-byte aligned");
+  SNISR_REQUIRE(npix >= 0, SNISR_ERR_ARG, "integrate_points: bad size");
+  if (npix == 0) return SNISR_OK;
+  hipLaunchKernelGGL(k_integrate_points, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     (const float4*)sample_results, integrate_steps, npix, early_stop, transparency, colors);
+  SNISR_LAUNCHED();
+  return SNISR_OK;
+}
+
+// Workspace bytes of snisr_sort_by_key for n keys.
+int64_t snisr_sort_by_key_workspace_size(int64_t n) {
+  size_t tmp = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const int16_t*)nullptr, (int16_t*)nullptr,
+                                         (const int32_t*)nullptr, (int32_t*)nullptr, (int)n) != hipSuccess)
+    return -1;
+  size_t tmp2 = 0;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, (const int32_t*)nullptr, (int32_t*)nullptr, (int)n) != hipSuccess)
+    return -1;
+  const int64_t a = 256;
+  auto up = [&](int64_t v) { return (v + a - 1) / a * a; };
+  return up((int64_t)std::max(tmp, tmp2)) + up(2 * n) + up(4 * n) + 3 * up(4 * n) + up(4);
+}
+
+// A8 sort_by_key (rendering/src/reorder.cu:11-23): stable ascending sort of keys i16[n] carrying
+// values i32[n], then the run heads: keys[0..m) = distinct keys, starts[0..m) = starts[] at the head
+// of each run (the reference passes starts = arange(n), giving each run's first position).  Returns
+// m in *n_unique (device-to-host copy, synchronises the stream as the reference's return value does).
+int snisr_sort_by_key(int16_t* keys, int32_t* values, int32_t* starts, int64_t n, void* workspace,
+                      int64_t workspace_bytes, int32_t* n_unique, void* stream) {
+  SNISR_REQUIRE(keys && values && starts && n_unique && workspace, SNISR_ERR_ARG, "sort_by_key: NULL pointer");
+  SNISR_REQUIRE(n >= 0 && n < (int64_t(1) << 31), SNISR_ERR_ARG, "sort_by_key: bad size");
+  if (n == 0) {
+    *n_unique = 0;
+    return SNISR_OK;
+  }
+  const int64_t need = snisr_sort_by_key_workspace_size(n);
+  SNISR_REQUIRE(need > 0 && workspace_bytes >= need, SNISR_ERR_CAPACITY, "sort_by_key: workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  size_t tmp = 0, tmp2 = 0;
+  SNISR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const int16_t*)nullptr, (int16_t*)nullptr,
+                                               (const int32_t*)nullptr, (int32_t*)nullptr, (int)n));
+  SNISR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, (const int32_t*)nullptr, (int32_t*)nullptr, (int)n));
+  const int64_t a = 256;
+  auto up = [&](int64_t v) { return (v + a - 1) / a * a; };
+  uint8_t* base = (uint8_t*)workspace;
+  void* t = base;
+  int16_t* keys2 = (int16_t*)(base + up((int64_t)std::max(tmp, tmp2)));
+  int32_t* vals2 = (int32_t*)((uint8_t*)keys2 + up(2 * n));
+  int32_t* flag = (int32_t*)((uint8_t*)vals2 + up(4 * n));
+  int32_t* rank = (int32_t*)((uint8_t*)flag + up(4 * n));
+  int32_t* starts2 = (int32_t*)((uint8_t*)rank + up(4 * n));
+  int32_t* cnt = (int32_t*)((uint8_t*)starts2 + up(4 * n));
+  size_t ts = std::max(tmp, tmp2);
+  SNISR_HIP(hipcub::DeviceRadixSort::SortPairs(t, ts, keys, keys2, values, vals2, (int)n, 0, 16, st));
+  SNISR_HIP(hipMemcpyAsync(values, vals2, 4 * n, hipMemcpyDeviceToDevice, st));
+  const dim3 g((unsigned)((n + 255) / 256));
+  hipLaunchKernelGGL(k_run_flags, g, dim3(256), 0, st, keys2, n, flag);
+  SNISR_LAUNCHED();
+  ts = std::max(tmp, tmp2);
+  SNISR_HIP(hipcub::DeviceScan::ExclusiveSum(t, ts, flag, rank, (int)n, st));
+  SNISR_HIP(hipMemcpyAsync(starts2, starts, 4 * n, hipMemcpyDeviceToDevice, st));
+  // keys keep the sorted order beyond the first m entries, as thrust::unique_by_key leaves them
+  SNISR_HIP(hipMemcpyAsync(keys, keys2, 2 * n, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_run_compact, g, dim3(256), 0, st, keys2, starts2, flag, rank, n, keys, starts, cnt);
+  SNISR_LAUNCHED();
+  SNISR_HIP(hipMemcpyAsync(n_unique, cnt, 4, hipMemcpyDeviceToHost, st));
+  SNISR_HIP(hipStreamSynchronize(st));
+  return SNISR_OK;
+}
+
+int snisr_assign_pixels(const int16_t* net, const int32_t* pixel_starts, const int32_t* block_starts, int64_t n_net,
+                        int16_t* new_net, int32_t* new_pixel_starts, void* stream) {
+  SNISR_REQUIRE(net && pixel_starts && block_starts && new_net && new_pixel_starts, SNISR_ERR_ARG,
+                "assign_pixels: NULL pointer");
+  if (n_net <= 0) return SNISR_OK;
+  hipLaunchKernelGGL(k_assign_pixels, dim3((unsigned)((n_net + 255) / 256)), dim3(256), 0, (hipStream_t)stream, net,
+                     pixel_starts, block_starts, n_net, new_net, new_pixel_starts);
+  SNISR_LAUNCHED();
+  return SNISR_OK;
+}
+
+}  // extern "C"

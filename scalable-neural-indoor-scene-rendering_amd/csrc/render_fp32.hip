@@ -155,7 +155,12 @@ int snisr_compute_rays(const float* intrinsic, const float* c2w, int32_t H, int3
 
 int snisr_trace_tiles(const snisr_scene* scene, const float* origin, const float* rays_d, int64_t n, int32_t K,
                       int32_t* visited, void* stream) {
-  if (int e = check_scene(scene, false)) return e;
+  // only the tile grid is read: tile_shape, scene_min, tile_size, tile_index_map
+  SNISR_REQUIRE(scene, SNISR_ERR_ARG, "scene descriptor is NULL");
+  SNISR_REQUIRE(scene->tile_shape[0] > 0 && scene->tile_shape[1] > 0 && scene->tile_shape[2] > 0, SNISR_ERR_ARG,
+                "tile_shape must be positive");
+  SNISR_REQUIRE(scene->tile_size > 0.0f, SNISR_ERR_ARG, "tile_size must be positive");
+  SNISR_REQUIRE(scene->tile_index_map, SNISR_ERR_ARG, "tile_index_map is NULL");
   SNISR_REQUIRE(origin && (n == 0 || (rays_d && visited)), SNISR_ERR_ARG, "trace_tiles: NULL pointer");
   SNISR_REQUIRE(n >= 0 && K >= 0, SNISR_ERR_ARG, "trace_tiles: negative size");
   if (n == 0 || K == 0) return SNISR_OK;

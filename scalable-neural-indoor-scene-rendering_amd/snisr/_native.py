@@ -62,6 +62,16 @@ SIGNATURES = [
                                   C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("snisr_mesh_first_hit", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int32,
                                        C.POINTER(RaysDesc), C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("snisr_inverse_z_sampling", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
+                                           C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
+    ("snisr_integrate_points", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]),
+    ("snisr_sort_by_key_workspace_size", C.c_int64, [C.c_int64]),
+    ("snisr_sort_by_key", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                    C.POINTER(C.c_int32), C.c_void_p]),
+    ("snisr_assign_pixels", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
     ("snisr_train_workspace_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int64, C.c_int64]),
     ("snisr_train_forward", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
                                       C.POINTER(RenderOut), C.c_void_p, C.c_int64, C.c_void_p]),

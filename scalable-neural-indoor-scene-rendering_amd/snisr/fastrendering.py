@@ -1,0 +1,167 @@
+"""Operator-for-operator stand-in for the reference's ``FASTRENDERING`` extension module.
+
+Same names, argument order and in-place semantics as the pybind11 module of
+rendering/src/binding.cpp:13-27 (signatures in rendering/src/include/*.h), so a driver written
+like ``FRender.render`` (rendering/src/__init__.py:247-341) can import this module in its place.
+Every op runs in the HIP kernels of libsnisr_hip.so through the C ABI (include/snisr.h); there is
+no CPU path.
+
+Differences from the reference, all deliberate:
+
+* ops launch on torch's current stream (the reference uses the legacy default stream);
+* ``num_thread`` is accepted for signature compatibility and ignored: launch shapes are chosen by
+  the library;
+* outputs must be contiguous tensors of the stated dtype on the GPU.  The reference writes a
+  non-contiguous output through ``.contiguous()`` into a temporary and silently loses the result;
+  here that raises ``ValueError``;
+* failures raise ``RuntimeError`` carrying the library's message, as ``AT_CUDA_CHECK`` does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .device import _ptr, _require_cuda, _stream
+
+
+def _host_floats(t, n: int, name: str) -> list[float]:
+    v = t.detach().reshape(-1).to("cpu", torch.float32).tolist() if isinstance(t, torch.Tensor) else list(t)
+    if len(v) != n:
+        raise ValueError(f"{name} must hold {n} values")
+    return [float(x) for x in v]
+
+
+def _host_ints(t, n: int, name: str) -> list[int]:
+    v = t.detach().reshape(-1).to("cpu", torch.int64).tolist() if isinstance(t, torch.Tensor) else list(t)
+    if len(v) != n:
+        raise ValueError(f"{name} must hold {n} values")
+    return [int(x) for x in v]
+
+
+def compute_rays(intrinsic: torch.Tensor, c2w_rotation: torch.Tensor, num_thread: int, rays_d: torch.Tensor) -> None:
+    """rendering/src/compute_rays.cu:46-68: rays_d[y, x] = R [(x - cx) / f, (y - cy) / f, 1] with
+    intrinsic = (cx, cy, 1/f); H, W from rays_d's shape; directions are not normalised."""
+    _require_cuda(rays_d, "rays_d", torch.float32)
+    if rays_d.dim() != 3 or rays_d.shape[2] != 3:
+        raise ValueError("rays_d must be [H, W, 3]")
+    intr = intrinsic.to(rays_d.device, torch.float32).contiguous()
+    rot = c2w_rotation.to(rays_d.device, torch.float32).contiguous()
+    if intr.numel() != 3 or rot.numel() != 9:
+        raise ValueError("intrinsic must hold 3 values and c2w_rotation 9")
+    H, W = rays_d.shape[0], rays_d.shape[1]
+    N.check(N.lib().snisr_compute_rays(_ptr(intr), _ptr(rot), H, W, _ptr(rays_d), _stream()), "compute_rays")
+
+
+def tracing_tiles(rays_d: torch.Tensor, IndexMap: torch.Tensor, origin: torch.Tensor, tile_shape, scene_min_corner,
+                  scene_size, tile_size: float, num_thread: int, visitedTiles: torch.Tensor) -> None:
+    """rendering/src/raytile_tracing.cu:63-95: the occupied tiles each ray crosses, in order,
+    -1-padded to visitedTiles.shape[-1] entries.  scene_size must equal tile_shape * tile_size
+    (as the reference's callers build it); the library derives it from those two."""
+    _require_cuda(visitedTiles, "visitedTiles", torch.int32)
+    shape = _host_ints(tile_shape, 3, "tile_shape")
+    smin = _host_floats(scene_min_corner, 3, "scene_min_corner")
+    ssz = np.asarray(_host_floats(scene_size, 3, "scene_size"), np.float32)
+    want = np.asarray(shape, np.float32) * np.float32(tile_size)
+    if not np.array_equal(ssz, want):
+        raise ValueError(f"scene_size {ssz.tolist()} differs from tile_shape * tile_size {want.tolist()}")
+    imap = IndexMap.to(visitedTiles.device, torch.int32).contiguous()
+    if imap.numel() != shape[0] * shape[1] * shape[2]:
+        raise ValueError("IndexMap must hold prod(tile_shape) entries")
+    rd = rays_d.to(visitedTiles.device, torch.float32).contiguous()
+    org = origin.to(visitedTiles.device, torch.float32).contiguous()
+    n = rd.numel() // 3
+    K = visitedTiles.shape[-1]
+    if visitedTiles.numel() != n * K:
+        raise ValueError("visitedTiles must be [H, W, K] with H*W rays")
+    d = N.SceneDesc()
+    d.tile_shape[:] = shape
+    d.scene_min[:] = smin
+    d.tile_size = float(tile_size)
+    d.tile_index_map = imap.data_ptr()
+    N.check(N.lib().snisr_trace_tiles(C.byref(d), _ptr(org), _ptr(rd), n, K, _ptr(visitedTiles), _stream()),
+            "tracing_tiles")
+
+
+def inverse_z_sampling(origin: torch.Tensor, rays_d: torch.Tensor, inverse_near: torch.Tensor,
+                       inverse_bound: torch.Tensor, num_thread: int, offset: int, sample_per_ray: int,
+                       sample_points: torch.Tensor, integrate_steps: torch.Tensor) -> None:
+    """rendering/src/sampling.cu:48-81: samples offset .. offset+sample_per_ray-1 of 64 uniform in
+    inverse depth.  rays_d may be a strided view such as ``rays_d[::2, ::2]`` (read in place)."""
+    _require_cuda(sample_points, "sample_points", torch.float32)
+    _require_cuda(integrate_steps, "integrate_steps", torch.float32)
+    H, W = inverse_near.shape[0], inverse_near.shape[1]
+    if rays_d.dim() != 3 or tuple(rays_d.shape) != (H, W, 3) or rays_d.dtype != torch.float32 or not rays_d.is_cuda:
+        raise ValueError("rays_d must be a float32 [H, W, 3] GPU tensor matching inverse_near")
+    if rays_d.stride(2) != 1:
+        rays_d = rays_d.contiguous()
+    inn = inverse_near.to(torch.float32).contiguous()
+    inb = inverse_bound.to(torch.float32).contiguous()
+    if inb.numel() != H * W:
+        raise ValueError("inverse_bound must hold H*W values")
+    if sample_points.numel() != H * W * sample_per_ray * 3 or integrate_steps.numel() != H * W * sample_per_ray:
+        raise ValueError("sample_points / integrate_steps sizes do not match H*W*sample_per_ray")
+    org = origin.to(rays_d.device, torch.float32).contiguous()
+    N.check(N.lib().snisr_inverse_z_sampling(_ptr(org), _ptr(rays_d), rays_d.stride(0), rays_d.stride(1), H, W,
+                                             _ptr(inn), _ptr(inb), int(offset), int(sample_per_ray),
+                                             _ptr(sample_points), _ptr(integrate_steps), _stream()),
+            "inverse_z_sampling")
+
+
+def integrate_points(sample_results: torch.Tensor, intergrate_steps: torch.Tensor, num_thread: int,
+                     early_step: float, transparency: torch.Tensor, colors_buffer: torch.Tensor) -> None:
+    """rendering/src/volume_rendering.cu:60-87: composites 16 samples per pixel into
+    colors_buffer [H, W, 3] and transparency (H*W values), in place."""
+    _require_cuda(transparency, "transparency", torch.float32)
+    _require_cuda(colors_buffer, "colors_buffer", torch.float32)
+    npix = colors_buffer.shape[0] * colors_buffer.shape[1]
+    res = sample_results.to(torch.float32).contiguous()
+    st = intergrate_steps.to(torch.float32).contiguous()
+    if res.numel() != npix * 64 or st.numel() != npix * 16 or transparency.numel() != npix:
+        raise ValueError("integrate_points expects 16 samples per pixel")
+    N.check(N.lib().snisr_integrate_points(_ptr(res), _ptr(st), npix, float(early_step), _ptr(transparency),
+                                           _ptr(colors_buffer), _stream()), "integrate_points")
+
+
+_sort_ws: dict = {}
+
+
+def sort_by_key(keys_tensor: torch.Tensor, values_tensor: torch.Tensor, starts_tensor: torch.Tensor) -> int:
+    """rendering/src/reorder.cu:11-23: stable sort of int16 keys carrying int32 values, then the
+    run heads; returns the number of distinct keys m (keys[:m], starts[:m] hold the runs)."""
+    _require_cuda(keys_tensor, "keys", torch.int16)
+    _require_cuda(values_tensor, "values", torch.int32)
+    _require_cuda(starts_tensor, "starts", torch.int32)
+    n = keys_tensor.numel()
+    if values_tensor.numel() != n or starts_tensor.numel() != n:
+        raise ValueError("keys, values and starts must have the same length")
+    need = N.lib().snisr_sort_by_key_workspace_size(n)
+    if need < 0:
+        raise RuntimeError("sort_by_key: workspace query failed")
+    dev = keys_tensor.device
+    ws = _sort_ws.get(dev)
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(max(need, 1 << 16), dtype=torch.uint8, device=dev)
+        _sort_ws[dev] = ws
+    m = C.c_int32(0)
+    N.check(N.lib().snisr_sort_by_key(_ptr(keys_tensor), _ptr(values_tensor), _ptr(starts_tensor), n, _ptr(ws),
+                                      ws.numel(), C.byref(m), _stream()), "sort_by_key")
+    return m.value
+
+
+def assign_pixels(netIdxs: torch.Tensor, pixel_starts: torch.Tensor, block_starts: torch.Tensor, num_thread: int,
+                  new_netIdxs: torch.Tensor, new_pixel_starts: torch.Tensor) -> None:
+    """rendering/src/reorder.cu:69-94: splits each group's pixel range into
+    block_starts[i+1]-block_starts[i] chunks (sizes differing by at most one, larger first)."""
+    _require_cuda(new_netIdxs, "new_netIdxs", torch.int16)
+    _require_cuda(new_pixel_starts, "new_pixel_starts", torch.int32)
+    net = netIdxs.to(torch.int16).contiguous()
+    ps = pixel_starts.to(torch.int32).contiguous()
+    bs = block_starts.to(torch.int32).contiguous()
+    n = net.numel()
+    if ps.numel() < n + 1 or bs.numel() < n + 1:
+        raise ValueError("pixel_starts and block_starts need numNet + 1 entries")
+    N.check(N.lib().snisr_assign_pixels(_ptr(net), _ptr(ps), _ptr(bs), n, _ptr(new_netIdxs), _ptr(new_pixel_starts),
+                                        _stream()), "assign_pixels")

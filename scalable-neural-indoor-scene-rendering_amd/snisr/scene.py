@@ -287,6 +287,35 @@ def config_c2(seed: int = 2, H: int = 720, W: int = 1280, n_triangles: int = 200
     return sc, cam, mesh
 
 
+def config_c3_rays(sc: Scene, seed: int = 3, rays_per_tile: int = 8192, room_lo=(0.0, 0.0, 0.0),
+                   room_hi=(4.0, 4.0, 1.0)):
+    """Training batch of the 16-tile room: for every tile, ``rays_per_tile`` rays from seeded points
+    in the room towards seeded points inside that tile (origins f32[N,3], dirs f32[N,3])."""
+    rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence(seed).spawn(9)[8]))
+    lo = np.asarray(room_lo, np.float64)
+    hi = np.asarray(room_hi, np.float64)
+    o, d = [], []
+    for t in range(sc.n_tiles):
+        c = sc.tile_centers[t].astype(np.float64)
+        org = lo + (hi - lo) * rng.uniform(0.02, 0.98, (rays_per_tile, 3))
+        tgt = c + rng.uniform(-0.5, 0.5, (rays_per_tile, 3)) * sc.tile_size
+        o.append(org)
+        d.append(tgt - org)
+    return np.concatenate(o).astype(np.float32), np.concatenate(d).astype(np.float32)
+
+
+def perturbed_copy(sc: Scene, seed: int, rel: float = 0.01) -> Scene:
+    """Target scene of the training config: features and MLP weights perturbed by
+    N(0, rel^2) * |value| (density unchanged)."""
+    rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence(seed).spawn(10)[9]))
+    c = dataclasses.replace(sc)
+    f = sc.feats.astype(np.float32)
+    c.feats = (f + rng.standard_normal(f.shape, dtype=np.float32) * np.float32(rel) * np.abs(f)).astype(sc.feats.dtype)
+    m = sc.mlp_params
+    c.mlp_params = (m + rng.standard_normal(m.shape, dtype=np.float32) * np.float32(rel) * np.abs(m)).astype(np.float32)
+    return c
+
+
 def config_c5(seed: int = 5, H: int = 1080, W: int = 1920, n_views: int = 4, n_triangles: int = 2_000_000):
     """256 tiles (16x16 floor plan) of 128^3 at 4%, F=32 FP16, width-128x4 MLP, 2M-triangle mesh,
     n_views seeded 1920x1080 views."""
