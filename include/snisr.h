@@ -116,7 +116,7 @@ typedef struct snisr_render_out {
 
 const char* snisr_last_error(void);
 int snisr_version(void);
-/* sizeof of the descriptors (0 scene, 1 march params, 2 rays, 3 render out) so that foreign
+/* sizeof of the descriptors (0 scene, 1 march params, 2 rays, 3 render out, 4 mlp query) so that foreign
  * bindings can check their struct layouts; no reference counterpart. */
 int64_t snisr_abi_sizeof(int32_t which);
 
@@ -201,6 +201,36 @@ int snisr_sort_by_key(int16_t* keys, int32_t* values, int32_t* starts, int64_t n
 /* A8 — assign_pixels (rendering/src/reorder.cu:69-94, include/reorder.h:10-15). */
 int snisr_assign_pixels(const int16_t* net, const int32_t* pixel_starts, const int32_t* block_starts, int64_t n_net,
                         int16_t* new_net, int32_t* new_pixel_starts, void* stream);
+
+/* A7 — inference_mlp (rendering/src/rayreflection_render.cu:111-335, include/rayreflection_render.h:7-19).
+ * The reflection MLP of every sample of the listed pixel blocks:  block b covers sorted pixels
+ * [pixel_starts[b], pixel_starts[b+1]) of query_pixel_indices and uses group net_idx[b] (-1: skip);
+ * a pixel whose transparency is below trans_stop is skipped; sample j of original pixel o reads
+ * samples[o*spr + j] and writes sample_results[o*spr + j] = (rgb, sigma).  Parameters come packed by
+ * snisr_pack_inference_mlp from the reference's 22,556 halves per group.  FP16 MFMA operands, FP32
+ * accumulation. */
+typedef struct snisr_mlp_query {
+  int32_t samples_per_ray;            /* max_samples_per_ray: samples per pixel in this call */
+  int32_t n_groups;                   /* groups in packed_params / group_centers */
+  int64_t n_blocks;                   /* num_block: entries of net_idx; pixel_starts holds n_blocks + 1 */
+  float inverse_far, trans_stop;
+  const int16_t* net_idx;             /* [n_blocks] */
+  const int32_t* pixel_starts;        /* [n_blocks + 1], positions in query_pixel_indices */
+  const int32_t* query_pixel_indices; /* [n_query] sorted position -> original pixel (y*W + x) */
+  int64_t n_query;
+  const float* transparency;          /* [H*W] */
+  const float* group_centers;         /* [n_groups, 3] */
+  const float* samples;               /* [H*W, samples_per_ray, 3] */
+  const float* rays_d;                /* element (y, x, c) at rays_d[y*d_row_stride + x*d_col_stride + c] */
+  int64_t d_row_stride, d_col_stride;
+  int32_t H, W;
+  const void* packed_params;          /* snisr_pack_inference_mlp output, 16-byte aligned */
+  float* sample_results;              /* [H*W, samples_per_ray, 4], 16-byte aligned; skipped samples untouched */
+} snisr_mlp_query;
+
+int64_t snisr_inference_mlp_packed_size(int64_t n_groups);
+int snisr_pack_inference_mlp(const void* params_f16, int64_t n_groups, void* packed, void* stream);
+int snisr_inference_mlp(const snisr_mlp_query* query, void* stream);
 
 /* A11 — training (FP32 features and MLP).  snisr_train_forward renders like snisr_render_fp32
  * (identical traversal and samples) and keeps per-sample records in the workspace;

@@ -47,6 +47,8 @@ def lib():
         _lib.orc_num_threads.restype = C.c_int
         _lib.orc_half_to_float_c.restype = C.c_float
         _lib.orc_half_to_float_c.argtypes = [C.c_uint16]
+        _lib.orc_round_half_f64.restype = C.c_double
+        _lib.orc_round_half_f64.argtypes = [C.c_double]
     return _lib
 
 
@@ -239,6 +241,44 @@ def assign_pixels(net, pixel_starts, block_starts):
     new_starts = np.empty(nb, np.int32)
     lib().orc_assign_pixels(_p(net), _p(ps), _p(bs), C.c_int(net.shape[0]), _p(new_net), _p(new_starts))
     return new_net, new_starts
+
+
+REF_MLP_PARAMS = 22556  # halves per group: per layer bias[out] then W[in][out]; 63-64-64-64-64-64-28
+
+
+def inference_mlp(spr, net_idx, pixel_starts, query_pixel_indices, transparency, group_centers, samples, rays_d,
+                  params_f16, inverse_far, trans_stop, results, quant16=False):
+    """FP64 restatement of inference_mlp_kernel; writes results f32[H*W*spr, 4] in place.
+    rays_d is f32[H, W, 3] (any strides of a numpy view are honoured)."""
+    net = np.ascontiguousarray(net_idx, np.int16)
+    ps = np.ascontiguousarray(pixel_starts, np.int32)
+    q = np.ascontiguousarray(query_pixel_indices, np.int32)
+    T = np.ascontiguousarray(transparency, np.float32).ravel()
+    ctr = np.ascontiguousarray(group_centers, np.float32)
+    smp = np.ascontiguousarray(samples, np.float32)
+    rd = np.asarray(rays_d, np.float32)
+    H, W = rd.shape[0], rd.shape[1]
+    if rd.strides[2] != 4:
+        rd = np.ascontiguousarray(rd)
+    prm = np.ascontiguousarray(params_f16, np.float16).view(np.uint16)
+    n_groups = prm.size // REF_MLP_PARAMS
+    assert results.dtype == np.float32 and results.flags.c_contiguous
+    lib().orc_inference_mlp_f64(C.c_int(spr), C.c_longlong(net.size), C.c_float(inverse_far), C.c_float(trans_stop),
+                                _p(net), _p(T), _p(q), _p(ps), _p(ctr), C.c_int(n_groups), _p(smp),
+                                C.c_void_p(rd.ctypes.data), C.c_longlong(rd.strides[0] // 4),
+                                C.c_longlong(rd.strides[1] // 4), C.c_int(W), C.c_longlong(H * W), _p(prm),
+                                C.c_int(1 if quant16 else 0), _p(results))
+
+
+def sh2_basis(dirs):
+    d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    out = np.empty((d.shape[0], 9), np.float64)
+    lib().orc_sh2_basis_f64(_p(d), C.c_longlong(d.shape[0]), _p(out))
+    return out
+
+
+def round_half(v: float) -> float:
+    return lib().orc_round_half_f64(C.c_double(v))
 
 
 def train_grad(osc: OracleScene, origins, dirs, g, t_hit=None, precision="f64", **pkw):

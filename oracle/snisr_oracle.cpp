@@ -591,6 +591,113 @@ static void mesh_hit_one(const float* V, const int* Fc, int nf, V3<R> o, V3<R> d
   *id_out = bid;
 }
 
+// A7: inference_mlp_kernel (rendering/src/rayreflection_render.cu:111-300) with inference_SH
+// (61-108), leakyRelu (38-49) and Sigmoid (51-58), in FP64.  Per sample: x = (p - centre) *
+// inverse_far in FP32 as the reference computes it, positional encoding of L = 10 bands
+// (x, sin 2^k x, cos 2^k x; 63 inputs), 63 -> 64 -> 4 x (64 -> 64) with LeakyReLU(0.01) -> 28;
+// sigma = max(out[27], 0); rgb = sigmoid(SH degree 2 at the normalised ray direction with
+// coefficient r = 3 * basis + channel).  Parameters per group: 22,556 halves, each layer's bias
+// (out) followed by W as [in][out].  quant16 != 0 rounds the 63 inputs and every hidden activation
+// to FP16 (round to nearest even) where the MFMA kernel's operands are FP16; 0 is exact math on
+// the FP16 parameters.
+static inline double round_half(double v) {
+  if (!(std::fabs(v) < 65520.0)) return std::isnan(v) ? v : std::copysign(INFINITY, v);
+  int e;
+  std::frexp(v, &e);                        // |v| = m 2^e, m in [0.5, 1)
+  const int q = std::max(e - 11, -24);      // 11 significant bits; subnormal quantum 2^-24
+  return std::ldexp(std::nearbyint(std::ldexp(v, -q)), q);
+}
+
+constexpr int kRefIn = 63, kRefWidth = 64, kRefOut = 28, kRefLayers = 6, kRefParams = 22556;
+
+static void sh2_basis(const double* d, double* b) {
+  const double pi = 3.14159265358979323846;
+  const double c0 = 0.5 / std::sqrt(pi), c1 = std::sqrt(3.0) * c0, c2 = std::sqrt(15.0) * c0,
+               c22 = std::sqrt(5.0) * 0.5 * c0, c24 = std::sqrt(15.0) * 0.5 * c0;
+  const double x = d[0], y = d[1], z = d[2];
+  b[0] = c0;
+  b[1] = -c1 * y; b[2] = c1 * z; b[3] = -c1 * x;
+  b[4] = c2 * x * y; b[5] = -c2 * y * z; b[6] = c22 * (2.0 * z * z - x * x - y * y);
+  b[7] = -c2 * x * z; b[8] = c24 * (x * x - y * y);
+}
+
+static void ref_mlp_sample(const uint16_t* P, const float* point, const double* dir_unit, int quant16, float* out4) {
+  double a[kRefWidth], b[kRefWidth];
+  a[0] = point[0]; a[1] = point[1]; a[2] = point[2];
+  for (int k = 0; k < 10; ++k)
+    for (int r = 0; r < 3; ++r) {
+      const double t = std::ldexp((double)point[r], k);
+      a[3 + 6 * k + r] = std::sin(t);
+      a[6 + 6 * k + r] = std::cos(t);
+    }
+  if (quant16)
+    for (int i = 0; i < kRefIn; ++i) a[i] = round_half(a[i]);
+  int off = 0, in = kRefIn;
+  double* x = a;
+  double* y = b;
+  for (int l = 0; l < kRefLayers; ++l) {
+    const int out = l == kRefLayers - 1 ? kRefOut : kRefWidth;
+    const uint16_t* bias = P + off;
+    const uint16_t* Wt = bias + out;  // [in][out]
+    for (int j = 0; j < out; ++j) {
+      double acc = half_to_float(bias[j]);
+      for (int i = 0; i < in; ++i) acc += x[i] * (double)half_to_float(Wt[i * out + j]);
+      y[j] = acc;
+    }
+    if (l < kRefLayers - 1)
+      for (int j = 0; j < out; ++j) {
+        y[j] = y[j] < 0.0 ? 0.01 * y[j] : y[j];
+        if (quant16) y[j] = round_half(y[j]);
+      }
+    off += out + in * out;
+    in = out;
+    std::swap(x, y);
+  }
+  double basis[9];
+  sh2_basis(dir_unit, basis);
+  for (int c = 0; c < 3; ++c) {
+    double raw = 0.0;
+    for (int k = 0; k < 9; ++k) raw += basis[k] * x[3 * k + c];
+    out4[c] = (float)(1.0 / (1.0 + std::exp(-raw)));
+  }
+  out4[3] = (float)std::max(x[kRefOut - 1], 0.0);
+}
+
+}  // namespace orc
+extern "C" {
+
+void orc_inference_mlp_f64(int spr, long long nblocks, float inverse_far, float trans_stop, const int16_t* net,
+                           const float* T, const int* qpi, const int* pixel_starts, const float* centers,
+                           int n_groups, const float* samples, const float* rays_d, long long d_row, long long d_col,
+                           int W, long long npix, const uint16_t* params, int quant16, float* results) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long long b = 0; b < nblocks; ++b) {
+    const int g = net[b];
+    if (g < 0 || g >= n_groups) continue;
+    const uint16_t* P = params + (long long)g * orc::kRefParams;
+    for (long long t = (long long)pixel_starts[b] * spr; t < (long long)pixel_starts[b + 1] * spr; ++t) {
+      const long long ori = qpi[t / spr];
+      if (ori < 0 || ori >= npix) continue;
+      if (T[ori] < trans_stop) continue;
+      const long long task = ori * spr + t % spr;
+      float p[3];
+      for (int i = 0; i < 3; ++i) p[i] = (samples[task * 3 + i] - centers[g * 3 + i]) * inverse_far;
+      const float* d = rays_d + (ori / W) * d_row + (ori % W) * d_col;
+      const double n = std::sqrt((double)d[0] * d[0] + (double)d[1] * d[1] + (double)d[2] * d[2]);
+      const double du[3] = {d[0] / n, d[1] / n, d[2] / n};
+      orc::ref_mlp_sample(P, p, du, quant16, results + task * 4);
+    }
+  }
+}
+
+void orc_sh2_basis_f64(const double* dirs, long long n, double* out) {
+  for (long long i = 0; i < n; ++i) orc::sh2_basis(dirs + 3 * i, out + 9 * i);
+}
+
+double orc_round_half_f64(double v) { return orc::round_half(v); }
+
+}  // extern "C"
+namespace orc {
 }  // namespace orc
 
 using namespace orc;

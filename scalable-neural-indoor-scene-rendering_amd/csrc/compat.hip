@@ -105,11 +105,11 @@ extern "C" {
 int snisr_inverse_z_sampling(const float* origin, const float* rays_d, int64_t d_row_stride, int64_t d_col_stride,
                              int32_t H, int32_t W, const float* inverse_near, const float* inverse_bound, int32_t offset,
                              int32_t sample_per_ray, float* sample_points, float* integrate_steps, void* stream) {
-  SNISR_REQUIRE(origin && rays_d && inverse_near && inverse_bound && sample_points && integrate_steps, SNISR_ERR_ARG,
-                "inverse_z_sampling: NULL pointer");
   SNISR_REQUIRE(H >= 0 && W >= 0 && sample_per_ray > 0, SNISR_ERR_ARG, "inverse_z_sampling: bad size");
   const int64_t total = (int64_t)H * W * sample_per_ray;
   if (total == 0) return SNISR_OK;
+  SNISR_REQUIRE(origin && rays_d && inverse_near && inverse_bound && sample_points && integrate_steps, SNISR_ERR_ARG,
+                "inverse_z_sampling: NULL pointer");
   hipLaunchKernelGGL(k_inverse_z, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, origin,
                      rays_d, d_row_stride, d_col_stride, W, inverse_near, inverse_bound, (int64_t)H * W, offset,
                      sample_per_ray, sample_points, integrate_steps);
@@ -119,11 +119,11 @@ int snisr_inverse_z_sampling(const float* origin, const float* rays_d, int64_t d
 
 int snisr_integrate_points(const float* sample_results, const float* integrate_steps, int64_t npix, float early_stop,
                            float* transparency, float* colors, void* stream) {
+  SNISR_REQUIRE(npix >= 0, SNISR_ERR_ARG, "integrate_points: bad size");
+  if (npix == 0) return SNISR_OK;
   SNISR_REQUIRE(sample_results && integrate_steps && transparency && colors, SNISR_ERR_ARG,
                 "integrate_points: NULL pointer");
   SNISR_REQUIRE(aligned(sample_results, 16), SNISR_ERR_ALIGN, "integrate_points: sample_results must be 16-byte aligned");
-  SNISR_REQUIRE(npix >= 0, SNISR_ERR_ARG, "integrate_points: bad size");
-  if (npix == 0) return SNISR_OK;
   hipLaunchKernelGGL(k_integrate_points, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                      (const float4*)sample_results, integrate_steps, npix, early_stop, transparency, colors);
   SNISR_LAUNCHED();
@@ -150,12 +150,13 @@ int64_t snisr_sort_by_key_workspace_size(int64_t n) {
 // m in *n_unique (device-to-host copy, synchronises the stream as the reference's return value does).
 int snisr_sort_by_key(int16_t* keys, int32_t* values, int32_t* starts, int64_t n, void* workspace,
                       int64_t workspace_bytes, int32_t* n_unique, void* stream) {
-  SNISR_REQUIRE(keys && values && starts && n_unique && workspace, SNISR_ERR_ARG, "sort_by_key: NULL pointer");
+  SNISR_REQUIRE(n_unique, SNISR_ERR_ARG, "sort_by_key: n_unique is NULL");
   SNISR_REQUIRE(n >= 0 && n < (int64_t(1) << 31), SNISR_ERR_ARG, "sort_by_key: bad size");
   if (n == 0) {
     *n_unique = 0;
     return SNISR_OK;
   }
+  SNISR_REQUIRE(keys && values && starts && workspace, SNISR_ERR_ARG, "sort_by_key: NULL pointer");
   const int64_t need = snisr_sort_by_key_workspace_size(n);
   SNISR_REQUIRE(need > 0 && workspace_bytes >= need, SNISR_ERR_CAPACITY, "sort_by_key: workspace too small");
   hipStream_t st = (hipStream_t)stream;
@@ -193,9 +194,9 @@ int snisr_sort_by_key(int16_t* keys, int32_t* values, int32_t* starts, int64_t n
 
 int snisr_assign_pixels(const int16_t* net, const int32_t* pixel_starts, const int32_t* block_starts, int64_t n_net,
                         int16_t* new_net, int32_t* new_pixel_starts, void* stream) {
+  if (n_net <= 0) return SNISR_OK;
   SNISR_REQUIRE(net && pixel_starts && block_starts && new_net && new_pixel_starts, SNISR_ERR_ARG,
                 "assign_pixels: NULL pointer");
-  if (n_net <= 0) return SNISR_OK;
   hipLaunchKernelGGL(k_assign_pixels, dim3((unsigned)((n_net + 255) / 256)), dim3(256), 0, (hipStream_t)stream, net,
                      pixel_starts, block_starts, n_net, new_net, new_pixel_starts);
   SNISR_LAUNCHED();

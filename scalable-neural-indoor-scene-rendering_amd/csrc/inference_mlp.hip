@@ -1,0 +1,288 @@
+// A7 — the reference-format reflection MLP (inference_mlp) on v_mfma_f32_32x32x16_f16.
+//
+// Per sample: x = (p - group centre) * inverse_far, positional encoding with 10 bands (63 inputs),
+// 63 -> 64 -> 4 x (64 -> 64) LeakyReLU(0.01) -> 28, sigma = max(out[27], 0), rgb = sigmoid of the
+// degree-2 spherical-harmonics colour at the normalised ray direction with coefficient
+// r = 3 * basis + channel (rendering/src/rayreflection_render.cu:111-300, inference_SH 61-108).
+// The reference evaluates it with scalar FP16 FMAs and FP16 accumulation; here weights and
+// activations are FP16 MFMA operands, accumulation, LeakyReLU, SH and sigmoid are FP32.
+//
+// Work split: the blocks of pixels (net_idx[b], pixel range [pixel_starts[b], pixel_starts[b+1]) of
+// the sorted query list) are cut into equal contiguous spans, one per workgroup.  A workgroup stages
+// its current group's packed weights (46,464 B) in LDS and reloads them only when the group changes
+// (blocks arrive sorted by group).  Pixels whose transparency is below trans_stop are compacted away
+// with a wave ballot, so every MFMA column is a live sample.  One wavefront evaluates 64 samples:
+// two 32-column blocks; all six layers keep their activations in registers in the B-operand layout.
+//
+// Positional encoding: lane half h = 0 holds x, y and bands 0..4, h = 1 holds z and bands 5..9
+// (weights of layer 0 are packed in that order).  Each half evaluates sincos once per coordinate,
+// at 2^(5h) x, then four double-angle steps (sin 2a = 2 sin a cos a, cos 2a = cos^2 a - sin^2 a);
+// the FP32 error of the recurrence stays near 1e-6, far below the FP16 rounding of the inputs.
+#include "mfma_ops.hpp"
+#include "snisr_common.hpp"
+#include "snisr_device.hpp"
+
+namespace snisr {
+
+constexpr int kRefIn = 63, kRefWidth = 64, kRefOut = 28, kRefLayers = 6;
+constexpr int kRefParams = 22556;            // halves per group in the reference packing
+constexpr int kRefHiddenBytes = 2 * (4 * 1024 + 128);
+constexpr int kRefGroupBytes = (kRefLayers - 1) * kRefHiddenBytes + (4 * 1024 + 128);  // 46,464
+
+__host__ __device__ constexpr int ref_in(int l) { return l == 0 ? kRefIn : kRefWidth; }
+__host__ __device__ constexpr int ref_out(int l) { return l == kRefLayers - 1 ? kRefOut : kRefWidth; }
+// halves before layer l in the reference packing (per layer: bias[out], then W[in][out])
+__host__ __device__ constexpr int ref_src_offset(int l) {
+  int o = 0;
+  for (int m = 0; m < l; ++m) o += ref_out(m) + ref_in(m) * ref_out(m);
+  return o;
+}
+static_assert(ref_src_offset(kRefLayers) == kRefParams, "reference parameter count");
+static_assert(kRefGroupBytes % 16 == 0, "group bytes");
+
+// layer-0 input index held in k slot s = 8 kc + e of lane half h (63 = zero padding)
+__host__ __device__ constexpr int pe_input_index(int h, int s) { return h == 0 ? (s < 2 ? s : s + 1) : (s == 0 ? 2 : s + 32); }
+
+__device__ __forceinline__ float half_bits_to_float(uint16_t b) {
+  _Float16 v;
+  __builtin_memcpy(&v, &b, 2);
+  return (float)v;
+}
+
+// one thread per packed weight (16-bit, copied bit-exact) or bias (FP32) of one layer of one group
+__global__ void k_pack_ref_layer(const uint16_t* __restrict__ src, uint8_t* __restrict__ dst, int l, int64_t n_groups) {
+  const int in = ref_in(l), out = ref_out(l), nob = (out + 31) / 32;
+  const int64_t nw = (int64_t)nob * 4 * 512, per = nw + nob * 32;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= per * n_groups) return;
+  const int64_t grp = g / per, r = g - grp * per;
+  const uint16_t* bias = src + grp * kRefParams + ref_src_offset(l);
+  const uint16_t* Wt = bias + out;  // [in][out]
+  uint8_t* D = dst + grp * kRefGroupBytes + (int64_t)l * kRefHiddenBytes;
+  if (r < nw) {
+    const int frag = (int)(r / 512), within = (int)(r % 512);
+    const int ob = frag / 4, kc = frag % 4, lane = within / 8, e = within % 8;
+    const int h = lane / 32, row = 32 * ob + lane % 32;
+    const int x = l == 0 ? pe_input_index(h, 8 * kc + e) : pack_input_index(l, kc, h, e);
+    reinterpret_cast<uint16_t*>(D)[r] = (row < out && x < in) ? Wt[x * out + row] : (uint16_t)0;
+  } else {
+    const int q = (int)(r - nw);
+    const int ob = q / 32, h = (q / 16) % 2, i = q % 16;
+    const int row = 32 * ob + 8 * (i / 4) + 4 * h + (i % 4);
+    reinterpret_cast<float*>(D + nob * 4 * 1024)[q] = row < out ? half_bits_to_float(bias[row]) : 0.0f;
+  }
+}
+
+// the 32 layer-0 inputs of lane half h for one sample, in k-slot order
+__device__ __forceinline__ void pe_half(const float (&p)[3], int h, float (&v)[32]) {
+  float sn[5][3], cs[5][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const float a = h ? 32.0f * p[r] : p[r];
+    float sa, ca;
+    sincosf(a, &sa, &ca);
+    sn[0][r] = sa;
+    cs[0][r] = ca;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) {
+      sn[k][r] = 2.0f * sn[k - 1][r] * cs[k - 1][r];
+      cs[k][r] = cs[k - 1][r] * cs[k - 1][r] - sn[k - 1][r] * sn[k - 1][r];
+    }
+  }
+  auto band = [&](int i) { return (i % 6) < 3 ? sn[i / 6][i % 6] : cs[i / 6][i % 6 - 3]; };
+#pragma unroll
+  for (int s = 0; s < 32; ++s) {
+    const float a0 = s < 2 ? p[s] : band(s - 2);
+    const float a1 = s == 0 ? p[2] : (s < 31 ? band(s - 1) : 0.0f);
+    v[s] = h ? a1 : a0;
+  }
+}
+
+__device__ __forceinline__ void leaky_cvt(const f32x16& acc, f16x8& lo, f16x8& hi) {
+  float v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = fmaxf(acc[i], 0.01f * acc[i]);
+  lo = Mfma<SNISR_F16>::cvt8(v);
+  hi = Mfma<SNISR_F16>::cvt8(v + 8);
+}
+
+// real SH of degree <= 2 (orthonormal on the sphere; the reference's sign convention)
+__device__ __forceinline__ void sh2_basis(float x, float y, float z, float (&b)[9]) {
+  constexpr float c0 = 0.28209479177387814f;   // 1 / (2 sqrt(pi))
+  constexpr float c1 = 0.4886025119029199f;    // sqrt(3) c0
+  constexpr float c2 = 1.0925484305920792f;    // sqrt(15) c0
+  constexpr float c22 = 0.31539156525252005f;  // sqrt(5) c0 / 2
+  constexpr float c24 = 0.5462742152960396f;   // sqrt(15) c0 / 2
+  b[0] = c0;
+  b[1] = -c1 * y; b[2] = c1 * z; b[3] = -c1 * x;
+  b[4] = c2 * (x * y); b[5] = -c2 * (y * z); b[6] = c22 * (2.0f * (z * z) - x * x - y * y);
+  b[7] = -c2 * (x * z); b[8] = c24 * (x * x - y * y);
+}
+
+// partial SH colour from the 16 output rows this lane holds (rows 8(i/4) + 4h + i%4; r < 27 are
+// coefficients 3 * basis + channel)
+template <int H>
+__device__ __forceinline__ void sh_partial(const f32x16& acc, const float (&b)[9], float (&pr)[3]) {
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = 8 * (i / 4) + 4 * H + (i % 4);
+    if (r < 27) pr[r % 3] += b[r / 3] * acc[i];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
+  __shared__ __align__(16) uint8_t lw[kRefGroupBytes];
+  __shared__ int act[64];
+  __shared__ float ctr[3];
+  __shared__ int nact_s;
+  using M = Mfma<SNISR_F16>;
+  const int lane = threadIdx.x & 63, h = lane >> 5, s = lane & 31;
+  const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  const int spr = Q.samples_per_ray;
+  const int64_t npix = (int64_t)Q.H * Q.W;
+  const int64_t per = (Q.n_blocks + gridDim.x - 1) / gridDim.x;
+  const int64_t b0 = (int64_t)blockIdx.x * per;
+  const int64_t b1 = b0 + per < Q.n_blocks ? b0 + per : Q.n_blocks;
+  const uint8_t* __restrict__ gw = reinterpret_cast<const uint8_t*>(Q.packed_params);
+  int loaded = -1;
+  for (int64_t b = b0; b < b1; ++b) {
+    const int g = Q.net_idx[b];
+    if (g < 0 || g >= Q.n_groups) continue;  // -1: pixels of no group
+    if (g != loaded) {
+      __syncthreads();
+      const uint4* src = reinterpret_cast<const uint4*>(gw + (int64_t)g * kRefGroupBytes);
+      uint4* dst = reinterpret_cast<uint4*>(lw);
+      for (int k = threadIdx.x; k < kRefGroupBytes / 16; k += blockDim.x) dst[k] = src[k];
+      if (threadIdx.x < 3) ctr[threadIdx.x] = Q.group_centers[3 * g + threadIdx.x];
+      __syncthreads();
+      loaded = g;
+    }
+    const int64_t p_end = Q.pixel_starts[b + 1] < Q.n_query ? Q.pixel_starts[b + 1] : Q.n_query;
+    for (int64_t pb = Q.pixel_starts[b]; pb < p_end; pb += 64) {
+      __syncthreads();  // act[] is free
+      if (threadIdx.x < 64) {
+        const int64_t p = pb + threadIdx.x;
+        int ori = -1;
+        bool ok = false;
+        if (p >= 0 && p < p_end) {
+          ori = Q.query_pixel_indices[p];
+          ok = ori >= 0 && ori < npix && !(Q.transparency[ori] < Q.trans_stop);
+        }
+        const uint64_t m = __ballot(ok);
+        if (ok) act[__popcll(m & ((1ull << threadIdx.x) - 1ull))] = ori;
+        if (threadIdx.x == 0) nact_s = __popcll(m);
+      }
+      __syncthreads();
+      const int ncol = nact_s * spr;
+      for (int c0 = wv * 64; c0 < ncol; c0 += nwv * 64) {
+        f16x8 xa[2][4], xb[2][4];
+        int64_t task[2];
+        int orip[2];
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+          const int c = c0 + cb * 32 + s;
+          float pt[3] = {0.0f, 0.0f, 0.0f};
+          task[cb] = -1;
+          orip[cb] = 0;
+          if (c < ncol) {
+            const int ori = act[c / spr];
+            orip[cb] = ori;
+            task[cb] = (int64_t)ori * spr + c % spr;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) pt[i] = (Q.samples[task[cb] * 3 + i] - ctr[i]) * Q.inverse_far;
+          }
+          float v[32];
+          pe_half(pt, h, v);
+#pragma unroll
+          for (int kc = 0; kc < 4; ++kc) xa[cb][kc] = M::cvt8(v + 8 * kc);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        // five hidden layers (A fragments from LDS), activations ping-pong xa <-> xb
+#pragma unroll
+        for (int l = 0; l < kRefLayers - 1; ++l) {
+          auto& src = (l & 1) ? xb : xa;
+          auto& dst = (l & 1) ? xa : xb;
+          const uint8_t* Lp = lw + l * kRefHiddenBytes;
+          const float* bias = reinterpret_cast<const float*>(Lp + 2 * 4 * 1024);
+#pragma unroll
+          for (int ob = 0; ob < 2; ++ob) {
+            f32x16 acc0 = bias_block(bias, ob, h), acc1 = acc0;
+            mma_block<M, 4>(Lp + ob * 4 * 1024, lane, src[0], src[1], acc0, acc1);
+            leaky_cvt(acc0, dst[0][2 * ob], dst[0][2 * ob + 1]);
+            leaky_cvt(acc1, dst[1][2 * ob], dst[1][2 * ob + 1]);
+          }
+        }
+        // output layer: 28 rows in one 32-row block; the input is the output of layer 4 (in xb)
+        const uint8_t* Lp = lw + (kRefLayers - 1) * kRefHiddenBytes;
+        const float* bias = reinterpret_cast<const float*>(Lp + 4 * 1024);
+        f32x16 acc[2];
+        acc[0] = bias_block(bias, 0, h);
+        acc[1] = acc[0];
+        mma_block<M, 4>(Lp, lane, xb[0], xb[1], acc[0], acc[1]);
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+          float pr[3] = {0.0f, 0.0f, 0.0f};
+          if (task[cb] >= 0) {
+            const int64_t py = orip[cb] / Q.W, px = orip[cb] - py * Q.W;
+            const float* d = Q.rays_d + py * Q.d_row_stride + px * Q.d_col_stride;
+            const V3 u = normalize_dir(v3(d[0], d[1], d[2]));
+            float bs[9];
+            sh2_basis(u.x, u.y, u.z, bs);
+            if (h == 0) sh_partial<0>(acc[cb], bs, pr);
+            else sh_partial<1>(acc[cb], bs, pr);
+          }
+          float raw[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) raw[k] = pr[k] + __shfl_xor(pr[k], 32);
+          if (h == 0 && task[cb] >= 0)
+            reinterpret_cast<float4*>(Q.sample_results)[task[cb]] =
+                make_float4(sigmoid_cw(raw[0]), sigmoid_cw(raw[1]), sigmoid_cw(raw[2]), fmaxf(acc[cb][15], 0.0f));
+        }
+      }
+    }
+  }
+}
+
+}  // namespace snisr
+
+using namespace snisr;
+
+extern "C" {
+
+int64_t snisr_inference_mlp_packed_size(int64_t n_groups) { return n_groups < 0 ? -1 : n_groups * kRefGroupBytes; }
+
+int snisr_pack_inference_mlp(const void* params_f16, int64_t n_groups, void* packed, void* stream) {
+  SNISR_REQUIRE(n_groups >= 0, SNISR_ERR_ARG, "pack_inference_mlp: n_groups < 0");
+  if (n_groups == 0) return SNISR_OK;
+  SNISR_REQUIRE(params_f16 && packed, SNISR_ERR_ARG, "pack_inference_mlp: NULL pointer");
+  SNISR_REQUIRE(aligned(packed, 16), SNISR_ERR_ALIGN, "pack_inference_mlp: packed must be 16-byte aligned");
+  for (int l = 0; l < kRefLayers; ++l) {
+    const int nob = (ref_out(l) + 31) / 32;
+    const int64_t total = ((int64_t)nob * 4 * 512 + nob * 32) * n_groups;
+    hipLaunchKernelGGL(k_pack_ref_layer, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint16_t*)params_f16, (uint8_t*)packed, l, n_groups);
+    SNISR_LAUNCHED();
+  }
+  return SNISR_OK;
+}
+
+int snisr_inference_mlp(const snisr_mlp_query* q, void* stream) {
+  SNISR_REQUIRE(q, SNISR_ERR_ARG, "inference_mlp: query is NULL");
+  SNISR_REQUIRE(q->samples_per_ray > 0 && q->n_blocks >= 0 && q->n_groups >= 0 && q->H >= 0 && q->W >= 0 &&
+                    q->n_query >= 0,
+                SNISR_ERR_ARG, "inference_mlp: bad size");
+  if (q->n_blocks == 0 || q->n_groups == 0 || (int64_t)q->H * q->W == 0) return SNISR_OK;
+  SNISR_REQUIRE(q->net_idx && q->pixel_starts && q->query_pixel_indices && q->transparency && q->group_centers &&
+                    q->samples && q->rays_d && q->packed_params && q->sample_results,
+                SNISR_ERR_ARG, "inference_mlp: NULL pointer");
+  SNISR_REQUIRE(aligned(q->packed_params, 16) && aligned(q->sample_results, 16), SNISR_ERR_ALIGN,
+                "inference_mlp: packed_params and sample_results must be 16-byte aligned");
+  SNISR_REQUIRE((int64_t)q->H * q->W * q->samples_per_ray < (int64_t(1) << 31), SNISR_ERR_SHAPE,
+                "inference_mlp: H*W*samples_per_ray must stay below 2^31");
+  const int64_t grid = q->n_blocks < 3 * (int64_t)device_cus() ? q->n_blocks : 3 * (int64_t)device_cus();
+  hipLaunchKernelGGL(k_inference_mlp, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, *q);
+  SNISR_LAUNCHED();
+  return SNISR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// MFMA building blocks shared by the per-tile MLP kernels: v_mfma_f32_32x32x16_{f16,bf16}
+// wrappers, fragment loads, the LDS-fed multiply of one 32-row output block, bias broadcast and the
+// activation + conversion that turns an accumulator into the next layer's B operand.
+//
+// Lane layouts of v_mfma_f32_32x32x16 (64 lanes, h = lane / 32, s = lane % 32):
+//   A (weights):     lane l holds W[row l%32][k slots 8h..8h+7]
+//   B (activations): lane l holds X[k slots 8h..8h+7][column s]
+//   D (result):      lane l holds rows 8(i/4) + 4h + (i%4), i = 0..15, of column s
+// A layer's D for output block ob is the next layer's B for k-chunks 2ob and 2ob+1 once the next
+// layer's weight columns are permuted by pack_input_index.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "snisr.h"
+
+namespace snisr {
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+// input index held in k slot e (0..7) of lane half h for k-chunk kc of layer l
+__host__ __device__ constexpr int pack_input_index(int l, int kc, int h, int e) {
+  return l == 0 ? 16 * kc + 8 * h + e : 32 * (kc / 2) + 8 * (2 * (kc % 2) + e / 4) + 4 * h + (e % 4);
+}
+
+template <int DT>
+struct Mfma;
+template <>
+struct Mfma<SNISR_F16> {
+  typedef f16x8 vec;
+  __device__ __forceinline__ static f32x16 mma(vec a, vec b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  }
+  __device__ __forceinline__ static vec cvt8(const float* v) {
+    vec r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = (_Float16)v[e];
+    return r;
+  }
+};
+template <>
+struct Mfma<SNISR_BF16> {
+  typedef bf16x8 vec;
+  __device__ __forceinline__ static f32x16 mma(vec a, vec b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+  __device__ __forceinline__ static vec cvt8(const float* v) {
+    vec r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = (__bf16)v[e];
+    return r;
+  }
+};
+
+template <class V>
+__device__ __forceinline__ V load_frag(const uint8_t* __restrict__ p) {
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  return *reinterpret_cast<const V*>(&q);
+}
+
+// acc{0,1} += sum_kc A(ob, kc) * src{0,1}[kc] with A fragments read from LDS one step ahead;
+// the scheduling barrier keeps at most two fragments in registers.
+template <class M, int NK>
+__device__ __forceinline__ void mma_block(const uint8_t* Ab, int lane, const typename M::vec (&s0)[NK],
+                                          const typename M::vec (&s1)[NK], f32x16& acc0, f32x16& acc1) {
+  using vec = typename M::vec;
+  vec nxt = load_frag<vec>(Ab + lane * 16);
+#pragma unroll
+  for (int kc = 0; kc < NK; ++kc) {
+    const vec a = nxt;
+    if (kc + 1 < NK) nxt = load_frag<vec>(Ab + (kc + 1) * 1024 + lane * 16);
+    acc0 = M::mma(a, s0[kc], acc0);
+    acc1 = M::mma(a, s1[kc], acc1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+__device__ __forceinline__ f32x16 bias_block(const float* bias, int ob, int h) {
+  f32x16 acc;
+  const float4* bp = reinterpret_cast<const float4*>(bias + (ob * 2 + h) * 16);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 t4 = bp[q];
+    acc[4 * q] = t4.x; acc[4 * q + 1] = t4.y; acc[4 * q + 2] = t4.z; acc[4 * q + 3] = t4.w;
+  }
+  return acc;
+}
+
+template <class M>
+__device__ __forceinline__ void relu_cvt(const f32x16& acc, typename M::vec& lo, typename M::vec& hi) {
+  float v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = acc[i] > 0.0f ? acc[i] : 0.0f;
+  lo = M::cvt8(v);
+  hi = M::cvt8(v + 8);
+}
+
+// FP16 operands: convert first, then ReLU on packed halves (max(x, 0) commutes with rounding)
+template <>
+__device__ __forceinline__ void relu_cvt<Mfma<SNISR_F16>>(const f32x16& acc, f16x8& lo, f16x8& hi) {
+  float v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = acc[i];
+  const f16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+  lo = __builtin_elementwise_max(Mfma<SNISR_F16>::cvt8(v), z);
+  hi = __builtin_elementwise_max(Mfma<SNISR_F16>::cvt8(v + 8), z);
+}
+
+}  // namespace snisr

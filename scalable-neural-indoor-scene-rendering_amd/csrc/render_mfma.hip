@@ -16,12 +16,10 @@
 
 #include "snisr_common.hpp"
 #include "snisr_device.hpp"
+#include "mfma_ops.hpp"
 
 namespace snisr {
 
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // ------------------------------------------------------------------------- packed layout
 __host__ __device__ constexpr int nkc_of(int in) { return (in + 15) / 16; }
@@ -38,10 +36,6 @@ __host__ __device__ constexpr int64_t layer_offset(int in0, int wd, int L, int l
 }
 __host__ __device__ constexpr int64_t tile_bytes(int in0, int wd, int L) { return layer_offset(in0, wd, L, L); }
 
-// input index held in k slot e (0..7) of lane half h for k-chunk kc of layer l
-__host__ __device__ constexpr int pack_input_index(int l, int kc, int h, int e) {
-  return l == 0 ? 16 * kc + 8 * h + e : 32 * (kc / 2) + 8 * (2 * (kc % 2) + e / 4) + 4 * h + (e % 4);
-}
 
 __device__ __forceinline__ uint16_t to_bits(float x, int dtype) {
   if (dtype == SNISR_BF16) {
@@ -319,41 +313,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_add(int64_t* __restrict__ o
     if (base + k < n) offsets[base + k] += add;
 }
 
-// ------------------------------------------------------------------------- MFMA MLP
-template <int DT>
-struct Mfma;
-template <>
-struct Mfma<SNISR_F16> {
-  typedef f16x8 vec;
-  __device__ __forceinline__ static f32x16 mma(vec a, vec b, f32x16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
-  }
-  __device__ __forceinline__ static vec cvt8(const float* v) {
-    vec r;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) r[e] = (_Float16)v[e];
-    return r;
-  }
-};
-template <>
-struct Mfma<SNISR_BF16> {
-  typedef bf16x8 vec;
-  __device__ __forceinline__ static f32x16 mma(vec a, vec b, f32x16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
-  }
-  __device__ __forceinline__ static vec cvt8(const float* v) {
-    vec r;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) r[e] = (__bf16)v[e];
-    return r;
-  }
-};
-
-template <class V>
-__device__ __forceinline__ V load_frag(const uint8_t* __restrict__ p) {
-  const uint4 q = *reinterpret_cast<const uint4*>(p);
-  return *reinterpret_cast<const V*>(&q);
-}
 
 // 8 consecutive features of one corner voxel as floats
 template <bool HALF>
@@ -445,24 +404,7 @@ __global__ __launch_bounds__(kSortBlock) void k_tile_scatter(Workspace W) {
 }
 
 
-// acc{0,1} += sum_kc A(ob, kc) * src{0,1}[kc] with A fragments read from LDS one step ahead;
-// the scheduling barrier keeps at most two fragments in registers.
-template <class M, int NK>
-__device__ __forceinline__ void mma_block(const uint8_t* Ab, int lane, const typename M::vec (&s0)[NK],
-                                          const typename M::vec (&s1)[NK], f32x16& acc0, f32x16& acc1) {
-  using vec = typename M::vec;
-  vec nxt = load_frag<vec>(Ab + lane * 16);
-#pragma unroll
-  for (int kc = 0; kc < NK; ++kc) {
-    const vec a = nxt;
-    if (kc + 1 < NK) nxt = load_frag<vec>(Ab + (kc + 1) * 1024 + lane * 16);
-    acc0 = M::mma(a, s0[kc], acc0);
-    acc1 = M::mma(a, s1[kc], acc1);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-}
 
-typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
 // Eight trilinear corner weights (corner order x0y0z0, x1y0z0, ... as in tri_combine) as half2
 // broadcasts, 0 for empty corners; the slots of empty corners are redirected to slot 0 so every
@@ -519,36 +461,6 @@ __device__ __forceinline__ f16x8 interp8_h2<Mfma<SNISR_F16>>(const _Float16* __r
   return r;
 }
 
-__device__ __forceinline__ f32x16 bias_block(const float* bias, int ob, int h) {
-  f32x16 acc;
-  const float4* bp = reinterpret_cast<const float4*>(bias + (ob * 2 + h) * 16);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const float4 t4 = bp[q];
-    acc[4 * q] = t4.x; acc[4 * q + 1] = t4.y; acc[4 * q + 2] = t4.z; acc[4 * q + 3] = t4.w;
-  }
-  return acc;
-}
-
-template <class M>
-__device__ __forceinline__ void relu_cvt(const f32x16& acc, typename M::vec& lo, typename M::vec& hi) {
-  float v[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = acc[i] > 0.0f ? acc[i] : 0.0f;
-  lo = M::cvt8(v);
-  hi = M::cvt8(v + 8);
-}
-
-// FP16 operands: convert first, then ReLU on packed halves (max(x, 0) commutes with rounding)
-template <>
-__device__ __forceinline__ void relu_cvt<Mfma<SNISR_F16>>(const f32x16& acc, f16x8& lo, f16x8& hi) {
-  float v[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = acc[i];
-  const f16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
-  lo = __builtin_elementwise_max(Mfma<SNISR_F16>::cvt8(v), z);
-  hi = __builtin_elementwise_max(Mfma<SNISR_F16>::cvt8(v + 8), z);
-}
 
 // ------------------------------------------------------------------------- MFMA MLP
 // Persistent: one 512-thread workgroup per CU; workgroup g owns an equal contiguous span of the
@@ -986,7 +898,8 @@ __global__ void k_fill_zero(float* __restrict__ p, int64_t n) {
 int packet_width(const snisr_rays& R);  // render_fp32.hip
 
 // compute units of the current device (a per-device cache of a constant attribute)
-static int device_cus() {
+// compute units of the current device (cached per device)
+int device_cus() {
   static std::atomic<int> cus[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;

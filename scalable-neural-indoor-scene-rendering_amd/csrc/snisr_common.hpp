@@ -59,5 +59,6 @@ int64_t mlp_f32t_stride(int in, int width, int layers);
 // MFMA weight packing (implemented with the MFMA kernels).
 int64_t mfma_pack_bytes(const snisr_scene* scene, int dtype);
 int mfma_pack(const snisr_scene* scene, int dtype, void* packed, void* stream);
+int device_cus();  // compute units of the current device (render_mfma.hip)
 
 }  // namespace snisr

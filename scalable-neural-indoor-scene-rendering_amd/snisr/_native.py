@@ -41,6 +41,15 @@ class RenderOut(C.Structure):
                 ("stage_events", C.c_void_p)]
 
 
+class MlpQuery(C.Structure):
+    _fields_ = [("samples_per_ray", C.c_int32), ("n_groups", C.c_int32), ("n_blocks", C.c_int64),
+                ("inverse_far", C.c_float), ("trans_stop", C.c_float), ("net_idx", C.c_void_p),
+                ("pixel_starts", C.c_void_p), ("query_pixel_indices", C.c_void_p), ("n_query", C.c_int64),
+                ("transparency", C.c_void_p), ("group_centers", C.c_void_p), ("samples", C.c_void_p),
+                ("rays_d", C.c_void_p), ("d_row_stride", C.c_int64), ("d_col_stride", C.c_int64),
+                ("H", C.c_int32), ("W", C.c_int32), ("packed_params", C.c_void_p), ("sample_results", C.c_void_p)]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every exported entry point declared in include/snisr.h
@@ -72,6 +81,9 @@ SIGNATURES = [
                                     C.POINTER(C.c_int32), C.c_void_p]),
     ("snisr_assign_pixels", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                       C.c_void_p]),
+    ("snisr_inference_mlp_packed_size", C.c_int64, [C.c_int64]),
+    ("snisr_pack_inference_mlp", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    ("snisr_inference_mlp", C.c_int, [C.POINTER(MlpQuery), C.c_void_p]),
     ("snisr_train_workspace_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int64, C.c_int64]),
     ("snisr_train_forward", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
                                       C.POINTER(RenderOut), C.c_void_p, C.c_int64, C.c_void_p]),
@@ -95,7 +107,7 @@ def lib():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    for which, st in enumerate((SceneDesc, MarchParams, RaysDesc, RenderOut)):
+    for which, st in enumerate((SceneDesc, MarchParams, RaysDesc, RenderOut, MlpQuery)):
         got = L.snisr_abi_sizeof(which)
         if got != C.sizeof(st):
             raise RuntimeError(f"snisr: ABI mismatch for {st.__name__}: C {got} vs ctypes {C.sizeof(st)}")

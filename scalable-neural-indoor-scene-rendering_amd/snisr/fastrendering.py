@@ -26,6 +26,8 @@ import torch
 from . import _native as N
 from .device import _ptr, _require_cuda, _stream
 
+REF_MLP_PARAMS = 22556  # halves per group of the reflection MLP (rendering/src/rayreflection_render.cu:138)
+
 
 def _host_floats(t, n: int, name: str) -> list[float]:
     v = t.detach().reshape(-1).to("cpu", torch.float32).tolist() if isinstance(t, torch.Tensor) else list(t)
@@ -165,3 +167,69 @@ def assign_pixels(netIdxs: torch.Tensor, pixel_starts: torch.Tensor, block_start
         raise ValueError("pixel_starts and block_starts need numNet + 1 entries")
     N.check(N.lib().snisr_assign_pixels(_ptr(net), _ptr(ps), _ptr(bs), n, _ptr(new_netIdxs), _ptr(new_pixel_starts),
                                         _stream()), "assign_pixels")
+
+
+_packed_cache: dict = {}
+
+
+def pack_inference_params(params: torch.Tensor) -> torch.Tensor:
+    """The MFMA fragment layout of the reference's per-group parameters (22,556 halves per group),
+    cached per parameter tensor and version."""
+    _require_cuda(params, "params", torch.float16)
+    if params.numel() % REF_MLP_PARAMS:
+        raise ValueError(f"params must hold a multiple of {REF_MLP_PARAMS} halves")
+    G = params.numel() // REF_MLP_PARAMS
+    key = (params.device, params.data_ptr(), params.numel())
+    hit = _packed_cache.get(key)
+    if hit is not None and hit[0] == params._version:
+        return hit[1]
+    nbytes = N.lib().snisr_inference_mlp_packed_size(G)
+    packed = torch.empty(max(nbytes, 16), dtype=torch.uint8, device=params.device)
+    N.check(N.lib().snisr_pack_inference_mlp(_ptr(params), G, _ptr(packed), _stream()), "pack_inference_mlp")
+    _packed_cache.clear()
+    _packed_cache[key] = (params._version, packed)
+    return packed
+
+
+
+def inference_mlp(max_samples_per_ray: int, num_block: int, num_thread: int, inverse_far: float, trans_stop: float,
+                  netIdxs: torch.Tensor, transparency: torch.Tensor, query_pixel_indices: torch.Tensor,
+                  pixel_starts: torch.Tensor, group_centers: torch.Tensor, samples: torch.Tensor,
+                  rays_d: torch.Tensor, params: torch.Tensor, sample_results: torch.Tensor) -> None:
+    """rendering/src/rayreflection_render.cu:303-335: the reflection MLP of every live sample of the
+    num_block pixel blocks; writes (rgb, sigma) into sample_results [H, W, spr, 4] at the samples'
+    original pixels.  rays_d may be a strided [H, W, 3] view such as ``rays_d[::2, ::2]``."""
+    _require_cuda(sample_results, "sample_results", torch.float32)
+    H, W = samples.shape[0], samples.shape[1]
+    spr = int(max_samples_per_ray)
+    if samples.numel() != H * W * spr * 3 or sample_results.numel() != H * W * spr * 4:
+        raise ValueError("samples / sample_results must be [H, W, max_samples_per_ray, 3 / 4]")
+    if rays_d.dim() != 3 or tuple(rays_d.shape) != (H, W, 3) or rays_d.dtype != torch.float32 or not rays_d.is_cuda:
+        raise ValueError("rays_d must be a float32 [H, W, 3] GPU tensor")
+    if rays_d.stride(2) != 1:
+        rays_d = rays_d.contiguous()
+    net = netIdxs.to(torch.int16).contiguous()
+    ps = pixel_starts.to(torch.int32).contiguous()
+    q = query_pixel_indices.to(torch.int32).contiguous()
+    T = transparency.to(torch.float32).contiguous()
+    ctr = group_centers.to(torch.float32).contiguous()
+    smp = samples.to(torch.float32).contiguous()
+    nb = int(num_block)
+    if net.numel() < nb or ps.numel() < nb + 1:
+        raise ValueError("netIdxs needs num_block entries and pixel_starts num_block + 1")
+    if T.numel() != H * W:
+        raise ValueError("transparency must hold H*W values")
+    packed = pack_inference_params(params)
+    G = params.numel() // REF_MLP_PARAMS
+    if ctr.numel() < 3 * G:
+        raise ValueError("group_centers must be [num_groups, 3]")
+    Q = N.MlpQuery()
+    Q.samples_per_ray, Q.n_groups, Q.n_blocks = spr, G, nb
+    Q.inverse_far, Q.trans_stop = float(inverse_far), float(trans_stop)
+    Q.net_idx, Q.pixel_starts, Q.query_pixel_indices = net.data_ptr(), ps.data_ptr(), q.data_ptr()
+    Q.n_query = q.numel()
+    Q.transparency, Q.group_centers, Q.samples = T.data_ptr(), ctr.data_ptr(), smp.data_ptr()
+    Q.rays_d, Q.d_row_stride, Q.d_col_stride = rays_d.data_ptr(), rays_d.stride(0), rays_d.stride(1)
+    Q.H, Q.W = H, W
+    Q.packed_params, Q.sample_results = packed.data_ptr(), sample_results.data_ptr()
+    N.check(N.lib().snisr_inference_mlp(C.byref(Q), _stream()), "inference_mlp")

@@ -335,3 +335,98 @@ def camera_rays_np(cam: Camera) -> np.ndarray:
     R = cam.c2w.reshape(3, 3)
     d = np.stack([R[r, 0] * x + R[r, 1] * y + R[r, 2] for r in range(3)], -1)
     return d.reshape(-1, 3).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------- reflection pass
+# The reference's second render pass (rendering/src/__init__.py:272-335): per downscaled pixel, 64
+# inverse-depth samples behind the first surface, a per-group MLP (inference_mlp) and compositing.
+
+REF_MLP_DIMS = ((63, 64), (64, 64), (64, 64), (64, 64), (64, 64), (64, 28))
+REF_MLP_PARAMS = sum(o + i * o for i, o in REF_MLP_DIMS)  # 22,556 halves per group
+
+
+@dataclasses.dataclass
+class ReflectionFrame:
+    origin: np.ndarray          # f32[3]
+    rays_d: np.ndarray          # f32[H, W, 3] full-resolution directions (compute_rays, not normalised)
+    inverse_near: np.ndarray    # f32[DH, DW] 1 / depth of the first surface (1e-8: no surface)
+    net_idx: np.ndarray         # i16[DH, DW] group of each downscaled pixel, -1 = none
+    params: np.ndarray          # f16[G * 22556] reference packing: per layer bias[out], then W[in][out]
+    centers: np.ndarray         # f32[G, 3]
+    far: float                  # scene diagonal (rendering/src/__init__.py:132)
+    trans_stop: float           # early_stop_R (rendering/src/__init__.py:39)
+    scale: int                  # reflection_scale
+
+
+def make_reflection_groups(seed: int, n_groups: int, room_lo, room_hi):
+    """Per group a reflection MLP in the reference packing (Kaiming-uniform hidden weights, Xavier
+    output layer, biases ~ N(0, 0.05^2), all rounded to FP16) and a centre inside the room."""
+    rs = np.random.SeedSequence(seed).spawn(2)
+    rng = np.random.Generator(np.random.PCG64(rs[0]))
+    out = np.empty(n_groups * REF_MLP_PARAMS, np.float16)
+    off = 0
+    for _ in range(n_groups):
+        for li, (i, o) in enumerate(REF_MLP_DIMS):
+            a = math.sqrt(6.0 / i) if li < len(REF_MLP_DIMS) - 1 else math.sqrt(6.0 / (i + o))
+            out[off:off + o] = rng.normal(0.0, 0.05, o)
+            off += o
+            out[off:off + i * o] = rng.uniform(-a, a, i * o)
+            off += i * o
+    lo = np.asarray(room_lo, np.float64)
+    hi = np.asarray(room_hi, np.float64)
+    crng = np.random.Generator(np.random.PCG64(rs[1]))
+    centers = (lo + (hi - lo) * crng.uniform(0.1, 0.9, (n_groups, 3))).astype(np.float32)
+    return out, centers
+
+
+def reflection_frame(seed: int = 6, H: int = 720, W: int = 1280, n_groups: int = 48, scale: int = 2,
+                     room_lo=(0.0, 0.0, 0.0), room_hi=(4.0, 4.0, 1.0), no_hit: float = 0.05) -> ReflectionFrame:
+    """Synthetic inputs of the reflection pass at H x W (downscaled by ``scale``): the pixel groups
+    are the Voronoi cells of ``n_groups`` seeded image points (screen regions, as a group of tiles
+    covers on a real frame), each cell at a seeded depth in [0.8, 4] plus a gentle slope; a
+    ``no_hit`` fraction of cells has no surface (net -1, inverse_near 1e-8, the reference's init)."""
+    rs = np.random.SeedSequence(seed).spawn(3)
+    rng = np.random.Generator(np.random.PCG64(rs[0]))
+    cam = room_camera(seed, room_lo, room_hi, H, W)
+    rays = camera_rays_np(cam).reshape(H, W, 3)
+    DH, DW = H // scale, W // scale
+    n_cells = n_groups + max(1, int(round(no_hit * n_groups)))
+    pts = rng.uniform(0.0, 1.0, (n_cells, 2)) * np.array([DH, DW])
+    yy, xx = np.meshgrid(np.arange(DH, dtype=np.float64), np.arange(DW, dtype=np.float64), indexing="ij")
+    d2 = (yy[..., None] - pts[:, 0]) ** 2 + (xx[..., None] - pts[:, 1]) ** 2
+    cell = np.argmin(d2, axis=-1)
+    group_of_cell = np.concatenate([rng.permutation(n_groups), -np.ones(n_cells - n_groups, np.int64)])
+    net = group_of_cell[cell].astype(np.int16)
+    depth0 = rng.uniform(0.8, 4.0, n_cells)
+    slope = rng.uniform(-1.0, 1.0, (n_cells, 2)) / max(DH, DW)
+    depth = depth0[cell] + slope[cell, 0] * (yy - pts[cell, 0]) + slope[cell, 1] * (xx - pts[cell, 1])
+    inv_near = (1.0 / np.maximum(depth, 0.3)).astype(np.float32)
+    inv_near[net < 0] = np.float32(1e-8)
+    params, centers = make_reflection_groups(seed + 1000, n_groups, room_lo, room_hi)
+    far = float(np.linalg.norm(np.asarray(room_hi, np.float64) - np.asarray(room_lo, np.float64)))
+    return ReflectionFrame(cam.origin.astype(np.float32), rays, inv_near, net, params, centers, far, 0.01, scale)
+
+
+def reflection_blocks(net_flat: np.ndarray, max_pixels_per_block: int = 64):
+    """numpy form of the reference's pixel regrouping (rendering/src/__init__.py:272-291): stable sort
+    of the downscaled pixels by group; each group's pixels cut into ceil(n/64) blocks of sizes that
+    differ by at most one.  Returns query_pixel_indices i32[P], net i16[B], pixel_starts i32[B+1]
+    (groups include -1, whose blocks the MLP skips)."""
+    net_flat = np.asarray(net_flat, np.int16).ravel()
+    P = net_flat.size
+    order = np.argsort(net_flat, kind="stable").astype(np.int32)
+    ks = net_flat[order]
+    heads = np.flatnonzero(np.concatenate([[True], ks[1:] != ks[:-1]])) if P else np.zeros(0, np.int64)
+    starts = np.concatenate([heads, [P]]).astype(np.int64)
+    nets, pstarts = [], []
+    for g in range(heads.size):
+        a, b = starts[g], starts[g + 1]
+        nb = -(-(b - a) // max_pixels_per_block)
+        base, add = divmod(b - a, nb)
+        p = a
+        for k in range(nb):
+            nets.append(ks[a])
+            pstarts.append(p)
+            p += base + 1 if k < add else base
+    pstarts.append(P)
+    return order, np.asarray(nets, np.int16), np.asarray(pstarts, np.int32)
