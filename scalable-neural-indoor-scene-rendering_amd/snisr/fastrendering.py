@@ -200,12 +200,12 @@ def inference_mlp(max_samples_per_ray: int, num_block: int, num_thread: int, inv
     num_block pixel blocks; writes (rgb, sigma) into sample_results [H, W, spr, 4] at the samples'
     original pixels.  rays_d may be a strided [H, W, 3] view such as ``rays_d[::2, ::2]``."""
     _require_cuda(sample_results, "sample_results", torch.float32)
-    H, W = samples.shape[0], samples.shape[1]
+    if rays_d.dim() != 3 or rays_d.shape[2] != 3 or rays_d.dtype != torch.float32 or not rays_d.is_cuda:
+        raise ValueError("rays_d must be a float32 [H, W, 3] GPU tensor")
+    H, W = rays_d.shape[0], rays_d.shape[1]
     spr = int(max_samples_per_ray)
     if samples.numel() != H * W * spr * 3 or sample_results.numel() != H * W * spr * 4:
-        raise ValueError("samples / sample_results must be [H, W, max_samples_per_ray, 3 / 4]")
-    if rays_d.dim() != 3 or tuple(rays_d.shape) != (H, W, 3) or rays_d.dtype != torch.float32 or not rays_d.is_cuda:
-        raise ValueError("rays_d must be a float32 [H, W, 3] GPU tensor")
+        raise ValueError("samples / sample_results must hold H*W*max_samples_per_ray points / results")
     if rays_d.stride(2) != 1:
         rays_d = rays_d.contiguous()
     net = netIdxs.to(torch.int16).contiguous()
