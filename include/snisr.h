@@ -116,7 +116,7 @@ typedef struct snisr_render_out {
 
 const char* snisr_last_error(void);
 int snisr_version(void);
-/* sizeof of the descriptors (0 scene, 1 march params, 2 rays, 3 render out, 4 mlp query) so that foreign
+/* sizeof of the descriptors (0 scene, 1 march params, 2 rays, 3 render out, 4 mlp query, 5 octree) so that foreign
  * bindings can check their struct layouts; no reference counterpart. */
 int64_t snisr_abi_sizeof(int32_t which);
 
@@ -173,6 +173,31 @@ int snisr_bvh_build(const float* vertices, int64_t n_vertices, const int32_t* fa
                     int64_t node_bytes, void* tris_out, int64_t tri_bytes, int32_t* n_nodes, int32_t* depth);
 int snisr_mesh_first_hit(const void* nodes, int32_t n_nodes, const void* tris, int64_t n_tris, int32_t depth,
                          const snisr_rays* rays, float* t_out, int32_t* id_out, void* stream);
+
+/* A3 — rendering_diffuse_octree_fp16 (rendering/src/raydiffuse_render_octree_fp16.cu:238-287,
+ * include/raydiffuse_render_octree.h:7-21) on the reference's octree scene format.  Tiles hold 4^3
+ * blocks; an occupied block is a node of num_voxel^3 dilated cells (reference: 18). */
+typedef struct snisr_octree {
+  const int32_t* group_map;          /* tile2group [n_tiles] */
+  const void* data_voxels;           /* half4 (r, g, b, sigma), 8-byte aligned */
+  const int32_t* block_index_map;    /* [n_tiles * 64] node of each block, -1 = empty */
+  const int16_t* nodes_index_map;    /* [n_nodes * num_voxel^3] voxel of each cell within its node, -1 = empty */
+  const uint8_t* nodes_sample_flag;  /* [n_nodes * num_voxel^3] cells that are sampled (bool) */
+  const int64_t* voxels_start;       /* [n_nodes] first voxel of each node in data_voxels */
+  const float* centers;              /* [n_tiles * 3] tile centres */
+  float tile_size, voxel_size, sample_step;
+  int32_t num_voxel;
+} snisr_octree;
+
+/* Rays: origin f32[3], rays_d f32[n,3] (not necessarily unit), visited i32[n,K] from
+ * snisr_trace_tiles.  Outputs per ray: frame_diffuse f32[n,3] (always written), inverse_near f32[n]
+ * and net_idx i16[n] (written at the first sample with T < trans_th, or by soft_hit; otherwise left
+ * as the caller initialised them: the reference uses 1e-8 and -1); n_samples i32[n] optional (NULL).
+ * image_width > 0 lets the kernel walk 8x8 pixel packets (n a multiple of 8 rows of that width). */
+int snisr_render_octree(const snisr_octree* octree, const float* origin, const float* rays_d, const int32_t* visited,
+                        int64_t n, int32_t K, int32_t image_width, float trans_th, float early_stop, int32_t soft_hit,
+                        float* frame_diffuse, float* inverse_near, int16_t* net_idx, int32_t* n_samples,
+                        void* stream);
 
 /* A6 — inverse_z_sampling (rendering/src/sampling.cu:48-81, include/sampling.h:7-15).  For pixel p
  * (row-major H x W) and point j < sample_per_ray: z = 1/((offset+j) ib[p] + in[p]), step =

@@ -303,3 +303,28 @@ def train_grad(osc: OracleScene, origins, dirs, g, t_hit=None, precision="f64", 
        C_.c_int(ostride), _p(dirs), _p(t_hit), C_.c_longlong(N), _p(g), _p(Cc),
        _p(dsig), _p(dfeat), _p(dmlp))
     return dict(rgb=Cc, dsigma=dsig, dfeats=dfeat, dmlp=dmlp)
+
+
+def render_octree(osc, origin, rays_d, visited, trans_th=0.8, early_stop=0.01, soft_hit=False,
+                  inverse_near=None, net_idx=None):
+    """FP32 restatement of rendering_diffuse_octree_fp16 on a ``snisr.scene.OctreeScene``.
+    Returns frame f32[n,3], inverse_near f32[n], net_idx i16[n], n_samples i32[n]; inverse_near and
+    net_idx start from the given arrays (default 1e-8 and -1, the reference's init)."""
+    d = np.ascontiguousarray(rays_d, np.float32).reshape(-1, 3)
+    n = d.shape[0]
+    vis = np.ascontiguousarray(visited, np.int32).reshape(n, -1)
+    K = vis.shape[1]
+    frame = np.empty((n, 3), np.float32)
+    inv = np.full(n, 1e-8, np.float32) if inverse_near is None else np.array(inverse_near, np.float32).ravel()
+    net = np.full(n, -1, np.int16) if net_idx is None else np.array(net_idx, np.int16).ravel()
+    ns = np.empty(n, np.int32)
+    arrs = [np.ascontiguousarray(osc.group_map, np.int32), np.ascontiguousarray(osc.data_voxels, np.float16),
+            np.ascontiguousarray(osc.block_index_map, np.int32), np.ascontiguousarray(osc.nodes_index_map, np.int16),
+            np.ascontiguousarray(osc.nodes_sample_flag, np.uint8), np.ascontiguousarray(osc.voxels_start, np.int64),
+            np.ascontiguousarray(osc.centers, np.float32)]
+    lib().orc_render_octree_f32(*[_p(a) for a in arrs], C.c_float(osc.tile_size), C.c_float(osc.voxel_size),
+                                C.c_float(osc.sample_step), C.c_int(osc.num_voxel),
+                                _p(np.ascontiguousarray(origin, np.float32)), _p(d), _p(vis), C.c_longlong(n),
+                                C.c_int(K), C.c_float(trans_th), C.c_float(early_stop), C.c_int(1 if soft_hit else 0),
+                                _p(frame), _p(inv), _p(net), _p(ns))
+    return frame, inv, net, ns

@@ -949,3 +949,179 @@ void orc_assign_pixels(const int16_t* net, const int* pixel_starts, const int* b
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ A3: reference octree format
+// rendering_diffuse_octree_fp16_kernel (rendering/src/raydiffuse_render_octree_fp16.cu:141-235) with
+// block_traversal_rendering (20-139) and the FP16 trilinear_base (rendering/src/include/half_ope.h:
+// 246-284), restated in FP32 with the reference's operation order: float3 / s is a multiply by the
+// float reciprocal 1/s (cutil_math.h:397-401), make_int3 truncates, the march advances by
+// zval += sample_step.  Arithmetic the reference does in FP16 (interpolation, alpha, weights,
+// colour, transmittance) is FP32 here, exp is orc_expf.  Three guards the reference does not have:
+// nodes_IndexMap entries of -1 read as an all-zero voxel, voxel / corner coordinates are clamped
+// into the dilated node grid, and one block's march takes at most 2^16 steps.
+namespace orc {
+
+struct OctreeC {
+  const int32_t* group_map;       // [n_tiles]
+  const uint16_t* data_voxels;    // half4 [*]
+  const int32_t* block_index_map; // [n_tiles * 64]
+  const int16_t* nodes_index_map; // [n_nodes * nv^3]
+  const uint8_t* nodes_sample_flag; // [n_nodes * nv^3]
+  const int64_t* voxels_start;    // [n_nodes]
+  const float* centers;           // [n_tiles * 3]
+  float tile_size, voxel_size, sample_step;
+  int num_voxel;
+};
+
+static inline void octree_voxel(const OctreeC& S, int64_t start, const int16_t* nodes, int idx, float* v) {
+  const int k = nodes[idx];
+  if (k < 0) { v[0] = v[1] = v[2] = v[3] = 0.0f; return; }
+  const uint16_t* p = S.data_voxels + 4 * (start + k);
+  for (int c = 0; c < 4; ++c) v[c] = half_to_float(p[c]);
+}
+
+static inline float lerpf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+static void octree_trilinear(const OctreeC& S, int64_t start, const int16_t* nodes, const float* fidx, float* out) {
+  const int M = S.num_voxel, MYZ = M * M;
+  int i0[3];
+  float w[3];
+  for (int a = 0; a < 3; ++a) {
+    i0[a] = std::min(std::max((int)fidx[a], 0), M - 2);
+    w[a] = fidx[a] - (float)(int)fidx[a];
+    if (w[a] < 0.0f) w[a] = 0.0f;
+  }
+  float c[8][4];
+  for (int q = 0; q < 8; ++q) {
+    const int x = i0[0] + (q & 1), y = i0[1] + ((q >> 1) & 1), z = i0[2] + (q >> 2);
+    octree_voxel(S, start, nodes, x * MYZ + y * M + z, c[q]);
+  }
+  for (int ch = 0; ch < 4; ++ch) {
+    const float s = lerpf(lerpf(c[0][ch], c[1][ch], w[0]), lerpf(c[2][ch], c[3][ch], w[0]), w[1]);
+    const float t = lerpf(lerpf(c[4][ch], c[5][ch], w[0]), lerpf(c[6][ch], c[7][ch], w[0]), w[1]);
+    out[ch] = lerpf(s, t, w[2]);
+  }
+}
+
+struct OctreeRay {
+  float color[3];
+  float T;
+  bool first_hit;
+  int32_t n_samples;
+};
+
+static void octree_blocks(const OctreeC& S, const float* o, const float* d, int tile, float tile_near, int group,
+                          float trans_th, OctreeRay& R, float* inv_near, int16_t* net) {
+  constexpr int NB = 4;
+  const int M = S.num_voxel, total = M * M * M;
+  const float bs = S.tile_size / (float)NB;
+  const float inv_bs = 1.0f / bs, inv_vs = 1.0f / S.voxel_size;
+  int cur[3], step[3];
+  float tmax[3], tdel[3];
+  for (int a = 0; a < 3; ++a) {
+    cur[a] = (int)(o[a] * inv_bs);
+    step[a] = d[a] >= 0.0f ? 1 : -1;
+    float nb = (float)(cur[a] + step[a]) * bs;
+    if (step[a] < 0) nb += bs;
+    tmax[a] = safe_div(nb - o[a], d[a]);
+    tdel[a] = safe_div(bs * (float)step[a], d[a]);
+  }
+  float near = 0.0f, far = std::fmin(std::fmin(tmax[0], tmax[1]), tmax[2]);
+  while (true) {
+    const int bidx = cur[0] * NB * NB + cur[1] * NB + cur[2];
+    const int nidx = S.block_index_map[tile * NB * NB * NB + bidx];
+    if (nidx != -1) {
+      const int16_t* nodes = S.nodes_index_map + (int64_t)nidx * total;
+      const uint8_t* flags = S.nodes_sample_flag + (int64_t)nidx * total;
+      const int64_t start = S.voxels_start[nidx];
+      float corner[3];
+      for (int a = 0; a < 3; ++a) corner[a] = (float)cur[a] * bs - S.voxel_size;
+      float zval = near;
+      for (int guard = 0; zval < far && guard < (1 << 16); ++guard) {  // step cap of the GPU kernel
+        float rel[3];
+        int vl[3];
+        for (int a = 0; a < 3; ++a) {
+          const float p = o[a] + zval * d[a];
+          rel[a] = (p - corner[a]) * inv_vs;
+          vl[a] = std::min(std::max((int)rel[a], 0), M - 1);
+        }
+        const int vidx = vl[0] * M * M + vl[1] * M + vl[2];
+        if (!flags[vidx]) {
+          zval += S.sample_step;
+          continue;
+        }
+        if (!R.first_hit && R.T < trans_th) {
+          R.first_hit = true;
+          *inv_near = 1.0f / (zval + tile_near);
+          *net = (int16_t)group;
+        }
+        float fidx[3] = {rel[0] - 0.5f, rel[1] - 0.5f, rel[2] - 0.5f};
+        float v[4];
+        octree_trilinear(S, start, nodes, fidx, v);
+        const float interval = std::fmin(S.sample_step, far - zval);
+        const float alpha = 1.0f - orc_expf(-1.0f * v[3] * interval);
+        const float wgt = alpha * R.T;
+        for (int c = 0; c < 3; ++c) R.color[c] = R.color[c] + wgt * v[c];
+        R.T = R.T * (1.0f - alpha);
+        ++R.n_samples;
+        zval += S.sample_step;
+      }
+    }
+    if (tmax[0] < tmax[1]) {
+      if (tmax[0] < tmax[2]) { cur[0] += step[0]; tmax[0] += tdel[0]; }
+      else { cur[2] += step[2]; tmax[2] += tdel[2]; }
+    } else {
+      if (tmax[1] < tmax[2]) { cur[1] += step[1]; tmax[1] += tdel[1]; }
+      else { cur[2] += step[2]; tmax[2] += tdel[2]; }
+    }
+    near = far;
+    far = std::fmin(std::fmin(tmax[0], tmax[1]), tmax[2]);
+    if (cur[0] < 0 || cur[1] < 0 || cur[2] < 0 || cur[0] >= NB || cur[1] >= NB || cur[2] >= NB) break;
+  }
+}
+
+}  // namespace orc
+extern "C" {
+
+void orc_render_octree_f32(const int32_t* group_map, const uint16_t* data_voxels, const int32_t* block_index_map,
+                           const int16_t* nodes_index_map, const uint8_t* nodes_sample_flag,
+                           const int64_t* voxels_start, const float* centers, float tile_size, float voxel_size,
+                           float sample_step, int num_voxel, const float* origin, const float* rays_d,
+                           const int32_t* visited, long long n, int K, float trans_th, float early_stop,
+                           int soft_hit, float* frame, float* inverse_near, int16_t* net_idx, int32_t* n_samples) {
+  const orc::OctreeC S{group_map, data_voxels, block_index_map, nodes_index_map, nodes_sample_flag, voxels_start,
+                       centers, tile_size, voxel_size, sample_step, num_voxel};
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long r = 0; r < n; ++r) {
+    orc::OctreeRay R{{0.0f, 0.0f, 0.0f}, 1.0f, false, 0};
+    const float d[3] = {rays_d[3 * r], rays_d[3 * r + 1], rays_d[3 * r + 2]};
+    const orc::V3<float> dv{d[0], d[1], d[2]}, ov{origin[0], origin[1], origin[2]};
+    int group = -1;
+    float bl = -1.0f, bh = -1.0f;
+    for (int i = 0; i < K; ++i) {
+      const int tile = visited[r * K + i];
+      if (tile == -1) break;
+      if (R.T < early_stop) break;
+      group = group_map[tile];
+      const orc::V3<float> c{centers[3 * tile], centers[3 * tile + 1], centers[3 * tile + 2]};
+      const float hs = tile_size / 2.0f;
+      orc::ray_aabb(ov, dv, c, orc::V3<float>{hs, hs, hs}, bl, bh);
+      if (bl == -1.0f) continue;
+      float ol[3];
+      const float cc[3] = {c.x, c.y, c.z};
+      for (int a = 0; a < 3; ++a) {
+        const float corner = cc[a] - hs;
+        ol[a] = orc::clampr(origin[a] + bl * d[a] - corner, 0.0f, tile_size - 0.00001f);
+      }
+      orc::octree_blocks(S, ol, d, tile, bl, group, trans_th, R, inverse_near + r, net_idx + r);
+    }
+    if (soft_hit && !R.first_hit && group != -1) {
+      inverse_near[r] = 1.0f / bh;
+      net_idx[r] = (int16_t)group;
+    }
+    for (int c = 0; c < 3; ++c) frame[3 * r + c] = R.color[c];
+    if (n_samples) n_samples[r] = R.n_samples;
+  }
+}
+
+}  // extern "C"

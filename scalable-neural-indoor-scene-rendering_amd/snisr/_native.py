@@ -50,6 +50,13 @@ class MlpQuery(C.Structure):
                 ("H", C.c_int32), ("W", C.c_int32), ("packed_params", C.c_void_p), ("sample_results", C.c_void_p)]
 
 
+class OctreeDesc(C.Structure):
+    _fields_ = [("group_map", C.c_void_p), ("data_voxels", C.c_void_p), ("block_index_map", C.c_void_p),
+                ("nodes_index_map", C.c_void_p), ("nodes_sample_flag", C.c_void_p), ("voxels_start", C.c_void_p),
+                ("centers", C.c_void_p), ("tile_size", C.c_float), ("voxel_size", C.c_float),
+                ("sample_step", C.c_float), ("num_voxel", C.c_int32)]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every exported entry point declared in include/snisr.h
@@ -81,6 +88,9 @@ SIGNATURES = [
                                     C.POINTER(C.c_int32), C.c_void_p]),
     ("snisr_assign_pixels", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                       C.c_void_p]),
+    ("snisr_render_octree", C.c_int, [C.POINTER(OctreeDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                      C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_int32, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("snisr_inference_mlp_packed_size", C.c_int64, [C.c_int64]),
     ("snisr_pack_inference_mlp", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     ("snisr_inference_mlp", C.c_int, [C.POINTER(MlpQuery), C.c_void_p]),
@@ -107,7 +117,7 @@ def lib():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    for which, st in enumerate((SceneDesc, MarchParams, RaysDesc, RenderOut, MlpQuery)):
+    for which, st in enumerate((SceneDesc, MarchParams, RaysDesc, RenderOut, MlpQuery, OctreeDesc)):
         got = L.snisr_abi_sizeof(which)
         if got != C.sizeof(st):
             raise RuntimeError(f"snisr: ABI mismatch for {st.__name__}: C {got} vs ctypes {C.sizeof(st)}")

@@ -233,3 +233,52 @@ def inference_mlp(max_samples_per_ray: int, num_block: int, num_thread: int, inv
     Q.H, Q.W = H, W
     Q.packed_params, Q.sample_results = packed.data_ptr(), sample_results.data_ptr()
     N.check(N.lib().snisr_inference_mlp(C.byref(Q), _stream()), "inference_mlp")
+
+
+def rendering_diffuse_octree_fp16(rays_d: torch.Tensor, visitedTiles: torch.Tensor, origin: torch.Tensor,
+                                  groupMap: torch.Tensor, data_voxels: torch.Tensor, block_IndexMap: torch.Tensor,
+                                  nodes_IndexMap: torch.Tensor, nodes_sampleFlag: torch.Tensor,
+                                  voxels_start: torch.Tensor, centers: torch.Tensor, tile_size: float,
+                                  voxel_size: float, sample_step: float, num_voxel: int, num_thread: int,
+                                  trans_th: float, early_stop: float, soft_hit: bool, frame_diffuse: torch.Tensor,
+                                  inverse_near: torch.Tensor, netIdxs: torch.Tensor,
+                                  n_samples: torch.Tensor | None = None) -> None:
+    """rendering/src/raydiffuse_render_octree_fp16.cu:238-287: diffuse colour of every ray over its
+    visited tiles of the octree scene (frame_diffuse [H, W, 3]); the first sample with T < trans_th
+    writes inverse_near and netIdxs (the caller initialises them, the reference with 1e-8 and -1).
+    Interpolation and compositing run in FP32 (the reference: FP16).  ``n_samples`` (optional, int32
+    [H, W]) receives the per-ray sample count."""
+    _require_cuda(frame_diffuse, "frame_diffuse", torch.float32)
+    _require_cuda(inverse_near, "inverse_near", torch.float32)
+    _require_cuda(netIdxs, "netIdxs", torch.int16)
+    dev = frame_diffuse.device
+    rd = rays_d.to(dev, torch.float32).contiguous()
+    n = rd.numel() // 3
+    vis = visitedTiles.to(dev, torch.int32).contiguous()
+    K = vis.shape[-1] if vis.dim() else 0
+    if vis.numel() != n * K or frame_diffuse.numel() != 3 * n or inverse_near.numel() != n or netIdxs.numel() != n:
+        raise ValueError("rays_d [H,W,3], visitedTiles [H,W,K], frame_diffuse [H,W,3], inverse_near/netIdxs [H,W]")
+    if n_samples is not None:
+        _require_cuda(n_samples, "n_samples", torch.int32)
+    keep = [t.to(dev).contiguous() for t in (groupMap, data_voxels, block_IndexMap, nodes_IndexMap, voxels_start,
+                                             centers, origin)]
+    gm, dv, bm, nm, vs_, ct, org = keep
+    if gm.dtype != torch.int32 or dv.dtype != torch.float16 or bm.dtype != torch.int32 or nm.dtype != torch.int16 \
+            or vs_.dtype != torch.int64 or ct.dtype != torch.float32:
+        raise ValueError("dtypes: groupMap i32, data_voxels f16, block_IndexMap i32, nodes_IndexMap i16, "
+                         "voxels_start i64, centers f32")
+    flags = nodes_sampleFlag.to(dev).contiguous()
+    if flags.dtype == torch.bool:
+        flags = flags.view(torch.uint8)
+    org = org.to(torch.float32)
+    O = N.OctreeDesc()
+    O.group_map, O.data_voxels, O.block_index_map = gm.data_ptr(), dv.data_ptr(), bm.data_ptr()
+    O.nodes_index_map, O.nodes_sample_flag, O.voxels_start = nm.data_ptr(), flags.data_ptr(), vs_.data_ptr()
+    O.centers = ct.data_ptr()
+    O.tile_size, O.voxel_size, O.sample_step, O.num_voxel = float(tile_size), float(voxel_size), float(sample_step), \
+        int(num_voxel)
+    width = rays_d.shape[1] if rays_d.dim() == 3 else 0
+    N.check(N.lib().snisr_render_octree(C.byref(O), _ptr(org), _ptr(rd), _ptr(vis), n, K, width, float(trans_th),
+                                        float(early_stop), 1 if soft_hit else 0, _ptr(frame_diffuse),
+                                        _ptr(inverse_near), _ptr(netIdxs), _ptr(n_samples), _stream()),
+            "rendering_diffuse_octree_fp16")

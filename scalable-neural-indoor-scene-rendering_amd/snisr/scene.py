@@ -430,3 +430,113 @@ def reflection_blocks(net_flat: np.ndarray, max_pixels_per_block: int = 64):
             p += base + 1 if k < add else base
     pstarts.append(P)
     return order, np.asarray(nets, np.int16), np.asarray(pstarts, np.int32)
+
+
+# ---------------------------------------------------------------------------- octree format
+# The reference's own diffuse scene format (rendering/src/__init__.py:111-121, consumed by
+# rendering_diffuse_octree_fp16): per tile 4^3 blocks; an occupied block is a node holding an
+# 18^3 dilated cell grid (16^3 voxels plus a one-voxel border copied from the neighbours).
+
+OCT_BLOCKS = 4
+OCT_NV = 16
+
+
+@dataclasses.dataclass
+class OctreeScene:
+    tile_shape: tuple
+    tile_size: float
+    scene_min: np.ndarray        # f32[3]
+    tile_index_map: np.ndarray   # i32[prod(tile_shape)] dense tile -> tile id, -1 = empty
+    centers: np.ndarray          # f32[T, 3]
+    group_map: np.ndarray        # i32[T] tile -> group
+    block_index_map: np.ndarray  # i32[T * 64] block -> node, -1 = empty
+    nodes_index_map: np.ndarray  # i16[nodes * 18^3] cell -> voxel within the node, -1 = empty
+    nodes_sample_flag: np.ndarray  # bool[nodes * 18^3]
+    voxels_start: np.ndarray     # i64[nodes]
+    data_voxels: np.ndarray      # f16[V, 4] (r, g, b, sigma)
+    voxel_size: float
+    sample_step: float
+    num_voxel: int = OCT_NV + 2
+
+    @property
+    def n_tiles(self):
+        return self.centers.shape[0]
+
+    @property
+    def n_nodes(self):
+        return self.voxels_start.shape[0]
+
+
+def make_octree_scene(seed: int, tile_shape=(4, 4, 1), tile_size=1.0, scene_min=(0.0, 0.0, 0.0),
+                      block_occupancy=0.3, voxel_occupancy=0.5, n_groups=4, sample_step_scale=0.2,
+                      field=None, flag_border=False) -> OctreeScene:
+    """Seeded scene in the reference's octree format.  Blocks are occupied ~ Bernoulli(block_occupancy),
+    voxels of an occupied block ~ Bernoulli(voxel_occupancy); rgb ~ U(0,1), sigma = exp(N(0,1)) /
+    voxel_size (FP16).  A cell is sampled when it is an interior cell whose voxel is occupied.  The
+    dilated border holds the neighbouring voxels of the same tile (zero beyond the tile).
+    ``field(tile, ijk) -> f32[...,4]`` overrides the voxel values (used by closed-form tests); with
+    it every voxel of an occupied block is occupied and the border beyond the tile holds the field's
+    values at voxel coordinates -1 and 64 (the field extends past the tile).  ``flag_border`` also
+    flags the border cells, so that a sample whose cell rounds onto the border is taken, not skipped."""
+    rs = np.random.SeedSequence(seed).spawn(4)
+    rng_b = np.random.Generator(np.random.PCG64(rs[0]))
+    rng_v = np.random.Generator(np.random.PCG64(rs[1]))
+    rng_d = np.random.Generator(np.random.PCG64(rs[2]))
+    rng_g = np.random.Generator(np.random.PCG64(rs[3]))
+    tile_shape = tuple(int(v) for v in tile_shape)
+    nt = int(np.prod(tile_shape))
+    smin = np.asarray(scene_min, np.float32)
+    R = OCT_BLOCKS * OCT_NV
+    vs = float(np.float32(tile_size) / np.float32(R))
+    centers = np.empty((nt, 3), np.float32)
+    for t in range(nt):
+        ix, iy, iz = t // (tile_shape[1] * tile_shape[2]), (t // tile_shape[2]) % tile_shape[1], t % tile_shape[2]
+        centers[t] = smin + (np.array([ix, iy, iz], np.float32) + np.float32(0.5)) * np.float32(tile_size)
+    M = OCT_NV + 2
+    bmap = np.full(nt * OCT_BLOCKS ** 3, -1, np.int32)
+    nodes_idx, nodes_flag, starts, data = [], [], [], []
+    n_vox = 0
+    ijk = np.stack(np.meshgrid(np.arange(M), np.arange(M), np.arange(M), indexing="ij"), -1).reshape(-1, 3)
+    for t in range(nt):
+        occ_b = rng_b.random(OCT_BLOCKS ** 3) < block_occupancy
+        vox_occ = np.zeros((R + 2,) * 3, bool)   # tile voxels with a zero border
+        vox_val = np.zeros((R + 2,) * 3 + (4,), np.float32)
+        for b in np.flatnonzero(occ_b):
+            bx, by, bz = b // 16, (b // 4) % 4, b % 4
+            sl = tuple(slice(1 + OCT_NV * c, 1 + OCT_NV * (c + 1)) for c in (bx, by, bz))
+            if field is None:
+                vox_occ[sl] = rng_v.random((OCT_NV,) * 3) < voxel_occupancy
+            else:
+                vox_occ[sl] = True
+        if field is not None:
+            vox_occ[:] = True
+        occ_idx = np.argwhere(vox_occ)
+        if field is None:
+            vals = np.empty((occ_idx.shape[0], 4), np.float32)
+            vals[:, :3] = rng_d.random((occ_idx.shape[0], 3))
+            vals[:, 3] = np.exp(rng_d.standard_normal(occ_idx.shape[0])) / vs
+        else:
+            vals = np.asarray(field(t, occ_idx - 1), np.float32)
+        vox_val[tuple(occ_idx.T)] = vals
+        for b in np.flatnonzero(occ_b):
+            bx, by, bz = b // 16, (b // 4) % 4, b % 4
+            cells = ijk + np.array([OCT_NV * bx, OCT_NV * by, OCT_NV * bz])   # padded tile coordinates
+            occ = vox_occ[tuple(cells.T)]
+            idx = np.full(M ** 3, -1, np.int16)
+            idx[occ] = np.arange(int(occ.sum()), dtype=np.int16)
+            interior = np.all((ijk >= 1) & (ijk <= OCT_NV), axis=1)
+            bmap[t * OCT_BLOCKS ** 3 + b] = len(starts)
+            nodes_idx.append(idx)
+            nodes_flag.append((interior | flag_border) & occ)
+            starts.append(n_vox)
+            data.append(vox_val[tuple(cells[occ].T)])
+            n_vox += int(occ.sum())
+    groups = rng_g.integers(0, n_groups, nt).astype(np.int32)
+    data_v = (np.concatenate(data) if data else np.zeros((1, 4), np.float32)).astype(np.float16)
+    if data_v.shape[0] == 0:
+        data_v = np.zeros((1, 4), np.float16)
+    return OctreeScene(tile_shape, float(tile_size), smin, np.arange(nt, dtype=np.int32), centers, groups, bmap,
+                       np.concatenate(nodes_idx) if nodes_idx else np.full(M ** 3, -1, np.int16),
+                       np.concatenate(nodes_flag) if nodes_flag else np.zeros(M ** 3, bool),
+                       np.asarray(starts, np.int64) if starts else np.zeros(1, np.int64), data_v, vs,
+                       float(np.float32(sample_step_scale) * np.float32(vs)))
