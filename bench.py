@@ -110,6 +110,35 @@ def train_bench(args, dev):
                             "copy, fwd + hand-written adjoint + Adam"}
 
 
+def ref_pipeline_bench(args, dev):
+    """The reference's own frame pipeline (FRender.render form, rendering/src/__init__.py:247-341) at
+    1280x720 through the FASTRENDERING stand-in: octree room (4x4x1 tiles, 30% of blocks occupied,
+    16^3 voxels per block at 50%), 8 groups with 63-64x5-28 reflection MLPs, 4 x 16 reflection
+    samples per 2x-downscaled pixel.  Includes the pipeline's own two host synchronisations."""
+    import torch
+    from snisr import frender
+    from snisr import scene as S
+
+    osc = S.make_octree_scene(12, tile_shape=(4, 4, 1), block_occupancy=0.3, n_groups=8)
+    params, centers = S.make_reflection_groups(13, 8, (0, 0, 0), (4, 4, 1))
+    cam = S.room_camera(2, (0, 0, 0), (4, 4, 1), 720, 1280)
+    fr = frender.FRender(osc, params, centers, H=720, W=1280)
+    fr.set_camera(cam)
+    for _ in range(2):
+        fr.render()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.ref_frames):
+        fr.render()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.ref_frames
+    return {"ref_pipeline_ms_per_frame": round(ms, 3), "ref_pipeline_mrays_s": round(720 * 1280 / ms / 1e3, 2),
+            "ref_pipeline_reflection_blocks": int(fr.n_reflection_blocks),
+            "ref_pipeline_config": "FRender.render form at 1280x720: compute_rays, tracing_tiles (K=30), octree "
+                                   "diffuse pass, sort_by_key/assign_pixels, 4 x (inverse_z_sampling, inference_mlp "
+                                   "FP16 MFMA, integrate_points) at 640x360, bilinear upsample; seeds 12/13/2"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +148,8 @@ def main():
     ap.add_argument("--cpu-frac", type=float, default=0.05)
     ap.add_argument("--fp32-frames", type=int, default=3)
     ap.add_argument("--train-steps", type=int, default=10, help="timed C3 training steps (0 = skip)")
+    ap.add_argument("--ref-frames", type=int, default=10,
+                    help="timed frames of the reference-format FRender pipeline (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -214,6 +245,8 @@ def main():
 
     if args.train_steps > 0:
         extra.update(train_bench(args, dev))
+    if args.ref_frames > 0:
+        extra.update(ref_pipeline_bench(args, dev))
 
     cpu = None
     if not args.no_cpu:

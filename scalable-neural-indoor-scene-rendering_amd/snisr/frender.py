@@ -1,0 +1,120 @@
+"""Frame driver in the form of the reference's ``FRender.render`` (rendering/src/__init__.py:247-341),
+calling the FASTRENDERING stand-in operators: rays, tile tracing, the octree diffuse pass, pixel
+regrouping by group, four chunks of 16 inverse-depth reflection samples through the per-group MLP
+with compositing, bilinear upsampling of the reflection layer, and diffuse + reflection clamped to
+[0, 1].  The screen-space group boundary (compute_group_boundary) and the optional U-Net refinement
+are not part of this path and are left out.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+from . import fastrendering as FR
+from .scene import Camera, OctreeScene
+
+
+class FRender:
+    """Holds the scene tensors on the device, like the reference's render_prepare
+    (rendering/src/__init__.py:95-146).  ``params``: reflection MLPs, f16[G * 22556]; ``group_centers``:
+    f32[G, 3]."""
+
+    def __init__(self, osc: OctreeScene, params: np.ndarray, group_centers: np.ndarray, H=720, W=1280,
+                 max_tracing_tile=30, reflection_scale=2, device="cuda"):
+        dev = torch.device(device)
+        self.device = dev
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        self.H, self.W = H, W
+        self.max_tracing_tile = max_tracing_tile
+        self.reflection_scale = reflection_scale
+        self.DH, self.DW = H // reflection_scale, W // reflection_scale
+        self.downscale_numPixel = self.DH * self.DW
+        self.num_thread = 256
+        self.max_pixels_per_block = 64
+        self.trans_decay = 0.8
+        self.early_stop_R = 0.01
+        self.early_stop_D = 0.01
+        self.tile_IndexMap = t(osc.tile_index_map)
+        self.tile_shape = torch.tensor(osc.tile_shape, dtype=torch.int32)
+        self.scene_min_corner = torch.from_numpy(osc.scene_min)
+        self.tile_size = osc.tile_size
+        self.scene_size = torch.from_numpy(np.asarray(osc.tile_shape, np.float32) * np.float32(osc.tile_size))
+        self.tile2group = t(osc.group_map)
+        self.data_voxels = t(osc.data_voxels)
+        self.block_IndexMap = t(osc.block_index_map)
+        self.nodes_IndexMap = t(osc.nodes_index_map)
+        self.nodes_sampleFlag = t(osc.nodes_sample_flag)
+        self.voxels_start = t(osc.voxels_start)
+        self.centers = t(osc.centers)
+        self.voxel_size = osc.voxel_size
+        self.sample_step = osc.sample_step
+        self.block_num_voxel = osc.num_voxel
+        self.net_params = t(params)
+        self.group_centers = t(group_centers)
+        self.far = float(np.linalg.norm(osc.tile_size * np.asarray(osc.tile_shape, np.float64)))
+        self.inverse_far = 1.0 / self.far
+        self.frame = None
+
+    def set_camera(self, cam: Camera):
+        self.intrinsic = torch.from_numpy(cam.intrinsic).to(self.device)
+        self.c2w = torch.from_numpy(cam.c2w).to(self.device)
+        self.origin = torch.from_numpy(cam.origin).to(self.device)
+
+    @torch.no_grad()
+    def render(self):
+        dev = self.device
+        H, W, s = self.H, self.W, self.reflection_scale
+        rays_d = torch.full((H, W, 3), -1, device=dev, dtype=torch.float32)
+        visited = torch.full((H, W, self.max_tracing_tile), -1, device=dev, dtype=torch.int32)
+        self.frame_diffuse = torch.full((H, W, 3), -1, device=dev, dtype=torch.float32)
+        inverse_near = torch.full((H, W), 1e-8, device=dev, dtype=torch.float32)
+        netIdxs = torch.full((H, W), -1, device=dev, dtype=torch.int16)
+        FR.compute_rays(self.intrinsic, self.c2w, self.num_thread, rays_d)
+        FR.tracing_tiles(rays_d, self.tile_IndexMap, self.origin, self.tile_shape, self.scene_min_corner,
+                         self.scene_size, self.tile_size, self.num_thread, visited)
+        FR.rendering_diffuse_octree_fp16(rays_d, visited, self.origin, self.tile2group, self.data_voxels,
+                                         self.block_IndexMap, self.nodes_IndexMap, self.nodes_sampleFlag,
+                                         self.voxels_start, self.centers, self.tile_size, self.voxel_size,
+                                         self.sample_step, self.block_num_voxel, self.num_thread, self.trans_decay,
+                                         self.early_stop_D, False, self.frame_diffuse, inverse_near, netIdxs)
+        del visited
+        inverse_near = inverse_near[::s, ::s].contiguous()
+        netIdxs = netIdxs[::s, ::s].flatten()
+        npx = self.downscale_numPixel
+        query = torch.arange(0, npx, 1, device=dev, dtype=torch.int32)
+        pixel_starts = torch.arange(0, npx, 1, device=dev, dtype=torch.int32)
+        numNet = FR.sort_by_key(netIdxs, query, pixel_starts)
+        tail = torch.full((1,), npx, dtype=torch.int32, device=dev)
+        per_net = torch.cat([pixel_starts[1:numNet], tail]) - pixel_starts[:numNet]
+        pixel_starts = torch.cat([pixel_starts[:numNet], tail])
+        block_starts = torch.cat([torch.zeros(1, dtype=torch.int32, device=dev),
+                                  torch.cumsum(torch.ceil(per_net / self.max_pixels_per_block), dim=0).int()])
+        netIdxs = netIdxs[:numNet]
+        n_blocks = int(block_starts[-1])
+        new_net = torch.empty(n_blocks, device=dev, dtype=torch.int16)
+        new_starts = torch.empty(n_blocks, device=dev, dtype=torch.int32)
+        FR.assign_pixels(netIdxs, pixel_starts, block_starts, self.num_thread, new_net, new_starts)
+        new_starts = torch.cat([new_starts, tail])
+        sample_far = 1.0 / inverse_near + self.far
+        inverse_bound = (1.0 / sample_far - inverse_near) / 63
+        transparency = torch.ones((self.DH, self.DW, 1), device=dev, dtype=torch.float32)
+        spec = torch.zeros((self.DH, self.DW, 3), device=dev, dtype=torch.float32)
+        rays_ds = rays_d[::s, ::s]
+        for step in range(0, 64, 16):
+            sample_results = torch.zeros((self.DH, self.DW, 16, 4), device=dev, dtype=torch.float32)
+            samples = torch.full((self.DH, self.DW, 16, 3), -1, device=dev, dtype=torch.float32)
+            steps = torch.zeros((self.DH, self.DW, 16), device=dev, dtype=torch.float32)
+            FR.inverse_z_sampling(self.origin, rays_ds, inverse_near, inverse_bound, self.num_thread, step, 16,
+                                  samples, steps)
+            FR.inference_mlp(16, n_blocks, self.num_thread, self.inverse_far, self.early_stop_R, new_net,
+                             transparency, query, new_starts, self.group_centers, samples, rays_ds, self.net_params,
+                             sample_results)
+            FR.integrate_points(sample_results, steps, self.num_thread * 2, self.early_stop_R, transparency, spec)
+        self.frame_speculars = F.interpolate(spec[None].permute(0, 3, 1, 2), scale_factor=s, mode="bilinear",
+                                             align_corners=True)[0].permute(1, 2, 0)
+        self.frame = torch.clamp(self.frame_diffuse + self.frame_speculars, 0, 1)
+        self.n_reflection_blocks = n_blocks
+        return self.frame

@@ -1,0 +1,60 @@
+"""The FRender.render-form frame driver (rendering/src/__init__.py:247-341) on an octree room with
+per-group reflection MLPs, against the oracle chain: diffuse layer within 1e-4 (samples, inverse
+depth and groups bit-exact), reflection layer within the FP16-MLP tolerance of the reflection-pass
+test (2e-2 max, 1e-3 mean)."""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from snisr import frender
+from snisr import scene as S
+
+pytestmark = pytest.mark.gpu
+
+
+def test_frender_frame_matches_oracle_chain(oracle_mod):
+    osc = S.make_octree_scene(9, tile_shape=(4, 4, 1), block_occupancy=0.3, n_groups=6)
+    params, centers = S.make_reflection_groups(19, 6, (0, 0, 0), (4, 4, 1))
+    H, W = 96, 160
+    cam = S.room_camera(9, (0, 0, 0), (4, 4, 1), H, W, focal=95.0)
+    fr = frender.FRender(osc, params, centers, H=H, W=W)
+    fr.set_camera(cam)
+    frame = fr.render().cpu().numpy()
+    diffuse = fr.frame_diffuse.cpu().numpy().reshape(-1, 3)
+    spec_up = fr.frame_speculars.cpu().numpy()
+    # diffuse layer through the oracle (rays and tile lists are bit-exact, tested elsewhere)
+    dirs = oracle_mod.compute_rays(H, W, cam.intrinsic, cam.c2w)
+    vis = oracle_mod.trace_tiles(oracle_mod.OracleScene(_grid_scene(osc)), cam.origin, dirs, 30)
+    rf, ri, rn, rs = oracle_mod.render_octree(osc, cam.origin, dirs, vis, trans_th=0.8, early_stop=0.01)
+    assert np.abs(diffuse - rf).max() <= 1e-4
+    # reflection layer: the oracle chain on the downscaled grid
+    DH, DW = H // 2, W // 2
+    inv_near = ri.reshape(H, W)[::2, ::2].copy()
+    net = rn.reshape(H, W)[::2, ::2].ravel().copy()
+    qpi, bnet, bps = S.reflection_blocks(net)
+    far = np.float32(fr.far)
+    inv_bound = ((np.float32(1.0) / (np.float32(1.0) / inv_near + far) - inv_near) / np.float32(63)).astype(np.float32)
+    dirs_ds = np.ascontiguousarray(dirs.reshape(H, W, 3)[::2, ::2])
+    T = np.ones(DH * DW, np.float32)
+    col = np.zeros((DH * DW, 3), np.float32)
+    for step in range(0, 64, 16):
+        pts, st = oracle_mod.inverse_z_sampling(cam.origin, dirs_ds.reshape(-1, 3), inv_near, inv_bound, step, 16)
+        res = np.zeros((DH * DW * 16, 4), np.float32)
+        oracle_mod.inference_mlp(16, bnet, bps, qpi, T, centers, pts, dirs_ds, params, 1.0 / fr.far, 0.01, res,
+                                 quant16=True)
+        oracle_mod.integrate_points(res, st, np.float32(0.01), T, col)
+    up = F.interpolate(torch.from_numpy(col.reshape(1, DH, DW, 3)).permute(0, 3, 1, 2), scale_factor=2,
+                       mode="bilinear", align_corners=True)[0].permute(1, 2, 0).numpy()
+    err = np.abs(spec_up - up)
+    assert err.max() <= 2e-2 and err.mean() <= 1e-3, (err.max(), err.mean())
+    assert np.array_equal(frame, np.clip(diffuse.reshape(H, W, 3) + spec_up, 0, 1))
+    assert fr.n_reflection_blocks > 0 and (net >= 0).mean() > 0.3
+
+
+def _grid_scene(osc):
+    """A brick-format scene with the octree scene's tile grid, for the oracle's tile tracer."""
+    sc = S.make_scene(0, tile_shape=osc.tile_shape, tile_size=osc.tile_size, scene_min=tuple(osc.scene_min), res=8,
+                      occupancy=0.0)
+    sc.tile_index_map = osc.tile_index_map.copy()
+    return sc
