@@ -83,7 +83,7 @@ def train_bench(args, dev):
     sigma = ds.sigma.clone().requires_grad_(True)
     feats = ds.feats.clone().requires_grad_(True)
     mlp = ds.mlp_params.clone().requires_grad_(True)
-    opt = torch.optim.Adam([sigma, feats, mlp], lr=1e-3)
+    opt = torch.optim.Adam([sigma, feats, mlp], lr=1e-4)
     ws = TR.TrainWorkspace()
     losses = []
 
@@ -100,14 +100,14 @@ def train_bench(args, dev):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.train_steps):
-        losses.append(step())
+        losses.append(step().detach())
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.train_steps
     n = o.shape[0]
     return {"train_mrays_s": round(n / (ms * 1e-3) / 1e6, 3), "train_ms_per_step": round(ms, 3),
             "train_rays_per_step": int(n), "train_loss_first_last": [float(losses[0]), float(losses[-1])],
             "train_config": "C3: C2 scene with FP32 features/MLP, 16 tiles x 8192 rays, MSE vs a perturbed "
-                            "copy, fwd + hand-written adjoint + Adam"}
+                            "copy, fwd + hand-written adjoint + Adam (lr 1e-4)"}
 
 
 def ref_pipeline_bench(args, dev):

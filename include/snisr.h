@@ -116,7 +116,7 @@ typedef struct snisr_render_out {
 
 const char* snisr_last_error(void);
 int snisr_version(void);
-/* sizeof of the descriptors (0 scene, 1 march params, 2 rays, 3 render out, 4 mlp query, 5 octree) so that foreign
+/* sizeof of the descriptors (0 scene, 1 march params, 2 rays, 3 render out, 4 mlp query, 5 octree, 6 tuning) so that foreign
  * bindings can check their struct layouts; no reference counterpart. */
 int64_t snisr_abi_sizeof(int32_t which);
 
@@ -140,6 +140,18 @@ int snisr_trace_tiles(const snisr_scene* scene, const float* origin, const float
 int snisr_render_fp32(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                       const snisr_render_out* out, void* stream);
 
+/* Launch schedule of snisr_render_mfma_tuned (chosen per workload by an autotuner; no reference
+ * counterpart).  ray_order 0: rays in index order (8x8 pixel packets when image_width allows),
+ * 1: rays counting-sorted by (first occupied tile, direction octant) before the march passes;
+ * mlp_threads: 256 or 512 threads per MLP workgroup (one workgroup per CU); sort_grid: workgroups
+ * of the record / ray counting sorts.  Results do not depend on the schedule. */
+typedef struct snisr_tuning {
+  int32_t ray_order;
+  int32_t mlp_threads;
+  int32_t sort_grid;
+  int32_t reserved;
+} snisr_tuning;
+
 /* Workspace bytes for snisr_render_mfma on n rays with room for `sample_capacity` samples
  * (0 = a default of 48 samples per ray). */
 int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t sample_capacity);
@@ -152,6 +164,11 @@ int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, in
  * snisr_render_fp32; the pixels are deterministic. */
 int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                       const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* snisr_render_mfma with an explicit launch schedule (NULL = the defaults {0, 512, 512, 0}). */
+int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
+                            const snisr_render_out* out, const snisr_tuning* tuning, void* workspace,
+                            int64_t workspace_bytes, void* stream);
 
 /* Samples emitted by the last snisr_render_mfma on `workspace` and whether a ray overflowed the
  * record capacity (its rgb is then NaN).  Synchronises `stream`.  No reference counterpart. */

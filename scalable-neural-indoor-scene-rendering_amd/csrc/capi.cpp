@@ -86,6 +86,7 @@ int64_t snisr_abi_sizeof(int32_t which) {
     case 3: return (int64_t)sizeof(snisr_render_out);
     case 4: return (int64_t)sizeof(snisr_mlp_query);
     case 5: return (int64_t)sizeof(snisr_octree);
+    case 6: return (int64_t)sizeof(snisr_tuning);
     default: return -1;
   }
 }

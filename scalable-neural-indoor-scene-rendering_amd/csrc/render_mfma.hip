@@ -126,14 +126,24 @@ struct Workspace {
   float4* crec;         // [cap] training: unweighted colour of every record
   float* dsig;          // [cap] training: dL/dsigma of every record
   float* act;           // training: activations/deltas per sorted record (train_act_floats each)
+  int32_t* order;         // [n] ray schedule: thread -> ray (when the ray-order pass runs)
+  int32_t* ray_key;       // [n] schedule key of every ray
+  int32_t* bucket_count;  // [n_buckets]
+  int64_t* bucket_cursor; // [n_buckets]
   int64_t cap;
   int32_t n_tiles;
+  int32_t n_buckets;
 };
 
 constexpr int kScanItems = 8, kScanBlock = 256, kScanTile = kScanItems * kScanBlock;
 constexpr int kMaxTiles = 4096;  // tile histogram in LDS
 
 static inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+// schedule keys: first occupied tile along the ray (T when none) x 8 direction octants, or the tile
+// alone when that would exceed the LDS histogram
+constexpr int kMaxBuckets = 8192;
+__host__ __device__ constexpr int32_t key_buckets(int32_t T) { return 8 * (T + 1) <= kMaxBuckets ? 8 * (T + 1) : T + 1; }
 
 static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspace* w, int64_t train_floats = 0) {
   const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
@@ -145,6 +155,11 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
   const int64_t o_ts = off; off = align_up(off + 8 * ((int64_t)T + 1), 256);
   const int64_t o_cu = off; off = align_up(off + 8 * (int64_t)T, 256);
   const int64_t o_enc = off; off = align_up(off + 32 * n, 256);
+  const int32_t nbk = key_buckets(T);
+  const int64_t o_ord = off; off = align_up(off + 4 * n, 256);
+  const int64_t o_key = off; off = align_up(off + 4 * n, 256);
+  const int64_t o_bkc = off; off = align_up(off + 4 * (int64_t)nbk, 256);
+  const int64_t o_bku = off; off = align_up(off + 8 * (int64_t)nbk, 256);
   const int64_t o_pm = off; off = align_up(off + 4 * (cap + 64 * (int64_t)T), 256);
   const int64_t o_rf = off; off = align_up(off + 16 * cap, 256);
   const int64_t o_ti = off; off = align_up(off + 8 * cap, 256);
@@ -174,8 +189,13 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
     w->crec = train_floats > 0 ? reinterpret_cast<float4*>(base + o_cr) : nullptr;
     w->dsig = train_floats > 0 ? reinterpret_cast<float*>(base + o_ds) : nullptr;
     w->act = train_floats > 0 ? reinterpret_cast<float*>(base + o_act) : nullptr;
+    w->order = reinterpret_cast<int32_t*>(base + o_ord);
+    w->ray_key = reinterpret_cast<int32_t*>(base + o_key);
+    w->bucket_count = reinterpret_cast<int32_t*>(base + o_bkc);
+    w->bucket_cursor = reinterpret_cast<int64_t*>(base + o_bku);
     w->cap = cap;
     w->n_tiles = T;
+    w->n_buckets = nbk;
   }
   return off;
 }
@@ -185,10 +205,11 @@ static int64_t default_capacity(int64_t n) { return 48 * n; }
 // ------------------------------------------------------------------------- march passes
 template <int MODE>  // 0: count, 1: emit records, 2: emit records + training extras
 __global__ __launch_bounds__(64) void k_march_pass(snisr_scene S, snisr_march_params P, snisr_rays R,
-                                                   snisr_render_out O, Workspace W, int packet_w, int enc_dtype) {
+                                                   snisr_render_out O, Workspace W, int packet_w, int enc_dtype,
+                                                   const int32_t* __restrict__ order) {
   const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
   if (tid >= R.n) return;
-  const int64_t i = ray_of_thread(tid, R.n, packet_w);
+  const int64_t i = order ? (int64_t)order[tid] : ray_of_thread(tid, R.n, packet_w);
   const float* og = R.origins + i * R.origin_stride;
   const V3 origin = v3(og[0], og[1], og[2]);
   const V3 d = normalize_dir(v3(R.dirs[3 * i], R.dirs[3 * i + 1], R.dirs[3 * i + 2]));
@@ -405,6 +426,80 @@ __global__ __launch_bounds__(kSortBlock) void k_tile_scatter(Workspace W) {
 
 
 
+
+// ------------------------------------------------------------------------- ray schedule
+// Orders the rays by (first occupied tile, direction octant) so that the wavefronts of the march
+// passes walk the same tile in similar directions: a counting sort (LDS histograms, one scan,
+// atomic bucket cursors).  Only the thread -> ray map changes; every ray's records and pixel are
+// the same as without the schedule.
+__global__ __launch_bounds__(256) void k_ray_key(snisr_scene S, snisr_rays R, Workspace W) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < W.n_buckets) W.bucket_count[i] = 0;
+  if (i >= R.n) return;
+  const float* og = R.origins + i * R.origin_stride;
+  const V3 o = v3(og[0], og[1], og[2]);
+  const V3 d = normalize_dir(v3(R.dirs[3 * i], R.dirs[3 * i + 1], R.dirs[3 * i + 2]));
+  TileWalker w;
+  w.init(S, o, d, 1);
+  int t = w.next();
+  if (t < 0) t = W.n_tiles;
+  const int oct = (d.x < 0.0f ? 1 : 0) + (d.y < 0.0f ? 2 : 0) + (d.z < 0.0f ? 4 : 0);
+  W.ray_key[i] = W.n_buckets == 8 * (W.n_tiles + 1) ? 8 * t + oct : t;
+}
+
+__device__ __forceinline__ void ray_range(int64_t n, int64_t& b, int64_t& e) {
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  b = (int64_t)blockIdx.x * per;
+  e = b + per < n ? b + per : n;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_key_hist(Workspace W, int64_t n) {
+  __shared__ int hist[kMaxBuckets];
+  const int nb = W.n_buckets;
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
+  int64_t b, e;
+  ray_range(n, b, e);
+  for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) atomicAdd(&hist[W.ray_key[k]], 1);
+  __syncthreads();
+  for (int t = threadIdx.x; t < nb; t += blockDim.x)
+    if (hist[t]) atomicAdd(&W.bucket_count[t], hist[t]);
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_key_scan(Workspace W) {
+  const int nb = W.n_buckets;
+  int64_t carry = 0;
+  for (int start = 0; start < nb; start += kScanBlock) {
+    const int t = start + threadIdx.x;
+    const int64_t v = t < nb ? (int64_t)W.bucket_count[t] : 0;
+    int64_t tot;
+    const int64_t ex = block_excl_scan(v, &tot);
+    if (t < nb) W.bucket_cursor[t] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_key_scatter(Workspace W, int64_t n) {
+  __shared__ int hist[kMaxBuckets];
+  __shared__ int64_t base[kMaxBuckets];
+  const int nb = W.n_buckets;
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
+  int64_t b, e;
+  ray_range(n, b, e);
+  for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) atomicAdd(&hist[W.ray_key[k]], 1);
+  __syncthreads();
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+    base[t] = hist[t] ? atomicAdd((unsigned long long*)&W.bucket_cursor[t], (unsigned long long)hist[t]) : 0;
+    hist[t] = 0;
+  }
+  __syncthreads();
+  for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+    const int t = W.ray_key[k];
+    const int r = atomicAdd(&hist[t], 1);
+    W.order[base[t] + r] = (int32_t)k;
+  }
+}
 
 // Eight trilinear corner weights (corner order x0y0z0, x1y0z0, ... as in tri_combine) as half2
 // broadcasts, 0 for empty corners; the slots of empty corners are redirected to slot 0 so every
@@ -928,6 +1023,17 @@ int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, in
 
 int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                       const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream) {
+  return snisr_render_mfma_tuned(scene, params, rays, out, nullptr, workspace, workspace_bytes, stream);
+}
+
+int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
+                            const snisr_render_out* out, const snisr_tuning* tuning, void* workspace,
+                            int64_t workspace_bytes, void* stream) {
+  const snisr_tuning tn = tuning ? *tuning : snisr_tuning{0, 512, kSortGrid, 0};
+  SNISR_REQUIRE(tn.ray_order == 0 || tn.ray_order == 1, SNISR_ERR_ARG, "render_mfma: ray_order must be 0 or 1");
+  SNISR_REQUIRE(tn.mlp_threads == 256 || tn.mlp_threads == 512, SNISR_ERR_ARG,
+                "render_mfma: mlp_threads must be 256 or 512");
+  SNISR_REQUIRE(tn.sort_grid >= 1 && tn.sort_grid <= 65536, SNISR_ERR_ARG, "render_mfma: sort_grid out of range");
   SNISR_REQUIRE(out && out->rgb, SNISR_ERR_ARG, "render_mfma: out.rgb is required");
   if (int e = check_scene(scene, true)) return e;
   if (int e = check_params(params)) return e;
@@ -957,7 +1063,20 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
   const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
 
   SNISR_HIP(stage_event(out, 0, st));
-  hipLaunchKernelGGL(k_march_pass<0>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt);
+  const int32_t* order = nullptr;
+  if (tn.ray_order == 1) {
+    const int64_t kn = n > W.n_buckets ? n : W.n_buckets;
+    hipLaunchKernelGGL(k_ray_key, dim3((unsigned)((kn + 255) / 256)), dim3(256), 0, st, *scene, *rays, W);
+    SNISR_LAUNCHED();
+    hipLaunchKernelGGL(k_key_hist, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W, n);
+    SNISR_LAUNCHED();
+    hipLaunchKernelGGL(k_key_scan, dim3(1), dim3(kScanBlock), 0, st, W);
+    SNISR_LAUNCHED();
+    hipLaunchKernelGGL(k_key_scatter, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W, n);
+    SNISR_LAUNCHED();
+    order = W.order;
+  }
+  hipLaunchKernelGGL(k_march_pass<0>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt, order);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 1, st));
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.counts, n, W.offsets,
@@ -969,25 +1088,29 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.offsets, n, W.block_sums);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 2, st));
-  hipLaunchKernelGGL(k_march_pass<1>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt);
+  hipLaunchKernelGGL(k_march_pass<1>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt, order);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 3, st));
 
-  hipLaunchKernelGGL(k_tile_hist, dim3(kSortGrid), dim3(kSortBlock), 0, st, W);
+  hipLaunchKernelGGL(k_tile_hist, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W);
   SNISR_LAUNCHED();
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kScanBlock), 0, st, W);
   SNISR_LAUNCHED();
-  hipLaunchKernelGGL(k_tile_scatter, dim3(kSortGrid), dim3(kSortBlock), 0, st, W);
+  hipLaunchKernelGGL(k_tile_scatter, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 4, st));
 
-  const dim3 grid((unsigned)device_cus()), block(512);
+  const dim3 grid((unsigned)device_cus()), block((unsigned)tn.mlp_threads);
   bool launched = false;
 #define SNISR_MFMA_CASE(F_, W_, L_, H_, D_)                                                        \
   if (!launched && F == F_ && WD == W_ && L == L_ && half == H_ && dt == D_) {                      \
     auto kern = k_mlp_mfma<F_, W_, L_, H_, D_>;                                                     \
     const size_t lds = (size_t)tile_bytes(F_ + 15, W_, L_);                                         \
-    SNISR_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+    static std::atomic<bool> lds_set{false};                                                        \
+    if (!lds_set.load()) {                                                                          \
+      SNISR_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+      lds_set.store(true);                                                                          \
+    }                                                                                               \
     hipLaunchKernelGGL(kern, grid, block, lds, st, *scene, *rays, W);                               \
     launched = true;                                                                                \
   }
@@ -1067,7 +1190,8 @@ int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* para
 #undef SNISR_CHECK_SHAPE
   SNISR_REQUIRE(ok, SNISR_ERR_SHAPE, "train_forward: unsupported shape F=%d width=%d layers=%d", F, WD, L);
   SNISR_HIP(stage_event(out, 0, st));
-  hipLaunchKernelGGL(k_march_pass<0>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, (int)SNISR_F16);
+  hipLaunchKernelGGL(k_march_pass<0>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, (int)SNISR_F16,
+                     (const int32_t*)nullptr);
   SNISR_LAUNCHED();
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.counts, n, W.offsets,
                      W.block_sums);
@@ -1077,7 +1201,8 @@ int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* para
   SNISR_LAUNCHED();
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.offsets, n, W.block_sums);
   SNISR_LAUNCHED();
-  hipLaunchKernelGGL(k_march_pass<2>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, (int)SNISR_F16);
+  hipLaunchKernelGGL(k_march_pass<2>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, (int)SNISR_F16,
+                     (const int32_t*)nullptr);
   SNISR_LAUNCHED();
   hipLaunchKernelGGL(k_tile_hist, dim3(kSortGrid), dim3(kSortBlock), 0, st, W);
   SNISR_LAUNCHED();

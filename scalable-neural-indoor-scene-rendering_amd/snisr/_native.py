@@ -57,6 +57,11 @@ class OctreeDesc(C.Structure):
                 ("sample_step", C.c_float), ("num_voxel", C.c_int32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("ray_order", C.c_int32), ("mlp_threads", C.c_int32), ("sort_grid", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every exported entry point declared in include/snisr.h
@@ -72,6 +77,8 @@ SIGNATURES = [
     ("snisr_render_mfma_workspace_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int64, C.c_int64]),
     ("snisr_render_mfma", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
                                     C.POINTER(RenderOut), C.c_void_p, C.c_int64, C.c_void_p]),
+    ("snisr_render_mfma_tuned", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Tuning),
+                                          C.c_void_p, C.c_int64, C.c_void_p]),
     ("snisr_render_mfma_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_void_p]),
     ("snisr_bvh_build_bounds", C.c_int, [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("snisr_bvh_build", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
@@ -117,7 +124,7 @@ def lib():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    for which, st in enumerate((SceneDesc, MarchParams, RaysDesc, RenderOut, MlpQuery, OctreeDesc)):
+    for which, st in enumerate((SceneDesc, MarchParams, RaysDesc, RenderOut, MlpQuery, OctreeDesc, Tuning)):
         got = L.snisr_abi_sizeof(which)
         if got != C.sizeof(st):
             raise RuntimeError(f"snisr: ABI mismatch for {st.__name__}: C {got} vs ctypes {C.sizeof(st)}")

@@ -216,27 +216,42 @@ def mfma_stats(buf) -> tuple[int, int]:
     return total.value, over.value
 
 
+def tuning_struct(t: dict | None):
+    """snisr_tuning from a dict {ray_order, mlp_threads, sort_grid} (None = library defaults)."""
+    if t is None:
+        return None
+    return N.Tuning(int(t.get("ray_order", 0)), int(t.get("mlp_threads", 512)), int(t.get("sort_grid", 512)), 0)
+
+
 def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_width=0, brick_cap=0,
                 workspace: MfmaWorkspace | None = None, capacity: int = 0, check: bool = True, events=None,
-                **march_kw):
-    """Render with the per-tile MLP on MFMA (snisr_render_mfma).  With ``check`` the record
-    capacity is verified after the call and the render repeated with room for every sample."""
+                tuning: dict | None = None, outputs: dict | None = None, **march_kw):
+    """Render with the per-tile MLP on MFMA (snisr_render_mfma_tuned).  With ``check`` the record
+    capacity is verified after the call and the render repeated with room for every sample.
+    ``outputs`` (from a previous call) is written in place instead of allocating new tensors."""
     if ds.mlp_packed is None:
         raise RuntimeError("scene has no MFMA-packed MLP; call DeviceScene.pack_mfma()")
     r = _rays_desc(origins, dirs, t_hit, visited, image_width)
     p = ds.march_params(**march_kw)
-    out, o = _render_outputs(r.n, dirs.device, True, brick_cap)
+    if outputs is None:
+        out, o = _render_outputs(r.n, dirs.device, True, brick_cap)
+    else:
+        out, o = outputs, N.RenderOut()
+        o.rgb, o.T = out["rgb"].data_ptr(), out["T"].data_ptr()
+        o.n_samples, o.depth = out["n_samples"].data_ptr(), out["depth"].data_ptr()
     keep = _attach_events(o, events)  # noqa: F841
     ws = workspace or MfmaWorkspace()
     buf, need = ws.get(ds, r.n, capacity)
-    N.check(N.lib().snisr_render_mfma(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), _ptr(buf), need,
-                                      _stream()), "render_mfma")
+    tn = tuning_struct(tuning)
+    tp = C.byref(tn) if tn is not None else None
+    N.check(N.lib().snisr_render_mfma_tuned(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), tp, _ptr(buf),
+                                            need, _stream()), "render_mfma")
     if check:
         total, over = mfma_stats(buf)
         if over:
             buf, need = ws.get(ds, r.n, int(total * 1.25) + 64)
-            N.check(N.lib().snisr_render_mfma(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), _ptr(buf),
-                                              need, _stream()), "render_mfma")
+            N.check(N.lib().snisr_render_mfma_tuned(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), tp,
+                                                    _ptr(buf), need, _stream()), "render_mfma")
         out["total_samples"] = total
     return out
 

@@ -316,6 +316,30 @@ def perturbed_copy(sc: Scene, seed: int, rel: float = 0.01) -> Scene:
     return c
 
 
+def config_c4(seed: int = 4, n_rays: int = 4_000_000, zipf_s: float = 1.2, tile_shape=(4, 4, 4)):
+    """Batched uneven tiles: 64 tiles (4x4x4 of unit size) with grids drawn from {64, 96, 128}^3 and
+    brick occupancy from U(0.02, 0.15), F=32 FP16 features, per-tile width-128 x4 MLP (run in BF16
+    on MFMA).  Per-tile ray counts follow Zipf(zipf_s) over a seeded ranking of the tiles and sum to
+    n_rays; every ray starts at a seeded point of the scene volume and points at a seeded point inside
+    its tile.  Returns (scene, origins f32[N,3], dirs f32[N,3], rays per tile i64[T])."""
+    sc = make_scene(seed, tile_shape=tile_shape, tile_size=1.0, res_choices=(64, 96, 128),
+                    occupancy_range=(0.02, 0.15), feat_dim=32, feat_half=True, mlp_width=128, mlp_layers=4,
+                    sample_step=0.2 / 128, early_stop=1e-3)
+    rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence(seed).spawn(11)[10]))
+    T = sc.n_tiles
+    rank = rng.permutation(T) + 1
+    w = rank.astype(np.float64) ** (-zipf_s)
+    counts = np.floor(n_rays * w / w.sum()).astype(np.int64)
+    counts[np.argsort(rank)[: n_rays - int(counts.sum())]] += 1     # remainder to the top ranks
+    hi = np.asarray(tile_shape, np.float64) * sc.tile_size + sc.scene_min
+    lo = sc.scene_min.astype(np.float64)
+    tile_of_ray = np.repeat(np.arange(T), counts)
+    org = lo + (hi - lo) * rng.uniform(0.0, 1.0, (n_rays, 3))
+    tgt = sc.tile_centers[tile_of_ray].astype(np.float64) + rng.uniform(-0.5, 0.5, (n_rays, 3)) * sc.tile_size
+    perm = rng.permutation(n_rays)           # rays of all tiles interleaved, as a batch would arrive
+    return sc, org[perm].astype(np.float32), (tgt - org)[perm].astype(np.float32), counts
+
+
 def config_c5(seed: int = 5, H: int = 1080, W: int = 1920, n_views: int = 4, n_triangles: int = 2_000_000):
     """256 tiles (16x16 floor plan) of 128^3 at 4%, F=32 FP16, width-128x4 MLP, 2M-triangle mesh,
     n_views seeded 1920x1080 views."""
