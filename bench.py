@@ -4,7 +4,10 @@
 One step = one frame through the hot path with every input resident in HBM:
 camera rays -> BVH nearest hit on the 200k-triangle proxy mesh (bounds the march) -> tile DDA,
 brick DDA march and sigma compositing -> per-tile width-128x4 MLP on FP16 MFMA with FP32
-accumulation -> ordered per-ray reduction.  Prints one JSON line (see the field notes in main()).
+accumulation -> ordered per-ray reduction.  The timed steps replay the step captured once in a HIP
+graph; per-stage kernel times come from eager steps with HIP events between the stages.  Further
+legs (plain FP32 path, C3 training, the reference-format FRender pipeline, C4 batched uneven tiles,
+C5 large scene, CPU oracle baseline) add fields to the same JSON line.
 """
 from __future__ import annotations
 
@@ -139,6 +142,151 @@ def ref_pipeline_bench(args, dev):
                                    "FP16 MFMA, integrate_points) at 640x360, bilinear upsample; seeds 12/13/2"}
 
 
+def c4_bench(args, dev):
+    """C4: batched uneven tiles.  64 tiles with grids from {64,96,128}^3 and brick occupancy
+    U(0.02,0.15), width-128x4 MLP in BF16 on MFMA, 4M rays with Zipf(1.2) per-tile counts, rendered
+    by one launch sequence (persistent MLP kernel) captured in a HIP graph; the launch schedule is
+    chosen by the autotuner."""
+    import numpy as np
+    import torch
+    from snisr import _native as N
+    from snisr import autotune
+    from snisr import device as D
+    from snisr import graph as G
+    from snisr import scene as S
+
+    sc, o_np, d_np, counts = S.config_c4()
+    ds = D.DeviceScene(sc, mfma_dtype=N.SNISR_BF16)
+    o = torch.from_numpy(o_np).to(dev)
+    d = torch.from_numpy(d_np).to(dev)
+    n = d.shape[0]
+    ws = D.MfmaWorkspace()
+    best, table = autotune.tune_render(ds, o, d, workspace=ws, reps=2)
+    # eager (default schedule and tuned) and graph replay of the tuned schedule
+    def timed(fn, k):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / k
+    out_e = D.render_mfma(ds, o, d, workspace=ws, check=True)
+    eager_default = timed(lambda: D.render_mfma(ds, o, d, workspace=ws, check=False, outputs=out_e), args.c4_steps)
+    eager_tuned = timed(lambda: D.render_mfma(ds, o, d, workspace=ws, check=False, tuning=best, outputs=out_e),
+                        args.c4_steps)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
+    for e in ev:
+        e.record()
+    D.render_mfma(ds, o, d, workspace=ws, check=False, tuning=best, outputs=out_e, events=ev)
+    torch.cuda.synchronize()
+    stage = [ev[j].elapsed_time(ev[j + 1]) for j in range(6)]
+    g = G.RenderGraph(ds, o, d, tuning=best)
+    graph_ms = timed(g.replay, args.c4_steps)
+    rgb = g.replay()["rgb"]
+    total = int(out_e["total_samples"])
+    res = {"c4_mrays_s": round(n / graph_ms / 1e3, 2), "c4_ms_per_batch": round(graph_ms, 3),
+           "c4_eager_default_ms": round(eager_default, 3), "c4_eager_tuned_ms": round(eager_tuned, 3),
+           "c4_rays": int(n), "c4_samples": total, "c4_tuning": best,
+           "c4_stage_ms": {k: round(float(v), 3) for k, v in zip(
+               ["schedule+march_count", "scan", "march_emit", "tile_sort", "mlp_mfma", "reduce"], stage)},
+           "c4_tuning_table_ms": {k: round(v, 3) for k, v in table.items()},
+           "c4_config": "64 tiles (4x4x4), grids {64,96,128}^3, occupancy U(0.02,0.15), F=32 FP16, MLP 47-128x3-3 "
+                        "BF16 MFMA FP32 acc, 4M rays Zipf(1.2) per tile, HIP-graph replay, seed 4"}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        idx = np.sort(np.random.default_rng(4).choice(n, 2000, replace=False))
+        ref = O.render(O.OracleScene(sc), o_np, d_np, ray_index=idx, precision="f64")
+        mse = float(np.mean((rgb.cpu().numpy()[idx].astype(np.float64) - ref["rgb"]) ** 2))
+        res["c4_psnr_vs_fp64_db"] = round(10 * np.log10(1.0 / max(mse, 1e-30)), 2)
+    return res
+
+
+def c5_bench(args, dev):
+    """C5: large scene.  256 tiles (16x16 floor plan) of 128^3 at 4% occupancy, F=32 FP16 features,
+    per-tile width-128x4 MLP (FP16 MFMA, FP32 accumulation), 2M-triangle proxy mesh; 4 seeded
+    1920x1080 views batched into one render (8.3M rays, per-ray origins).  One step = rays of the 4
+    views, BVH first hit, the MFMA render; the step is captured once in a HIP graph and replayed."""
+    import numpy as np
+    import torch
+    from snisr import _native as N
+    from snisr import device as D
+    from snisr import scene as S
+
+    sc, cams, mesh = S.config_c5()
+    ds = D.DeviceScene(sc, mfma_dtype=N.SNISR_F16)
+    dm = D.DeviceMesh(mesh.vertices, mesh.faces)
+    V, H, W = len(cams), cams[0].H, cams[0].W
+    n = V * H * W
+    intr = [torch.from_numpy(c.intrinsic).to(dev) for c in cams]
+    c2w = [torch.from_numpy(c.c2w).to(dev) for c in cams]
+    origins = torch.from_numpy(np.repeat(np.stack([c.origin for c in cams]), H * W, axis=0)).to(dev)
+    rays = torch.empty((V, H, W, 3), dtype=torch.float32, device=dev)
+    t_hit = torch.empty(n, dtype=torch.float32, device=dev)
+    tri = torch.empty(n, dtype=torch.int32, device=dev)
+    ws = D.MfmaWorkspace()
+
+    def trace():
+        for v in range(V):
+            D.compute_rays(intr[v], c2w[v], H, W, out=rays[v])
+        dm.first_hit(origins, rays, image_width=W, t_out=t_hit, id_out=tri)
+
+    trace()
+    first = D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, check=True)
+    outs = {k: first[k] for k in ("rgb", "T", "n_samples", "depth")}
+    cap = int(first["total_samples"]) + 64
+
+    def step():
+        trace()
+        D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap, check=False,
+                      outputs=outs)
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
+    for e in ev:
+        e.record()
+    trace()
+    D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap, check=False,
+                  outputs=outs, events=ev)
+    torch.cuda.synchronize()
+    stage = [ev[j].elapsed_time(ev[j + 1]) for j in range(6)]
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        step()
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.c5_steps):
+        g.replay()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.c5_steps
+    total, over = D.mfma_stats(ws.buf)
+    assert over == 0
+    flops = total * mlp_flops_per_sample(sc.mlp_in, sc.mlp_width, sc.mlp_layers)
+    res = {"c5_mrays_s": round(n / ms / 1e3, 2), "c5_ms_per_step": round(ms, 3), "c5_rays": int(n),
+           "c5_samples": int(total),
+           "c5_stage_ms": {k: round(float(v), 3) for k, v in zip(
+               ["march_count", "scan", "march_emit", "tile_sort", "mlp_mfma", "reduce"], stage)},
+           "c5_mlp_tflops": round(flops / (stage[4] * 1e-3) / 1e12, 1),
+           "c5_config": "256 tiles (16x16x1) of 128^3 at 4%, F=32 FP16, MLP 47-128x3-3 FP16 MFMA FP32 acc, "
+                        "2M-triangle BVH first hit, 4 seeded 1920x1080 views in one batch, HIP-graph replay, seed 5"}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        idx = np.sort(np.random.default_rng(5).choice(n, 2000, replace=False))
+        ref = O.render(O.OracleScene(sc), origins.cpu().numpy(), rays.reshape(-1, 3).cpu().numpy(),
+                       t_hit=t_hit.cpu().numpy(), ray_index=idx, precision="f64")
+        mse = float(np.mean((outs["rgb"].cpu().numpy()[idx].astype(np.float64) - ref["rgb"]) ** 2))
+        res["c5_psnr_vs_fp64_db"] = round(10 * np.log10(1.0 / max(mse, 1e-30)), 2)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +296,8 @@ def main():
     ap.add_argument("--cpu-frac", type=float, default=0.05)
     ap.add_argument("--fp32-frames", type=int, default=3)
     ap.add_argument("--train-steps", type=int, default=10, help="timed C3 training steps (0 = skip)")
+    ap.add_argument("--c4-steps", type=int, default=5, help="timed C4 batches (0 = skip)")
+    ap.add_argument("--c5-steps", type=int, default=3, help="timed C5 steps (0 = skip)")
     ap.add_argument("--ref-frames", type=int, default=10,
                     help="timed frames of the reference-format FRender pipeline (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -203,6 +353,7 @@ def main():
     total_samples, overflow = D.mfma_stats(ws.buf)
     assert overflow == 0
 
+    # per-stage kernel times from eager steps with HIP events between the stages
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(args.steps)]
     for row in ev:
         for e in row:
@@ -212,8 +363,39 @@ def main():
     for k in range(args.steps):
         out = step(ev[k])
     torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the timed steps: the whole step (rays, mesh hit, render) captured once in a HIP graph
+    outs = {k: out[k] for k in ("rgb", "T", "n_samples", "depth")}
+    cap = int(total_samples) + 64
+    ws.get(ds, n_rays, cap)
+
+    def gstep():
+        D.compute_rays(intr, c2w, H, W, out=rays_d)
+        dm.first_hit(origin, rays_d, image_width=W, t_out=t_hit, id_out=tri_id)
+        D.render_mfma(ds, origin, rays_d, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap, check=False,
+                      outputs=outs)
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        gstep()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        gstep()
+    for _ in range(max(1, args.warmup)):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    assert D.mfma_stats(ws.buf)[1] == 0
+    assert torch.equal(outs["rgb"], out["rgb"])   # the replay renders the same frame
+    out = outs
     stage = np.array([[ev[k][j].elapsed_time(ev[k][j + 1]) for j in range(6)] for k in range(args.steps)])
     stage_ms = stage.mean(0)
     mlp_ms = float(stage_ms[4])
@@ -247,6 +429,10 @@ def main():
         extra.update(train_bench(args, dev))
     if args.ref_frames > 0:
         extra.update(ref_pipeline_bench(args, dev))
+    if args.c4_steps > 0:
+        extra.update(c4_bench(args, dev))
+    if args.c5_steps > 0:
+        extra.update(c5_bench(args, dev))
 
     cpu = None
     if not args.no_cpu:
@@ -276,6 +462,7 @@ def main():
             "samples_per_frame": int(total_samples),
             "mesh_triangles": int(mesh.faces.shape[0]),
         },
+        "eager_ms_per_step": round(eager_ms, 4),
         "stage_ms": {k: round(float(v), 4) for k, v in zip(
             ["march_count", "scan", "march_emit", "tile_sort", "mlp_mfma", "reduce"], stage_ms)},
         "roofline": roofline,
