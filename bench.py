@@ -189,7 +189,7 @@ def c4_bench(args, dev):
            "c4_eager_default_ms": round(eager_default, 3), "c4_eager_tuned_ms": round(eager_tuned, 3),
            "c4_rays": int(n), "c4_samples": total, "c4_tuning": best,
            "c4_stage_ms": {k: round(float(v), 3) for k, v in zip(
-               ["schedule+march_count", "scan", "march_emit", "tile_sort", "mlp_mfma", "reduce"], stage)},
+               ["schedule", "march", "tile_hist", "tile_scatter", "mlp_mfma", "reduce"], stage)},
            "c4_tuning_table_ms": {k: round(v, 3) for k, v in table.items()},
            "c4_config": "64 tiles (4x4x4), grids {64,96,128}^3, occupancy U(0.02,0.15), F=32 FP16, MLP 47-128x3-3 "
                         "BF16 MFMA FP32 acc, 4M rays Zipf(1.2) per tile, HIP-graph replay, seed 4"}
@@ -235,7 +235,7 @@ def c5_bench(args, dev):
     trace()
     first = D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, check=True)
     outs = {k: first[k] for k in ("rgb", "T", "n_samples", "depth")}
-    cap = int(first["total_samples"]) + 64
+    cap = D.records_for(first["max_samples_per_ray"])
 
     def step():
         trace()
@@ -272,7 +272,7 @@ def c5_bench(args, dev):
     res = {"c5_mrays_s": round(n / ms / 1e3, 2), "c5_ms_per_step": round(ms, 3), "c5_rays": int(n),
            "c5_samples": int(total),
            "c5_stage_ms": {k: round(float(v), 3) for k, v in zip(
-               ["march_count", "scan", "march_emit", "tile_sort", "mlp_mfma", "reduce"], stage)},
+               ["schedule", "march", "tile_hist", "tile_scatter", "mlp_mfma", "reduce"], stage)},
            "c5_mlp_tflops": round(flops / (stage[4] * 1e-3) / 1e12, 1),
            "c5_config": "256 tiles (16x16x1) of 128^3 at 4%, F=32 FP16, MLP 47-128x3-3 FP16 MFMA FP32 acc, "
                         "2M-triangle BVH first hit, 4 seeded 1920x1080 views in one batch, HIP-graph replay, seed 5"}
@@ -346,6 +346,7 @@ def main():
 
     # size the workspace once (exact sample count of this view), then warm up
     first = D.render_mfma(ds, origin, D.compute_rays(intr, c2w, H, W), image_width=W, workspace=ws, check=True)
+    max_per_ray = first["max_samples_per_ray"]
     del first
     for _ in range(args.warmup):
         out = step()
@@ -366,7 +367,7 @@ def main():
     eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     # the timed steps: the whole step (rays, mesh hit, render) captured once in a HIP graph
     outs = {k: out[k] for k in ("rgb", "T", "n_samples", "depth")}
-    cap = int(total_samples) + 64
+    cap = D.records_for(max_per_ray)
     ws.get(ds, n_rays, cap)
 
     def gstep():
@@ -464,7 +465,7 @@ def main():
         },
         "eager_ms_per_step": round(eager_ms, 4),
         "stage_ms": {k: round(float(v), 4) for k, v in zip(
-            ["march_count", "scan", "march_emit", "tile_sort", "mlp_mfma", "reduce"], stage_ms)},
+            ["schedule", "march", "tile_hist", "tile_scatter", "mlp_mfma", "reduce"], stage_ms)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extra,

@@ -152,16 +152,19 @@ typedef struct snisr_tuning {
   int32_t reserved;
 } snisr_tuning;
 
-/* Workspace bytes for snisr_render_mfma on n rays with room for `sample_capacity` samples
- * (0 = a default of 48 samples per ray). */
-int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t sample_capacity);
+/* Workspace bytes for snisr_render_mfma on n rays with `records_per_ray` sample records per ray
+ * (0 = a default of 64).  The march writes each ray's records at its own fixed slots (one pass),
+ * so the capacity is per ray; the render takes as many records per ray as workspace_bytes holds. */
+int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t records_per_ray);
 
 /* Same render with the per-tile MLP on MFMA (FP16 or BF16 operands, FP32 accumulation,
- * FP32 transmittance).  Stages: a sigma-only march counting samples per ray, a scan, a second
- * march emitting per-sample records contiguous per ray, a counting sort of the records by tile,
- * the per-tile MFMA MLP (weights staged in LDS once per tile and workgroup), and an ordered
+ * FP32 transmittance).  Stages: one sigma-only march writing every ray's sample records (voxel
+ * coordinate, compositing weight, tile) at the ray's own slots, a counting sort of the records by
+ * tile, the per-tile MFMA MLP (weights staged in LDS once per tile and workgroup), and an ordered
  * per-ray reduction.  Sample counts, T, depth and the brick lists are bit-identical to
- * snisr_render_fp32; the pixels are deterministic. */
+ * snisr_render_fp32; the pixels are deterministic.  Stage events (snisr_render_out): 0 start,
+ * 1 after the optional ray schedule, 2 after the march, 3 after the tile histogram, 4 after the
+ * record sort, 5 after the MLP, 6 after the reduction. */
 int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                       const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream);
 
@@ -170,9 +173,12 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
                             const snisr_render_out* out, const snisr_tuning* tuning, void* workspace,
                             int64_t workspace_bytes, void* stream);
 
-/* Samples emitted by the last snisr_render_mfma on `workspace` and whether a ray overflowed the
- * record capacity (its rgb is then NaN).  Synchronises `stream`.  No reference counterpart. */
-int snisr_render_mfma_stats(const void* workspace, int64_t* total, int32_t* overflow, void* stream);
+/* Samples of the last snisr_render_mfma on `workspace`, whether a ray had more samples than its
+ * record slots (its rgb is then NaN) and the largest per-ray sample count (max_count may be NULL;
+ * size the next workspace with records_per_ray >= max_count).  Synchronises `stream`.  No
+ * reference counterpart. */
+int snisr_render_mfma_stats(const void* workspace, int64_t* total, int32_t* overflow, int32_t* max_count,
+                            void* stream);
 
 /* A9 — proxy-mesh nearest hit.  Replaces ray_firstHit_inTile_kernel (training/src/cuda_func.cu:130-282,
  * per-tile per-block face lists BConFaceIdx/BConFaceNum, training/scripts/tools.py:364-379) and its

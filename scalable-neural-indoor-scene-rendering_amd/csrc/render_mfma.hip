@@ -126,13 +126,8 @@ struct Workspace {
   float4* crec;         // [cap] training: unweighted colour of every record
   float* dsig;          // [cap] training: dL/dsigma of every record
   float* act;           // training: activations/deltas per sorted record (train_act_floats each)
-  int32_t* order;         // [n] ray schedule: thread -> ray (when the ray-order pass runs)
-  int32_t* ray_key;       // [n] schedule key of every ray
-  int32_t* bucket_count;  // [n_buckets]
-  int64_t* bucket_cursor; // [n_buckets]
   int64_t cap;
   int32_t n_tiles;
-  int32_t n_buckets;
 };
 
 constexpr int kScanItems = 8, kScanBlock = 256, kScanTile = kScanItems * kScanBlock;
@@ -155,11 +150,6 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
   const int64_t o_ts = off; off = align_up(off + 8 * ((int64_t)T + 1), 256);
   const int64_t o_cu = off; off = align_up(off + 8 * (int64_t)T, 256);
   const int64_t o_enc = off; off = align_up(off + 32 * n, 256);
-  const int32_t nbk = key_buckets(T);
-  const int64_t o_ord = off; off = align_up(off + 4 * n, 256);
-  const int64_t o_key = off; off = align_up(off + 4 * n, 256);
-  const int64_t o_bkc = off; off = align_up(off + 4 * (int64_t)nbk, 256);
-  const int64_t o_bku = off; off = align_up(off + 8 * (int64_t)nbk, 256);
   const int64_t o_pm = off; off = align_up(off + 4 * (cap + 64 * (int64_t)T), 256);
   const int64_t o_rf = off; off = align_up(off + 16 * cap, 256);
   const int64_t o_ti = off; off = align_up(off + 8 * cap, 256);
@@ -189,18 +179,85 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
     w->crec = train_floats > 0 ? reinterpret_cast<float4*>(base + o_cr) : nullptr;
     w->dsig = train_floats > 0 ? reinterpret_cast<float*>(base + o_ds) : nullptr;
     w->act = train_floats > 0 ? reinterpret_cast<float*>(base + o_act) : nullptr;
-    w->order = reinterpret_cast<int32_t*>(base + o_ord);
-    w->ray_key = reinterpret_cast<int32_t*>(base + o_key);
-    w->bucket_count = reinterpret_cast<int32_t*>(base + o_bkc);
-    w->bucket_cursor = reinterpret_cast<int64_t*>(base + o_bku);
     w->cap = cap;
     w->n_tiles = T;
-    w->n_buckets = nbk;
   }
   return off;
 }
 
 static int64_t default_capacity(int64_t n) { return 48 * n; }
+
+// Render workspace: one pass of the march writes every ray's records at fixed slots, ray i owning
+// slots [i * cpr, (i + 1) * cpr) (cpr = record capacity per ray); no count pass, no scan.  A ray's
+// records come in runs of one tile each (the ray walks its tiles in order); the first record of a
+// run carries the tile and the run length, so the tile counting sort works per run, not per
+// record, and writes each run's slots contiguously into the tile's bucket of perm.  The MLP writes
+// each record's weighted colour back into its slot of rec_f and the reduction sums every ray's
+// slots in sample order.
+struct RenderWs {
+  int64_t* total;         // header: samples in total
+  int32_t* overflow;      // header: 1 if a ray had more than cpr samples
+  int32_t* max_count;     // header: largest per-ray sample count
+  int32_t* counts;        // [n]
+  uint16_t* enc;          // [n][16] per-ray view encoding in the MFMA operand type (15 values + 0)
+  int32_t* order;         // [n] ray schedule: thread -> ray
+  int32_t* ray_key;       // [n]
+  int32_t* bucket_count;  // [n_buckets]
+  int64_t* bucket_cursor; // [n_buckets]
+  int32_t* tile_count;    // [T]
+  int64_t* tile_start;    // [T+1] 64-aligned bucket starts
+  int64_t* tile_cursor;   // [T]
+  float4* rec_f;          // [n * cpr] voxel coordinate + weight; after the MLP: weighted colour
+  int32_t* rec_t;         // [n * cpr] at the first record of each tile run: tile | run length << 16
+  int32_t* perm;          // [n * cpr + 64 T] record slots in tile order, 64-aligned buckets, -1 = padding
+  int64_t n;
+  int32_t cpr;
+  int32_t n_tiles;
+  int32_t n_buckets;
+};
+
+constexpr int32_t kDefaultRecordsPerRay = 64;
+constexpr int64_t kRenderBytesPerSlot = 16 + 4 + 4;
+
+static int64_t render_layout(int64_t n, int64_t cpr, int32_t T, uint8_t* base, RenderWs* w) {
+  int64_t off = 256;
+  const int32_t nbk = key_buckets(T);
+  const int64_t slots = n * cpr, sorted = slots + 64 * (int64_t)T;
+  const int64_t o_cnt = off; off = align_up(off + 4 * n, 256);
+  const int64_t o_enc = off; off = align_up(off + 32 * n, 256);
+  const int64_t o_ord = off; off = align_up(off + 4 * n, 256);
+  const int64_t o_key = off; off = align_up(off + 4 * n, 256);
+  const int64_t o_bkc = off; off = align_up(off + 4 * (int64_t)nbk, 256);
+  const int64_t o_bku = off; off = align_up(off + 8 * (int64_t)nbk, 256);
+  const int64_t o_tc = off; off = align_up(off + 4 * (int64_t)T, 256);
+  const int64_t o_ts = off; off = align_up(off + 8 * ((int64_t)T + 1), 256);
+  const int64_t o_cu = off; off = align_up(off + 8 * (int64_t)T, 256);
+  const int64_t o_rf = off; off = align_up(off + 16 * slots, 256);
+  const int64_t o_rt = off; off = align_up(off + 4 * slots, 256);
+  const int64_t o_pm = off; off = align_up(off + 4 * sorted, 256);
+  if (w && base) {
+    w->total = reinterpret_cast<int64_t*>(base);
+    w->overflow = reinterpret_cast<int32_t*>(base + 8);
+    w->max_count = reinterpret_cast<int32_t*>(base + 12);
+    w->counts = reinterpret_cast<int32_t*>(base + o_cnt);
+    w->enc = reinterpret_cast<uint16_t*>(base + o_enc);
+    w->order = reinterpret_cast<int32_t*>(base + o_ord);
+    w->ray_key = reinterpret_cast<int32_t*>(base + o_key);
+    w->bucket_count = reinterpret_cast<int32_t*>(base + o_bkc);
+    w->bucket_cursor = reinterpret_cast<int64_t*>(base + o_bku);
+    w->tile_count = reinterpret_cast<int32_t*>(base + o_tc);
+    w->tile_start = reinterpret_cast<int64_t*>(base + o_ts);
+    w->tile_cursor = reinterpret_cast<int64_t*>(base + o_cu);
+    w->rec_f = reinterpret_cast<float4*>(base + o_rf);
+    w->rec_t = reinterpret_cast<int32_t*>(base + o_rt);
+    w->perm = reinterpret_cast<int32_t*>(base + o_pm);
+    w->n = n;
+    w->cpr = (int32_t)cpr;
+    w->n_tiles = T;
+    w->n_buckets = nbk;
+  }
+  return off;
+}
 
 // ------------------------------------------------------------------------- march passes
 template <int MODE>  // 0: count, 1: emit records, 2: emit records + training extras
@@ -432,7 +489,7 @@ __global__ __launch_bounds__(kSortBlock) void k_tile_scatter(Workspace W) {
 // passes walk the same tile in similar directions: a counting sort (LDS histograms, one scan,
 // atomic bucket cursors).  Only the thread -> ray map changes; every ray's records and pixel are
 // the same as without the schedule.
-__global__ __launch_bounds__(256) void k_ray_key(snisr_scene S, snisr_rays R, Workspace W) {
+__global__ __launch_bounds__(256) void k_ray_key(snisr_scene S, snisr_rays R, RenderWs W) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < W.n_buckets) W.bucket_count[i] = 0;
   if (i >= R.n) return;
@@ -453,7 +510,7 @@ __device__ __forceinline__ void ray_range(int64_t n, int64_t& b, int64_t& e) {
   e = b + per < n ? b + per : n;
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_key_hist(Workspace W, int64_t n) {
+__global__ __launch_bounds__(kSortBlock) void k_key_hist(RenderWs W, int64_t n) {
   __shared__ int hist[kMaxBuckets];
   const int nb = W.n_buckets;
   for (int t = threadIdx.x; t < nb; t += blockDim.x) hist[t] = 0;
@@ -466,7 +523,7 @@ __global__ __launch_bounds__(kSortBlock) void k_key_hist(Workspace W, int64_t n)
     if (hist[t]) atomicAdd(&W.bucket_count[t], hist[t]);
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_key_scan(Workspace W) {
+__global__ __launch_bounds__(kScanBlock) void k_key_scan(RenderWs W) {
   const int nb = W.n_buckets;
   int64_t carry = 0;
   for (int start = 0; start < nb; start += kScanBlock) {
@@ -479,7 +536,7 @@ __global__ __launch_bounds__(kScanBlock) void k_key_scan(Workspace W) {
   }
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_key_scatter(Workspace W, int64_t n) {
+__global__ __launch_bounds__(kSortBlock) void k_key_scatter(RenderWs W, int64_t n) {
   __shared__ int hist[kMaxBuckets];
   __shared__ int64_t base[kMaxBuckets];
   const int nb = W.n_buckets;
@@ -499,6 +556,194 @@ __global__ __launch_bounds__(kSortBlock) void k_key_scatter(Workspace W, int64_t
     const int r = atomicAdd(&hist[t], 1);
     W.order[base[t] + r] = (int32_t)k;
   }
+}
+
+// ------------------------------------------------------------------------- render: one march pass
+__global__ void k_render_begin(RenderWs W) {
+  if (threadIdx.x == 0) {
+    *W.total = 0;
+    *W.overflow = 0;
+    *W.max_count = 0;
+  }
+  for (int t = threadIdx.x; t < W.n_tiles; t += blockDim.x) W.tile_count[t] = 0;
+}
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One wavefront per 64 rays: march each ray once, writing its records (voxel coordinate + weight,
+// tile) at its own slots, its sample count, T, depth and view encoding; per-wave totals feed the
+// header (sample total, largest count, overflow when a ray has more samples than slots).
+__global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_params P, snisr_rays R,
+                                                     snisr_render_out O, RenderWs W, int packet_w, int enc_dtype,
+                                                     const int32_t* __restrict__ order) {
+  const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  int ns = 0;
+  if (tid < R.n) {
+    const int64_t i = order ? (int64_t)order[tid] : ray_of_thread(tid, R.n, packet_w);
+    const float* og = R.origins + i * R.origin_stride;
+    const V3 origin = v3(og[0], og[1], og[2]);
+    const V3 d = normalize_dir(v3(R.dirs[3 * i], R.dirs[3 * i + 1], R.dirs[3 * i + 2]));
+    const float t_hit = R.t_hit ? R.t_hit[i] : __builtin_inff();
+    const int32_t* vis = R.visited ? R.visited + i * P.max_tiles : nullptr;
+    const int cpr = W.cpr;
+    const int64_t slot0 = i * (int64_t)cpr;
+    int k = 0, run_tile = -1, run_start = 0;
+    MarchResult res;
+    int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
+    march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap,
+              [&](int tile, const V3& f, const Corners&, float w, float, float, float) {
+                if (k < cpr) {
+                  if (tile != run_tile) {
+                    if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((k - run_start) << 16);
+                    run_tile = tile;
+                    run_start = k;
+                  }
+                  W.rec_f[slot0 + k] = make_float4(f.x, f.y, f.z, w);
+                }
+                ++k;
+              },
+              res);
+    if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((min(k, cpr) - run_start) << 16);
+    if (bricks)
+      for (int b = res.n_bricks; b < O.brick_cap; ++b) bricks[b] = -1;
+    W.counts[i] = res.n_samples;
+    float enc[15];
+    view_encoding(d, enc);
+    uint16_t e16[16];
+#pragma unroll
+    for (int q = 0; q < 15; ++q) e16[q] = to_bits(enc[q], enc_dtype);
+    e16[15] = 0;
+    uint4* dst = reinterpret_cast<uint4*>(W.enc + i * 16);
+    dst[0] = *reinterpret_cast<const uint4*>(e16);
+    dst[1] = *reinterpret_cast<const uint4*>(e16 + 8);
+    if (O.T) O.T[i] = res.T;
+    if (O.n_samples) O.n_samples[i] = res.n_samples;
+    if (O.depth) O.depth[i] = res.depth;
+    ns = res.n_samples;
+  }
+  const int64_t sum = wave_sum_i64((int64_t)ns);
+  const int mx = wave_max_i32(ns);
+  if (threadIdx.x == 0) {
+    if (sum) atomicAdd((unsigned long long*)W.total, (unsigned long long)sum);
+    if (mx) atomicMax(W.max_count, mx);
+    if (mx > W.cpr) *W.overflow = 1;
+  }
+}
+
+// rays [b, e) of this workgroup
+__device__ __forceinline__ void wg_ray_range(int64_t n, int64_t& b, int64_t& e) {
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  b = (int64_t)blockIdx.x * per;
+  e = b + per < n ? b + per : n;
+}
+
+// per-tile record counts of this workgroup's rays into LDS, one atomic per tile run
+__device__ __forceinline__ void rtile_count_block(const RenderWs& W, int* hist, int64_t b, int64_t e) {
+  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    const int c = min(W.counts[i], W.cpr);
+    const int32_t* rt = W.rec_t + i * (int64_t)W.cpr;
+    for (int k = 0; k < c;) {
+      const int v = rt[k], len = v >> 16;
+      if (len <= 0) break;  // unreachable for runs written by the march; keeps the loop finite
+      atomicAdd(&hist[v & 0xffff], len);
+      k += len;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_rtile_hist(RenderWs W) {
+  __shared__ int hist[kMaxTiles];
+  const int T = W.n_tiles;
+  for (int t = threadIdx.x; t < T; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
+  int64_t b, e;
+  wg_ray_range(W.n, b, e);
+  rtile_count_block(W, hist, b, e);
+  __syncthreads();
+  for (int t = threadIdx.x; t < T; t += blockDim.x)
+    if (hist[t]) atomicAdd(&W.tile_count[t], hist[t]);
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_rtile_scan(RenderWs W) {
+  const int T = W.n_tiles;
+  int64_t carry = 0;
+  for (int start = 0; start < T; start += kScanBlock) {
+    const int t = start + threadIdx.x;
+    const int64_t v = t < T ? (int64_t)((W.tile_count[t] + 63) & ~63) : 0;
+    int64_t tot;
+    const int64_t ex = block_excl_scan(v, &tot);
+    if (t < T) {
+      W.tile_start[t] = carry + ex;
+      W.tile_cursor[t] = carry + ex;
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) W.tile_start[T] = carry;
+  __syncthreads();
+  // padding slots at the end of every bucket
+  for (int t = 0; t < T; ++t) {
+    const int64_t pb = W.tile_start[t] + W.tile_count[t], pe = W.tile_start[t + 1];
+    for (int64_t k = pb + threadIdx.x; k < pe; k += blockDim.x) W.perm[k] = -1;
+  }
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_rtile_scatter(RenderWs W) {
+  __shared__ int hist[kMaxTiles];
+  __shared__ int64_t base[kMaxTiles];
+  const int T = W.n_tiles;
+  for (int t = threadIdx.x; t < T; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
+  int64_t b, e;
+  wg_ray_range(W.n, b, e);
+  rtile_count_block(W, hist, b, e);
+  __syncthreads();
+  for (int t = threadIdx.x; t < T; t += blockDim.x) {
+    base[t] = hist[t] ? atomicAdd((unsigned long long*)&W.tile_cursor[t], (unsigned long long)hist[t]) : 0;
+    hist[t] = 0;
+  }
+  __syncthreads();
+  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    const int c = min(W.counts[i], W.cpr);
+    const int64_t slot0 = i * (int64_t)W.cpr;
+    for (int k = 0; k < c;) {
+      const int v = W.rec_t[slot0 + k], t = v & 0xffff, len = v >> 16;
+      if (len <= 0) break;  // unreachable for runs written by the march; keeps the loop finite
+      int32_t* dst = W.perm + base[t] + atomicAdd(&hist[t], len);
+      for (int j = 0; j < len; ++j) dst[j] = (int32_t)(slot0 + k + j);
+      k += len;
+    }
+  }
+}
+
+// ordered per-ray sum of the weighted colours (NaN for a ray whose samples exceeded its slots)
+__global__ __launch_bounds__(256) void k_render_reduce(RenderWs W, float* __restrict__ rgb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W.n) return;
+  const int c = W.counts[i];
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+  if (c > W.cpr) {
+    c0 = c1 = c2 = __builtin_nanf("");
+  } else {
+    const float4* v = W.rec_f + i * (int64_t)W.cpr;
+    for (int k = 0; k < c; ++k) {
+      const float4 q = v[k];
+      c0 = c0 + q.x;
+      c1 = c1 + q.y;
+      c2 = c2 + q.z;
+    }
+  }
+  rgb[3 * i] = c0;
+  rgb[3 * i + 1] = c1;
+  rgb[3 * i + 2] = c2;
 }
 
 // Eight trilinear corner weights (corner order x0y0z0, x1y0z0, ... as in tri_combine) as half2
@@ -565,7 +810,7 @@ __device__ __forceinline__ f16x8 interp8_h2<Mfma<SNISR_F16>>(const _Float16* __r
 // Per layer the accumulators of one 32-row out-block live at a time (32 VGPRs); the layer's input
 // and output activations stay in registers in the B-operand layout.
 template <int F, int WD, int L, bool HALF, int DT>
-__global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, Workspace W) {
+__global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, RenderWs W) {
   using M = Mfma<DT>;
   using vec = typename M::vec;
   constexpr int IN = F + 15;
@@ -595,6 +840,7 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, W
     }
     const int t = lo;
     const int64_t seg_end = (W.tile_start[t + 1] >> 6) < cend ? (W.tile_start[t + 1] >> 6) : cend;
+    const int32_t tile_base = S.tile_brick_base[t], tile_res = S.tile_res[t];
     // stage the tile's weights
     __syncthreads();
     {
@@ -608,11 +854,12 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, W
       // ---- gather: lane (h, s) builds its half of the input column of sample s of both blocks
       vec xin[2][NKC0];
       float wgt[2];
-      int32_t rec[2];
+      int32_t rec[2], ray[2];
 #pragma unroll
       for (int cbk = 0; cbk < 2; ++cbk) {
         const int32_t k = W.perm[c * 64 + cbk * 32 + s];
         rec[cbk] = k;
+        ray[cbk] = k >= 0 ? k / W.cpr : 0;
         wgt[cbk] = 0.0f;
         float xv[NKC0][8];
 #pragma unroll
@@ -621,10 +868,9 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, W
           for (int e = 0; e < 8; ++e) xv[kc][e] = 0.0f;
         if (k >= 0) {
           const float4 rf = W.rec_f[k];
-          const int2 ti = W.rec_ti[k];
           wgt[cbk] = rf.w;
           Corners cn;
-          find_corners(S.brick_map + S.tile_brick_base[ti.x], S.tile_res[ti.x], v3(rf.x, rf.y, rf.z), cn);
+          find_corners(S.brick_map + tile_base, tile_res, v3(rf.x, rf.y, rf.z), cn);
           if constexpr (HALF) {
             h2 w2[8];
             corner_weights_h2(cn, w2);
@@ -656,8 +902,7 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, W
         }
         // view encoding: precomputed per ray in the operand type by the count pass
         if (k >= 0) {
-          const int64_t ray = W.rec_ti[k].y;
-          xin[cbk][NFC] = load_frag<vec>(reinterpret_cast<const uint8_t*>(W.enc + ray * 16 + 8 * h));
+          xin[cbk][NFC] = load_frag<vec>(reinterpret_cast<const uint8_t*>(W.enc + (int64_t)ray[cbk] * 16 + 8 * h));
         } else {
           float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
           xin[cbk][NFC] = M::cvt8(z);
@@ -701,11 +946,11 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, W
         // rows 0..2 of the output block sit in elements 0..2 of the lanes with h == 0
         if (h == 0) {
           if (rec[0] >= 0)
-            W.rgbw[rec[0]] = make_float4(wgt[0] * sigmoid_cw(acc0[0]), wgt[0] * sigmoid_cw(acc0[1]),
-                                         wgt[0] * sigmoid_cw(acc0[2]), 0.0f);
+            W.rec_f[rec[0]] = make_float4(wgt[0] * sigmoid_cw(acc0[0]), wgt[0] * sigmoid_cw(acc0[1]),
+                                          wgt[0] * sigmoid_cw(acc0[2]), 0.0f);
           if (rec[1] >= 0)
-            W.rgbw[rec[1]] = make_float4(wgt[1] * sigmoid_cw(acc1[0]), wgt[1] * sigmoid_cw(acc1[1]),
-                                         wgt[1] * sigmoid_cw(acc1[2]), 0.0f);
+            W.rec_f[rec[1]] = make_float4(wgt[1] * sigmoid_cw(acc1[0]), wgt[1] * sigmoid_cw(acc1[1]),
+                                          wgt[1] * sigmoid_cw(acc1[2]), 0.0f);
         }
       }
     }
@@ -1012,13 +1257,13 @@ using namespace snisr;
 
 extern "C" {
 
-int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t sample_capacity) {
-  if (!scene || n < 0 || sample_capacity < 0) {
+int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t records_per_ray) {
+  if (!scene || n < 0 || records_per_ray < 0) {
     set_error("render_mfma_workspace_size: bad arguments");
     return -1;
   }
-  const int64_t cap = sample_capacity ? sample_capacity : default_capacity(n);
-  return layout(n, cap, scene->n_tiles, nullptr, nullptr);
+  const int64_t cpr = records_per_ray ? records_per_ray : kDefaultRecordsPerRay;
+  return render_layout(n, cpr, scene->n_tiles, nullptr, nullptr);
 }
 
 int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
@@ -1044,25 +1289,28 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
   SNISR_REQUIRE(!out->bricks || out->brick_cap > 0, SNISR_ERR_ARG, "render_mfma: bricks needs brick_cap > 0");
   const int64_t n = rays->n;
   if (n == 0) return SNISR_OK;
-  // capacity: everything after the fixed part of the layout
   SNISR_REQUIRE(scene->n_tiles <= kMaxTiles, SNISR_ERR_SHAPE, "render_mfma: at most %d tiles", kMaxTiles);
-  const int64_t fixed = layout(n, 0, scene->n_tiles, nullptr, nullptr);
-  SNISR_REQUIRE(workspace_bytes >= fixed, SNISR_ERR_CAPACITY, "render_mfma: workspace too small");
-  const int64_t cap = (workspace_bytes - fixed - 4 * 256) / 44;
-  Workspace W;
-  layout(n, cap, scene->n_tiles, (uint8_t*)workspace, &W);
-  SNISR_REQUIRE(cap > 0 && layout(n, cap, scene->n_tiles, nullptr, nullptr) <= workspace_bytes, SNISR_ERR_CAPACITY,
-                "render_mfma: workspace too small");
+  // records per ray: the most that fit the workspace
+  const int64_t fixed = render_layout(n, 0, scene->n_tiles, nullptr, nullptr);
+  SNISR_REQUIRE(workspace_bytes > fixed, SNISR_ERR_CAPACITY, "render_mfma: workspace too small");
+  int64_t cpr = (workspace_bytes - fixed) / (kRenderBytesPerSlot * n);
+  while (cpr > 0 && render_layout(n, cpr, scene->n_tiles, nullptr, nullptr) > workspace_bytes) --cpr;
+  SNISR_REQUIRE(cpr > 0, SNISR_ERR_CAPACITY, "render_mfma: workspace too small for one record per ray");
+  if (cpr > 4096) cpr = 4096;
+  if (n * cpr >= (int64_t(1) << 31)) cpr = ((int64_t(1) << 31) - 1) / n;  // slot ids are 32-bit
+  SNISR_REQUIRE(cpr > 0, SNISR_ERR_CAPACITY, "render_mfma: batch too large for 32-bit record slots");
+  RenderWs W;
+  render_layout(n, cpr, scene->n_tiles, (uint8_t*)workspace, &W);
 
   const int F = scene->feat_dim, WD = scene->mlp_width, L = scene->mlp_layers;
   const bool half = scene->feat_dtype == SNISR_F16;
   const int dt = scene->mlp_packed_dtype;
   hipStream_t st = (hipStream_t)stream;
   const int pw = packet_width(*rays);
-  const dim3 mgrid((unsigned)((n + 63) / 64));
-  const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
 
   SNISR_HIP(stage_event(out, 0, st));
+  hipLaunchKernelGGL(k_render_begin, dim3(1), dim3(256), 0, st, W);
+  SNISR_LAUNCHED();
   const int32_t* order = nullptr;
   if (tn.ray_order == 1) {
     const int64_t kn = n > W.n_buckets ? n : W.n_buckets;
@@ -1076,27 +1324,17 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
     SNISR_LAUNCHED();
     order = W.order;
   }
-  hipLaunchKernelGGL(k_march_pass<0>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt, order);
-  SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 1, st));
-  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.counts, n, W.offsets,
-                     W.block_sums);
-  SNISR_LAUNCHED();
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, st, W.block_sums, nblocks, W.offsets, n, W.total,
-                     W.overflow, W.tile_count, W.n_tiles);
-  SNISR_LAUNCHED();
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.offsets, n, W.block_sums);
+  hipLaunchKernelGGL(k_march_render, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, *scene, *params, *rays, *out,
+                     W, pw, dt, order);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 2, st));
-  hipLaunchKernelGGL(k_march_pass<1>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt, order);
+  hipLaunchKernelGGL(k_rtile_hist, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W);
+  SNISR_LAUNCHED();
+  hipLaunchKernelGGL(k_rtile_scan, dim3(1), dim3(kScanBlock), 0, st, W);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 3, st));
-
-  hipLaunchKernelGGL(k_tile_hist, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W);
-  SNISR_LAUNCHED();
-  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kScanBlock), 0, st, W);
-  SNISR_LAUNCHED();
-  hipLaunchKernelGGL(k_tile_scatter, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W);
+  hipLaunchKernelGGL(k_rtile_scatter, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 4, st));
 
@@ -1127,7 +1365,7 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
   }
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 5, st));
-  hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, W, n, out->rgb);
+  hipLaunchKernelGGL(k_render_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, W, out->rgb);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 6, st));
   return SNISR_OK;
@@ -1284,13 +1522,15 @@ int snisr_train_backward(const snisr_scene* scene, const snisr_march_params* par
 
 // Total samples of the last snisr_render_mfma on this workspace and whether any ray overflowed
 // the record capacity (synchronises the stream).
-int snisr_render_mfma_stats(const void* workspace, int64_t* total, int32_t* overflow, void* stream) {
+int snisr_render_mfma_stats(const void* workspace, int64_t* total, int32_t* overflow, int32_t* max_count,
+                            void* stream) {
   SNISR_REQUIRE(workspace && total && overflow, SNISR_ERR_ARG, "render_mfma_stats: NULL pointer");
   int64_t hdr[2];
   SNISR_HIP(hipMemcpyAsync(hdr, workspace, 16, hipMemcpyDeviceToHost, (hipStream_t)stream));
   SNISR_HIP(hipStreamSynchronize((hipStream_t)stream));
   *total = hdr[0];
   *overflow = (int32_t)(hdr[1] & 0xffffffff);
+  if (max_count) *max_count = (int32_t)((uint64_t)hdr[1] >> 32);
   return SNISR_OK;
 }
 

@@ -79,7 +79,8 @@ SIGNATURES = [
                                     C.POINTER(RenderOut), C.c_void_p, C.c_int64, C.c_void_p]),
     ("snisr_render_mfma_tuned", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Tuning),
                                           C.c_void_p, C.c_int64, C.c_void_p]),
-    ("snisr_render_mfma_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_void_p]),
+    ("snisr_render_mfma_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32), C.c_void_p]),
     ("snisr_bvh_build_bounds", C.c_int, [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("snisr_bvh_build", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                   C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),

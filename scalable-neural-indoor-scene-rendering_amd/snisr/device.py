@@ -194,26 +194,35 @@ def render_fp32(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_
 
 
 class MfmaWorkspace:
-    """Caller-owned workspace of snisr_render_mfma, grown on demand."""
+    """Caller-owned workspace of snisr_render_mfma, grown on demand.  ``records_per_ray`` is the
+    per-ray record capacity (0 = the library default of 64)."""
 
     def __init__(self):
         self.buf = None
 
-    def get(self, ds: DeviceScene, n: int, capacity: int = 0):
-        need = N.lib().snisr_render_mfma_workspace_size(C.byref(ds.desc), n, capacity)
+    def get(self, ds: DeviceScene, n: int, records_per_ray: int = 0):
+        need = N.lib().snisr_render_mfma_workspace_size(C.byref(ds.desc), n, records_per_ray)
         if need < 0:
             raise RuntimeError(N.lib().snisr_last_error().decode())
         if self.buf is None or self.buf.numel() < need:
+            self.buf = None
             self.buf = torch.empty(need, dtype=torch.uint8, device=ds.sigma.device)
-        return self.buf, need
+        # the whole buffer: the render takes as many records per ray as it holds
+        return self.buf, self.buf.numel()
 
 
-def mfma_stats(buf) -> tuple[int, int]:
-    """(total samples, overflow flag) of the last render on this workspace (synchronises)."""
-    total, over = C.c_int64(0), C.c_int32(0)
-    N.check(N.lib().snisr_render_mfma_stats(_ptr(buf), C.byref(total), C.byref(over), _stream()),
+def mfma_stats(buf, with_max: bool = False):
+    """(total samples, overflow flag[, largest per-ray count]) of the last render on this workspace
+    (synchronises)."""
+    total, over, mx = C.c_int64(0), C.c_int32(0), C.c_int32(0)
+    N.check(N.lib().snisr_render_mfma_stats(_ptr(buf), C.byref(total), C.byref(over), C.byref(mx), _stream()),
             "render_mfma_stats")
-    return total.value, over.value
+    return (total.value, over.value, mx.value) if with_max else (total.value, over.value)
+
+
+def records_for(max_count: int, headroom: float = 1.25) -> int:
+    """Per-ray record capacity for a workload whose largest ray has ``max_count`` samples."""
+    return max(8, -(-int(max_count * headroom + 1) // 8) * 8)
 
 
 def tuning_struct(t: dict | None):
@@ -226,9 +235,10 @@ def tuning_struct(t: dict | None):
 def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_width=0, brick_cap=0,
                 workspace: MfmaWorkspace | None = None, capacity: int = 0, check: bool = True, events=None,
                 tuning: dict | None = None, outputs: dict | None = None, **march_kw):
-    """Render with the per-tile MLP on MFMA (snisr_render_mfma_tuned).  With ``check`` the record
-    capacity is verified after the call and the render repeated with room for every sample.
-    ``outputs`` (from a previous call) is written in place instead of allocating new tensors."""
+    """Render with the per-tile MLP on MFMA (snisr_render_mfma_tuned).  ``capacity``: sample records
+    per ray (0 = default).  With ``check`` the capacity is verified after the call and the render
+    repeated with room for the largest ray.  ``outputs`` (from a previous call) is written in place
+    instead of allocating new tensors."""
     if ds.mlp_packed is None:
         raise RuntimeError("scene has no MFMA-packed MLP; call DeviceScene.pack_mfma()")
     r = _rays_desc(origins, dirs, t_hit, visited, image_width)
@@ -247,12 +257,15 @@ def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_
     N.check(N.lib().snisr_render_mfma_tuned(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), tp, _ptr(buf),
                                             need, _stream()), "render_mfma")
     if check:
-        total, over = mfma_stats(buf)
+        total, over, mx = mfma_stats(buf, with_max=True)
         if over:
-            buf, need = ws.get(ds, r.n, int(total * 1.25) + 64)
+            buf, need = ws.get(ds, r.n, records_for(mx))
             N.check(N.lib().snisr_render_mfma_tuned(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), tp,
                                                     _ptr(buf), need, _stream()), "render_mfma")
+            total, over, mx = mfma_stats(buf, with_max=True)
+            assert not over
         out["total_samples"] = total
+        out["max_samples_per_ray"] = mx
     return out
 
 

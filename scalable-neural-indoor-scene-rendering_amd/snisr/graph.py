@@ -21,7 +21,7 @@ class RenderGraph:
         # cannot grow it; overflowed() reports a replay that ran out of record space)
         first = D.render_mfma(ds, origins, dirs, t_hit=t_hit, image_width=image_width, workspace=self.ws,
                               check=True, tuning=tuning)
-        self.capacity = int(first["total_samples"] * capacity_factor) + 64
+        self.capacity = D.records_for(first["max_samples_per_ray"], capacity_factor)
         self.ws.get(ds, dirs.numel() // 3, self.capacity)
         self.outputs = {k: first[k] for k in ("rgb", "T", "n_samples", "depth")}
         self.graph = torch.cuda.CUDAGraph()
@@ -40,7 +40,7 @@ class RenderGraph:
                       outputs=self.outputs)
 
     def overflowed(self) -> bool:
-        """True if the last replay had more samples than the record capacity (synchronises)."""
+        """True if a ray of the last replay had more samples than its record slots (synchronises)."""
         return D.mfma_stats(self.ws.buf)[1] != 0
 
     def replay(self):
