@@ -236,18 +236,19 @@ def c5_bench(args, dev):
     first = D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, check=True)
     outs = {k: first[k] for k in ("rgb", "T", "n_samples", "depth")}
     cap = D.records_for(first["max_samples_per_ray"])
+    scap = int(first["total_samples"] * 1.05) + 64
 
     def step():
         trace()
-        D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap, check=False,
-                      outputs=outs)
+        D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap,
+                      sample_capacity=scap, check=False, outputs=outs)
 
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
     for e in ev:
         e.record()
     trace()
-    D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap, check=False,
-                  outputs=outs, events=ev)
+    D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap,
+                  sample_capacity=scap, check=False, outputs=outs, events=ev)
     torch.cuda.synchronize()
     stage = [ev[j].elapsed_time(ev[j + 1]) for j in range(6)]
     side = torch.cuda.Stream()
@@ -346,7 +347,7 @@ def main():
 
     # size the workspace once (exact sample count of this view), then warm up
     first = D.render_mfma(ds, origin, D.compute_rays(intr, c2w, H, W), image_width=W, workspace=ws, check=True)
-    max_per_ray = first["max_samples_per_ray"]
+    max_per_ray, frame_samples = first["max_samples_per_ray"], first["total_samples"]
     del first
     for _ in range(args.warmup):
         out = step()
@@ -368,13 +369,14 @@ def main():
     # the timed steps: the whole step (rays, mesh hit, render) captured once in a HIP graph
     outs = {k: out[k] for k in ("rgb", "T", "n_samples", "depth")}
     cap = D.records_for(max_per_ray)
-    ws.get(ds, n_rays, cap)
+    scap = int(frame_samples * 1.05) + 64
+    ws.get(ds, n_rays, cap, scap)
 
     def gstep():
         D.compute_rays(intr, c2w, H, W, out=rays_d)
         dm.first_hit(origin, rays_d, image_width=W, t_out=t_hit, id_out=tri_id)
-        D.render_mfma(ds, origin, rays_d, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap, check=False,
-                      outputs=outs)
+        D.render_mfma(ds, origin, rays_d, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap,
+                      sample_capacity=scap, check=False, outputs=outs)
 
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())

@@ -140,22 +140,26 @@ int snisr_trace_tiles(const snisr_scene* scene, const float* origin, const float
 int snisr_render_fp32(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                       const snisr_render_out* out, void* stream);
 
-/* Launch schedule of snisr_render_mfma_tuned (chosen per workload by an autotuner; no reference
- * counterpart).  ray_order 0: rays in index order (8x8 pixel packets when image_width allows),
- * 1: rays counting-sorted by (first occupied tile, direction octant) before the march passes;
- * mlp_threads: 256 or 512 threads per MLP workgroup (one workgroup per CU); sort_grid: workgroups
- * of the record / ray counting sorts.  Results do not depend on the schedule. */
+/* Launch schedule and record capacity of snisr_render_mfma_tuned (no reference counterpart).
+ * ray_order 0: rays in index order (8x8 pixel packets when image_width allows), 1: rays
+ * counting-sorted by (first occupied tile, direction octant) before the march; mlp_threads: 256 or
+ * 512 threads per MLP workgroup (one workgroup per CU); sort_grid: workgroups of the record / ray
+ * counting sorts (chosen per workload by an autotuner; results do not depend on them).
+ * records_per_ray: sample-record slots per ray (0 = 64); every other workspace byte goes to the
+ * tile-sorted sample arrays. */
 typedef struct snisr_tuning {
   int32_t ray_order;
   int32_t mlp_threads;
   int32_t sort_grid;
-  int32_t reserved;
+  int32_t records_per_ray;
 } snisr_tuning;
 
-/* Workspace bytes for snisr_render_mfma on n rays with `records_per_ray` sample records per ray
- * (0 = a default of 64).  The march writes each ray's records at its own fixed slots (one pass),
- * so the capacity is per ray; the render takes as many records per ray as workspace_bytes holds. */
-int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t records_per_ray);
+/* Workspace bytes for snisr_render_mfma on n rays with `records_per_ray` record slots per ray
+ * (0 = 64; the march writes each ray's records at its own slots, in one pass) and room for
+ * `sample_capacity` samples in the tile-sorted arrays (0 = 16 per ray).  A frame with more samples
+ * for one ray or in total is reported by snisr_render_mfma_stats (overflow; its rgb is NaN). */
+int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t records_per_ray,
+                                         int64_t sample_capacity);
 
 /* Same render with the per-tile MLP on MFMA (FP16 or BF16 operands, FP32 accumulation,
  * FP32 transmittance).  Stages: one sigma-only march writing every ray's sample records (voxel
@@ -164,18 +168,19 @@ int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, in
  * per-ray reduction.  Sample counts, T, depth and the brick lists are bit-identical to
  * snisr_render_fp32; the pixels are deterministic.  Stage events (snisr_render_out): 0 start,
  * 1 after the optional ray schedule, 2 after the march, 3 after the tile histogram, 4 after the
- * record sort, 5 after the MLP, 6 after the reduction. */
+ * record sort, 5 after the MLP (with its trilinear feature gather), 6 after the reduction. */
 int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                       const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream);
 
-/* snisr_render_mfma with an explicit launch schedule (NULL = the defaults {0, 512, 512, 0}). */
+/* snisr_render_mfma with an explicit schedule and record capacity (NULL = {0, 512, 512, 0}). */
 int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                             const snisr_render_out* out, const snisr_tuning* tuning, void* workspace,
                             int64_t workspace_bytes, void* stream);
 
-/* Samples of the last snisr_render_mfma on `workspace`, whether a ray had more samples than its
- * record slots (its rgb is then NaN) and the largest per-ray sample count (max_count may be NULL;
- * size the next workspace with records_per_ray >= max_count).  Synchronises `stream`.  No
+/* Samples of the last snisr_render_mfma on `workspace`, whether the frame overflowed (a ray had more
+ * samples than its record slots, or the frame more than the sample capacity; the affected rgb is
+ * NaN) and the largest per-ray sample count (max_count may be NULL; size the next workspace with
+ * records_per_ray >= max_count and sample_capacity >= total).  Synchronises `stream`.  No
  * reference counterpart. */
 int snisr_render_mfma_stats(const void* workspace, int64_t* total, int32_t* overflow, int32_t* max_count,
                             void* stream);

@@ -62,17 +62,20 @@ __device__ __forceinline__ V load_frag(const uint8_t* __restrict__ p) {
   return *reinterpret_cast<const V*>(&q);
 }
 
-// acc{0,1} += sum_kc A(ob, kc) * src{0,1}[kc] with A fragments read from LDS one step ahead;
-// the scheduling barrier keeps at most two fragments in registers.
-template <class M, int NK>
+// acc{0,1} += sum_kc A(ob, kc) * src{0,1}[kc] with the A fragments read from LDS PD steps ahead
+// (a ring of PD + 1 fragments); the scheduling barrier keeps that ring the only live fragments.
+template <class M, int NK, int PD = 1>
 __device__ __forceinline__ void mma_block(const uint8_t* Ab, int lane, const typename M::vec (&s0)[NK],
                                           const typename M::vec (&s1)[NK], f32x16& acc0, f32x16& acc1) {
   using vec = typename M::vec;
-  vec nxt = load_frag<vec>(Ab + lane * 16);
+  constexpr int R = PD + 1;
+  vec ring[R];
+#pragma unroll
+  for (int p = 0; p < PD && p < NK; ++p) ring[p] = load_frag<vec>(Ab + p * 1024 + lane * 16);
 #pragma unroll
   for (int kc = 0; kc < NK; ++kc) {
-    const vec a = nxt;
-    if (kc + 1 < NK) nxt = load_frag<vec>(Ab + (kc + 1) * 1024 + lane * 16);
+    if (kc + PD < NK) ring[(kc + PD) % R] = load_frag<vec>(Ab + (kc + PD) * 1024 + lane * 16);
+    const vec a = ring[kc % R];
     acc0 = M::mma(a, s0[kc], acc0);
     acc1 = M::mma(a, s1[kc], acc1);
     __builtin_amdgcn_sched_barrier(0);

@@ -209,20 +209,25 @@ struct RenderWs {
   int64_t* tile_cursor;   // [T]
   float4* rec_f;          // [n * cpr] voxel coordinate + weight; after the MLP: weighted colour
   int32_t* rec_t;         // [n * cpr] at the first record of each tile run: tile | run length << 16
-  int32_t* perm;          // [n * cpr + 64 T] record slots in tile order, 64-aligned buckets, -1 = padding
+  int32_t* perm;          // [scap] record slots in tile order, 64-aligned buckets, -1 = padding
+  int32_t* skip;          // header: 1 when the sorted samples exceed scap (render skipped, rgb NaN)
   int64_t n;
+  int64_t scap;           // sorted positions: sample capacity + 64 per tile of bucket padding
   int32_t cpr;
   int32_t n_tiles;
   int32_t n_buckets;
 };
 
 constexpr int32_t kDefaultRecordsPerRay = 64;
-constexpr int64_t kRenderBytesPerSlot = 16 + 4 + 4;
+constexpr int64_t kDefaultSamplesPerRay = 16;  // default sample capacity per ray of the sorted arrays
+constexpr int64_t kRenderBytesPerSlot = 16 + 4;
 
-static int64_t render_layout(int64_t n, int64_t cpr, int32_t T, uint8_t* base, RenderWs* w) {
+constexpr int64_t kSortedBytes = 4;  // bytes per tile-sorted position (perm)
+
+static int64_t render_layout(int64_t n, int64_t cpr, int64_t samples, int32_t T, uint8_t* base, RenderWs* w) {
   int64_t off = 256;
   const int32_t nbk = key_buckets(T);
-  const int64_t slots = n * cpr, sorted = slots + 64 * (int64_t)T;
+  const int64_t slots = n * cpr, sorted = samples + 64 * (int64_t)T;
   const int64_t o_cnt = off; off = align_up(off + 4 * n, 256);
   const int64_t o_enc = off; off = align_up(off + 32 * n, 256);
   const int64_t o_ord = off; off = align_up(off + 4 * n, 256);
@@ -239,6 +244,8 @@ static int64_t render_layout(int64_t n, int64_t cpr, int32_t T, uint8_t* base, R
     w->total = reinterpret_cast<int64_t*>(base);
     w->overflow = reinterpret_cast<int32_t*>(base + 8);
     w->max_count = reinterpret_cast<int32_t*>(base + 12);
+    w->skip = reinterpret_cast<int32_t*>(base + 16);
+    w->scap = sorted;
     w->counts = reinterpret_cast<int32_t*>(base + o_cnt);
     w->enc = reinterpret_cast<uint16_t*>(base + o_enc);
     w->order = reinterpret_cast<int32_t*>(base + o_ord);
@@ -564,6 +571,7 @@ __global__ void k_render_begin(RenderWs W) {
     *W.total = 0;
     *W.overflow = 0;
     *W.max_count = 0;
+    *W.skip = 0;
   }
   for (int t = threadIdx.x; t < W.n_tiles; t += blockDim.x) W.tile_count[t] = 0;
 }
@@ -688,6 +696,13 @@ __global__ __launch_bounds__(kScanBlock) void k_rtile_scan(RenderWs W) {
     carry += tot;
   }
   if (threadIdx.x == 0) W.tile_start[T] = carry;
+  if (carry > W.scap) {  // more samples than sorted positions: skip the rest of the render
+    if (threadIdx.x == 0) {
+      *W.skip = 1;
+      *W.overflow = 1;
+    }
+    return;
+  }
   __syncthreads();
   // padding slots at the end of every bucket
   for (int t = 0; t < T; ++t) {
@@ -699,6 +714,7 @@ __global__ __launch_bounds__(kScanBlock) void k_rtile_scan(RenderWs W) {
 __global__ __launch_bounds__(kSortBlock) void k_rtile_scatter(RenderWs W) {
   __shared__ int hist[kMaxTiles];
   __shared__ int64_t base[kMaxTiles];
+  if (*W.skip) return;
   const int T = W.n_tiles;
   for (int t = threadIdx.x; t < T; t += blockDim.x) hist[t] = 0;
   __syncthreads();
@@ -730,7 +746,7 @@ __global__ __launch_bounds__(256) void k_render_reduce(RenderWs W, float* __rest
   if (i >= W.n) return;
   const int c = W.counts[i];
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-  if (c > W.cpr) {
+  if (c > W.cpr || *W.skip) {
     c0 = c1 = c2 = __builtin_nanf("");
   } else {
     const float4* v = W.rec_f + i * (int64_t)W.cpr;
@@ -825,7 +841,7 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
   const int lane = threadIdx.x & 63, h = lane >> 5, s = lane & 31;
   const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   const int T = W.n_tiles;
-  const int64_t nchunks = W.tile_start[T] >> 6;
+  const int64_t nchunks = (*W.skip ? 0 : W.tile_start[T]) >> 6;
   const int64_t per = (nchunks + gridDim.x - 1) / gridDim.x;
   int64_t cb0 = (int64_t)blockIdx.x * per;
   const int64_t cend = cb0 + per < nchunks ? cb0 + per : nchunks;
@@ -850,14 +866,32 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
     }
     __syncthreads();
 
+    // records of this wave's first chunk; every later chunk's perm entries and records are
+    // loaded one chunk ahead (perm at the top of the previous iteration, the records after its
+    // gather), so that only the brick-table and feature loads are exposed
+    int32_t rk[2] = {-1, -1};
+    float4 rr[2] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+    if (cb0 + wv < seg_end) {
+#pragma unroll
+      for (int cbk = 0; cbk < 2; ++cbk) {
+        rk[cbk] = W.perm[(cb0 + wv) * 64 + cbk * 32 + s];
+        if (rk[cbk] >= 0) rr[cbk] = W.rec_f[rk[cbk]];
+      }
+    }
     for (int64_t c = cb0 + wv; c < seg_end; c += nwv) {
+      const bool has_next = c + nwv < seg_end;
+      int32_t nk[2] = {-1, -1};
+      if (has_next) {
+#pragma unroll
+        for (int cbk = 0; cbk < 2; ++cbk) nk[cbk] = W.perm[(c + nwv) * 64 + cbk * 32 + s];
+      }
       // ---- gather: lane (h, s) builds its half of the input column of sample s of both blocks
       vec xin[2][NKC0];
       float wgt[2];
       int32_t rec[2], ray[2];
 #pragma unroll
       for (int cbk = 0; cbk < 2; ++cbk) {
-        const int32_t k = W.perm[c * 64 + cbk * 32 + s];
+        const int32_t k = rk[cbk];
         rec[cbk] = k;
         ray[cbk] = k >= 0 ? k / W.cpr : 0;
         wgt[cbk] = 0.0f;
@@ -867,7 +901,7 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
 #pragma unroll
           for (int e = 0; e < 8; ++e) xv[kc][e] = 0.0f;
         if (k >= 0) {
-          const float4 rf = W.rec_f[k];
+          const float4 rf = rr[cbk];
           wgt[cbk] = rf.w;
           Corners cn;
           find_corners(S.brick_map + tile_base, tile_res, v3(rf.x, rf.y, rf.z), cn);
@@ -909,6 +943,11 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
         }
         __builtin_amdgcn_sched_barrier(0);
       }
+      float4 nr[2] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+#pragma unroll
+      for (int cbk = 0; cbk < 2; ++cbk)
+        if (nk[cbk] >= 0) nr[cbk] = W.rec_f[nk[cbk]];
+      __builtin_amdgcn_sched_barrier(0);
 
       // ---- MLP, A fragments from LDS
       vec xa[2][NKH], xb[2][NKH];
@@ -952,6 +991,11 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
             W.rec_f[rec[1]] = make_float4(wgt[1] * sigmoid_cw(acc1[0]), wgt[1] * sigmoid_cw(acc1[1]),
                                           wgt[1] * sigmoid_cw(acc1[2]), 0.0f);
         }
+      }
+#pragma unroll
+      for (int cbk = 0; cbk < 2; ++cbk) {
+        rk[cbk] = nk[cbk];
+        rr[cbk] = nr[cbk];
       }
     }
     cb0 = seg_end;
@@ -1257,13 +1301,15 @@ using namespace snisr;
 
 extern "C" {
 
-int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t records_per_ray) {
-  if (!scene || n < 0 || records_per_ray < 0) {
+int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t records_per_ray,
+                                         int64_t sample_capacity) {
+  if (!scene || n < 0 || records_per_ray < 0 || sample_capacity < 0 || scene->mlp_in <= 0) {
     set_error("render_mfma_workspace_size: bad arguments");
     return -1;
   }
   const int64_t cpr = records_per_ray ? records_per_ray : kDefaultRecordsPerRay;
-  return render_layout(n, cpr, scene->n_tiles, nullptr, nullptr);
+  const int64_t cap = sample_capacity ? sample_capacity : kDefaultSamplesPerRay * n;
+  return render_layout(n, cpr, cap, scene->n_tiles, nullptr, nullptr);
 }
 
 int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
@@ -1275,6 +1321,8 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
                             const snisr_render_out* out, const snisr_tuning* tuning, void* workspace,
                             int64_t workspace_bytes, void* stream) {
   const snisr_tuning tn = tuning ? *tuning : snisr_tuning{0, 512, kSortGrid, 0};
+  SNISR_REQUIRE(tn.records_per_ray >= 0 && tn.records_per_ray <= 4096, SNISR_ERR_ARG,
+                "render_mfma: records_per_ray out of range");
   SNISR_REQUIRE(tn.ray_order == 0 || tn.ray_order == 1, SNISR_ERR_ARG, "render_mfma: ray_order must be 0 or 1");
   SNISR_REQUIRE(tn.mlp_threads == 256 || tn.mlp_threads == 512, SNISR_ERR_ARG,
                 "render_mfma: mlp_threads must be 256 or 512");
@@ -1290,17 +1338,19 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
   const int64_t n = rays->n;
   if (n == 0) return SNISR_OK;
   SNISR_REQUIRE(scene->n_tiles <= kMaxTiles, SNISR_ERR_SHAPE, "render_mfma: at most %d tiles", kMaxTiles);
-  // records per ray: the most that fit the workspace
-  const int64_t fixed = render_layout(n, 0, scene->n_tiles, nullptr, nullptr);
-  SNISR_REQUIRE(workspace_bytes > fixed, SNISR_ERR_CAPACITY, "render_mfma: workspace too small");
-  int64_t cpr = (workspace_bytes - fixed) / (kRenderBytesPerSlot * n);
-  while (cpr > 0 && render_layout(n, cpr, scene->n_tiles, nullptr, nullptr) > workspace_bytes) --cpr;
-  SNISR_REQUIRE(cpr > 0, SNISR_ERR_CAPACITY, "render_mfma: workspace too small for one record per ray");
-  if (cpr > 4096) cpr = 4096;
+  // records per ray from the schedule (default 64); every remaining byte goes to the sorted arrays
+  int64_t cpr = tn.records_per_ray > 0 ? tn.records_per_ray : kDefaultRecordsPerRay;
   if (n * cpr >= (int64_t(1) << 31)) cpr = ((int64_t(1) << 31) - 1) / n;  // slot ids are 32-bit
   SNISR_REQUIRE(cpr > 0, SNISR_ERR_CAPACITY, "render_mfma: batch too large for 32-bit record slots");
+  const int64_t fixed = render_layout(n, cpr, 0, scene->n_tiles, nullptr, nullptr);
+  SNISR_REQUIRE(workspace_bytes > fixed, SNISR_ERR_CAPACITY,
+                "render_mfma: workspace too small for %lld records per ray", (long long)cpr);
+  int64_t samples = (workspace_bytes - fixed) / kSortedBytes;
+  while (samples > 0 && render_layout(n, cpr, samples, scene->n_tiles, nullptr, nullptr) > workspace_bytes)
+    --samples;
+  SNISR_REQUIRE(samples > 0, SNISR_ERR_CAPACITY, "render_mfma: workspace too small for the sorted samples");
   RenderWs W;
-  render_layout(n, cpr, scene->n_tiles, (uint8_t*)workspace, &W);
+  render_layout(n, cpr, samples, scene->n_tiles, (uint8_t*)workspace, &W);
 
   const int F = scene->feat_dim, WD = scene->mlp_width, L = scene->mlp_layers;
   const bool half = scene->feat_dtype == SNISR_F16;

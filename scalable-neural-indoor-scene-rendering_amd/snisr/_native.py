@@ -59,7 +59,7 @@ class OctreeDesc(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("ray_order", C.c_int32), ("mlp_threads", C.c_int32), ("sort_grid", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("records_per_ray", C.c_int32)]
 
 
 _lib = None
@@ -74,7 +74,7 @@ SIGNATURES = [
                                     C.c_void_p, C.c_void_p]),
     ("snisr_render_fp32", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
                                     C.POINTER(RenderOut), C.c_void_p]),
-    ("snisr_render_mfma_workspace_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int64, C.c_int64]),
+    ("snisr_render_mfma_workspace_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int64, C.c_int64, C.c_int64]),
     ("snisr_render_mfma", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
                                     C.POINTER(RenderOut), C.c_void_p, C.c_int64, C.c_void_p]),
     ("snisr_render_mfma_tuned", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Tuning),

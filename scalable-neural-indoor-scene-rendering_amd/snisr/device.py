@@ -194,20 +194,24 @@ def render_fp32(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_
 
 
 class MfmaWorkspace:
-    """Caller-owned workspace of snisr_render_mfma, grown on demand.  ``records_per_ray`` is the
-    per-ray record capacity (0 = the library default of 64)."""
+    """Caller-owned workspace of snisr_render_mfma, grown on demand.  It keeps the record slots per
+    ray (``cpr``) and the sorted-sample capacity it was last sized for; requests only grow them."""
 
     def __init__(self):
         self.buf = None
+        self.cpr = 0
+        self.samples = 0
 
-    def get(self, ds: DeviceScene, n: int, records_per_ray: int = 0):
-        need = N.lib().snisr_render_mfma_workspace_size(C.byref(ds.desc), n, records_per_ray)
+    def get(self, ds: DeviceScene, n: int, records_per_ray: int = 0, samples: int = 0):
+        cpr = max(int(records_per_ray), self.cpr) or 64
+        cap = max(int(samples), self.samples) or 16 * max(n, 1)
+        need = N.lib().snisr_render_mfma_workspace_size(C.byref(ds.desc), n, cpr, cap)
         if need < 0:
             raise RuntimeError(N.lib().snisr_last_error().decode())
         if self.buf is None or self.buf.numel() < need:
             self.buf = None
             self.buf = torch.empty(need, dtype=torch.uint8, device=ds.sigma.device)
-        # the whole buffer: the render takes as many records per ray as it holds
+        self.cpr, self.samples = cpr, cap
         return self.buf, self.buf.numel()
 
 
@@ -221,24 +225,25 @@ def mfma_stats(buf, with_max: bool = False):
 
 
 def records_for(max_count: int, headroom: float = 1.25) -> int:
-    """Per-ray record capacity for a workload whose largest ray has ``max_count`` samples."""
+    """Per-ray record slots for a workload whose largest ray has ``max_count`` samples."""
     return max(8, -(-int(max_count * headroom + 1) // 8) * 8)
 
 
-def tuning_struct(t: dict | None):
-    """snisr_tuning from a dict {ray_order, mlp_threads, sort_grid} (None = library defaults)."""
-    if t is None:
-        return None
-    return N.Tuning(int(t.get("ray_order", 0)), int(t.get("mlp_threads", 512)), int(t.get("sort_grid", 512)), 0)
+def tuning_struct(t: dict | None, records_per_ray: int = 0):
+    """snisr_tuning from a dict {ray_order, mlp_threads, sort_grid} and the record slots per ray."""
+    t = t or {}
+    return N.Tuning(int(t.get("ray_order", 0)), int(t.get("mlp_threads", 512)), int(t.get("sort_grid", 512)),
+                    int(records_per_ray))
 
 
 def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_width=0, brick_cap=0,
-                workspace: MfmaWorkspace | None = None, capacity: int = 0, check: bool = True, events=None,
-                tuning: dict | None = None, outputs: dict | None = None, **march_kw):
-    """Render with the per-tile MLP on MFMA (snisr_render_mfma_tuned).  ``capacity``: sample records
-    per ray (0 = default).  With ``check`` the capacity is verified after the call and the render
-    repeated with room for the largest ray.  ``outputs`` (from a previous call) is written in place
-    instead of allocating new tensors."""
+                workspace: MfmaWorkspace | None = None, capacity: int = 0, sample_capacity: int = 0,
+                check: bool = True, events=None, tuning: dict | None = None, outputs: dict | None = None,
+                **march_kw):
+    """Render with the per-tile MLP on MFMA (snisr_render_mfma_tuned).  ``capacity``: sample record
+    slots per ray, ``sample_capacity``: samples of the frame (0 = keep the workspace's).  With
+    ``check`` the capacities are verified after the call and the render repeated with room for the
+    largest ray and the whole frame.  ``outputs`` (from a previous call) is written in place."""
     if ds.mlp_packed is None:
         raise RuntimeError("scene has no MFMA-packed MLP; call DeviceScene.pack_mfma()")
     r = _rays_desc(origins, dirs, t_hit, visited, image_width)
@@ -251,17 +256,17 @@ def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_
         o.n_samples, o.depth = out["n_samples"].data_ptr(), out["depth"].data_ptr()
     keep = _attach_events(o, events)  # noqa: F841
     ws = workspace or MfmaWorkspace()
-    buf, need = ws.get(ds, r.n, capacity)
-    tn = tuning_struct(tuning)
-    tp = C.byref(tn) if tn is not None else None
-    N.check(N.lib().snisr_render_mfma_tuned(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), tp, _ptr(buf),
-                                            need, _stream()), "render_mfma")
+    buf, nbytes = ws.get(ds, r.n, capacity, sample_capacity)
+    tn = tuning_struct(tuning, ws.cpr)
+    N.check(N.lib().snisr_render_mfma_tuned(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), C.byref(tn),
+                                            _ptr(buf), nbytes, _stream()), "render_mfma")
     if check:
         total, over, mx = mfma_stats(buf, with_max=True)
         if over:
-            buf, need = ws.get(ds, r.n, records_for(mx))
-            N.check(N.lib().snisr_render_mfma_tuned(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), tp,
-                                                    _ptr(buf), need, _stream()), "render_mfma")
+            buf, nbytes = ws.get(ds, r.n, records_for(mx), int(total * 1.25) + 64)
+            tn = tuning_struct(tuning, ws.cpr)
+            N.check(N.lib().snisr_render_mfma_tuned(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o),
+                                                    C.byref(tn), _ptr(buf), nbytes, _stream()), "render_mfma")
             total, over, mx = mfma_stats(buf, with_max=True)
             assert not over
         out["total_samples"] = total

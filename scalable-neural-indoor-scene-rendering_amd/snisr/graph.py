@@ -22,7 +22,8 @@ class RenderGraph:
         first = D.render_mfma(ds, origins, dirs, t_hit=t_hit, image_width=image_width, workspace=self.ws,
                               check=True, tuning=tuning)
         self.capacity = D.records_for(first["max_samples_per_ray"], capacity_factor)
-        self.ws.get(ds, dirs.numel() // 3, self.capacity)
+        self.sample_capacity = int(first["total_samples"] * capacity_factor) + 64
+        self.ws.get(ds, dirs.numel() // 3, self.capacity, self.sample_capacity)
         self.outputs = {k: first[k] for k in ("rgb", "T", "n_samples", "depth")}
         self.graph = torch.cuda.CUDAGraph()
         side = torch.cuda.Stream()
@@ -36,11 +37,12 @@ class RenderGraph:
 
     def _launch(self):
         D.render_mfma(self.ds, self.origins, self.dirs, t_hit=self.t_hit, image_width=self.image_width,
-                      workspace=self.ws, capacity=self.capacity, check=False, tuning=self.tuning,
+                      workspace=self.ws, capacity=self.capacity, sample_capacity=self.sample_capacity,
+                      check=False, tuning=self.tuning,
                       outputs=self.outputs)
 
     def overflowed(self) -> bool:
-        """True if a ray of the last replay had more samples than its record slots (synchronises)."""
+        """True if the last replay ran out of record slots or sample capacity (synchronises)."""
         return D.mfma_stats(self.ws.buf)[1] != 0
 
     def replay(self):

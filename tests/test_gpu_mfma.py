@@ -80,6 +80,22 @@ def test_mfma_capacity_overflow_is_detected_and_recovered():
     assert torch.equal(full["rgb"], ref["rgb"])
 
 
+def test_mfma_sample_capacity_overflow_is_detected_and_recovered():
+    sc, cam, _ = S.config_c1()
+    ds = D.DeviceScene(sc, mfma_dtype=N.SNISR_F16)
+    dirs = S.camera_rays_np(cam)
+    o, d = to_dev(cam.origin), to_dev(dirs)
+    ref = D.render_mfma(ds, o, d)
+    ws = D.MfmaWorkspace()
+    small = D.render_mfma(ds, o, d, workspace=ws, capacity=128, sample_capacity=1000, check=False)
+    total, over = D.mfma_stats(ws.buf)
+    assert over == 1 and total == int(ref["n_samples"].sum())
+    assert torch.isnan(small["rgb"]).all()          # the frame is skipped, not partially rendered
+    assert torch.equal(small["n_samples"], ref["n_samples"]) and torch.equal(small["T"], ref["T"])
+    full = D.render_mfma(ds, o, d, workspace=ws, check=True)
+    assert torch.equal(full["rgb"], ref["rgb"])
+
+
 def test_mfma_deterministic():
     sc, cam, _ = S.config_c1()
     ds = D.DeviceScene(sc, mfma_dtype=N.SNISR_F16)
