@@ -117,6 +117,95 @@ __global__ __launch_bounds__(64) void k_mesh_first_hit(const float4* __restrict_
   id_out[i] = best_id;
 }
 
+// Packet traversal for coherent rays (camera 8x8 pixel packets): the wavefront walks one shared
+// traversal order, the union of the nodes its 64 rays need.  Each lane tests its own ray against
+// both child boxes of the current node with its own nearest t; a child is visited when any lane
+// hits it, the nearer one first by a lane vote; a leaf's triangles are tested by every lane once any
+// lane hits the leaf box.  The stack lives in LDS, one per wavefront, and every lane holds the same
+// stack pointer, so node fetches are wave-uniform (one 64-byte line for the whole wavefront).  The
+// nearest hit of each ray is the same as in the per-lane traversal: the order of visits never
+// changes the minimum t, and ties go to the smaller triangle id.
+__global__ __launch_bounds__(64) void k_mesh_first_hit_packet(const float4* __restrict__ nodes,
+                                                              const float4* __restrict__ tris, snisr_rays R,
+                                                              float* __restrict__ t_out, int32_t* __restrict__ id_out,
+                                                              int packet_w) {
+  __shared__ int stack[kBvhStack];
+  const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const bool live = tid < R.n;
+  const int64_t i = live ? ray_of_thread(tid, R.n, packet_w) : 0;
+  V3 o = v3(0.0f, 0.0f, 0.0f), d = v3(0.0f, 0.0f, 1.0f);
+  if (live) {
+    const float* og = R.origins + i * R.origin_stride;
+    o = v3(og[0], og[1], og[2]);
+    d = normalize_dir(v3(R.dirs[3 * i], R.dirs[3 * i + 1], R.dirs[3 * i + 2]));
+  }
+  const V3 inv = v3(d.x != 0.0f ? 1.0f / d.x : __builtin_copysignf(1e30f, d.x),
+                    d.y != 0.0f ? 1.0f / d.y : __builtin_copysignf(1e30f, d.y),
+                    d.z != 0.0f ? 1.0f / d.z : __builtin_copysignf(1e30f, d.z));
+  float best = live ? __builtin_inff() : -1.0f;  // a dead lane never wants a node
+  int best_id = -1;
+  int sp = 0;
+  int node = 0;
+  while (true) {
+    const float4 q0 = nodes[4 * node], q1 = nodes[4 * node + 1], q2 = nodes[4 * node + 2], q3 = nodes[4 * node + 3];
+    const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+    const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+    const int c[2] = {__float_as_int(q3.x), __float_as_int(q3.y)};
+    float tn[2], tf[2];
+    slab(b0, o, inv, tn[0], tf[0]);
+    slab(b1, o, inv, tn[1], tf[1]);
+    bool any[2];
+    bool mine[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      mine[k] = tn[k] <= tf[k] && tf[k] >= 0.0f && tn[k] <= best;
+      any[k] = __ballot(mine[k]) != 0;
+    }
+    int inner[2], ni = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!any[k]) continue;
+      if (c[k] >= 0) {
+        inner[ni++] = c[k];
+      } else {
+        const int code = ~c[k];
+        const int first = code >> 3, cnt = code & 7;
+        for (int j = 0; j < cnt; ++j) {
+          const float4 t0 = tris[3 * (first + j)], t1 = tris[3 * (first + j) + 1], t2 = tris[3 * (first + j) + 2];
+          const int id = __float_as_int(t0.w);
+          const float t = ray_triangle(o, d, v3(t0.x, t0.y, t0.z), v3(t1.x, t1.y, t1.z), v3(t2.x, t2.y, t2.z));
+          if (live && t > 0.0f && (t < best || (t == best && id < best_id))) {
+            best = t;
+            best_id = id;
+          }
+        }
+      }
+    }
+    if (ni == 2) {
+      // nearer child first, by the vote of the lanes that hit both
+      const bool both = mine[0] && mine[1];
+      const int n_both = __popcll(__ballot(both));
+      const int n_first1 = __popcll(__ballot(both && tn[1] < tn[0]));
+      const bool swap = 2 * n_first1 > n_both;
+      const int near_n = swap ? inner[1] : inner[0], far_n = swap ? inner[0] : inner[1];
+      if (threadIdx.x == 0) stack[sp] = far_n;  // sp < kBvhStack: the tree depth is below the stack size
+      ++sp;
+      node = near_n;
+    } else if (ni == 1) {
+      node = inner[0];
+    } else {
+      if (sp == 0) break;
+      --sp;
+      __syncthreads();  // one wavefront per workgroup, uniform branch: orders the stack write before the read
+      node = stack[sp];
+    }
+  }
+  if (live) {
+    t_out[i] = best;
+    id_out[i] = best_id;
+  }
+}
+
 int packet_width(const snisr_rays& R);  // render_fp32.hip
 
 }  // namespace snisr
@@ -134,8 +223,14 @@ int snisr_mesh_first_hit(const void* nodes, int32_t n_nodes, const void* tris, i
   SNISR_REQUIRE(aligned(nodes, 16) && aligned(tris, 16), SNISR_ERR_ALIGN, "mesh_first_hit: 16-byte alignment");
   if (int e = check_rays(rays, nullptr)) return e;
   if (rays->n == 0) return SNISR_OK;
-  hipLaunchKernelGGL(k_mesh_first_hit, dim3((unsigned)((rays->n + 63) / 64)), dim3(64), 0, (hipStream_t)stream,
-                     (const float4*)nodes, (const float4*)tris, *rays, t_out, id_out, packet_width(*rays));
+  // camera packets (8x8 pixels per wavefront) share one traversal; other rays walk one per lane
+  const int pw = packet_width(*rays);
+  if (pw > 0)
+    hipLaunchKernelGGL(k_mesh_first_hit_packet, dim3((unsigned)((rays->n + 63) / 64)), dim3(64), 0,
+                       (hipStream_t)stream, (const float4*)nodes, (const float4*)tris, *rays, t_out, id_out, pw);
+  else
+    hipLaunchKernelGGL(k_mesh_first_hit, dim3((unsigned)((rays->n + 63) / 64)), dim3(64), 0, (hipStream_t)stream,
+                       (const float4*)nodes, (const float4*)tris, *rays, t_out, id_out, pw);
   SNISR_LAUNCHED();
   return SNISR_OK;
 }

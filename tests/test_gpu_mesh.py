@@ -61,3 +61,30 @@ def test_mesh_hit_tiny_and_empty_meshes(oracle_mod):
     dm = D.DeviceMesh(empty.vertices, empty.faces)
     t, ids = dm.first_hit(to_dev(origins), to_dev(dirs))
     assert torch.all(ids == -1) and torch.all(torch.isinf(t))
+
+
+def test_packet_traversal_matches_per_lane_and_brute_force(oracle_mod):
+    """Camera 8x8 packets take the wavefront-shared traversal: the full C2 frame equals the per-lane
+    traversal bit-exactly, a subset equals the brute force, and incoherent rays laid out as
+    packets (the worst case for a shared traversal) equal the brute force."""
+    sc, cam, mesh = S.config_c2()
+    dm = D.DeviceMesh(mesh.vertices, mesh.faces)
+    dirs = S.camera_rays_np(cam)
+    o, d = to_dev(cam.origin), to_dev(dirs)
+    t_pk, id_pk = dm.first_hit(o, d, image_width=cam.W)
+    t_ln, id_ln = dm.first_hit(o, d)
+    assert torch.equal(id_pk, id_ln) and torch.equal(t_pk.view(torch.int32), t_ln.view(torch.int32))
+    idx = np.random.default_rng(3).choice(dirs.shape[0], 1500, replace=False)
+    t_ref, id_ref = oracle_mod.mesh_hit(mesh.vertices, mesh.faces, cam.origin, oracle_mod.normalize(dirs[idx]))
+    assert np.array_equal(id_pk.cpu().numpy()[idx], id_ref)
+    assert np.array_equal(t_pk.cpu().numpy()[idx].view(np.uint32), t_ref.view(np.uint32))
+    small = S.room_mesh(8, (0, 0, 0), (2, 2, 1), 12000)
+    dsm = D.DeviceMesh(small.vertices, small.faces)
+    rng = np.random.default_rng(4)
+    n = 96 * 64
+    origins = rng.uniform([-0.5, -0.5, -0.5], [2.5, 2.5, 1.5], (n, 3)).astype(np.float32)
+    rd = rng.standard_normal((n, 3)).astype(np.float32)
+    t, ids = dsm.first_hit(to_dev(origins), to_dev(rd), image_width=96)
+    t_ref, id_ref = oracle_mod.mesh_hit(small.vertices, small.faces, origins, oracle_mod.normalize(rd))
+    assert np.array_equal(ids.cpu().numpy(), id_ref)
+    assert np.array_equal(t.cpu().numpy().view(np.uint32), t_ref.view(np.uint32))
