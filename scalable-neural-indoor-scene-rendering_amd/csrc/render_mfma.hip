@@ -889,6 +889,36 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
       vec xin[2][NKC0];
       float wgt[2];
       int32_t rec[2], ray[2];
+      if constexpr (HALF) {
+        // branch-free for both column blocks (padding slots read valid memory with zero weights), so
+        // the brick-table and feature loads of the two blocks are in flight together
+        Corners cn[2];
+        h2 w2[2][8];
+#pragma unroll
+        for (int cbk = 0; cbk < 2; ++cbk) {
+          const int32_t k = rk[cbk];
+          rec[cbk] = k;
+          ray[cbk] = k >= 0 ? k / W.cpr : 0;
+          wgt[cbk] = k >= 0 ? rr[cbk].w : 0.0f;
+          find_corners(S.brick_map + tile_base, tile_res, v3(rr[cbk].x, rr[cbk].y, rr[cbk].z), cn[cbk]);
+          corner_weights_h2(cn[cbk], w2[cbk]);
+          if (k < 0) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) w2[cbk][q] = h2{0, 0};
+          }
+        }
+#pragma unroll
+        for (int kc = 0; kc < NFC; ++kc)
+#pragma unroll
+          for (int cbk = 0; cbk < 2; ++cbk)
+            xin[cbk][kc] = interp8_h2<M>(reinterpret_cast<const _Float16*>(S.feats), cn[cbk], w2[cbk], F, 16 * kc + 8 * h);
+#pragma unroll
+        for (int cbk = 0; cbk < 2; ++cbk) {
+          const vec e = load_frag<vec>(reinterpret_cast<const uint8_t*>(W.enc + (int64_t)ray[cbk] * 16 + 8 * h));
+          float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          xin[cbk][NFC] = rec[cbk] >= 0 ? e : M::cvt8(z);
+        }
+      } else {
 #pragma unroll
       for (int cbk = 0; cbk < 2; ++cbk) {
         const int32_t k = rk[cbk];
@@ -942,6 +972,7 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
           xin[cbk][NFC] = M::cvt8(z);
         }
         __builtin_amdgcn_sched_barrier(0);
+      }
       }
       float4 nr[2] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
 #pragma unroll
