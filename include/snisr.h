@@ -251,9 +251,10 @@ int64_t snisr_sort_by_key_workspace_size(int64_t n);
 int snisr_sort_by_key(int16_t* keys, int32_t* values, int32_t* starts, int64_t n, void* workspace,
                       int64_t workspace_bytes, int32_t* n_unique, void* stream);
 
-/* A8 — assign_pixels (rendering/src/reorder.cu:69-94, include/reorder.h:10-15). */
+/* A8 — assign_pixels (rendering/src/reorder.cu:69-94, include/reorder.h:10-15).  n_blocks =
+ * block_starts[n_net], the number of output chunks (the reference's caller sizes the outputs with it). */
 int snisr_assign_pixels(const int16_t* net, const int32_t* pixel_starts, const int32_t* block_starts, int64_t n_net,
-                        int16_t* new_net, int32_t* new_pixel_starts, void* stream);
+                        int64_t n_blocks, int16_t* new_net, int32_t* new_pixel_starts, void* stream);
 
 /* A7 — inference_mlp (rendering/src/rayreflection_render.cu:111-335, include/rayreflection_render.h:7-19).
  * The reflection MLP of every sample of the listed pixel blocks:  block b covers sorted pixels

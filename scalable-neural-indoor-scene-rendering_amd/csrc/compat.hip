@@ -30,19 +30,41 @@ __global__ __launch_bounds__(256) void k_inverse_z(const float* __restrict__ ori
 
 // A5: 16 samples per pixel; skip pixels with T < early_stop; alpha = 1 - exp(-sigma*step),
 // C += alpha*T*rgb, T *= 1 - alpha; T and colours updated in place
-// (rendering/src/volume_rendering.cu:5-58).
-__global__ __launch_bounds__(256) void k_integrate_points(const float4* __restrict__ res, const float* __restrict__ steps,
-                                                          int64_t npix, float early, float* __restrict__ T,
-                                                          float* __restrict__ color) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// (rendering/src/volume_rendering.cu:5-58).  One wave per 64 pixels: the 64 x 16 sample records
+// and steps are read with contiguous loads and transposed through LDS to sample-major rows
+// (padded to 65 entries, so both the stores and the per-pixel reads are bank-conflict free); each
+// lane then composites its pixel in sample order.
+constexpr int kIntegPix = 64, kIntegPad = kIntegPix + 1;
+__global__ __launch_bounds__(kIntegPix) void k_integrate_points(const float4* __restrict__ res,
+                                                                const float* __restrict__ steps, int64_t npix,
+                                                                float early, float* __restrict__ T,
+                                                                float* __restrict__ color) {
+  __shared__ float4 rs[16 * kIntegPad];
+  __shared__ float ss[16 * kIntegPad];
+  const int lane = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * kIntegPix;
+  const int64_t rem = npix - base;
+  const int nv = 16 * (int)(rem < kIntegPix ? rem : kIntegPix);  // records of this block
+  const float4* rb = res + base * 16;
+  const float* sb = steps + base * 16;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int g = k * kIntegPix + lane;  // record g: pixel g / 16, sample g % 16
+    if (g < nv) {
+      rs[(g & 15) * kIntegPad + (g >> 4)] = rb[g];
+      ss[(g & 15) * kIntegPad + (g >> 4)] = sb[g];
+    }
+  }
+  __syncthreads();
+  const int64_t p = base + lane;
   if (p >= npix) return;
   float t = T[p];
   if (t < early) return;
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    const float4 v = res[p * 16 + i];
-    const float alpha = 1.0f - exp_cw(-1.0f * v.w * steps[p * 16 + i]);
+    const float4 v = rs[i * kIntegPad + lane];
+    const float alpha = 1.0f - exp_cw(-1.0f * v.w * ss[i * kIntegPad + lane]);
     const float at = alpha * t;
     c0 = c0 + at * v.x;
     c1 = c1 + at * v.y;
@@ -77,23 +99,29 @@ __global__ __launch_bounds__(256) void k_run_compact(const int16_t* __restrict__
 }
 
 // A8: assign_pixels_kernel (rendering/src/reorder.cu:25-67): group i's pixel range is split into
-// block_starts[i+1]-block_starts[i] chunks whose sizes differ by at most one (larger first)
+// block_starts[i+1]-block_starts[i] chunks whose sizes differ by at most one (larger first).  One
+// thread per output chunk (the reference loops over a group's chunks in one thread): chunk b finds
+// its group by binary search over block_starts, then its start in closed form.
 __global__ __launch_bounds__(256) void k_assign_pixels(const int16_t* __restrict__ net, const int32_t* __restrict__ pixel_starts,
                                                        const int32_t* __restrict__ block_starts, int64_t nnet,
-                                                       int16_t* __restrict__ new_net, int32_t* __restrict__ new_starts) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nnet) return;
-  const int start_loc = block_starts[i];
-  int ps = pixel_starts[i];
-  const int npx = pixel_starts[i + 1] - ps;
-  const int nb = block_starts[i + 1] - start_loc;
-  if (nb <= 0) return;
-  const int base = npx / nb, add = npx - base * nb;
-  for (int b = 0; b < nb; ++b) {
-    new_net[start_loc + b] = net[i];
-    new_starts[start_loc + b] = ps;
-    ps += (b < add) ? base + 1 : base;
+                                                       int64_t nblocks, int16_t* __restrict__ new_net,
+                                                       int32_t* __restrict__ new_starts) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  // group i with block_starts[i] <= b < block_starts[i+1] (groups with no chunk are skipped)
+  int64_t lo = 0, hi = nnet;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (block_starts[mid] <= b) lo = mid; else hi = mid;
   }
+  const int64_t i = lo;
+  const int nb = block_starts[i + 1] - block_starts[i];
+  if (nb <= 0 || b < block_starts[i] || b >= block_starts[i + 1]) return;
+  const int ps = pixel_starts[i], npx = pixel_starts[i + 1] - ps;
+  const int base = npx / nb, add = npx - base * nb;
+  const int r = (int)(b - block_starts[i]);
+  new_net[b] = net[i];
+  new_starts[b] = ps + r * base + (r < add ? r : add);
 }
 
 }  // namespace snisr
@@ -124,7 +152,8 @@ int snisr_integrate_points(const float* sample_results, const float* integrate_s
   SNISR_REQUIRE(sample_results && integrate_steps && transparency && colors, SNISR_ERR_ARG,
                 "integrate_points: NULL pointer");
   SNISR_REQUIRE(aligned(sample_results, 16), SNISR_ERR_ALIGN, "integrate_points: sample_results must be 16-byte aligned");
-  hipLaunchKernelGGL(k_integrate_points, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(k_integrate_points, dim3((unsigned)((npix + kIntegPix - 1) / kIntegPix)), dim3(kIntegPix), 0,
+                     (hipStream_t)stream,
                      (const float4*)sample_results, integrate_steps, npix, early_stop, transparency, colors);
   SNISR_LAUNCHED();
   return SNISR_OK;
@@ -193,12 +222,12 @@ int snisr_sort_by_key(int16_t* keys, int32_t* values, int32_t* starts, int64_t n
 }
 
 int snisr_assign_pixels(const int16_t* net, const int32_t* pixel_starts, const int32_t* block_starts, int64_t n_net,
-                        int16_t* new_net, int32_t* new_pixel_starts, void* stream) {
-  if (n_net <= 0) return SNISR_OK;
+                        int64_t n_blocks, int16_t* new_net, int32_t* new_pixel_starts, void* stream) {
+  if (n_net <= 0 || n_blocks <= 0) return SNISR_OK;
   SNISR_REQUIRE(net && pixel_starts && block_starts && new_net && new_pixel_starts, SNISR_ERR_ARG,
                 "assign_pixels: NULL pointer");
-  hipLaunchKernelGGL(k_assign_pixels, dim3((unsigned)((n_net + 255) / 256)), dim3(256), 0, (hipStream_t)stream, net,
-                     pixel_starts, block_starts, n_net, new_net, new_pixel_starts);
+  hipLaunchKernelGGL(k_assign_pixels, dim3((unsigned)((n_blocks + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     net, pixel_starts, block_starts, n_net, n_blocks, new_net, new_pixel_starts);
   SNISR_LAUNCHED();
   return SNISR_OK;
 }

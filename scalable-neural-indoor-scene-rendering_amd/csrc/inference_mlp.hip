@@ -14,10 +14,13 @@
 // with a wave ballot, so every MFMA column is a live sample.  One wavefront evaluates 64 samples:
 // two 32-column blocks; all six layers keep their activations in registers in the B-operand layout.
 //
-// Positional encoding: lane half h = 0 holds x, y and bands 0..4, h = 1 holds z and bands 5..9
-// (weights of layer 0 are packed in that order).  Each half evaluates sincos once per coordinate,
-// at 2^(5h) x, then four double-angle steps (sin 2a = 2 sin a cos a, cos 2a = cos^2 a - sin^2 a);
-// the FP32 error of the recurrence stays near 1e-6, far below the FP16 rounding of the inputs.
+// Positional encoding: lane half h = 0 holds x, y and bands 0..4, h = 1 holds z, one zero and
+// bands 5..9 (weights of layer 0 are packed in that order, so k slot s >= 2 is band slot s - 2 in
+// both halves).  Each half evaluates the hardware sin / cos once per coordinate, at 2^(5h) x (the
+// reference uses the same fast intrinsics, __sinf / __cosf), then four double-angle steps
+// (sin 2a = 2 sin a cos a, cos 2a = cos^2 a - sin^2 a); the error stays near 1e-5, below the FP16
+// rounding of the inputs.  LeakyReLU runs on the FP16 values, as in the reference (x < 0 -> 0.01 x
+// in half precision), as packed-half multiply and max.
 #include "mfma_ops.hpp"
 #include "snisr_common.hpp"
 #include "snisr_device.hpp"
@@ -41,7 +44,17 @@ static_assert(ref_src_offset(kRefLayers) == kRefParams, "reference parameter cou
 static_assert(kRefGroupBytes % 16 == 0, "group bytes");
 
 // layer-0 input index held in k slot s = 8 kc + e of lane half h (63 = zero padding)
-__host__ __device__ constexpr int pe_input_index(int h, int s) { return h == 0 ? (s < 2 ? s : s + 1) : (s == 0 ? 2 : s + 32); }
+__host__ __device__ constexpr int pe_input_index(int h, int s) {
+  return h == 0 ? (s < 2 ? s : s + 1) : (s == 0 ? 2 : (s == 1 ? kRefIn : s + 31));
+}
+
+// a / b for 0 <= a < 2^24, b > 0, from the FP32 quotient corrected by one step
+__device__ __forceinline__ int div_small(int a, int b, float inv_b) {
+  int q = (int)((float)a * inv_b);
+  const int r = a - q * b;
+  q += r < 0 ? -1 : (r >= b ? 1 : 0);
+  return q;
+}
 
 __device__ __forceinline__ float half_bits_to_float(uint16_t b) {
   _Float16 v;
@@ -79,31 +92,31 @@ __device__ __forceinline__ void pe_half(const float (&p)[3], int h, float (&v)[3
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     const float a = h ? 32.0f * p[r] : p[r];
-    float sa, ca;
-    sincosf(a, &sa, &ca);
-    sn[0][r] = sa;
-    cs[0][r] = ca;
+    sn[0][r] = __sinf(a);
+    cs[0][r] = __cosf(a);
 #pragma unroll
     for (int k = 1; k < 5; ++k) {
       sn[k][r] = 2.0f * sn[k - 1][r] * cs[k - 1][r];
       cs[k][r] = cs[k - 1][r] * cs[k - 1][r] - sn[k - 1][r] * sn[k - 1][r];
     }
   }
-  auto band = [&](int i) { return (i % 6) < 3 ? sn[i / 6][i % 6] : cs[i / 6][i % 6 - 3]; };
+  v[0] = h ? p[2] : p[0];
+  v[1] = h ? 0.0f : p[1];
 #pragma unroll
-  for (int s = 0; s < 32; ++s) {
-    const float a0 = s < 2 ? p[s] : band(s - 2);
-    const float a1 = s == 0 ? p[2] : (s < 31 ? band(s - 1) : 0.0f);
-    v[s] = h ? a1 : a0;
+  for (int s = 2; s < 32; ++s) {
+    const int i = s - 2;
+    v[s] = (i % 6) < 3 ? sn[i / 6][i % 6] : cs[i / 6][i % 6 - 3];
   }
 }
 
 __device__ __forceinline__ void leaky_cvt(const f32x16& acc, f16x8& lo, f16x8& hi) {
   float v[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = fmaxf(acc[i], 0.01f * acc[i]);
-  lo = Mfma<SNISR_F16>::cvt8(v);
-  hi = Mfma<SNISR_F16>::cvt8(v + 8);
+  for (int i = 0; i < 16; ++i) v[i] = acc[i];
+  const f16x8 slope = {0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16};
+  const f16x8 a = Mfma<SNISR_F16>::cvt8(v), b = Mfma<SNISR_F16>::cvt8(v + 8);
+  lo = __builtin_elementwise_max(a, a * slope);
+  hi = __builtin_elementwise_max(b, b * slope);
 }
 
 // real SH of degree <= 2 (orthonormal on the sphere; the reference's sign convention)
@@ -139,6 +152,7 @@ __global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
   const int lane = threadIdx.x & 63, h = lane >> 5, s = lane & 31;
   const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   const int spr = Q.samples_per_ray;
+  const float inv_spr = 1.0f / (float)spr, inv_w = 1.0f / (float)Q.W;
   const int64_t npix = (int64_t)Q.H * Q.W;
   const int64_t per = (Q.n_blocks + gridDim.x - 1) / gridDim.x;
   const int64_t b0 = (int64_t)blockIdx.x * per;
@@ -185,9 +199,10 @@ __global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
           task[cb] = -1;
           orip[cb] = 0;
           if (c < ncol) {
-            const int ori = act[c / spr];
+            const int q = div_small(c, spr, inv_spr);
+            const int ori = act[q];
             orip[cb] = ori;
-            task[cb] = (int64_t)ori * spr + c % spr;
+            task[cb] = (int64_t)ori * spr + (c - q * spr);
 #pragma unroll
             for (int i = 0; i < 3; ++i) pt[i] = (Q.samples[task[cb] * 3 + i] - ctr[i]) * Q.inverse_far;
           }
@@ -223,8 +238,8 @@ __global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
         for (int cb = 0; cb < 2; ++cb) {
           float pr[3] = {0.0f, 0.0f, 0.0f};
           if (task[cb] >= 0) {
-            const int64_t py = orip[cb] / Q.W, px = orip[cb] - py * Q.W;
-            const float* d = Q.rays_d + py * Q.d_row_stride + px * Q.d_col_stride;
+            const int py = div_small(orip[cb], Q.W, inv_w), px = orip[cb] - py * Q.W;
+            const float* d = Q.rays_d + (int64_t)py * Q.d_row_stride + (int64_t)px * Q.d_col_stride;
             const V3 u = normalize_dir(v3(d[0], d[1], d[2]));
             float bs[9];
             sh2_basis(u.x, u.y, u.z, bs);
@@ -279,6 +294,8 @@ int snisr_inference_mlp(const snisr_mlp_query* q, void* stream) {
                 "inference_mlp: packed_params and sample_results must be 16-byte aligned");
   SNISR_REQUIRE((int64_t)q->H * q->W * q->samples_per_ray < (int64_t(1) << 31), SNISR_ERR_SHAPE,
                 "inference_mlp: H*W*samples_per_ray must stay below 2^31");
+  SNISR_REQUIRE((int64_t)q->H * q->W < (int64_t(1) << 24) && q->samples_per_ray <= (1 << 18), SNISR_ERR_SHAPE,
+                "inference_mlp: H*W must stay below 2^24 and samples_per_ray at most 2^18");
   const int64_t grid = q->n_blocks < 3 * (int64_t)device_cus() ? q->n_blocks : 3 * (int64_t)device_cus();
   hipLaunchKernelGGL(k_inference_mlp, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, *q);
   SNISR_LAUNCHED();

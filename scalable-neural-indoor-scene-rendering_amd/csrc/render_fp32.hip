@@ -18,17 +18,40 @@ __global__ __launch_bounds__(256) void k_compute_rays(const float* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------- A2
-__global__ __launch_bounds__(256) void k_trace_tiles(snisr_scene S, const float* __restrict__ origin,
-                                                     const float* __restrict__ rays_d, int64_t n, int K,
-                                                     int32_t* __restrict__ visited) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const V3 o = v3(origin[0], origin[1], origin[2]);
-  const V3 d = v3(rays_d[3 * i], rays_d[3 * i + 1], rays_d[3 * i + 2]);
-  TileWalker w;
-  w.init(S, o, d, K);
-  int32_t* out = visited + i * K;
-  for (int j = 0; j < K; ++j) out[j] = w.next();
+// One wave per 64 rays.  With STAGE, each lane writes its K entries into an LDS row (odd stride,
+// so the 64 rows hit distinct banks) and the wave then stores the block's 64*K contiguous entries
+// with coalesced writes; without it (K > kTraceStageMaxK) every lane writes its row directly.
+constexpr int kTraceStageMaxK = 127;
+template <bool STAGE>
+__global__ __launch_bounds__(64) void k_trace_tiles(snisr_scene S, const float* __restrict__ origin,
+                                                    const float* __restrict__ rays_d, int64_t n, int K,
+                                                    int32_t* __restrict__ visited) {
+  extern __shared__ int32_t trace_lds[];
+  const int64_t base = (int64_t)blockIdx.x * 64;
+  const int64_t i = base + threadIdx.x;
+  const int KP = K | 1;
+  if (i < n) {
+    const V3 o = v3(origin[0], origin[1], origin[2]);
+    const V3 d = v3(rays_d[3 * i], rays_d[3 * i + 1], rays_d[3 * i + 2]);
+    TileWalker w;
+    w.init(S, o, d, K);
+    if (STAGE) {
+      int32_t* row = trace_lds + threadIdx.x * KP;
+      for (int j = 0; j < K; ++j) row[j] = w.next();
+    } else {
+      int32_t* out = visited + i * K;
+      for (int j = 0; j < K; ++j) out[j] = w.next();
+    }
+  }
+  if (!STAGE) return;
+  __syncthreads();
+  const int64_t rows = n - base < 64 ? n - base : 64;
+  const int total = (int)rows * K;
+  int32_t* out = visited + base * K;
+  for (int q = threadIdx.x; q < total; q += 64) {
+    const int r = q / K;
+    out[q] = trace_lds[r * KP + (q - r * K)];
+  }
 }
 
 // ---------------------------------------------------------------------------- MLP packing (f32t)
@@ -164,8 +187,13 @@ int snisr_trace_tiles(const snisr_scene* scene, const float* origin, const float
   SNISR_REQUIRE(origin && (n == 0 || (rays_d && visited)), SNISR_ERR_ARG, "trace_tiles: NULL pointer");
   SNISR_REQUIRE(n >= 0 && K >= 0, SNISR_ERR_ARG, "trace_tiles: negative size");
   if (n == 0 || K == 0) return SNISR_OK;
-  hipLaunchKernelGGL(k_trace_tiles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *scene,
-                     origin, rays_d, n, K, visited);
+  const dim3 grid((unsigned)((n + 63) / 64));
+  if (K <= kTraceStageMaxK)
+    hipLaunchKernelGGL(k_trace_tiles<true>, grid, dim3(64), (size_t)64 * (K | 1) * sizeof(int32_t),
+                       (hipStream_t)stream, *scene, origin, rays_d, n, K, visited);
+  else
+    hipLaunchKernelGGL(k_trace_tiles<false>, grid, dim3(64), 0, (hipStream_t)stream, *scene, origin, rays_d, n, K,
+                       visited);
   SNISR_LAUNCHED();
   return SNISR_OK;
 }

@@ -28,19 +28,7 @@ constexpr int kOctBlocks = 4;  // blocks per tile edge (num_block, raydiffuse_re
 // The cap bounds the loop for degenerate rays (zero direction: far = 1e8) and for steps below
 // zval's FP32 resolution, where zval += step would never reach far.
 constexpr int kOctMaxBlockSteps = 1 << 16;
-
-__device__ __forceinline__ void octree_voxel(const snisr_octree& O, int64_t start, const int16_t* __restrict__ nodes,
-                                             int idx, float (&v)[4]) {
-  const int k = nodes[idx];
-  if (k < 0) {
-    v[0] = v[1] = v[2] = v[3] = 0.0f;
-    return;
-  }
-  const uint2 raw = reinterpret_cast<const uint2*>(O.data_voxels)[start + k];
-  const _Float16* h = reinterpret_cast<const _Float16*>(&raw);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) v[c] = (float)h[c];
-}
+constexpr int kOctBatch = 4;  // samples whose loads are issued together
 
 __device__ __forceinline__ float lerp_ab(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 
@@ -99,54 +87,93 @@ __global__ __launch_bounds__(64) void k_render_octree(snisr_octree O, const floa
         float corner[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) corner[a] = (float)cur[a] * bs - vs;
+        // samples are taken kOctBatch at a time: positions (zval advanced by repeated += step, as
+        // in the sequential march), then all their flag loads, index loads and voxel loads are
+        // issued together, then the samples are composited in order
         float zval = near;
-        for (int guard = 0; zval < far && guard < kOctMaxBlockSteps; ++guard) {
-          float rel[3];
-          int vl[3];
+        int guard = 0;
+        while (zval < far && guard < kOctMaxBlockSteps) {
+          float zs[kOctBatch], rel[kOctBatch][3];
+          int cell[kOctBatch];
+          bool live[kOctBatch];
+          float z = zval;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const float p = ol[a] + zval * d[a];
-            rel[a] = (p - corner[a]) * inv_vs;
-            vl[a] = min(max((int)rel[a], 0), M - 1);
-          }
-          if (!flags[vl[0] * MM + vl[1] * M + vl[2]]) {
-            zval += step;
-            continue;
-          }
-          if (!first_hit && T < trans_th) {
-            first_hit = true;
-            inverse_near[r] = 1.0f / (zval + bl);
-            net_idx[r] = (int16_t)group;
-          }
-          // trilinear over the dilated grid (x, then y, then z)
-          int i0[3];
-          float w[3];
+          for (int b = 0; b < kOctBatch; ++b) {
+            zs[b] = z;
+            live[b] = z < far && guard + b < kOctMaxBlockSteps;
+            int vl[3];
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const float f = rel[a] - 0.5f;
-            i0[a] = min(max((int)f, 0), M - 2);
-            w[a] = f - (float)(int)f;
-            w[a] = w[a] < 0.0f ? 0.0f : w[a];
+            for (int a = 0; a < 3; ++a) {
+              const float p = ol[a] + z * d[a];
+              rel[b][a] = (p - corner[a]) * inv_vs;
+              vl[a] = min(max((int)rel[b][a], 0), M - 1);
+            }
+            cell[b] = vl[0] * MM + vl[1] * M + vl[2];
+            z += step;
           }
-          float cv[8][4];
+          bool take[kOctBatch];
 #pragma unroll
-          for (int q = 0; q < 8; ++q)
-            octree_voxel(O, start, nodes, (i0[0] + (q & 1)) * MM + (i0[1] + ((q >> 1) & 1)) * M + i0[2] + (q >> 2), cv[q]);
-          float v[4];
+          for (int b = 0; b < kOctBatch; ++b) take[b] = live[b] && flags[cell[b]] != 0;
+          // trilinear corners over the dilated grid (x, then y, then z)
+          int base[kOctBatch];
+          float w[kOctBatch][3];
 #pragma unroll
-          for (int ch = 0; ch < 4; ++ch) {
-            const float s = lerp_ab(lerp_ab(cv[0][ch], cv[1][ch], w[0]), lerp_ab(cv[2][ch], cv[3][ch], w[0]), w[1]);
-            const float t = lerp_ab(lerp_ab(cv[4][ch], cv[5][ch], w[0]), lerp_ab(cv[6][ch], cv[7][ch], w[0]), w[1]);
-            v[ch] = lerp_ab(s, t, w[2]);
+          for (int b = 0; b < kOctBatch; ++b) {
+            int i0[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const float f = rel[b][a] - 0.5f;
+              i0[a] = min(max((int)f, 0), M - 2);
+              const float t = f - (float)(int)f;
+              w[b][a] = t < 0.0f ? 0.0f : t;
+            }
+            base[b] = i0[0] * MM + i0[1] * M + i0[2];
           }
-          const float interval = fminf(step, far - zval);
-          const float alpha = 1.0f - exp_cw(-1.0f * v[3] * interval);
-          const float wgt = alpha * T;
+          int16_t vid[kOctBatch][8];
 #pragma unroll
-          for (int ch = 0; ch < 3; ++ch) col[ch] = col[ch] + wgt * v[ch];
-          T = T * (1.0f - alpha);
-          ++ns;
-          zval += step;
+          for (int b = 0; b < kOctBatch; ++b)
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+              vid[b][q] = take[b] ? nodes[base[b] + (q & 1) * MM + ((q >> 1) & 1) * M + (q >> 2)] : (int16_t)-1;
+          uint2 raw[kOctBatch][8];
+#pragma unroll
+          for (int b = 0; b < kOctBatch; ++b)
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+              raw[b][q] = vid[b][q] >= 0 ? reinterpret_cast<const uint2*>(O.data_voxels)[start + vid[b][q]]
+                                         : make_uint2(0u, 0u);
+#pragma unroll
+          for (int b = 0; b < kOctBatch; ++b) {
+            if (!take[b]) continue;
+            if (!first_hit && T < trans_th) {
+              first_hit = true;
+              inverse_near[r] = 1.0f / (zs[b] + bl);
+              net_idx[r] = (int16_t)group;
+            }
+            float cv[8][4];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              const _Float16* hv = reinterpret_cast<const _Float16*>(&raw[b][q]);
+#pragma unroll
+              for (int c = 0; c < 4; ++c) cv[q][c] = (float)hv[c];
+            }
+            float v[4];
+#pragma unroll
+            for (int ch = 0; ch < 4; ++ch) {
+              const float s0 = lerp_ab(lerp_ab(cv[0][ch], cv[1][ch], w[b][0]), lerp_ab(cv[2][ch], cv[3][ch], w[b][0]), w[b][1]);
+              const float s1 = lerp_ab(lerp_ab(cv[4][ch], cv[5][ch], w[b][0]), lerp_ab(cv[6][ch], cv[7][ch], w[b][0]), w[b][1]);
+              v[ch] = lerp_ab(s0, s1, w[b][2]);
+            }
+            const float interval = fminf(step, far - zs[b]);
+            const float alpha = 1.0f - exp_cw(-1.0f * v[3] * interval);
+            const float wgt = alpha * T;
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) col[ch] = col[ch] + wgt * v[ch];
+            T = T * (1.0f - alpha);
+            ++ns;
+          }
+          zval = z;
+          guard += kOctBatch;
         }
       }
       if (tmax[0] < tmax[1]) {

@@ -25,6 +25,9 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"-I{INCLUDE}", f"-I{CSRC}"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# per-source flags: the reflection MLP keeps its MFMA accumulators in VGPRs, which removes one
+# accumulator-to-VGPR move per activation before the LeakyReLU / conversion
+EXTRA = {"inference_mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def sources():
@@ -33,7 +36,7 @@ def sources():
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + COMMON + DEVICE + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + COMMON + DEVICE + EXTRA.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC] + COMMON + ["-x", "hip"] + DEVICE + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -165,8 +165,9 @@ def assign_pixels(netIdxs: torch.Tensor, pixel_starts: torch.Tensor, block_start
     n = net.numel()
     if ps.numel() < n + 1 or bs.numel() < n + 1:
         raise ValueError("pixel_starts and block_starts need numNet + 1 entries")
-    N.check(N.lib().snisr_assign_pixels(_ptr(net), _ptr(ps), _ptr(bs), n, _ptr(new_netIdxs), _ptr(new_pixel_starts),
-                                        _stream()), "assign_pixels")
+    nb = min(new_netIdxs.numel(), new_pixel_starts.numel())   # the reference sizes them block_starts[-1]
+    N.check(N.lib().snisr_assign_pixels(_ptr(net), _ptr(ps), _ptr(bs), n, nb, _ptr(new_netIdxs),
+                                        _ptr(new_pixel_starts), _stream()), "assign_pixels")
 
 
 _packed_cache: dict = {}

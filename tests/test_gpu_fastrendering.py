@@ -90,6 +90,15 @@ def test_integrate_points_in_place(oracle_mod):
         oracle_mod.integrate_points(res, st, np.float32(0.01), T.reshape(-1), Cc.reshape(-1, 3))
         assert np.array_equal(bits(Tg.cpu().numpy()), bits(T)), chunk
         assert np.array_equal(bits(Cg.cpu().numpy()), bits(Cc)), chunk
+    # a single pixel (one partial 64-pixel block)
+    T1, C1 = np.ones((1, 1, 1), np.float32), np.zeros((1, 1, 3), np.float32)
+    res = rng.random((1, 1, 16, 4)).astype(np.float32)
+    st = rng.uniform(0.0, 0.2, (1, 1, 16)).astype(np.float32)
+    T1g, C1g = to_dev(T1), to_dev(C1)
+    FR.integrate_points(to_dev(res), to_dev(st), 512, 0.01, T1g, C1g)
+    oracle_mod.integrate_points(res, st, np.float32(0.01), T1.reshape(-1), C1.reshape(-1, 3))
+    assert np.array_equal(bits(T1g.cpu().numpy()), bits(T1))
+    assert np.array_equal(bits(C1g.cpu().numpy()), bits(C1))
 
 
 @pytest.mark.parametrize("n,nkeys", [(1, 1), (230400, 40), (230400, 1), (1000, 700), (1 << 20, 3000)])

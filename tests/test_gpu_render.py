@@ -54,7 +54,7 @@ def test_trace_tiles_bit_exact(oracle_mod):
     dirs[:100, 1:] = 0.0   # axis-parallel
     dirs[100:200, 0] = -0.0
     for origin in (np.array([3.1, 2.2, 1.3], np.float32), np.array([-4.0, 9.0, -2.0], np.float32)):
-        for K in (1, 4, 30):
+        for K in (1, 4, 30, 64, 130):   # K > 127: lanes write their rows directly (no LDS staging)
             got = D.trace_tiles(ds, to_dev(origin), to_dev(dirs), K).cpu().numpy()
             ref = oracle_mod.trace_tiles(osc, origin, dirs, K)
             assert np.array_equal(got, ref), (origin, K)
