@@ -126,6 +126,7 @@ struct Workspace {
   float4* crec;         // [cap] training: unweighted colour of every record
   float* dsig;          // [cap] training: dL/dsigma of every record
   float* act;           // training: activations/deltas per sorted record (train_act_floats each)
+  float* frag;          // training: per-tile FP32 MFMA weight fragments (forward and transposed)
   int64_t cap;
   int32_t n_tiles;
 };
@@ -140,7 +141,8 @@ static inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * 
 constexpr int kMaxBuckets = 8192;
 __host__ __device__ constexpr int32_t key_buckets(int32_t T) { return 8 * (T + 1) <= kMaxBuckets ? 8 * (T + 1) : T + 1; }
 
-static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspace* w, int64_t train_floats = 0) {
+static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspace* w, int64_t train_floats = 0,
+                      int64_t frag_floats = 0) {
   const int64_t nblocks = (n + kScanTile - 1) / kScanTile;
   int64_t off = 256;
   const int64_t o_counts = off; off = align_up(off + 4 * n, 256);
@@ -154,8 +156,9 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
   const int64_t o_rf = off; off = align_up(off + 16 * cap, 256);
   const int64_t o_ti = off; off = align_up(off + 8 * cap, 256);
   const int64_t o_rgb = off; off = align_up(off + 16 * cap, 256);
-  int64_t o_rb = 0, o_cr = 0, o_ds = 0, o_act = 0;
+  int64_t o_rb = 0, o_cr = 0, o_ds = 0, o_act = 0, o_fr = 0;
   if (train_floats > 0) {
+    o_fr = off; off = align_up(off + 4 * frag_floats, 256);
     o_rb = off; off = align_up(off + 16 * cap, 256);
     o_cr = off; off = align_up(off + 16 * cap, 256);
     o_ds = off; off = align_up(off + 4 * cap, 256);
@@ -179,6 +182,7 @@ static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspac
     w->crec = train_floats > 0 ? reinterpret_cast<float4*>(base + o_cr) : nullptr;
     w->dsig = train_floats > 0 ? reinterpret_cast<float*>(base + o_ds) : nullptr;
     w->act = train_floats > 0 ? reinterpret_cast<float*>(base + o_act) : nullptr;
+    w->frag = train_floats > 0 ? reinterpret_cast<float*>(base + o_fr) : nullptr;
     w->cap = cap;
     w->n_tiles = T;
   }
@@ -1057,11 +1061,11 @@ __global__ __launch_bounds__(256) void k_reduce(Workspace W, int64_t n, float* _
 
 // ------------------------------------------------------------------------- training (FP32)
 // Forward: same count/scan/emit (+ interval, T after, sigma per record) and tile sort as the render,
-// then a per-record FP32 MLP.  Backward: a per-ray reverse pass gives dL/dc and dL/dsigma of
-// every record; a per-record kernel recomputes the MLP forward (segment recomputation), writes
-// each layer's input and output delta rows, backpropagates to the features and scatter-adds the
-// trilinear adjoint into the feature and density gradients; a segmented reduction over each tile's
-// rows gives dW = sum delta (x) input and db = sum delta.
+// then the per-record FP32 MLP on FP32 MFMA (k_train_mma).  Backward: a per-ray reverse pass gives
+// dL/dc and dL/dsigma of every record; k_train_mma recomputes the MLP forward of every record
+// (segment recomputation), writes each layer's input rows and output-delta rows, backpropagates to
+// the features and scatter-adds the trilinear adjoint into the feature and density gradients; a
+// reduction over each tile's rows gives dW = sum delta (x) input and db = sum delta (k_train_dw).
 __host__ __device__ constexpr int in_pad_of(int in0, int wd, int l) { return l == 0 ? ((in0 + 3) & ~3) : wd; }
 __host__ __device__ constexpr int out_pad_of(int wd, int L, int l) { return l == L - 1 ? 4 : wd; }
 __host__ __device__ constexpr int64_t train_act_floats(int in0, int wd, int L) {
@@ -1089,147 +1093,6 @@ __host__ __device__ constexpr int64_t param_w_off(int in0, int wd, int L, int l)
   }
   return s;
 }
-// float offset of layer l's Wt block in the f32t layout
-__host__ __device__ constexpr int64_t f32t_w_off(int in0, int wd, int L, int l) {
-  int64_t s = 0;
-  for (int m = 0; m < l; ++m) {
-    const int i = m == 0 ? in0 : wd, o4 = m == L - 1 ? 4 : wd;
-    s += (int64_t)i * o4 + o4;
-  }
-  return s;
-}
-
-// delta of layer l's input: out[i] = sum_o Wt[i][o] dl[o] for i < IN_L, masked by a[i] > 0 when
-// `a` is given; written to the LDS column and, when given, to a global row
-template <int IN_L, int OUT4>
-__device__ __forceinline__ void backprop_layer(const float* __restrict__ Wt, const float (&dl)[OUT4],
-                                               const float* a, float* drow, float* xs) {
-#pragma unroll 1
-  for (int i = 0; i < IN_L; ++i) {
-    const float4* w = reinterpret_cast<const float4*>(Wt + i * OUT4);
-    float acc = 0.0f;
-#pragma unroll
-    for (int q = 0; q < OUT4 / 4; ++q) {
-      const float4 t = w[q];
-      acc = __builtin_fmaf(t.x, dl[4 * q], acc);
-      acc = __builtin_fmaf(t.y, dl[4 * q + 1], acc);
-      acc = __builtin_fmaf(t.z, dl[4 * q + 2], acc);
-      acc = __builtin_fmaf(t.w, dl[4 * q + 3], acc);
-    }
-    if (a) acc = a[i] > 0.0f ? acc : 0.0f;
-    if (drow) drow[i] = acc;
-    xs[i * 64] = acc;
-  }
-}
-
-template <int F, int WD, int L, bool HALF, bool BWD>
-__global__ __launch_bounds__(64) void k_train_mlp(snisr_scene S, snisr_rays R, Workspace W, float* __restrict__ dsigma,
-                                                  float* __restrict__ dfeats) {
-  constexpr int IN = F + 15;
-  constexpr int XD = IN > WD ? IN : WD;
-  constexpr int64_t WS = f32t_stride(IN, WD, L);
-  __shared__ float lds[XD * 64];
-  float* xs = lds + threadIdx.x;
-  const int T = W.n_tiles;
-  const int64_t NJ = W.cap + 64 * (int64_t)T;
-  const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
-  if (j >= W.tile_start[T]) return;
-  const int32_t k = W.perm[j];
-  if (k < 0) return;
-  const float4 rf = W.rec_f[k];
-  const int2 ti = W.rec_ti[k];
-  const int tile = ti.x;
-  Corners cn;
-  find_corners(S.brick_map + S.tile_brick_base[tile], S.tile_res[tile], v3(rf.x, rf.y, rf.z), cn);
-  gather_features<F, HALF>(S.feats, cn, xs);
-  const int64_t ray = ti.y;
-  const V3 d = normalize_dir(v3(R.dirs[3 * ray], R.dirs[3 * ray + 1], R.dirs[3 * ray + 2]));
-  float enc[15];
-  view_encoding(d, enc);
-#pragma unroll
-  for (int q = 0; q < 15; ++q) xs[(F + q) * 64] = enc[q];
-  const float* P = S.mlp_f32t + (int64_t)tile * WS;
-  if (!BWD) {
-    float rgb[3];
-    mlp_f32<IN, WD, L>(P, xs, rgb);
-    W.crec[k] = make_float4(rgb[0], rgb[1], rgb[2], 0.0f);
-    W.rgbw[k] = make_float4(rf.w * rgb[0], rf.w * rgb[1], rf.w * rgb[2], 0.0f);
-    return;
-  }
-  // ---- recompute the forward, storing every layer's input row
-  float* act = W.act;
-  constexpr int IP0 = in_pad_of(IN, WD, 0);
-  float* A0 = act + NJ * act_a_off(IN, WD, L, 0) + j * IP0;
-#pragma unroll 1
-  for (int i = 0; i < IP0; ++i) A0[i] = i < IN ? xs[i * 64] : 0.0f;
-  float y[WD];
-  dense_f32<IN, WD>(P, P + IN * WD, xs, y);
-  {
-    float* A1 = act + NJ * act_a_off(IN, WD, L, 1) + j * WD;
-#pragma unroll
-    for (int o = 0; o < WD; ++o) {
-      const float v = y[o] > 0.0f ? y[o] : 0.0f;
-      xs[o * 64] = v;
-      A1[o] = v;
-    }
-  }
-#pragma unroll 1
-  for (int l = 1; l < L - 1; ++l) {
-    const float* Pl = P + f32t_w_off(IN, WD, L, l);
-    dense_f32<WD, WD>(Pl, Pl + WD * WD, xs, y);
-    float* An = act + NJ * act_a_off(IN, WD, L, l + 1) + j * WD;
-#pragma unroll
-    for (int o = 0; o < WD; ++o) {
-      const float v = y[o] > 0.0f ? y[o] : 0.0f;
-      xs[o * 64] = v;
-      An[o] = v;
-    }
-  }
-  const float* PL = P + f32t_w_off(IN, WD, L, L - 1);
-  float z[4];
-  dense_f32<WD, 4>(PL, PL + WD * 4, xs, z);
-  // ---- output delta from dL/dc (written into rgbw by the compositing adjoint)
-  const float4 dc = W.rgbw[k];
-  float dl4[4];
-  {
-    const float c0 = sigmoid_cw(z[0]), c1 = sigmoid_cw(z[1]), c2 = sigmoid_cw(z[2]);
-    dl4[0] = dc.x * c0 * (1.0f - c0);
-    dl4[1] = dc.y * c1 * (1.0f - c1);
-    dl4[2] = dc.z * c2 * (1.0f - c2);
-    dl4[3] = 0.0f;
-    float* DL = act + NJ * act_d_off(IN, WD, L, L - 1) + j * 4;
-    DL[0] = dl4[0]; DL[1] = dl4[1]; DL[2] = dl4[2]; DL[3] = 0.0f;
-  }
-  // ---- backpropagate through the hidden layers
-  backprop_layer<WD, 4>(PL, dl4, act + NJ * act_a_off(IN, WD, L, L - 1) + j * WD,
-                        act + NJ * act_d_off(IN, WD, L, L - 2) + j * WD, xs);
-#pragma unroll 1
-  for (int l = L - 2; l >= 1; --l) {
-    float dl[WD];
-#pragma unroll
-    for (int o = 0; o < WD; ++o) dl[o] = xs[o * 64];
-    backprop_layer<WD, WD>(P + f32t_w_off(IN, WD, L, l), dl, act + NJ * act_a_off(IN, WD, L, l) + j * WD,
-                           act + NJ * act_d_off(IN, WD, L, l - 1) + j * WD, xs);
-  }
-  {
-    float dl[WD];
-#pragma unroll
-    for (int o = 0; o < WD; ++o) dl[o] = xs[o * 64];
-    backprop_layer<F, WD>(P, dl, nullptr, nullptr, xs);  // dL/d(features) only
-  }
-  // ---- trilinear adjoint
-  const float dsg = W.dsig[k];
-  const float wx[2] = {1.0f - cn.wx, cn.wx}, wy[2] = {1.0f - cn.wy, cn.wy}, wz[2] = {1.0f - cn.wz, cn.wz};
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int slot = cn.slot[q];
-    if (slot < 0) continue;
-    const float wq = wx[q & 1] * wy[(q >> 1) & 1] * wz[q >> 2];
-    if (dsg != 0.0f) atomicAdd(&dsigma[slot], wq * dsg);
-#pragma unroll 4
-    for (int e = 0; e < F; ++e) atomicAdd(&dfeats[(int64_t)slot * F + e], wq * xs[e * 64]);
-  }
-}
 
 // dL/dsigma and dL/dc of every record of ray i, in reverse sample order
 __global__ __launch_bounds__(256) void k_train_comp_bwd(Workspace W, int64_t n, const float* __restrict__ g) {
@@ -1252,57 +1115,428 @@ __global__ __launch_bounds__(256) void k_train_comp_bwd(Workspace W, int64_t n, 
   }
 }
 
-// dW_l[o][i] += sum_j D_l[j][o] A_l[j][i], db_l[o] += sum_j D_l[j][o] over one split of one tile's rows;
-// a 256-thread block owns a 32x32 output block (4 outputs per thread).
-template <int IN, int WD, int L>
-__global__ __launch_bounds__(256) void k_train_dw(Workspace W, int64_t mlp_stride, float* __restrict__ dmlp) {
-  __shared__ float Ds[32][33];
-  __shared__ float As[32][33];
-  const int T = W.n_tiles;
-  const int tile = blockIdx.z / L, l = blockIdx.z % L;
-  const int in_l = l == 0 ? IN : WD, out_l = l == L - 1 ? 3 : WD;
-  const int ip = l == 0 ? in_pad_of(IN, WD, 0) : WD, op = l == L - 1 ? 4 : WD;
-  const int nib = (in_l + 31) / 32, nob = (out_l + 31) / 32;
-  if ((int)blockIdx.y >= nib * nob) return;
-  const int ob = blockIdx.y / nib, ib = blockIdx.y % nib;
-  const int64_t NJ = W.cap + 64 * (int64_t)T;
-  const float* A = W.act + NJ * (l == 0 ? 0 : act_a_off(IN, WD, L, 1) + (int64_t)(l - 1) * WD);
-  const float* D = W.act + NJ * act_d_off(IN, WD, L, l);
-  const int64_t j0 = W.tile_start[tile], cnt = W.tile_count[tile];
-  const int64_t per = (cnt + gridDim.x - 1) / gridDim.x;
-  const int64_t jb = j0 + (int64_t)blockIdx.x * per;
-  const int64_t je = jb + per < j0 + cnt ? jb + per : j0 + cnt;
-  const int ty = threadIdx.x >> 3, tx = threadIdx.x & 7;
-  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, accb = 0.0f;
-  for (int64_t r0 = jb; r0 < je; r0 += 32) {
-    __syncthreads();
-    for (int q = threadIdx.x; q < 1024; q += 256) {
-      const int rr = q >> 5, cc = q & 31;
-      const int64_t jr = r0 + rr;
-      const int o = ob * 32 + cc, i = ib * 32 + cc;
-      Ds[rr][cc] = (jr < je && o < op) ? D[jr * op + o] : 0.0f;
-      As[rr][cc] = (jr < je && i < ip) ? A[jr * ip + i] : 0.0f;
+// dW_l[o][i] += sum_j D_l[j][o] A_l[j][i] and db_l[o] += sum_j D_l[j][o] over one split of one tile's
+// sorted records, on v_mfma_f32_32x32x2_f32 with the records as the K dimension.  A workgroup owns
+// every (out-block, in-block) pair of the layer (at most 16 pairs of 32 x 32; wave w takes pairs
+// w, w + 4, ...) and walks its records in tiles of 32 staged through LDS: the A operand of step s
+// is D rows 2s + h at columns ob*32 + c, the B operand A rows 2s + h at columns ib*32 + c.  Rows are
+// padded to 160 floats so that the two lane halves read disjoint banks.  The next tile's rows are
+// loaded into registers while the current tile's products run.
+constexpr int kDwRows = 32, kDwPitch = 160;
+template <int IP, int OP, int INL, int OUTL>  // padded row widths, real in / out sizes
+__device__ __forceinline__ void dw_layer(const float* __restrict__ A, const float* __restrict__ D, int64_t jb,
+                                         int64_t je, float* __restrict__ G, float* Ds, float* As) {
+  constexpr int IP4 = IP / 4, OP4 = OP / 4;
+  constexpr int ND = (kDwRows * OP4 + 255) / 256, NA = (kDwRows * IP4 + 255) / 256;
+  constexpr int NIB = (INL + 31) / 32, NOB = (OUTL + 31) / 32, NPAIR = NIB * NOB;
+  static_assert(NPAIR <= 16, "at most 4 pairs per wave");
+  const int lane = threadIdx.x & 63, c = lane & 31, h = lane >> 5, wv = threadIdx.x >> 6;
+  f32x16 acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = f32x16{};
+  float accb = 0.0f;
+  float4 rd[ND], ra[NA];
+  auto fetch = [&](int64_t r0) {
+#pragma unroll
+    for (int u = 0; u < ND; ++u) {
+      const int q = threadIdx.x + 256 * u, rr = q / OP4, cc = 4 * (q % OP4);
+      rd[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (q < kDwRows * OP4 && r0 + rr < je) rd[u] = *reinterpret_cast<const float4*>(D + (r0 + rr) * OP + cc);
+    }
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int q = threadIdx.x + 256 * u, rr = q / IP4, cc = 4 * (q % IP4);
+      ra[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (q < kDwRows * IP4 && r0 + rr < je) ra[u] = *reinterpret_cast<const float4*>(A + (r0 + rr) * IP + cc);
+    }
+  };
+  if (jb < je) fetch(jb);
+  for (int64_t r0 = jb; r0 < je; r0 += kDwRows) {
+    __syncthreads();  // the previous tile's reads of Ds / As are done
+#pragma unroll
+    for (int u = 0; u < ND; ++u) {
+      const int q = threadIdx.x + 256 * u;
+      if (q < kDwRows * OP4) *reinterpret_cast<float4*>(Ds + (q / OP4) * kDwPitch + 4 * (q % OP4)) = rd[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int q = threadIdx.x + 256 * u;
+      if (q < kDwRows * IP4) *reinterpret_cast<float4*>(As + (q / IP4) * kDwPitch + 4 * (q % IP4)) = ra[u];
     }
     __syncthreads();
+    if (r0 + kDwRows < je) fetch(r0 + kDwRows);
+    if (threadIdx.x < OP) {
 #pragma unroll 8
-    for (int rr = 0; rr < 32; ++rr) {
-      const float dv = Ds[rr][ty];
-      acc[0] = __builtin_fmaf(dv, As[rr][tx * 4 + 0], acc[0]);
-      acc[1] = __builtin_fmaf(dv, As[rr][tx * 4 + 1], acc[1]);
-      acc[2] = __builtin_fmaf(dv, As[rr][tx * 4 + 2], acc[2]);
-      acc[3] = __builtin_fmaf(dv, As[rr][tx * 4 + 3], acc[3]);
-      accb += dv;
+      for (int rr = 0; rr < kDwRows; ++rr) accb += Ds[rr * kDwPitch + threadIdx.x];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int p = wv + 4 * q;
+      if (p < NPAIR) {  // wave-uniform
+        const int ob = p / NIB, ib = p % NIB;
+        f32x16 a = acc[q];
+#pragma unroll
+        for (int st = 0; st < kDwRows / 2; ++st) {
+          const float x = Ds[(2 * st + h) * kDwPitch + ob * 32 + c];
+          const float y = As[(2 * st + h) * kDwPitch + ib * 32 + c];
+          a = __builtin_amdgcn_mfma_f32_32x32x2f32(x, y, a, 0, 0, 0);
+        }
+        acc[q] = a;
+      }
     }
   }
-  const int o = ob * 32 + ty;
-  if (o >= out_l) return;
-  float* G = dmlp + (int64_t)tile * mlp_stride + param_w_off(IN, WD, L, l);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int i = ib * 32 + tx * 4 + q;
-    if (i < in_l && acc[q] != 0.0f) atomicAdd(&G[(int64_t)o * in_l + i], acc[q]);
+    const int p = wv + 4 * q;
+    if (p >= NPAIR) break;
+    const int ob = p / NIB, ib = p % NIB;
+    const int i = ib * 32 + c;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int o = ob * 32 + 8 * (r / 4) + 4 * h + (r % 4);
+      if (o < OUTL && i < INL && acc[q][r] != 0.0f) atomicAdd(&G[(int64_t)o * INL + i], acc[q][r]);
+    }
   }
-  if (ib == 0 && tx == 0 && accb != 0.0f) atomicAdd(&G[(int64_t)out_l * in_l + o], accb);
+  if (threadIdx.x < OUTL && accb != 0.0f) atomicAdd(&G[(int64_t)OUTL * INL + threadIdx.x], accb);
+}
+
+template <int IN, int WD, int L>
+__global__ __launch_bounds__(256) void k_train_dw(Workspace W, int64_t mlp_stride, float* __restrict__ dmlp) {
+  static_assert(WD <= 128 && (IN + 3) / 4 * 4 <= 128, "rows fit the LDS pitch");
+  __shared__ __align__(16) float Ds[kDwRows * kDwPitch];
+  __shared__ __align__(16) float As[kDwRows * kDwPitch];
+  const int T = W.n_tiles;
+  const int tile = blockIdx.y / L, l = blockIdx.y % L;
+  const int64_t NJ = W.cap + 64 * (int64_t)T;
+  const float* A = W.act + NJ * act_a_off(IN, WD, L, l);
+  const float* D = W.act + NJ * act_d_off(IN, WD, L, l);
+  const int64_t j0 = W.tile_start[tile], cnt = W.tile_count[tile];
+  const int64_t per = ((cnt + gridDim.x - 1) / gridDim.x + kDwRows - 1) / kDwRows * kDwRows;
+  const int64_t jb = j0 + (int64_t)blockIdx.x * per;
+  const int64_t je = jb + per < j0 + cnt ? jb + per : j0 + cnt;
+  float* G = dmlp + (int64_t)tile * mlp_stride + param_w_off(IN, WD, L, l);
+  constexpr int IP0 = in_pad_of(IN, WD, 0);
+  if (l == 0) dw_layer<IP0, WD, IN, WD>(A, D, jb, je, G, Ds, As);
+  else if (l == L - 1) dw_layer<WD, 4, WD, 3>(A, D, jb, je, G, Ds, As);
+  else dw_layer<WD, WD, WD, WD>(A, D, jb, je, G, Ds, As);
+}
+
+// ---------------------------------------------------------------------------- FP32 MFMA training MLP
+// The per-record MLP of training as v_mfma_f32_32x32x2_f32 products: one wavefront takes 32 sorted
+// records (one tile: buckets are 64-aligned) as the 32 columns, so every layer is W (out x in) times
+// a 32-column block of activations.  Lane layouts (h = lane / 32, c = lane % 32):
+//   A:  lane holds A[row c][k = h]        B:  lane holds B[k = h][column c]
+//   D:  lane holds rows 8(r/4) + 4h + (r%4), r = 0..15, of column c
+// Step s of a product over an input block pairs row 8(r/4) + (r%4) (lane half 0) with row
+// 8(r/4) + 4 + (r%4) (lane half 1), r = s % 16: the B operand of step s is then register r of the
+// previous layer's D block s / 16, so activations and deltas never leave registers.  The weights are
+// packed per tile in that step order: forward fragments A[o][i] = W[o][i] and, for the backward,
+// transposed fragments A[i][o] = W[o][i]; a lane's values of 4 consecutive steps are contiguous
+// (float offset 256 (s / 4) + 4 lane + s % 4 inside a block), so one 16-byte load per lane feeds
+// four products and a wavefront reads 1 KB per load from global memory (L2-resident).  Layer 0's
+// inputs (features, then the 15-value view encoding) follow the same row order; its last block
+// only spans the steps that hold real inputs.
+__host__ __device__ constexpr int tr_ns0(int in0) { return 16 * (in0 / 32) + 4 * ((in0 % 32 + 7) / 8); }
+__host__ __device__ constexpr int tr_nob(int wd, int L, int l) { return l == L - 1 ? 1 : wd / 32; }
+__host__ __device__ constexpr int tr_nsf(int in0, int wd, int l) { return l == 0 ? tr_ns0(in0) : wd / 2; }
+__host__ __device__ constexpr int tr_nib(int wd, int l) { return l == 0 ? 1 : wd / 32; }
+__host__ __device__ constexpr int tr_nsb(int wd, int L, int l) { return l == L - 1 ? 4 : wd / 2; }
+// float offsets inside one tile's fragment block: forward layer l (weights, then 32 bias floats per
+// out-block in the D layout), then the transposed fragments of every layer
+__host__ __device__ constexpr int64_t tr_fwd_off(int in0, int wd, int L, int l) {
+  int64_t o = 0;
+  for (int m = 0; m < l; ++m) o += (int64_t)tr_nob(wd, L, m) * (tr_nsf(in0, wd, m) * 64 + 32);
+  return o;
+}
+__host__ __device__ constexpr int64_t tr_bias_off(int in0, int wd, int L, int l) {
+  return tr_fwd_off(in0, wd, L, l) + (int64_t)tr_nob(wd, L, l) * tr_nsf(in0, wd, l) * 64;
+}
+__host__ __device__ constexpr int64_t tr_bwd_off(int in0, int wd, int L, int l) {
+  int64_t o = tr_fwd_off(in0, wd, L, L);
+  for (int m = 0; m < l; ++m) o += (int64_t)tr_nib(wd, m) * tr_nsb(wd, L, m) * 64;
+  return o;
+}
+__host__ __device__ constexpr int64_t tr_stride(int in0, int wd, int L) { return (tr_bwd_off(in0, wd, L, L) + 63) / 64 * 64; }
+// input row held by lane half h at step s of an input block sequence
+__host__ __device__ constexpr int tr_row(int s, int h) { return 32 * (s / 16) + 8 * ((s % 16) / 4) + 4 * h + (s % 4); }
+
+// one thread per packed float of one tile (mlp_params layout: per layer W[out][in], then b[out])
+__global__ void k_pack_train_frag(const float* __restrict__ src, int64_t src_stride, float* __restrict__ dst,
+                                  int in0, int wd, int L, int n_tiles) {
+  const int64_t per = tr_stride(in0, wd, L);
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= per * n_tiles) return;
+  const int64_t tile = g / per;
+  const int64_t r = g - tile * per;
+  float v = 0.0f;
+  const float* P = src + tile * src_stride;
+  int64_t poff = 0;
+  for (int l = 0; l < L; ++l) {
+    const int in = l == 0 ? in0 : wd, out = l == L - 1 ? 3 : wd;
+    const float* Wl = P + poff;
+    const float* bl = Wl + (int64_t)out * in;
+    const int64_t fo = tr_fwd_off(in0, wd, L, l), bo = tr_bias_off(in0, wd, L, l);
+    const int nsf = tr_nsf(in0, wd, l), nob = tr_nob(wd, L, l);
+    if (r >= fo && r < bo) {  // forward fragment (ob, s, lane)
+      const int64_t q = r - fo, qq = q % (64 * nsf);
+      const int lane = (int)((qq % 256) / 4), st = (int)(4 * (qq / 256) + qq % 4), ob = (int)(q / (64 * nsf));
+      const int o = 32 * ob + lane % 32, i = tr_row(st, lane / 32);
+      if (o < out && i < in) v = Wl[(int64_t)o * in + i];
+    } else if (r >= bo && r < bo + 32 * nob) {  // bias in the D layout
+      const int q = (int)(r - bo), ob = q / 32, h = (q / 16) % 2, rr = q % 16;
+      const int o = 32 * ob + 8 * (rr / 4) + 4 * h + rr % 4;
+      if (o < out) v = bl[o];
+    }
+    const int64_t wo = tr_bwd_off(in0, wd, L, l), we = tr_bwd_off(in0, wd, L, l + 1);
+    if (r >= wo && r < we) {  // transposed fragment (ib, s, lane)
+      const int nsb = tr_nsb(wd, L, l);
+      const int64_t q = r - wo, qq = q % (64 * nsb);
+      const int lane = (int)((qq % 256) / 4), st = (int)(4 * (qq / 256) + qq % 4), ib = (int)(q / (64 * nsb));
+      const int i = 32 * ib + lane % 32, o = tr_row(st, lane / 32);
+      if (o < out && i < in) v = Wl[(int64_t)o * in + i];
+    }
+    poff += (int64_t)out * in + out;
+  }
+  dst[g] = v;
+}
+
+// acc += sum_s A[s] x[s] over NS steps (NS % 4 == 0); each lane loads 4 steps' A values at once,
+// PD such loads ahead of the products (a ring)
+template <int NS, int PD = 6>
+__device__ __forceinline__ f32x16 mma_f32_chain(const float* __restrict__ A, const float* x, f32x16 acc, int lane) {
+  static_assert(NS % 4 == 0, "steps come in groups of 4");
+  constexpr int N4 = NS / 4, D = PD < N4 ? PD : N4;
+  const float4* p = reinterpret_cast<const float4*>(A) + lane;
+  float4 ring[D];
+#pragma unroll
+  for (int q = 0; q < D; ++q) ring[q] = p[q * 64];
+#pragma unroll
+  for (int q = 0; q < N4; ++q) {
+    const float4 a = ring[q % D];
+    if (q + D < N4) ring[q % D] = p[(q + D) * 64];
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, x[4 * q], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, x[4 * q + 1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, x[4 * q + 2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, x[4 * q + 3], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ void store4(float* p, const float* v) {
+  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+template <int F, int WD, int L, bool HALF, bool BWD>
+__global__ __launch_bounds__(256, 2) void k_train_mma(snisr_scene S, snisr_rays R, Workspace W, float* __restrict__ dsigma,
+                                                   float* __restrict__ dfeats) {
+  constexpr int IN = F + 15, NS0 = tr_ns0(IN), NB = WD / 32, NSH = WD / 2;
+  static_assert(F % 8 == 0 && F <= 32, "feature rows must fill whole 8-row groups of the first block");
+  static_assert(WD % 32 == 0 && WD <= 128 && L >= 2 && L <= 8, "MLP shape (one 64-bit ReLU mask per layer)");
+  constexpr int64_t STRIDE = tr_stride(IN, WD, L);
+  constexpr int kTrLdsPerWave = 32 * (F + 1) + 2 * 32 * 8 + 32;
+  __shared__ float tr_lds[BWD ? 4 * kTrLdsPerWave : 1];
+  const int lane = threadIdx.x & 63, col = lane & 31, h = lane >> 5;
+  const int T = W.n_tiles;
+  const int64_t NJ = W.cap + 64 * (int64_t)T;
+  const int64_t j = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32 + col;
+  int32_t k = -1;
+  if (j < W.tile_start[T]) k = W.perm[j];
+  const bool valid = k >= 0;
+  const uint64_t vm = __ballot(valid);
+  if (vm == 0) return;  // a chunk of padding only (no barriers below)
+  float4 rf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  int2 ti = make_int2(0, 0);
+  if (valid) {
+    rf = W.rec_f[k];
+    ti = W.rec_ti[k];
+  }
+  const int tile = __shfl(ti.x, (int)__builtin_ctzll(vm));
+  Corners cn;
+  if (valid) {
+    find_corners(S.brick_map + S.tile_brick_base[tile], S.tile_res[tile], v3(rf.x, rf.y, rf.z), cn);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) cn.slot[q] = -1;
+    cn.wx = cn.wy = cn.wz = 0.0f;
+  }
+  // ---- layer-0 inputs: rows tr_row(s, h)
+  float enc[15];
+  {
+    V3 d = v3(1.0f, 0.0f, 0.0f);
+    if (valid) d = normalize_dir(v3(R.dirs[3 * (int64_t)ti.y], R.dirs[3 * (int64_t)ti.y + 1], R.dirs[3 * (int64_t)ti.y + 2]));
+    view_encoding(d, enc);
+  }
+  float x0[NS0];
+#pragma unroll
+  for (int g = 0; g < NS0 / 4; ++g) {
+    const int base = 32 * (g / 4) + 8 * (g % 4);  // first row of the 8-row group (lane half 0)
+    if (base < F) {
+      float v[8][4];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (cn.slot[q] >= 0) {
+          const int64_t off = (int64_t)cn.slot[q] * F + base + 4 * h;
+          if (HALF) {
+            const uint2 raw = *reinterpret_cast<const uint2*>(reinterpret_cast<const _Float16*>(S.feats) + off);
+            const _Float16* hv = reinterpret_cast<const _Float16*>(&raw);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[q][t] = (float)hv[t];
+          } else {
+            const float4 f4 = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(S.feats) + off);
+            v[q][0] = f4.x; v[q][1] = f4.y; v[q][2] = f4.z; v[q][3] = f4.w;
+          }
+        } else {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) v[q][t] = 0.0f;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float qv[8] = {v[0][t], v[1][t], v[2][t], v[3][t], v[4][t], v[5][t], v[6][t], v[7][t]};
+        x0[4 * g + t] = tri_combine(qv, cn.wx, cn.wy, cn.wz);
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int e0 = base - F + t, e1 = e0 + 4;  // view-encoding index for lane half 0 / 1
+        const float v0 = e0 < 15 ? enc[e0 < 15 ? e0 : 0] : 0.0f;
+        const float v1 = e1 < 15 ? enc[e1 < 15 ? e1 : 0] : 0.0f;
+        x0[4 * g + t] = h ? v1 : v0;
+      }
+    }
+  }
+  const float* P = W.frag + (int64_t)tile * STRIDE;
+  float* act = W.act;
+  if (BWD && valid) {
+    float* A0 = act + NJ * act_a_off(IN, WD, L, 0) + j * in_pad_of(IN, WD, 0);
+#pragma unroll
+    for (int g = 0; g < NS0 / 4; ++g) store4(A0 + 32 * (g / 4) + 8 * (g % 4) + 4 * h, x0 + 4 * g);
+  }
+  // ---- forward: hidden layers (ReLU), masks of every hidden input for the backward
+  float a[NB * 16];
+  uint64_t mask[L];
+#pragma unroll
+  for (int ob = 0; ob < NB; ++ob) {
+    f32x16 acc = bias_block(P + tr_bias_off(IN, WD, L, 0), ob, h);
+    acc = mma_f32_chain<NS0>(P + tr_fwd_off(IN, WD, L, 0) + ob * NS0 * 64, x0, acc, lane);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) a[ob * 16 + r] = acc[r] > 0.0f ? acc[r] : 0.0f;
+  }
+#pragma unroll
+  for (int l = 1; l < L; ++l) {
+    mask[l] = 0;
+#pragma unroll
+    for (int r = 0; r < NB * 16; ++r) mask[l] |= (uint64_t)(a[r] > 0.0f) << (r & 63);
+    if (BWD && valid) {
+      float* Al = act + NJ * act_a_off(IN, WD, L, l) + j * WD;
+#pragma unroll
+      for (int g = 0; g < NB * 4; ++g) store4(Al + 32 * (g / 4) + 8 * (g % 4) + 4 * h, a + 4 * g);
+    }
+    if (l == L - 1) break;
+    float b[NB * 16];
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+      f32x16 acc = bias_block(P + tr_bias_off(IN, WD, L, l), ob, h);
+      acc = mma_f32_chain<NSH>(P + tr_fwd_off(IN, WD, L, l) + ob * NSH * 64, a, acc, lane);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) b[ob * 16 + r] = acc[r] > 0.0f ? acc[r] : 0.0f;
+    }
+#pragma unroll
+    for (int r = 0; r < NB * 16; ++r) a[r] = b[r];
+  }
+  // ---- output layer: rows 0..2 (lane half 0, registers 0..2)
+  f32x16 z = bias_block(P + tr_bias_off(IN, WD, L, L - 1), 0, h);
+  z = mma_f32_chain<NSH>(P + tr_fwd_off(IN, WD, L, L - 1), a, z, lane);
+  if (!BWD) {
+    if (valid && h == 0) {
+      const float c0 = sigmoid_cw(z[0]), c1 = sigmoid_cw(z[1]), c2 = sigmoid_cw(z[2]);
+      W.crec[k] = make_float4(c0, c1, c2, 0.0f);
+      W.rgbw[k] = make_float4(rf.w * c0, rf.w * c1, rf.w * c2, 0.0f);
+    }
+    return;
+  }
+  // ---- output delta from dL/dc (written into rgbw by the compositing adjoint)
+  float dout[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (valid && h == 0) {
+    const float4 dc = W.rgbw[k];
+    const float c0 = sigmoid_cw(z[0]), c1 = sigmoid_cw(z[1]), c2 = sigmoid_cw(z[2]);
+    dout[0] = dc.x * c0 * (1.0f - c0);
+    dout[1] = dc.y * c1 * (1.0f - c1);
+    dout[2] = dc.z * c2 * (1.0f - c2);
+    store4(act + NJ * act_d_off(IN, WD, L, L - 1) + j * 4, dout);
+  }
+  // ---- backpropagate: d_{l-1} = mask_l (W_l^T d_l)
+  float d[NB * 16];
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib) {
+    f32x16 acc = {};
+    acc = mma_f32_chain<4>(P + tr_bwd_off(IN, WD, L, L - 1) + ib * 4 * 64, dout, acc, lane);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) d[ib * 16 + r] = ((mask[L - 1] >> ((ib * 16 + r) & 63)) & 1) ? acc[r] : 0.0f;
+  }
+#pragma unroll
+  for (int l = L - 2; l >= 0; --l) {
+    if (valid) {
+      float* Dl = act + NJ * act_d_off(IN, WD, L, l) + j * WD;
+#pragma unroll
+      for (int g = 0; g < NB * 4; ++g) store4(Dl + 32 * (g / 4) + 8 * (g % 4) + 4 * h, d + 4 * g);
+    }
+    if (l == 0) break;
+    float e[NB * 16];
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+      f32x16 acc = {};
+      acc = mma_f32_chain<NSH>(P + tr_bwd_off(IN, WD, L, l) + ib * NSH * 64, d, acc, lane);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) e[ib * 16 + r] = ((mask[l] >> ((ib * 16 + r) & 63)) & 1) ? acc[r] : 0.0f;
+    }
+#pragma unroll
+    for (int r = 0; r < NB * 16; ++r) d[r] = e[r];
+  }
+  // ---- dL/d(features): rows 0..F-1 of W_0^T d_0
+  f32x16 df = {};
+  df = mma_f32_chain<NSH>(P + tr_bwd_off(IN, WD, L, 0), d, df, lane);
+  // ---- trilinear adjoint.  The wave's 32 records go through LDS so that each atomic instruction
+  // covers whole feature rows (64 / F records x F contiguous features) instead of 64 scattered
+  // values; padding records have no corners.
+  float* dfl = tr_lds + (threadIdx.x >> 6) * kTrLdsPerWave;  // [32][F + 1] feature gradients
+  float* wl = dfl + 32 * (F + 1);                             // [32][8] corner weights
+  int* sl = reinterpret_cast<int*>(wl + 32 * 8);              // [32][8] corner slots
+  float* gl = wl + 2 * 32 * 8;                                // [32] dL/dsigma
+#pragma unroll
+  for (int g = 0; g < F / 8; ++g)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) dfl[col * (F + 1) + 8 * g + 4 * h + t] = df[4 * g + t];
+  if (h == 0) {
+    const float wx[2] = {1.0f - cn.wx, cn.wx}, wy[2] = {1.0f - cn.wy, cn.wy}, wz[2] = {1.0f - cn.wz, cn.wz};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      wl[col * 8 + q] = wx[q & 1] * wy[(q >> 1) & 1] * wz[q >> 2];
+      sl[col * 8 + q] = cn.slot[q];
+    }
+    gl[col] = valid ? W.dsig[k] : 0.0f;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  constexpr int RPI = 64 / F;  // records per atomic instruction
+  const int fr = lane % F, rr = lane / F;
+#pragma unroll 2
+  for (int q = 0; q < 8; ++q) {
+#pragma unroll
+    for (int m = 0; m < 32 / RPI; ++m) {
+      const int rec = m * RPI + rr;
+      const int slot = sl[rec * 8 + q];
+      if (slot >= 0) atomicAdd(&dfeats[(int64_t)slot * F + fr], wl[rec * 8 + q] * dfl[rec * (F + 1) + fr]);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int pr = m * 64 + lane, rec = pr >> 3, q = pr & 7;
+    const int slot = sl[rec * 8 + q];
+    const float gs = gl[rec];
+    if (slot >= 0 && gs != 0.0f) atomicAdd(&dsigma[slot], wl[rec * 8 + q] * gs);
+  }
 }
 
 __global__ void k_fill_zero(float* __restrict__ p, int64_t n) {
@@ -1453,6 +1687,16 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
 }
 
 
+// record splits per (tile, layer) of the weight-gradient reduction: about four workgroups per CU
+static int dw_splits(int64_t tile_layers) {
+  const int64_t s = 4 * (int64_t)device_cus() / (tile_layers > 0 ? tile_layers : 1);
+  return (int)(s < 1 ? 1 : (s > 64 ? 64 : s));
+}
+
+static int64_t train_frag_floats(const snisr_scene* s) {
+  return (int64_t)s->n_tiles * tr_stride(s->mlp_in, s->mlp_width, s->mlp_layers);
+}
+
 int64_t snisr_train_workspace_size(const snisr_scene* scene, int64_t n, int64_t sample_capacity) {
   if (!scene || n < 0 || sample_capacity < 0) {
     set_error("train_workspace_size: bad arguments");
@@ -1460,16 +1704,17 @@ int64_t snisr_train_workspace_size(const snisr_scene* scene, int64_t n, int64_t 
   }
   const int64_t cap = sample_capacity ? sample_capacity : default_capacity(n);
   return layout(n, cap, scene->n_tiles, nullptr, nullptr,
-                train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers));
+                train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers), train_frag_floats(scene));
 }
 
 static int train_capacity(const snisr_scene* scene, int64_t n, int64_t bytes, int64_t* cap_out) {
   const int64_t tf = train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers);
-  const int64_t fixed = layout(n, 0, scene->n_tiles, nullptr, nullptr, tf);
+  const int64_t ff = train_frag_floats(scene);
+  const int64_t fixed = layout(n, 0, scene->n_tiles, nullptr, nullptr, tf, ff);
   SNISR_REQUIRE(bytes >= fixed, SNISR_ERR_CAPACITY, "train: workspace too small");
   const int64_t per = 44 + 36 + 4 * tf;  // render record arrays + training arrays + activation rows
   int64_t cap = (bytes - fixed - 8 * 256) / per;
-  while (cap > 0 && layout(n, cap, scene->n_tiles, nullptr, nullptr, tf) > bytes) --cap;
+  while (cap > 0 && layout(n, cap, scene->n_tiles, nullptr, nullptr, tf, ff) > bytes) --cap;
   SNISR_REQUIRE(cap > 0, SNISR_ERR_CAPACITY, "train: workspace too small");
   *cap_out = cap;
   return SNISR_OK;
@@ -1486,8 +1731,7 @@ int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* para
   if (int e = check_scene(scene, true)) return e;
   if (int e = check_params(params)) return e;
   if (int e = check_rays(rays, params)) return e;
-  SNISR_REQUIRE(scene->mlp_f32t && aligned(scene->mlp_f32t, 16), SNISR_ERR_ARG,
-                "train_forward: scene.mlp_f32t (snisr_pack_mlp(SNISR_F32)) is required");
+  SNISR_REQUIRE(scene->mlp_params, SNISR_ERR_ARG, "train_forward: scene.mlp_params is NULL");
   SNISR_REQUIRE(workspace && aligned(workspace, 256), SNISR_ERR_ARG, "train_forward: workspace must be 256-byte aligned");
   SNISR_REQUIRE(scene->n_tiles <= kMaxTiles, SNISR_ERR_SHAPE, "train_forward: at most %d tiles", kMaxTiles);
   const int64_t n = rays->n;
@@ -1496,7 +1740,7 @@ int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* para
   if (int e = train_capacity(scene, n, workspace_bytes, &cap)) return e;
   Workspace W;
   layout(n, cap, scene->n_tiles, (uint8_t*)workspace, &W,
-         train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers));
+         train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers), train_frag_floats(scene));
   const int F = scene->feat_dim, WD = scene->mlp_width, L = scene->mlp_layers;
   const bool half = scene->feat_dtype == SNISR_F16;
   hipStream_t st = (hipStream_t)stream;
@@ -1530,10 +1774,16 @@ int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* para
   hipLaunchKernelGGL(k_tile_scatter, dim3(kSortGrid), dim3(kSortBlock), 0, st, W);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 1, st));
-  const dim3 jgrid((unsigned)((cap + 64 * (int64_t)scene->n_tiles + 63) / 64));
-#define SNISR_FWD(F_, W_, L_, H_)                                                                     \
-  if (F == F_ && WD == W_ && L == L_ && half == H_)                                                     \
-    hipLaunchKernelGGL((k_train_mlp<F_, W_, L_, H_, false>), jgrid, dim3(64), 0, st, *scene, *rays, W, \
+  {  // this step's weights as FP32 MFMA fragments (the backward reads the same copy)
+    const int64_t tot = train_frag_floats(scene);
+    hipLaunchKernelGGL(k_pack_train_frag, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, scene->mlp_params,
+                       scene->mlp_stride, W.frag, scene->mlp_in, WD, L, scene->n_tiles);
+    SNISR_LAUNCHED();
+  }
+  const dim3 jgrid((unsigned)((cap + 64 * (int64_t)scene->n_tiles + 127) / 128));
+#define SNISR_FWD(F_, W_, L_, H_)                                                                      \
+  if (F == F_ && WD == W_ && L == L_ && half == H_)                                                      \
+    hipLaunchKernelGGL((k_train_mma<F_, W_, L_, H_, false>), jgrid, dim3(256), 0, st, *scene, *rays, W, \
                        (float*)nullptr, (float*)nullptr);
   SNISR_TRAIN_SHAPES(SNISR_FWD)
 #undef SNISR_FWD
@@ -1577,19 +1827,19 @@ int snisr_train_backward(const snisr_scene* scene, const snisr_march_params* par
   if (int e = train_capacity(scene, n, workspace_bytes, &cap)) return e;
   Workspace W;
   layout(n, cap, scene->n_tiles, (uint8_t*)workspace, &W,
-         train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers));
+         train_act_floats(scene->mlp_in, scene->mlp_width, scene->mlp_layers), train_frag_floats(scene));
   hipLaunchKernelGGL(k_train_comp_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, W, n, grad_rgb);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(&ev, 1, st));
-  const dim3 jgrid((unsigned)((cap + 64 * (int64_t)scene->n_tiles + 63) / 64));
+  const dim3 jgrid((unsigned)((cap + 64 * (int64_t)scene->n_tiles + 127) / 128));
   bool ok = false;
-#define SNISR_BWD(F_, W_, L_, H_)                                                                           \
-  if (F == F_ && WD == W_ && L == L_ && half == H_) {                                                       \
-    hipLaunchKernelGGL((k_train_mlp<F_, W_, L_, H_, true>), jgrid, dim3(64), 0, st, *scene, *rays, W, dsigma, \
-                       dfeats);                                                                             \
+#define SNISR_BWD(F_, W_, L_, H_)                                                                            \
+  if (F == F_ && WD == W_ && L == L_ && half == H_) {                                                        \
+    hipLaunchKernelGGL((k_train_mma<F_, W_, L_, H_, true>), jgrid, dim3(256), 0, st, *scene, *rays, W, dsigma, \
+                       dfeats);                                                                              \
     SNISR_LAUNCHED();                                                                                       \
     SNISR_HIP(stage_event(&ev, 2, st));                                                                     \
-    const dim3 g(8, 16, (unsigned)(scene->n_tiles * L_));                                                   \
+    const dim3 g((unsigned)dw_splits(scene->n_tiles * L_), (unsigned)(scene->n_tiles * L_));               \
     hipLaunchKernelGGL((k_train_dw<F_ + 15, W_, L_>), g, dim3(256), 0, st, W, scene->mlp_stride, dmlp);     \
     SNISR_LAUNCHED();                                                                                       \
     ok = true;                                                                                              \
