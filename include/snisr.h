@@ -295,8 +295,9 @@ int snisr_inference_mlp(const snisr_mlp_query* query, void* stream);
  * (training/scripts/tile/tile.py:631-820): compositing of utils.volume_rendering
  * (training/scripts/utils.py:823-837), grid_sample trilinear gathers (tile.py:225-236) and the MLP
  * (training/scripts/networks.py:17-41).  The workspace must not be reused between a forward and its
- * backward.  train_forward's stage_events (4): start, after march+sort, after the MLP, after the
- * reduction. */
+ * backward; the forward packs the MLP weights of scene->mlp_params into it and the backward uses
+ * that copy, so the parameters may change only after the backward.  train_forward's stage_events
+ * (4): start, after march+sort, after the MLP, after the reduction. */
 int64_t snisr_train_workspace_size(const snisr_scene* scene, int64_t n, int64_t sample_capacity);
 int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                         const snisr_render_out* out, void* workspace, int64_t workspace_bytes, void* stream);

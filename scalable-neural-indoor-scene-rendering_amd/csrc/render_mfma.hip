@@ -1322,6 +1322,56 @@ __device__ __forceinline__ f32x16 mma_f32_chain(const float* __restrict__ A, con
   return acc;
 }
 
+// two independent chains over the same B operands (two output blocks), interleaved so that
+// consecutive products never depend on each other
+template <int NS, int PD = 4>
+__device__ __forceinline__ void mma_f32_chain2(const float* __restrict__ A0, const float* __restrict__ A1,
+                                               const float* x, f32x16& acc0, f32x16& acc1, int lane) {
+  static_assert(NS % 4 == 0, "steps come in groups of 4");
+  constexpr int N4 = NS / 4, D = PD < N4 ? PD : N4;
+  const float4* p0 = reinterpret_cast<const float4*>(A0) + lane;
+  const float4* p1 = reinterpret_cast<const float4*>(A1) + lane;
+  float4 r0[D], r1[D];
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    r0[q] = p0[q * 64];
+    r1[q] = p1[q * 64];
+  }
+#pragma unroll
+  for (int q = 0; q < N4; ++q) {
+    const float4 a0 = r0[q % D], a1 = r1[q % D];
+    if (q + D < N4) {
+      r0[q % D] = p0[(q + D) * 64];
+      r1[q % D] = p1[(q + D) * 64];
+    }
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, x[4 * q], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, x[4 * q], acc1, 0, 0, 0);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, x[4 * q + 1], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, x[4 * q + 1], acc1, 0, 0, 0);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, x[4 * q + 2], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, x[4 * q + 2], acc1, 0, 0, 0);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, x[4 * q + 3], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, x[4 * q + 3], acc1, 0, 0, 0);
+  }
+}
+
+// out[blk * 16 + r] = f(block blk of sum_s A_blk[s] x[s] + init_blk) for NBLK blocks, two at a time
+template <int NS, int NBLK, class Init, class Out>
+__device__ __forceinline__ void mma_blocks(const float* __restrict__ A, const float* x, int lane, Init init, Out out) {
+#pragma unroll
+  for (int b = 0; b + 1 < NBLK; b += 2) {
+    f32x16 a0 = init(b), a1 = init(b + 1);
+    mma_f32_chain2<NS>(A + b * NS * 64, A + (b + 1) * NS * 64, x, a0, a1, lane);
+    out(b, a0);
+    out(b + 1, a1);
+  }
+  if (NBLK % 2) {
+    f32x16 a0 = init(NBLK - 1);
+    a0 = mma_f32_chain<NS>(A + (NBLK - 1) * NS * 64, x, a0, lane);
+    out(NBLK - 1, a0);
+  }
+}
+
 __device__ __forceinline__ void store4(float* p, const float* v) {
   *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
 }
@@ -1415,13 +1465,12 @@ __global__ __launch_bounds__(256, 2) void k_train_mma(snisr_scene S, snisr_rays 
   // ---- forward: hidden layers (ReLU), masks of every hidden input for the backward
   float a[NB * 16];
   uint64_t mask[L];
+  mma_blocks<NS0, NB>(
+      P + tr_fwd_off(IN, WD, L, 0), x0, lane, [&](int ob) { return bias_block(P + tr_bias_off(IN, WD, L, 0), ob, h); },
+      [&](int ob, const f32x16& acc) {
 #pragma unroll
-  for (int ob = 0; ob < NB; ++ob) {
-    f32x16 acc = bias_block(P + tr_bias_off(IN, WD, L, 0), ob, h);
-    acc = mma_f32_chain<NS0>(P + tr_fwd_off(IN, WD, L, 0) + ob * NS0 * 64, x0, acc, lane);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) a[ob * 16 + r] = acc[r] > 0.0f ? acc[r] : 0.0f;
-  }
+        for (int r = 0; r < 16; ++r) a[ob * 16 + r] = acc[r] > 0.0f ? acc[r] : 0.0f;
+      });
 #pragma unroll
   for (int l = 1; l < L; ++l) {
     mask[l] = 0;
@@ -1434,13 +1483,12 @@ __global__ __launch_bounds__(256, 2) void k_train_mma(snisr_scene S, snisr_rays 
     }
     if (l == L - 1) break;
     float b[NB * 16];
+    mma_blocks<NSH, NB>(
+        P + tr_fwd_off(IN, WD, L, l), a, lane, [&](int ob) { return bias_block(P + tr_bias_off(IN, WD, L, l), ob, h); },
+        [&](int ob, const f32x16& acc) {
 #pragma unroll
-    for (int ob = 0; ob < NB; ++ob) {
-      f32x16 acc = bias_block(P + tr_bias_off(IN, WD, L, l), ob, h);
-      acc = mma_f32_chain<NSH>(P + tr_fwd_off(IN, WD, L, l) + ob * NSH * 64, a, acc, lane);
-#pragma unroll
-      for (int r = 0; r < 16; ++r) b[ob * 16 + r] = acc[r] > 0.0f ? acc[r] : 0.0f;
-    }
+          for (int r = 0; r < 16; ++r) b[ob * 16 + r] = acc[r] > 0.0f ? acc[r] : 0.0f;
+        });
 #pragma unroll
     for (int r = 0; r < NB * 16; ++r) a[r] = b[r];
   }
@@ -1467,13 +1515,12 @@ __global__ __launch_bounds__(256, 2) void k_train_mma(snisr_scene S, snisr_rays 
   }
   // ---- backpropagate: d_{l-1} = mask_l (W_l^T d_l)
   float d[NB * 16];
+  mma_blocks<4, NB>(
+      P + tr_bwd_off(IN, WD, L, L - 1), dout, lane, [&](int) { return f32x16{}; },
+      [&](int ib, const f32x16& acc) {
 #pragma unroll
-  for (int ib = 0; ib < NB; ++ib) {
-    f32x16 acc = {};
-    acc = mma_f32_chain<4>(P + tr_bwd_off(IN, WD, L, L - 1) + ib * 4 * 64, dout, acc, lane);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) d[ib * 16 + r] = ((mask[L - 1] >> ((ib * 16 + r) & 63)) & 1) ? acc[r] : 0.0f;
-  }
+        for (int r = 0; r < 16; ++r) d[ib * 16 + r] = ((mask[L - 1] >> ((ib * 16 + r) & 63)) & 1) ? acc[r] : 0.0f;
+      });
 #pragma unroll
   for (int l = L - 2; l >= 0; --l) {
     if (valid) {
@@ -1483,13 +1530,12 @@ __global__ __launch_bounds__(256, 2) void k_train_mma(snisr_scene S, snisr_rays 
     }
     if (l == 0) break;
     float e[NB * 16];
+    mma_blocks<NSH, NB>(
+        P + tr_bwd_off(IN, WD, L, l), d, lane, [&](int) { return f32x16{}; },
+        [&](int ib, const f32x16& acc) {
 #pragma unroll
-    for (int ib = 0; ib < NB; ++ib) {
-      f32x16 acc = {};
-      acc = mma_f32_chain<NSH>(P + tr_bwd_off(IN, WD, L, l) + ib * NSH * 64, d, acc, lane);
-#pragma unroll
-      for (int r = 0; r < 16; ++r) e[ib * 16 + r] = ((mask[l] >> ((ib * 16 + r) & 63)) & 1) ? acc[r] : 0.0f;
-    }
+          for (int r = 0; r < 16; ++r) e[ib * 16 + r] = ((mask[l] >> ((ib * 16 + r) & 63)) & 1) ? acc[r] : 0.0f;
+        });
 #pragma unroll
     for (int r = 0; r < NB * 16; ++r) d[r] = e[r];
   }
@@ -1753,8 +1799,10 @@ int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* para
 #undef SNISR_CHECK_SHAPE
   SNISR_REQUIRE(ok, SNISR_ERR_SHAPE, "train_forward: unsupported shape F=%d width=%d layers=%d", F, WD, L);
   SNISR_HIP(stage_event(out, 0, st));
+  // no ray schedule: sorting arbitrary training rays by (first tile, octant) made the march slower
+  const int32_t* order = nullptr;
   hipLaunchKernelGGL(k_march_pass<0>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, (int)SNISR_F16,
-                     (const int32_t*)nullptr);
+                     order);
   SNISR_LAUNCHED();
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.counts, n, W.offsets,
                      W.block_sums);
@@ -1765,7 +1813,7 @@ int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* para
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nblocks), dim3(kScanBlock), 0, st, W.offsets, n, W.block_sums);
   SNISR_LAUNCHED();
   hipLaunchKernelGGL(k_march_pass<2>, mgrid, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, (int)SNISR_F16,
-                     (const int32_t*)nullptr);
+                     order);
   SNISR_LAUNCHED();
   hipLaunchKernelGGL(k_tile_hist, dim3(kSortGrid), dim3(kSortBlock), 0, st, W);
   SNISR_LAUNCHED();
