@@ -288,7 +288,7 @@ __global__ __launch_bounds__(64) void k_march_pass(snisr_scene S, snisr_march_pa
     const int64_t base = W.offsets[i];
     int64_t k = base;
     bool over = false;
-    march_ray(S, P, origin, d, t_hit, vis, nullptr, 0,
+    march_ray_any(packet_w > 0 && !order, S, P, origin, d, t_hit, vis, nullptr, 0,
               [&](int tile, const V3& f, const Corners&, float w, float sig, float interval, float T_after) {
                 if (k < W.cap) {
                   W.rec_f[k] = make_float4(f.x, f.y, f.z, w);
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(64) void k_march_pass(snisr_scene S, snisr_march_pa
     if (over) *W.overflow = 1;
   } else {
     int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
-    march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, [&](int, const V3&, const Corners&, float, float, float, float) {}, res);
+    march_ray_any(packet_w > 0 && !order, S, P, origin, d, t_hit, vis, bricks, O.brick_cap, [&](int, const V3&, const Corners&, float, float, float, float) {}, res);
     if (bricks)
       for (int k = res.n_bricks; k < O.brick_cap; ++k) bricks[k] = -1;
     W.counts[i] = res.n_samples;
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
     int k = 0, run_tile = -1, run_start = 0;
     MarchResult res;
     int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
-    march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap,
+    march_ray_any(packet_w > 0 && !order, S, P, origin, d, t_hit, vis, bricks, O.brick_cap,
               [&](int tile, const V3& f, const Corners&, float w, float, float, float) {
                 if (k < cpr) {
                   if (tile != run_tile) {
