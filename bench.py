@@ -25,8 +25,14 @@ MFMA_PEAK_F16_TFLOPS = 2500.0   # MI355X dense FP16 MFMA
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_mlp_mfma launch on this workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (factors from a 256 MiB device copy), see profiles/c2_render_kernels.md
-MLP_HBM_BYTES_PER_LAUNCH = 211.0e6 + 129.5e6
+MLP_HBM_BYTES_PER_LAUNCH = 212.1e6 + 129.6e6
 
+_T0 = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """One line on stderr per finished leg (stdout carries only the result line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 def mlp_flops_per_sample(mlp_in, width, layers):
     from snisr.scene import mlp_layer_dims
@@ -428,21 +434,28 @@ def main():
         extra["mfma_vs_fp32_path_render_speedup"] = round(fp32_ms / render_ms, 2)
         extra["counts_equal_fp32_path"] = bool(torch.equal(o32["n_samples"], out["n_samples"]))
 
+    progress("C2 headline and FP32 path done")
     if args.train_steps > 0:
         extra.update(train_bench(args, dev))
+        progress("C3 training done")
     if args.ref_frames > 0:
         extra.update(ref_pipeline_bench(args, dev))
+        progress("reference pipeline done")
     if args.c4_steps > 0:
         extra.update(c4_bench(args, dev))
+        progress("C4 done")
     if args.c5_steps > 0:
         extra.update(c5_bench(args, dev))
+        progress("C5 done")
 
     cpu = None
     if not args.no_cpu:
         dirs_np = rays_d.reshape(-1, 3).cpu().numpy()
         th_np = t_hit.cpu().numpy()
         extra["psnr_vs_fp64_db"] = round(psnr_vs_fp64(sc, cam, dirs_np, th_np, rgb), 2)
+        progress("C2 PSNR vs FP64 done")
         cpu = cpu_baseline(sc, cam, dirs_np, th_np, args.cpu_frac)
+        progress("CPU baseline done")
 
     result = {
         "metric": METRIC,
