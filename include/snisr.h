@@ -101,8 +101,9 @@ typedef struct snisr_rays {
  * depth f32[N] (t of the first sample reached with T < trans_th, -1 if none);
  * bricks i32[N,brick_cap] occupied bricks marched in order, -1 padded (debug).
  * stage_events: optional array of n_stage_events hipEvent_t recorded on `stream` at the stage
- * boundaries (render_mfma: before the count march, after it, after the scan, after the emit march,
- * after the tile sort, after the MLP, after the reduction; render_fp32: before and after its kernel). */
+ * boundaries (render_mfma: start, after the ray schedule, after the march, after the tile histogram
+ * and scan, after the tile scatter, after the MLP, after the reduction; render_fp32: before and after
+ * its kernel). */
 typedef struct snisr_render_out {
   float* rgb;
   float* T;
@@ -304,6 +305,29 @@ int snisr_train_forward(const snisr_scene* scene, const snisr_march_params* para
 int snisr_train_backward(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                          const float* grad_rgb, float* dsigma, float* dfeats, float* dmlp, void* workspace,
                          int64_t workspace_bytes, void* const* stage_events, int32_t n_stage_events, void* stream);
+
+/* Training samplers of the reference's CUDA_EXT (training/src/Sample.cpp:97-158, kernels in
+ * training/src/Sample_kernel.cu): per ray of one tile (centre cx, cy, cz; edge tile_size), the sample
+ * depths of its training batch, written in place into z_vals f32[n, n_samples] (and dists for
+ * sparse); rays the reference skips keep their values.  node_flags i16[num_voxel^3] (1 = sampled
+ * voxel, index (x*nv + y)*nv + z); voxels f32[num_voxel^3][4] (r, g, b, sigma).
+ *  - sample_sparse      Sample_sparse (Sample_kernel.cu:181-434): length-proportional samples over
+ *                       runs of flagged voxels, 4 before the first run and 4 after the last;
+ *  - sample_reflection  Sample_reflection (16-179): inverse-depth samples from the exit of the voxel
+ *                       where transmittance reaches T_th to `far`;
+ *  - sample_uniform     Sample_uniform (436-479, 544-578): uniform over the tile's chord;
+ *  - sample_bg          Sample_bg (481-542, 580-616): uniform over sample_range around bg_zval[i],
+ *                       behind the tile's exit; tile_center is a device f32[3]. */
+int snisr_sample_sparse(const float* rays_o, const float* rays_d, int64_t n, const int16_t* node_flags, float cx,
+                        float cy, float cz, float tile_size, int32_t num_voxel, int32_t n_samples, float* z_vals,
+                        float* dists, void* stream);
+int snisr_sample_reflection(const float* rays_o, const float* rays_d, int64_t n, const int16_t* node_flags,
+                            const float* voxels, float cx, float cy, float cz, float tile_size, float far,
+                            int32_t num_voxel, float T_th, int32_t n_samples, float* z_vals, void* stream);
+int snisr_sample_uniform(const float* rays_o, const float* rays_d, int64_t n, float cx, float cy, float cz,
+                         float tile_size, int32_t n_samples, float* z_vals, void* stream);
+int snisr_sample_bg(const float* rays_o, const float* rays_d, const float* bg_zval, int64_t n, const float* tile_center,
+                    float tile_size, float sample_range, int32_t n_samples, float* z_vals, void* stream);
 
 /* Bytes of a packed weight buffer of the scene's MLP shape for `dtype`, and the packing itself
  * (device to device, from scene->mlp_params into `packed`): SNISR_F32 gives the mlp_f32t layout,

@@ -328,3 +328,38 @@ def render_octree(osc, origin, rays_d, visited, trans_th=0.8, early_stop=0.01, s
                                 C.c_int(K), C.c_float(trans_th), C.c_float(early_stop), C.c_int(1 if soft_hit else 0),
                                 _p(frame), _p(inv), _p(net), _p(ns))
     return frame, inv, net, ns
+
+
+# ---------------------------------------------------------------------------- training samplers
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def sample_sparse(rays_o, rays_d, node_flags, center, tile_size, num_voxel, z_vals, dists):
+    """In place on z_vals / dists f32[n, N] (rows of rays that miss or see no flagged voxel are left)."""
+    o, d = _f32(rays_o), _f32(rays_d)
+    fl = np.ascontiguousarray(node_flags, np.int16)
+    lib().orc_sample_sparse(_p(o), _p(d), C.c_longlong(o.shape[0]), _p(fl), C.c_float(center[0]),
+                            C.c_float(center[1]), C.c_float(center[2]), C.c_float(tile_size), C.c_int(num_voxel),
+                            C.c_int(z_vals.shape[1]), _p(z_vals), _p(dists))
+
+
+def sample_uniform(rays_o, rays_d, center, tile_size, z_vals):
+    o, d = _f32(rays_o), _f32(rays_d)
+    lib().orc_sample_uniform(_p(o), _p(d), C.c_longlong(o.shape[0]), C.c_float(center[0]), C.c_float(center[1]),
+                             C.c_float(center[2]), C.c_float(tile_size), C.c_int(z_vals.shape[1]), _p(z_vals))
+
+
+def sample_bg(rays_o, rays_d, bg_zval, tile_center, tile_size, sample_range, z_vals):
+    o, d, bg, tc = _f32(rays_o), _f32(rays_d), _f32(bg_zval), _f32(tile_center)
+    lib().orc_sample_bg(_p(o), _p(d), _p(bg), C.c_longlong(o.shape[0]), _p(tc), C.c_float(tile_size),
+                        C.c_float(sample_range), C.c_int(z_vals.shape[1]), _p(z_vals))
+
+
+def sample_reflection(rays_o, rays_d, node_flags, voxels, center, tile_size, far, num_voxel, T_th, z_vals):
+    o, d = _f32(rays_o), _f32(rays_d)
+    fl = np.ascontiguousarray(node_flags, np.int16)
+    vx = _f32(voxels)
+    lib().orc_sample_reflection(_p(o), _p(d), C.c_longlong(o.shape[0]), _p(fl), _p(vx), C.c_float(center[0]),
+                                C.c_float(center[1]), C.c_float(center[2]), C.c_float(tile_size), C.c_float(far),
+                                C.c_int(num_voxel), C.c_float(T_th), C.c_int(z_vals.shape[1]), _p(z_vals))

@@ -1125,3 +1125,263 @@ void orc_render_octree_f32(const int32_t* group_map, const uint16_t* data_voxels
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ training samplers
+// Restatement of training/src/Sample_kernel.cu (Sample_sparse 181-395, Sample_reflection 16-137,
+// Sample_uniform 436-479, Sample_bg 481-542) and the sample helpers of
+// training/src/include/cuda_utils.h (inverse_z_sample_bound 61-75, uniform_sample_bound 77-88,
+// uniform_sample_bound_v3 101-113), FP32, one ray at a time.  Operation order as written there: a
+// float3 divided by a scalar is a multiply by the FP32 reciprocal, make_int3 truncates.
+namespace orc {
+struct SmpTile {
+  float c[3], ts, hs, vs, mn[3];
+  int nv;
+};
+static inline SmpTile smp_tile(float cx, float cy, float cz, float tile_size, int num_voxel) {
+  SmpTile t;
+  t.c[0] = cx; t.c[1] = cy; t.c[2] = cz;
+  t.ts = tile_size;
+  t.hs = tile_size * 0.5f;
+  t.nv = num_voxel;
+  t.vs = tile_size / (float)num_voxel;
+  for (int a = 0; a < 3; ++a) t.mn[a] = t.c[a] - t.hs;
+  return t;
+}
+static inline void smp_aabb(const float* o, const float* d, const float* c, float h, float& lo, float& hi) {
+  ray_aabb(V3<float>{o[0], o[1], o[2]}, V3<float>{d[0], d[1], d[2]}, V3<float>{c[0], c[1], c[2]},
+           V3<float>{h, h, h}, lo, hi);
+}
+// voxel DDA state over the tile's num_voxel^3 grid from the entry point at t = near
+struct SmpDda {
+  int cur[3], stp[3];
+  float nb[3], tmax[3], tdel[3], ol[3];
+};
+static inline void smp_dda_init(const SmpTile& T, const float* o, const float* d, float near, SmpDda& D) {
+  const float inv = 1.0f / T.vs;
+  for (int a = 0; a < 3; ++a) {
+    D.ol[a] = clampr(o[a] + near * d[a] - T.mn[a], 0.0f, T.ts - 0.000001f);
+    D.cur[a] = std::min(std::max((int)(D.ol[a] * inv), 0), T.nv - 1);
+    D.stp[a] = d[a] >= 0.0f ? 1 : -1;
+    D.nb[a] = (float)(D.cur[a] + D.stp[a]) * T.vs;
+    if (D.stp[a] < 0) D.nb[a] += T.vs;
+    D.tmax[a] = safe_div(D.nb[a] - D.ol[a], d[a]);
+    D.tdel[a] = safe_div((float)D.stp[a] * T.vs, d[a]);
+  }
+}
+static inline bool smp_dda_step(const SmpTile& T, SmpDda& D) {
+  int a;
+  if (D.tmax[0] < D.tmax[1]) a = D.tmax[0] < D.tmax[2] ? 0 : 2;
+  else a = D.tmax[1] < D.tmax[2] ? 1 : 2;
+  D.cur[a] += D.stp[a];
+  D.tmax[a] += D.tdel[a];
+  return D.cur[0] >= 0 && D.cur[1] >= 0 && D.cur[2] >= 0 && D.cur[0] < T.nv && D.cur[1] < T.nv && D.cur[2] < T.nv;
+}
+static inline void smp_voxel_bound(const SmpTile& T, const float* o, const float* d, const int* cur, float& lo,
+                                   float& hi) {
+  float vc[3];
+  for (int a = 0; a < 3; ++a) vc[a] = T.mn[a] + (float)cur[a] * T.vs + 0.5f * T.vs;
+  smp_aabb(o, d, vc, 0.5f * T.vs, lo, hi);
+}
+// uniform_sample_bound_v3 writing at most `room` entries (the reference writes n; n <= room there
+// except for rows shorter than 8 samples)
+static inline void smp_uniform_v3(float* z, float* dist, float near, float far, int n, int room) {
+  const float interval = (far - near) / (float)n;
+  for (int i = 0; i < n && i < room; ++i) {
+    z[i] = near + (float)i * interval;
+    dist[i] = interval;
+  }
+}
+static inline void smp_uniform(float* z, float near, float far, int n) {
+  const float interval = (far - near) / (float)(n - 1);
+  for (int i = 0; i < n; ++i) z[i] = near + (float)i * interval;
+}
+// overlap_interpolation + trilinear_base (training/src/include/interpolation.h:43-134) over a dense
+// num_voxel^3 float4 grid; corner indices are clamped into the grid (the reference reads out of
+// range for points on the upper face)
+static inline void smp_overlap(const float* vox, const SmpTile& T, const float* p, float* out) {
+  const float inv = 1.0f / T.vs;
+  const int M = T.nv;
+  float f[3], w[3];
+  int i0[3], i1[3];
+  for (int a = 0; a < 3; ++a) {
+    const float corner = T.c[a] - T.ts / 2.0f;
+    f[a] = (p[a] - corner) * inv + 0.5f;
+    i0[a] = (int)f[a];
+    i1[a] = i0[a] + 1;
+    w[a] = f[a] - (float)i0[a];
+    if (w[a] < 0.0f) w[a] = 0.0f;
+  }
+  auto V = [&](int x, int y, int z, int c) {
+    x = std::min(std::max(x, 0), M - 1); y = std::min(std::max(y, 0), M - 1); z = std::min(std::max(z, 0), M - 1);
+    return vox[(((int64_t)x * M + y) * M + z) * 4 + c];
+  };
+  for (int c = 0; c < 4; ++c) {
+    const bool ex = i1[0] >= M, ey = i1[1] >= M, ez = i1[2] >= M;
+    const float v000 = V(i0[0], i0[1], i0[2], c);
+    float r;
+    if (ex) {
+      if (ey) r = ez ? v000 : lerp(v000, V(i0[0], i0[1], i1[2], c), w[2]);
+      else if (ez) r = lerp(v000, V(i0[0], i1[1], i0[2], c), w[1]);
+      else r = lerp(lerp(v000, V(i0[0], i1[1], i0[2], c), w[1]), lerp(V(i0[0], i0[1], i1[2], c), V(i0[0], i1[1], i1[2], c), w[1]), w[2]);
+    } else if (ey) {
+      if (ez) r = lerp(v000, V(i1[0], i0[1], i0[2], c), w[0]);
+      else r = lerp(lerp(v000, V(i1[0], i0[1], i0[2], c), w[0]), lerp(V(i0[0], i0[1], i1[2], c), V(i1[0], i0[1], i1[2], c), w[0]), w[2]);
+    } else if (ez) {
+      r = lerp(lerp(v000, V(i1[0], i0[1], i0[2], c), w[0]), lerp(V(i0[0], i1[1], i0[2], c), V(i1[0], i1[1], i0[2], c), w[0]), w[1]);
+    } else {
+      const float s = lerp(lerp(v000, V(i1[0], i0[1], i0[2], c), w[0]), lerp(V(i0[0], i1[1], i0[2], c), V(i1[0], i1[1], i0[2], c), w[0]), w[1]);
+      const float t = lerp(lerp(V(i0[0], i0[1], i1[2], c), V(i1[0], i0[1], i1[2], c), w[0]), lerp(V(i0[0], i1[1], i1[2], c), V(i1[0], i1[1], i1[2], c), w[0]), w[1]);
+      r = lerp(s, t, w[2]);
+    }
+    out[c] = r;
+  }
+}
+}  // namespace orc
+
+extern "C" {
+
+void orc_sample_sparse(const float* rays_o, const float* rays_d, long long n, const int16_t* flags, float cx, float cy,
+                       float cz, float tile_size, int num_voxel, int nsamp, float* z_vals, float* dists) {
+  using namespace orc;
+  const SmpTile T = smp_tile(cx, cy, cz, tile_size, num_voxel);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long r = 0; r < n; ++r) {
+    const float* o = rays_o + 3 * r;
+    const float* d = rays_d + 3 * r;
+    float* z = z_vals + r * nsamp;
+    float* ds = dists + r * nsamp;
+    float near, far;
+    smp_aabb(o, d, T.c, T.hs, near, far);
+    if (near == -1.0f && far == -1.0f) continue;
+    SmpDda D;
+    smp_dda_init(T, o, d, near, D);
+    const SmpDda D0 = D;
+    float total = 0.0f;
+    while (true) {
+      const int v = (D.cur[0] * T.nv + D.cur[1]) * T.nv + D.cur[2];
+      if (flags[v] == 1) {
+        float lo, hi;
+        smp_voxel_bound(T, o, d, D.cur, lo, hi);
+        total = total + (hi - lo);
+      }
+      if (!smp_dda_step(T, D)) break;
+    }
+    if (total == 0.0f) continue;
+    D = D0;
+    int count = 0;
+    bool acc = false, first = false;
+    const int bsmp = 4, ins = nsamp - 2 * bsmp;
+    while (true) {
+      const int v = (D.cur[0] * T.nv + D.cur[1]) * T.nv + D.cur[2];
+      if (flags[v] == 1) {
+        float lo, hi;
+        smp_voxel_bound(T, o, d, D.cur, lo, hi);
+        if (lo != -1.0f && hi != -1.0f) {
+          if (acc) far = hi;
+          else {
+            near = lo;
+            far = hi;
+            acc = true;
+            if (!first) {
+              first = true;
+              smp_uniform_v3(z + count, ds + count, near - T.vs, near - 0.000001f, bsmp, nsamp - count);
+              count = std::min(count + bsmp, nsamp);
+            }
+          }
+        }
+      } else if (acc) {
+        int k = (int)std::ceil((float)ins * (far - near) / total);
+        k = std::max(std::min(nsamp - count, k), 0);
+        smp_uniform_v3(z + count, ds + count, near, far, k, nsamp - count);
+        count = count + k;
+        acc = false;
+      }
+      if (!smp_dda_step(T, D)) break;
+    }
+    if (acc) {
+      int k = (int)std::ceil((float)ins * (far - near) / total);
+      k = std::max(std::min(nsamp - count, k), 0);
+      smp_uniform_v3(z + count, ds + count, near, far, k, nsamp - count);
+      count = count + k;
+    }
+    const int tail = std::min(nsamp - count, bsmp);
+    smp_uniform_v3(z + count, ds + count, far + 0.000001f, far + T.vs, tail, nsamp - count);
+  }
+}
+
+void orc_sample_uniform(const float* rays_o, const float* rays_d, long long n, float cx, float cy, float cz,
+                        float tile_size, int nsamp, float* z_vals) {
+  using namespace orc;
+  const float c[3] = {cx, cy, cz};
+  const float hs = tile_size / 2.0f;
+#pragma omp parallel for schedule(static)
+  for (long long r = 0; r < n; ++r) {
+    float near, far;
+    smp_aabb(rays_o + 3 * r, rays_d + 3 * r, c, hs, near, far);
+    smp_uniform(z_vals + r * nsamp, near, far, nsamp);
+  }
+}
+
+void orc_sample_bg(const float* rays_o, const float* rays_d, const float* bg_zval, long long n, const float* tile_center,
+                   float tile_size, float sample_range, int nsamp, float* z_vals) {
+  using namespace orc;
+#pragma omp parallel for schedule(static)
+  for (long long r = 0; r < n; ++r) {
+    const float bg = bg_zval[r];
+    if (bg == -1.0f) continue;
+    const float half_range = sample_range / 2.0f;
+    float lo, hi;
+    smp_aabb(rays_o + 3 * r, rays_d + 3 * r, tile_center, tile_size / 2.0f, lo, hi);
+    float near = bg - half_range, far = bg + half_range;
+    if (near <= hi) {
+      near = hi + 0.00001f;
+      far = near + sample_range;
+    }
+    smp_uniform(z_vals + r * nsamp, near, far, nsamp);
+  }
+}
+
+void orc_sample_reflection(const float* rays_o, const float* rays_d, long long n, const int16_t* flags,
+                           const float* voxels, float cx, float cy, float cz, float tile_size, float far, int num_voxel,
+                           float T_th, int nsamp, float* z_vals) {
+  using namespace orc;
+  const SmpTile T = smp_tile(cx, cy, cz, tile_size, num_voxel);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long r = 0; r < n; ++r) {
+    const float* o = rays_o + 3 * r;
+    const float* d = rays_d + 3 * r;
+    float bl, bh;
+    smp_aabb(o, d, T.c, T.hs, bl, bh);
+    if (bl == -1.0f && bh == -1.0f) continue;
+    SmpDda D;
+    smp_dda_init(T, o, d, bl, D);
+    float trans = 1.0f, near = -1.0f;
+    while (true) {
+      const int v = (D.cur[0] * T.nv + D.cur[1]) * T.nv + D.cur[2];
+      if (flags[v] == 1) {
+        float lo, hi;
+        smp_voxel_bound(T, o, d, D.cur, lo, hi);
+        if (lo != -1.0f && hi != -1.0f) {
+          float p[3], rgba[4];
+          for (int a = 0; a < 3; ++a) p[a] = o[a] + lo * d[a];
+          smp_overlap(voxels, T, p, rgba);
+          const float dist = hi - lo;
+          const float alpha = 1.0f - orc_expf(-1.0f * rgba[3] * dist);
+          trans = trans * (1.0f - alpha);
+          if (trans <= T_th) {
+            near = hi;
+            break;
+          }
+        }
+      }
+      if (!smp_dda_step(T, D)) break;
+    }
+    if (near == -1.0f) continue;
+    float* z = z_vals + r * nsamp;
+    const float inv_near = 1.0f / near, inv_far = 1.0f / far, inv_bound = inv_far - inv_near;
+    const float step = 1.0f / (float)(nsamp - 1);
+    for (int i = 0; i < nsamp; ++i) z[i] = 1.0f / (step * (float)i * inv_bound + inv_near);
+  }
+}
+
+}  // extern "C"

@@ -94,6 +94,16 @@ SIGNATURES = [
     ("snisr_sort_by_key_workspace_size", C.c_int64, [C.c_int64]),
     ("snisr_sort_by_key", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                     C.POINTER(C.c_int32), C.c_void_p]),
+    ("snisr_sample_sparse", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_float, C.c_float,
+                                      C.c_float, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    ("snisr_sample_reflection", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_float,
+                                          C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32, C.c_float,
+                                          C.c_int32, C.c_void_p, C.c_void_p]),
+    ("snisr_sample_uniform", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_float,
+                                       C.c_float, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("snisr_sample_bg", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_float, C.c_float,
+                                  C.c_int32, C.c_void_p, C.c_void_p]),
     ("snisr_assign_pixels", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
     ("snisr_render_octree", C.c_int, [C.POINTER(OctreeDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,

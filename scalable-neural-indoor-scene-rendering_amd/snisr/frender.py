@@ -58,6 +58,16 @@ class FRender:
         self.inverse_far = 1.0 / self.far
         self.frame = None
 
+    @classmethod
+    def from_render_data(cls, path, sample_step_scale: float = 0.2, **kw) -> "FRender":
+        """FRender of a renderData.npz scene (the reference's render_prepare,
+        rendering/src/__init__.py:98-135)."""
+        from .render_data import load_render_data
+        rd = load_render_data(path, sample_step_scale)
+        fr = cls(rd.scene, rd.net_params, rd.group_centers, **kw)
+        fr.group_names = rd.group_names
+        return fr
+
     def set_camera(self, cam: Camera):
         self.intrinsic = torch.from_numpy(cam.intrinsic).to(self.device)
         self.c2w = torch.from_numpy(cam.c2w).to(self.device)

@@ -58,3 +58,22 @@ def _grid_scene(osc):
                       occupancy=0.0)
     sc.tile_index_map = osc.tile_index_map.copy()
     return sc
+
+
+def test_frender_from_render_data_file(tmp_path):
+    """A scene written to renderData.npz and read back renders the identical frame."""
+    from snisr import render_data as RD
+    osc = S.make_octree_scene(10, tile_shape=(4, 4, 1), block_occupancy=0.3, n_groups=4)
+    params, centers = S.make_reflection_groups(20, 4, (0, 0, 0), (4, 4, 1))
+    H, W = 64, 96
+    cam = S.room_camera(10, (0, 0, 0), (4, 4, 1), H, W, focal=70.0)
+    a = frender.FRender(osc, params, centers, H=H, W=W)
+    a.set_camera(cam)
+    fa = a.render().cpu()
+    f = tmp_path / "renderData.npz"
+    RD.save_render_data(f, osc, params, centers)
+    b = frender.FRender.from_render_data(f, H=H, W=W)
+    b.set_camera(cam)
+    fb = b.render().cpu()
+    assert torch.equal(fa, fb)
+    assert len(b.group_names) == 4
