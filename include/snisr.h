@@ -252,6 +252,12 @@ int64_t snisr_sort_by_key_workspace_size(int64_t n);
 int snisr_sort_by_key(int16_t* keys, int32_t* values, int32_t* starts, int64_t n, void* workspace,
                       int64_t workspace_bytes, int32_t* n_unique, void* stream);
 
+/* compute_group_boundary (rendering/src/compute_group_boundary.cu:53-75, include/compute_group_boundary.h):
+ * boundary u8[H*W] (bool) is set to 1 where a pixel of the ksize x ksize window around the pixel
+ * (clipped to the image) has another group id than the pixel; set flags stay set.  net_idx i16[H*W]. */
+int snisr_compute_group_boundary(const int16_t* net_idx, int32_t H, int32_t W, int32_t ksize, uint8_t* boundary,
+                                 void* stream);
+
 /* A8 — assign_pixels (rendering/src/reorder.cu:69-94, include/reorder.h:10-15).  n_blocks =
  * block_starts[n_net], the number of output chunks (the reference's caller sizes the outputs with it). */
 int snisr_assign_pixels(const int16_t* net, const int32_t* pixel_starts, const int32_t* block_starts, int64_t n_net,

@@ -363,3 +363,12 @@ def sample_reflection(rays_o, rays_d, node_flags, voxels, center, tile_size, far
     lib().orc_sample_reflection(_p(o), _p(d), C.c_longlong(o.shape[0]), _p(fl), _p(vx), C.c_float(center[0]),
                                 C.c_float(center[1]), C.c_float(center[2]), C.c_float(tile_size), C.c_float(far),
                                 C.c_int(num_voxel), C.c_float(T_th), C.c_int(z_vals.shape[1]), _p(z_vals))
+
+
+def group_boundary(net_idx, ksize, boundary=None):
+    """bool[H, W] (in place on `boundary` when given)."""
+    net = np.ascontiguousarray(net_idx, np.int16)
+    H, W = net.shape
+    out = np.zeros((H, W), np.uint8) if boundary is None else boundary
+    lib().orc_group_boundary(_p(net), C.c_int(H), C.c_int(W), C.c_int(ksize), _p(out))
+    return out

@@ -1385,3 +1385,20 @@ void orc_sample_reflection(const float* rays_o, const float* rays_d, long long n
 }
 
 }  // extern "C"
+
+extern "C" {
+// compute_group_boundary_kernel (rendering/src/compute_group_boundary.cu:4-50), one pixel at a time
+void orc_group_boundary(const int16_t* net, int H, int W, int ksize, uint8_t* boundary) {
+  const int r = ksize / 2;
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)H * W; ++i) {
+    const int py = (int)(i / W), px = (int)(i - (long long)py * W);
+    const int16_t c = net[i];
+    const int x0 = std::max(px - r, 0), x1 = std::min(px + r, W - 1);
+    const int y0 = std::max(py - r, 0), y1 = std::min(py + r, H - 1);
+    for (int y = y0; y <= y1 && !boundary[i]; ++y)
+      for (int x = x0; x <= x1; ++x)
+        if (net[(long long)y * W + x] != c) { boundary[i] = 1; break; }
+  }
+}
+}  // extern "C"

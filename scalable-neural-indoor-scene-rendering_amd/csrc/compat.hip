@@ -124,6 +124,37 @@ __global__ __launch_bounds__(256) void k_assign_pixels(const int16_t* __restrict
   new_starts[b] = ps + r * base + (r < add ? r : add);
 }
 
+// compute_group_boundary_kernel (rendering/src/compute_group_boundary.cu:4-50): a pixel is on a
+// group boundary when any pixel of its ksize x ksize window (clipped to the image) has another group
+// id; the flag is only ever set (an already-set flag stays set).  A 16 x 16 block of pixels stages
+// its window rows of group ids in LDS once.
+constexpr int kGbTile = 16, kGbMaxRadius = 15;
+__global__ __launch_bounds__(kGbTile * kGbTile) void k_group_boundary(const int16_t* __restrict__ net, int H, int W,
+                                                                      int radius, uint8_t* __restrict__ boundary) {
+  __shared__ int16_t tile[kGbTile + 2 * kGbMaxRadius][kGbTile + 2 * kGbMaxRadius];
+  const int x0 = blockIdx.x * kGbTile - radius, y0 = blockIdx.y * kGbTile - radius;
+  const int span = kGbTile + 2 * radius;
+  for (int q = threadIdx.x; q < span * span; q += blockDim.x) {
+    const int ty = q / span, tx = q - ty * span;
+    const int gy = min(max(y0 + ty, 0), H - 1), gx = min(max(x0 + tx, 0), W - 1);
+    tile[ty][tx] = net[(int64_t)gy * W + gx];
+  }
+  __syncthreads();
+  const int lx = threadIdx.x % kGbTile, ly = threadIdx.x / kGbTile;
+  const int px = blockIdx.x * kGbTile + lx, py = blockIdx.y * kGbTile + ly;
+  if (px >= W || py >= H) return;
+  const int64_t i = (int64_t)py * W + px;
+  if (boundary[i]) return;
+  const int16_t c = tile[ly + radius][lx + radius];
+  // clipped window: rows / columns outside the image are not part of it (clamped copies of the
+  // edge would compare equal to the edge pixel, which is itself in the window)
+  bool diff = false;
+  for (int dy = -radius; dy <= radius && !diff; ++dy)
+    for (int dx = -radius; dx <= radius; ++dx)
+      if (tile[ly + radius + dy][lx + radius + dx] != c) { diff = true; break; }
+  if (diff) boundary[i] = 1;
+}
+
 }  // namespace snisr
 
 using namespace snisr;
@@ -218,6 +249,20 @@ int snisr_sort_by_key(int16_t* keys, int32_t* values, int32_t* starts, int64_t n
   SNISR_LAUNCHED();
   SNISR_HIP(hipMemcpyAsync(n_unique, cnt, 4, hipMemcpyDeviceToHost, st));
   SNISR_HIP(hipStreamSynchronize(st));
+  return SNISR_OK;
+}
+
+int snisr_compute_group_boundary(const int16_t* net_idx, int32_t H, int32_t W, int32_t ksize, uint8_t* boundary,
+                                 void* stream) {
+  SNISR_REQUIRE(H >= 0 && W >= 0 && ksize >= 0, SNISR_ERR_ARG, "compute_group_boundary: negative size");
+  if ((int64_t)H * W == 0) return SNISR_OK;
+  SNISR_REQUIRE(net_idx && boundary, SNISR_ERR_ARG, "compute_group_boundary: NULL pointer");
+  SNISR_REQUIRE(ksize / 2 <= kGbMaxRadius, SNISR_ERR_SHAPE, "compute_group_boundary: ksize above %d",
+                2 * kGbMaxRadius + 1);
+  const dim3 grid((unsigned)((W + kGbTile - 1) / kGbTile), (unsigned)((H + kGbTile - 1) / kGbTile));
+  hipLaunchKernelGGL(k_group_boundary, grid, dim3(kGbTile * kGbTile), 0, (hipStream_t)stream, net_idx, H, W,
+                     ksize / 2, boundary);
+  SNISR_LAUNCHED();
   return SNISR_OK;
 }
 

@@ -104,6 +104,8 @@ SIGNATURES = [
                                        C.c_float, C.c_int32, C.c_void_p, C.c_void_p]),
     ("snisr_sample_bg", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_float, C.c_float,
                                   C.c_int32, C.c_void_p, C.c_void_p]),
+    ("snisr_compute_group_boundary", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                               C.c_void_p]),
     ("snisr_assign_pixels", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
     ("snisr_render_octree", C.c_int, [C.POINTER(OctreeDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,

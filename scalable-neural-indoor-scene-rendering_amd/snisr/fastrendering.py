@@ -153,6 +153,18 @@ def sort_by_key(keys_tensor: torch.Tensor, values_tensor: torch.Tensor, starts_t
     return m.value
 
 
+def compute_group_boundary(netIdx: torch.Tensor, boundary: torch.Tensor, ksize: int, num_thread: int) -> None:
+    """rendering/src/compute_group_boundary.cu:53-75: boundary (bool [H, W, 1] or [H, W]) is set where
+    the ksize x ksize window around a pixel holds another group id of netIdx (int16 [H, W])."""
+    _require_cuda(netIdx, "netIdx", torch.int16)
+    _require_cuda(boundary, "boundary", torch.bool)
+    if netIdx.dim() != 2 or boundary.numel() != netIdx.numel():
+        raise ValueError("netIdx must be [H, W] and boundary hold H*W flags")
+    H, W = netIdx.shape
+    N.check(N.lib().snisr_compute_group_boundary(_ptr(netIdx), H, W, int(ksize), _ptr(boundary), _stream()),
+            "compute_group_boundary")
+
+
 def assign_pixels(netIdxs: torch.Tensor, pixel_starts: torch.Tensor, block_starts: torch.Tensor, num_thread: int,
                   new_netIdxs: torch.Tensor, new_pixel_starts: torch.Tensor) -> None:
     """rendering/src/reorder.cu:69-94: splits each group's pixel range into

@@ -1,9 +1,8 @@
 """Frame driver in the form of the reference's ``FRender.render`` (rendering/src/__init__.py:247-341),
 calling the FASTRENDERING stand-in operators: rays, tile tracing, the octree diffuse pass, pixel
-regrouping by group, four chunks of 16 inverse-depth reflection samples through the per-group MLP
-with compositing, bilinear upsampling of the reflection layer, and diffuse + reflection clamped to
-[0, 1].  The screen-space group boundary (compute_group_boundary) and the optional U-Net refinement
-are not part of this path and are left out.
+regrouping by group, the screen-space group boundary, four chunks of 16 inverse-depth reflection
+samples through the per-group MLP with compositing, bilinear upsampling of the reflection layer,
+diffuse + reflection clamped to [0, 1], and optionally the U-Net refinement (load_cnn).
 """
 from __future__ import annotations
 
@@ -56,7 +55,20 @@ class FRender:
         self.group_centers = t(group_centers)
         self.far = float(np.linalg.norm(osc.tile_size * np.asarray(osc.tile_shape, np.float64)))
         self.inverse_far = 1.0 / self.far
+        self.ksize_boundary = 3
+        self.cnn = None
         self.frame = None
+
+    def load_cnn(self, state_dict=None, half: bool = True):
+        """Enables the screen-space refinement network (the reference's CNN / Unet,
+        rendering/src/__init__.py:79-90); ``state_dict`` as saved by the reference (None keeps the
+        random initialisation)."""
+        from .cnn import CNN
+        net = CNN()
+        if state_dict is not None:
+            net.load_state_dict(state_dict)
+        net = net.to(self.device).eval()
+        self.cnn = net.half() if half else net
 
     @classmethod
     def from_render_data(cls, path, sample_step_scale: float = 0.2, **kw) -> "FRender":
@@ -91,6 +103,8 @@ class FRender:
                                          self.sample_step, self.block_num_voxel, self.num_thread, self.trans_decay,
                                          self.early_stop_D, False, self.frame_diffuse, inverse_near, netIdxs)
         del visited
+        self.large_inverse_near = inverse_near.clone()
+        self.large_netIdx = netIdxs.clone()
         inverse_near = inverse_near[::s, ::s].contiguous()
         netIdxs = netIdxs[::s, ::s].flatten()
         npx = self.downscale_numPixel
@@ -108,6 +122,8 @@ class FRender:
         new_starts = torch.empty(n_blocks, device=dev, dtype=torch.int32)
         FR.assign_pixels(netIdxs, pixel_starts, block_starts, self.num_thread, new_net, new_starts)
         new_starts = torch.cat([new_starts, tail])
+        self.boundary = torch.zeros((H, W, 1), dtype=torch.bool, device=dev)
+        FR.compute_group_boundary(self.large_netIdx, self.boundary, self.ksize_boundary, self.num_thread * 2)
         sample_far = 1.0 / inverse_near + self.far
         inverse_bound = (1.0 / sample_far - inverse_near) / 63
         transparency = torch.ones((self.DH, self.DW, 1), device=dev, dtype=torch.float32)
@@ -126,5 +142,8 @@ class FRender:
         self.frame_speculars = F.interpolate(spec[None].permute(0, 3, 1, 2), scale_factor=s, mode="bilinear",
                                              align_corners=True)[0].permute(1, 2, 0)
         self.frame = torch.clamp(self.frame_diffuse + self.frame_speculars, 0, 1)
+        if self.cnn is not None:
+            from .cnn import refine
+            self.frame = refine(self.cnn, self.frame, self.large_inverse_near, self.boundary)
         self.n_reflection_blocks = n_blocks
         return self.frame

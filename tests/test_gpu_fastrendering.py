@@ -154,3 +154,17 @@ def test_reorder_pipeline_matches_oracle(oracle_mod):
     edges = np.concatenate([new_ps.cpu().numpy(), [npx]])
     sizes = np.diff(edges)
     assert sizes.min() >= 1 and sizes.max() <= 64 and edges[0] == 0
+
+
+@pytest.mark.parametrize("H,W,ksize", [(720, 1280, 3), (45, 77, 5), (17, 33, 9), (8, 8, 1)])
+def test_compute_group_boundary_bit_exact(oracle_mod, H, W, ksize):
+    rng = np.random.default_rng(H + W)
+    blocks = rng.integers(-1, 7, ((H + 5) // 6, (W + 6) // 7))
+    net = np.repeat(np.repeat(blocks, 6, 0), 7, 1)[:H, :W].astype(np.int16)
+    pre = np.zeros((H, W, 1), bool)
+    pre[H // 2, W // 3] = True
+    b = to_dev(pre)
+    FR.compute_group_boundary(to_dev(net), b, ksize, 512)
+    ref = oracle_mod.group_boundary(net, ksize, pre.reshape(H, W).astype(np.uint8).copy()).astype(bool)
+    assert np.array_equal(b.cpu().numpy().reshape(H, W), ref)
+    assert ref.any() and not ref.all()

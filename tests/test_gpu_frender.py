@@ -77,3 +77,28 @@ def test_frender_from_render_data_file(tmp_path):
     fb = b.render().cpu()
     assert torch.equal(fa, fb)
     assert len(b.group_names) == 4
+
+
+def test_frender_cnn_refinement_matches_fp32_module():
+    """With the refinement network enabled (FP16 on the device), the frame equals the FP32 module
+    applied on the CPU to the same 5-channel input within FP16 tolerance."""
+    from snisr import cnn as CN
+    osc = S.make_octree_scene(11, tile_shape=(4, 4, 1), block_occupancy=0.3, n_groups=4)
+    params, centers = S.make_reflection_groups(21, 4, (0, 0, 0), (4, 4, 1))
+    H, W = 64, 96
+    cam = S.room_camera(11, (0, 0, 0), (4, 4, 1), H, W, focal=70.0)
+    fr = frender.FRender(osc, params, centers, H=H, W=W)
+    fr.set_camera(cam)
+    base = fr.render().clone()
+    torch.manual_seed(0)
+    ref_net = CN.CNN().eval()
+    fr.load_cnn(ref_net.state_dict())
+    out = fr.render()
+    x = torch.cat([base.permute(2, 0, 1), fr.large_inverse_near.reshape(1, H, W),
+                   fr.boundary.reshape(1, H, W).float()], 0)[None].cpu()
+    with torch.no_grad():
+        ref = ref_net(x)[0].permute(1, 2, 0).clamp(0, 1)
+    err = (out.cpu() - ref).abs()
+    assert err.max() < 2e-2 and err.mean() < 2e-3, (err.max(), err.mean())
+    net = fr.large_netIdx
+    assert bool(fr.boundary.any()) == bool((net != net.flatten()[0]).any())   # flags iff the groups vary

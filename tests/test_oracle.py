@@ -407,3 +407,21 @@ def test_assign_pixels_partitions_groups(oracle_mod):
         sizes = ends[sel] - new_starts[sel]
         assert new_starts[sel][0] == ps[g] and sizes.sum() == counts[g]
         assert sizes.max() - sizes.min() <= 1 and sizes.max() <= 64
+
+
+@pytest.mark.parametrize("ksize", [1, 3, 5, 8])
+def test_group_boundary_matches_window_extrema(oracle_mod, ksize):
+    """compute_group_boundary (rendering/src/compute_group_boundary.cu:4-50): a pixel is on a boundary
+    iff its clipped window is not constant, i.e. the window maximum differs from the minimum
+    (scipy's maximum/minimum filters with edge replication give the clipped-window extrema)."""
+    from scipy import ndimage
+    rng = np.random.default_rng(ksize)
+    net = np.repeat(np.repeat(rng.integers(-1, 5, (9, 13)), 4, 0), 5, 1)[:33, :61].astype(np.int16)
+    got = oracle_mod.group_boundary(net, ksize).astype(bool)
+    size = 2 * (ksize // 2) + 1
+    ref = ndimage.maximum_filter(net, size=size, mode="nearest") != ndimage.minimum_filter(net, size=size, mode="nearest")
+    assert np.array_equal(got, ref)
+    pre = np.zeros(net.shape, np.uint8)
+    pre[5, 7] = 1                                    # flags that are already set stay set
+    out = oracle_mod.group_boundary(net, ksize, pre).astype(bool)
+    assert out[5, 7] and np.array_equal(out | ref, out)
