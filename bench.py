@@ -25,7 +25,7 @@ MFMA_PEAK_F16_TFLOPS = 2500.0   # MI355X dense FP16 MFMA
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_mlp_mfma launch on this workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (factors from a 256 MiB device copy), see profiles/c2_render_kernels.md
-MLP_HBM_BYTES_PER_LAUNCH = 212.1e6 + 129.6e6
+MLP_HBM_BYTES_PER_LAUNCH = 212.3e6 + 129.6e6
 
 _T0 = time.perf_counter()
 
@@ -141,11 +141,24 @@ def ref_pipeline_bench(args, dev):
         fr.render()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.ref_frames
+    # the same frame with the U-Net refinement enabled (FP16, random weights of the reference's shape)
+    torch.manual_seed(0)
+    fr.load_cnn()
+    for _ in range(2):
+        fr.render()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.ref_frames):
+        fr.render()
+    torch.cuda.synchronize()
+    ms_cnn = (time.perf_counter() - t0) * 1e3 / args.ref_frames
     return {"ref_pipeline_ms_per_frame": round(ms, 3), "ref_pipeline_mrays_s": round(720 * 1280 / ms / 1e3, 2),
+            "ref_pipeline_with_unet_ms_per_frame": round(ms_cnn, 3),
             "ref_pipeline_reflection_blocks": int(fr.n_reflection_blocks),
             "ref_pipeline_config": "FRender.render form at 1280x720: compute_rays, tracing_tiles (K=30), octree "
-                                   "diffuse pass, sort_by_key/assign_pixels, 4 x (inverse_z_sampling, inference_mlp "
-                                   "FP16 MFMA, integrate_points) at 640x360, bilinear upsample; seeds 12/13/2"}
+                                   "diffuse pass, sort_by_key/assign_pixels, group boundary, 4 x (inverse_z_sampling, "
+                                   "inference_mlp FP16 MFMA, integrate_points) at 640x360, bilinear upsample; "
+                                   "with_unet adds the 5-channel U-Net refinement in FP16; seeds 12/13/2"}
 
 
 def c4_bench(args, dev):
