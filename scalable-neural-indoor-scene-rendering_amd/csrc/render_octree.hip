@@ -28,7 +28,7 @@ constexpr int kOctBlocks = 4;  // blocks per tile edge (num_block, raydiffuse_re
 // The cap bounds the loop for degenerate rays (zero direction: far = 1e8) and for steps below
 // zval's FP32 resolution, where zval += step would never reach far.
 constexpr int kOctMaxBlockSteps = 1 << 16;
-constexpr int kOctBatch = 4;  // samples whose loads are issued together
+constexpr int kOctBatch = 2;  // samples whose loads are issued together (2: 417 us, 4: 456 us, 8: 576 us per 720p frame)
 
 __device__ __forceinline__ float lerp_ab(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 
