@@ -113,4 +113,42 @@ __device__ __forceinline__ void relu_cvt<Mfma<SNISR_F16>>(const f32x16& acc, f16
   hi = __builtin_elementwise_max(Mfma<SNISR_F16>::cvt8(v + 8), z);
 }
 
+// One layer over two 32-column blocks: d{0,1}[2ob], d{0,1}[2ob + 1] = ReLU(W(ob) s{0,1} + b(ob))
+// for the NOB 32-row output blocks.  Software-pipelined over the output blocks: the ReLU and
+// conversion of block ob - 1 are issued between the first MFMAs of block ob (vector work under
+// the running matrix pipe instead of after it), and the bias of block ob + 1 is read from LDS
+// before the last MFMA pair of block ob.  A fragments come from LDS one step ahead.
+template <class M, int NK, int NOB>
+__device__ __forceinline__ void mma_layer(const uint8_t* A, const float* bias, int lane, int h,
+                                          const typename M::vec (&s0)[NK], const typename M::vec (&s1)[NK],
+                                          typename M::vec (&d0)[2 * NOB], typename M::vec (&d1)[2 * NOB]) {
+  using vec = typename M::vec;
+  static_assert(NK >= 2, "two k-steps at least");
+  constexpr int K0 = 1, K1 = NK > 2 ? 2 : 1;  // k-steps that carry the pending activations
+  f32x16 p0, p1;
+  f32x16 bn = bias_block(bias, 0, h);
+#pragma unroll
+  for (int ob = 0; ob < NOB; ++ob) {
+    const uint8_t* Ab = A + ob * NK * 1024 + lane * 16;
+    vec ring[2];
+    ring[0] = load_frag<vec>(Ab);
+    f32x16 acc0 = bn, acc1 = bn;
+#pragma unroll
+    for (int kc = 0; kc < NK; ++kc) {
+      if (kc + 1 < NK) ring[(kc + 1) & 1] = load_frag<vec>(Ab + (kc + 1) * 1024);
+      if (kc == NK - 1 && ob + 1 < NOB) bn = bias_block(bias, ob + 1, h);
+      const vec a = ring[kc & 1];
+      acc0 = M::mma(a, s0[kc], acc0);
+      acc1 = M::mma(a, s1[kc], acc1);
+      if (ob > 0 && kc == K0) relu_cvt<M>(p0, d0[2 * ob - 2], d0[2 * ob - 1]);
+      if (ob > 0 && kc == K1) relu_cvt<M>(p1, d1[2 * ob - 2], d1[2 * ob - 1]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    p0 = acc0;
+    p1 = acc1;
+  }
+  relu_cvt<M>(p0, d0[2 * NOB - 2], d0[2 * NOB - 1]);
+  relu_cvt<M>(p1, d1[2 * NOB - 2], d1[2 * NOB - 1]);
+}
+
 }  // namespace snisr

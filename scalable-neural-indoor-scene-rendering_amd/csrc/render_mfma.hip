@@ -405,17 +405,12 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_add(int64_t* __restrict__ o
 
 // 8 consecutive features of one corner voxel as floats
 template <bool HALF>
-__device__ __forceinline__ void load8(const void* __restrict__ feats, int64_t off, float* v) {
+__device__ __forceinline__ void load8(const void* __restrict__ feats, int64_t off, uint32_t m, float* v) {
   if (HALF) {
-    const uint4 q = *reinterpret_cast<const uint4*>(reinterpret_cast<const _Float16*>(feats) + off);
-    const _Float16* h = reinterpret_cast<const _Float16*>(&q);
-#pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = (float)h[e];
+    FeatLoad<true>::load(feats, off, m, v);
   } else {
-    const float4* p = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(feats) + off);
-    const float4 a = p[0], b = p[1];
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    FeatLoad<false>::load(feats, off, m, v);
+    FeatLoad<false>::load(feats, off + 4, m, v + 4);
   }
 }
 
@@ -905,11 +900,13 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
           ray[cbk] = k >= 0 ? k / W.cpr : 0;
           wgt[cbk] = k >= 0 ? rr[cbk].w : 0.0f;
           find_corners(S.brick_map + tile_base, tile_res, v3(rr[cbk].x, rr[cbk].y, rr[cbk].z), cn[cbk]);
-          corner_weights_h2(cn[cbk], w2[cbk]);
-          if (k < 0) {
+        }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) w2[cbk][q] = h2{0, 0};
-          }
+        for (int cbk = 0; cbk < 2; ++cbk) {
+          corner_weights_h2(cn[cbk], w2[cbk]);
+          const _Float16 keep = rk[cbk] >= 0 ? (_Float16)1.0f : (_Float16)0.0f;  // padding slots: zero weights
+#pragma unroll
+          for (int q = 0; q < 8; ++q) w2[cbk][q] = w2[cbk][q] * h2{keep, keep};
         }
 #pragma unroll
         for (int kc = 0; kc < NFC; ++kc)
@@ -946,17 +943,15 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
             for (int kc = 0; kc < NFC; ++kc)
               xin[cbk][kc] = interp8_h2<M>(reinterpret_cast<const _Float16*>(S.feats), cn, w2, F, 16 * kc + 8 * h);
           }
+          uint32_t cmask[8];  // ahead of the loads: the masking statements bound scheduling regions
+#pragma unroll
+          for (int q = 0; q < 8; ++q) cmask[q] = corner_mask(cn.slot[q] >= 0);
 #pragma unroll
           for (int kc = 0; kc < (HALF ? 0 : NFC); ++kc) {
             float v[8][8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-              if (cn.slot[q] >= 0) load8<HALF>(S.feats, (int64_t)cn.slot[q] * F + 16 * kc + 8 * h, v[q]);
-              else {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) v[q][e] = 0.0f;
-              }
-            }
+            for (int q = 0; q < 8; ++q)  // unconditional loads, empty corners masked to zero
+              load8<HALF>(S.feats, (int64_t)(cn.slot[q] >= 0 ? cn.slot[q] : 0) * F + 16 * kc + 8 * h, cmask[q], v[q]);
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
               const float qv[8] = {v[0][e], v[1][e], v[2][e], v[3][e], v[4][e], v[5][e], v[6][e], v[7][e]};
@@ -988,28 +983,16 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
       vec xa[2][NKH], xb[2][NKH];
       {
         const uint8_t* Lp = lds_w + layer_offset(IN, WD, L, 0);
-        const float* bias = reinterpret_cast<const float*>(Lp + NOB * NKC0 * 1024);
-#pragma unroll
-        for (int ob = 0; ob < NOB; ++ob) {
-          f32x16 acc0 = bias_block(bias, ob, h), acc1 = acc0;
-          mma_block<M, NKC0>(Lp + ob * NKC0 * 1024, lane, xin[0], xin[1], acc0, acc1);
-          relu_cvt<M>(acc0, xa[0][2 * ob], xa[0][2 * ob + 1]);
-          relu_cvt<M>(acc1, xa[1][2 * ob], xa[1][2 * ob + 1]);
-        }
+        mma_layer<M, NKC0, NOB>(Lp, reinterpret_cast<const float*>(Lp + NOB * NKC0 * 1024), lane, h, xin[0], xin[1],
+                                xa[0], xa[1]);
       }
 #pragma unroll
       for (int l = 1; l < L - 1; ++l) {
         auto& src = (l & 1) ? xa : xb;
         auto& dst = (l & 1) ? xb : xa;
         const uint8_t* Lp = lds_w + layer_offset(IN, WD, L, l);
-        const float* bias = reinterpret_cast<const float*>(Lp + NOB * NKH * 1024);
-#pragma unroll
-        for (int ob = 0; ob < NOB; ++ob) {
-          f32x16 acc0 = bias_block(bias, ob, h), acc1 = acc0;
-          mma_block<M, NKH>(Lp + ob * NKH * 1024, lane, src[0], src[1], acc0, acc1);
-          relu_cvt<M>(acc0, dst[0][2 * ob], dst[0][2 * ob + 1]);
-          relu_cvt<M>(acc1, dst[1][2 * ob], dst[1][2 * ob + 1]);
-        }
+        mma_layer<M, NKH, NOB>(Lp, reinterpret_cast<const float*>(Lp + NOB * NKH * 1024), lane, h, src[0], src[1],
+                               dst[0], dst[1]);
       }
       {
         auto& src = ((L - 2) & 1) ? xb : xa;  // output of layer L-2
@@ -1409,6 +1392,9 @@ __global__ __launch_bounds__(256, 2) void k_train_mma(snisr_scene S, snisr_rays 
     for (int q = 0; q < 8; ++q) cn.slot[q] = -1;
     cn.wx = cn.wy = cn.wz = 0.0f;
   }
+  uint32_t cmask[8];  // computed ahead of the loads: the masking statements bound scheduling regions
+#pragma unroll
+  for (int q = 0; q < 8; ++q) cmask[q] = corner_mask(cn.slot[q] >= 0);
   // ---- layer-0 inputs: rows tr_row(s, h)
   float enc[15];
   {
@@ -1423,21 +1409,20 @@ __global__ __launch_bounds__(256, 2) void k_train_mma(snisr_scene S, snisr_rays 
     if (base < F) {
       float v[8][4];
 #pragma unroll
+      // unconditional loads (empty corners read slot 0, which exists whenever a lane of the
+      // chunk is valid, and are zeroed after), so the eight loads are in flight together
       for (int q = 0; q < 8; ++q) {
-        if (cn.slot[q] >= 0) {
-          const int64_t off = (int64_t)cn.slot[q] * F + base + 4 * h;
-          if (HALF) {
-            const uint2 raw = *reinterpret_cast<const uint2*>(reinterpret_cast<const _Float16*>(S.feats) + off);
-            const _Float16* hv = reinterpret_cast<const _Float16*>(&raw);
+        const int64_t off = (int64_t)(cn.slot[q] >= 0 ? cn.slot[q] : 0) * F + base + 4 * h;
+        const uint32_t m = cmask[q];
+        if (HALF) {
+          uint2 raw = *reinterpret_cast<const uint2*>(reinterpret_cast<const _Float16*>(S.feats) + off);
+          raw.x &= m;
+          raw.y &= m;
+          const _Float16* hv = reinterpret_cast<const _Float16*>(&raw);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) v[q][t] = (float)hv[t];
-          } else {
-            const float4 f4 = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(S.feats) + off);
-            v[q][0] = f4.x; v[q][1] = f4.y; v[q][2] = f4.z; v[q][3] = f4.w;
-          }
+          for (int t = 0; t < 4; ++t) v[q][t] = (float)hv[t];
         } else {
-#pragma unroll
-          for (int t = 0; t < 4; ++t) v[q][t] = 0.0f;
+          FeatLoad<false>::load(S.feats, off, m, v[q]);
         }
       }
 #pragma unroll

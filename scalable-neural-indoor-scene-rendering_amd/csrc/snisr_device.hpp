@@ -158,6 +158,9 @@ struct Corners {
   float wx, wy, wz;
 };
 
+// Branch-free: all eight brick-table loads are issued together (a corner outside the tile reads
+// the tile's first entry, always present since n >= 8, and is then discarded), so one memory
+// round trip resolves the sample instead of eight dependent ones.
 __device__ __forceinline__ void find_corners(const int32_t* __restrict__ bmap, int n, V3 f, Corners& c) {
   const float fx = __builtin_floorf(f.x), fy = __builtin_floorf(f.y), fz = __builtin_floorf(f.z);
   const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
@@ -165,15 +168,18 @@ __device__ __forceinline__ void find_corners(const int32_t* __restrict__ bmap, i
   c.wy = f.y - fy;
   c.wz = f.z - fz;
   const int nb = n >> 3;
+  bool in[8];
+  int32_t b[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int x = ix + (k & 1), y = iy + ((k >> 1) & 1), z = iz + (k >> 2);
-    int s = -1;
-    if ((unsigned)x < (unsigned)n && (unsigned)y < (unsigned)n && (unsigned)z < (unsigned)n) {
-      const int b = bmap[((x >> 3) * nb + (y >> 3)) * nb + (z >> 3)];
-      if (b >= 0) s = b * 512 + (((x & 7) * 8 + (y & 7)) * 8 + (z & 7));
-    }
-    c.slot[k] = s;
+    in[k] = (unsigned)x < (unsigned)n && (unsigned)y < (unsigned)n && (unsigned)z < (unsigned)n;
+    b[k] = bmap[in[k] ? ((x >> 3) * nb + (y >> 3)) * nb + (z >> 3) : 0];
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int x = ix + (k & 1), y = iy + ((k >> 1) & 1), z = iz + (k >> 2);
+    c.slot[k] = in[k] && b[k] >= 0 ? b[k] * 512 + (((x & 7) * 8 + (y & 7)) * 8 + (z & 7)) : -1;
   }
 }
 
@@ -186,10 +192,25 @@ __device__ __forceinline__ float tri_combine(const float v[8], float wx, float w
   return lerpx(s, t, wz);
 }
 
+// All ones for an occupied corner, zero for an empty one.  The empty asm statement hides the
+// select from the optimizer: ANDed into the raw bits of an unconditional load, it zeroes the empty
+// corners without the load being moved into a branch of its own (where every corner would wait
+// for its own memory round trip).
+__device__ __forceinline__ uint32_t corner_mask(bool occ) {
+  uint32_t m = occ ? 0xffffffffu : 0u;
+  asm volatile("" : "+v"(m));
+  return m;
+}
+
+// Only called for samples inside an occupied brick, so the scene holds a brick and slot 0 (read
+// for the empty corners, then discarded) exists; the eight loads are unconditional.
 __device__ __forceinline__ float tri_sigma(const float* __restrict__ sigma, const Corners& c) {
   float v[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = c.slot[k] >= 0 ? sigma[c.slot[k]] : 0.0f;
+  for (int k = 0; k < 8; ++k) {
+    const float s = sigma[c.slot[k] >= 0 ? c.slot[k] : 0];
+    v[k] = c.slot[k] >= 0 ? s : 0.0f;
+  }
   return tri_combine(v, c.wx, c.wy, c.wz);
 }
 
@@ -490,21 +511,23 @@ __device__ __forceinline__ void mlp_f32(const float* __restrict__ P, float* xs, 
 template <bool HALF>
 struct FeatLoad;
 template <>
-struct FeatLoad<true> {  // 8 halves per 16-byte load
+struct FeatLoad<true> {  // 8 halves per 16-byte load, raw bits ANDed with m
   static constexpr int kVec = 8;
-  __device__ __forceinline__ static void load(const void* base, int64_t off, float* v) {
-    const uint4 q = *reinterpret_cast<const uint4*>(reinterpret_cast<const _Float16*>(base) + off);
+  __device__ __forceinline__ static void load(const void* base, int64_t off, uint32_t m, float* v) {
+    uint4 q = *reinterpret_cast<const uint4*>(reinterpret_cast<const _Float16*>(base) + off);
+    q.x &= m; q.y &= m; q.z &= m; q.w &= m;
     const _Float16* h = reinterpret_cast<const _Float16*>(&q);
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = (float)h[k];
   }
 };
 template <>
-struct FeatLoad<false> {  // 4 floats per 16-byte load
+struct FeatLoad<false> {  // 4 floats per 16-byte load, raw bits ANDed with m
   static constexpr int kVec = 4;
-  __device__ __forceinline__ static void load(const void* base, int64_t off, float* v) {
-    const float4 q = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(base) + off);
-    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  __device__ __forceinline__ static void load(const void* base, int64_t off, uint32_t m, float* v) {
+    const uint4 q = *reinterpret_cast<const uint4*>(reinterpret_cast<const float*>(base) + off);
+    v[0] = __uint_as_float(q.x & m); v[1] = __uint_as_float(q.y & m);
+    v[2] = __uint_as_float(q.z & m); v[3] = __uint_as_float(q.w & m);
   }
 };
 
@@ -514,17 +537,15 @@ __device__ __forceinline__ void gather_features(const void* __restrict__ feats, 
   using LD = FeatLoad<HALF>;
   constexpr int V = LD::kVec;
   static_assert(F % V == 0, "feature dim must fill whole 16-byte vectors");
+  uint32_t m[8];  // computed ahead of the loads: the masking statements bound scheduling regions
+#pragma unroll
+  for (int k = 0; k < 8; ++k) m[k] = corner_mask(c.slot[k] >= 0);
 #pragma unroll
   for (int ch = 0; ch < F / V; ++ch) {
     float v[8][V];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (c.slot[k] >= 0) LD::load(feats, (int64_t)c.slot[k] * F + ch * V, v[k]);
-      else {
-#pragma unroll
-        for (int e = 0; e < V; ++e) v[k][e] = 0.0f;
-      }
-    }
+    for (int k = 0; k < 8; ++k)  // unconditional loads (slot 0 for empty corners), masked to zero
+      LD::load(feats, (int64_t)(c.slot[k] >= 0 ? c.slot[k] : 0) * F + ch * V, m[k], v[k]);
 #pragma unroll
     for (int e = 0; e < V; ++e) {
       const float q[8] = {v[0][e], v[1][e], v[2][e], v[3][e], v[4][e], v[5][e], v[6][e], v[7][e]};
