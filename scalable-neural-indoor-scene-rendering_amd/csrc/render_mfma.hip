@@ -776,13 +776,13 @@ __device__ __forceinline__ void corner_weights_h2(Corners& c, h2 (&w2)[8]) {
 }
 
 // 8 consecutive FP16 features interpolated as sum_q w_q v_q with packed half FMAs
+// (cp[q]: the corner rows, already offset to this lane's features; off: further bytes, a constant)
 template <class M>
-__device__ __forceinline__ typename M::vec interp8_h2(const _Float16* __restrict__ feats, const Corners& c,
-                                                      const h2 (&w2)[8], int F, int off) {
+__device__ __forceinline__ typename M::vec interp8_h2(const uint8_t* const (&cp)[8], const h2 (&w2)[8], int off) {
   h2 acc[4];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const uint4 raw = *reinterpret_cast<const uint4*>(feats + (int64_t)c.slot[q] * F + off);
+    const uint4 raw = *reinterpret_cast<const uint4*>(cp[q] + off);
     const h2* v = reinterpret_cast<const h2*>(&raw);
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] = q == 0 ? w2[0] * v[j] : __builtin_elementwise_fma(w2[q], v[j], acc[j]);
@@ -797,12 +797,12 @@ __device__ __forceinline__ typename M::vec interp8_h2(const _Float16* __restrict
 }
 
 template <>
-__device__ __forceinline__ f16x8 interp8_h2<Mfma<SNISR_F16>>(const _Float16* __restrict__ feats, const Corners& c,
-                                                             const h2 (&w2)[8], int F, int off) {
+__device__ __forceinline__ f16x8 interp8_h2<Mfma<SNISR_F16>>(const uint8_t* const (&cp)[8], const h2 (&w2)[8],
+                                                             int off) {
   h2 acc[4];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const uint4 raw = *reinterpret_cast<const uint4*>(feats + (int64_t)c.slot[q] * F + off);
+    const uint4 raw = *reinterpret_cast<const uint4*>(cp[q] + off);
     const h2* v = reinterpret_cast<const h2*>(&raw);
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] = q == 0 ? w2[0] * v[j] : __builtin_elementwise_fma(w2[q], v[j], acc[j]);
@@ -816,6 +816,25 @@ __device__ __forceinline__ f16x8 interp8_h2<Mfma<SNISR_F16>>(const _Float16* __r
   return r;
 }
 
+
+// v_permlane32_swap on one value held by every lane: lo receives the value of lanes 0..31 and hi
+// that of lanes 32..63, both in all 64 lanes (lane l and lane l ^ 32 end up with the same pair).
+template <class T>
+__device__ __forceinline__ void swap_halves(T own, T& lo, T& hi) {
+  const uint32_t v = __builtin_bit_cast(uint32_t, own);
+  const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  lo = __builtin_bit_cast(T, (uint32_t)r[0]);
+  hi = __builtin_bit_cast(T, (uint32_t)r[1]);
+}
+
+// v_permlane32_swap of two values: lanes 32..63 of a are exchanged with lanes 0..31 of b, so
+// a_out = (a of lanes 0..31, b of lanes 0..31) and b_out = (a of lanes 32..63, b of lanes 32..63).
+__device__ __forceinline__ void swap_lanes(float a, float b, float& a_out, float& b_out) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
+                                                  false, false);
+  a_out = __builtin_bit_cast(float, (uint32_t)r[0]);
+  b_out = __builtin_bit_cast(float, (uint32_t)r[1]);
+}
 
 // ------------------------------------------------------------------------- MFMA MLP
 // Persistent: one 512-thread workgroup per CU; workgroup g owns an equal contiguous span of the
@@ -868,115 +887,100 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
     // records of this wave's first chunk; every later chunk's perm entries and records are
     // loaded one chunk ahead (perm at the top of the previous iteration, the records after its
     // gather), so that only the brick-table and feature loads are exposed
-    int32_t rk[2] = {-1, -1};
-    float4 rr[2] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+    // lane (h, s) owns record s of column block h, i.e. perm entry 64 c + lane of chunk c: it loads
+    // that record, resolves its corners and writes its colour; the corners of both blocks reach
+    // every lane through v_permlane32_swap.  The next chunk's perm entry and record are loaded one
+    // chunk ahead (perm at the top of the previous iteration, the record after its gather), so
+    // that only the brick-table and feature loads are exposed.
+    int32_t ko = -1;
+    float4 ro = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (cb0 + wv < seg_end) {
-#pragma unroll
-      for (int cbk = 0; cbk < 2; ++cbk) {
-        rk[cbk] = W.perm[(cb0 + wv) * 64 + cbk * 32 + s];
-        if (rk[cbk] >= 0) rr[cbk] = W.rec_f[rk[cbk]];
-      }
+      ko = W.perm[(cb0 + wv) * 64 + lane];
+      if (ko >= 0) ro = W.rec_f[ko];
     }
     for (int64_t c = cb0 + wv; c < seg_end; c += nwv) {
-      const bool has_next = c + nwv < seg_end;
-      int32_t nk[2] = {-1, -1};
-      if (has_next) {
-#pragma unroll
-        for (int cbk = 0; cbk < 2; ++cbk) nk[cbk] = W.perm[(c + nwv) * 64 + cbk * 32 + s];
-      }
+      const int32_t nko = c + nwv < seg_end ? W.perm[(c + nwv) * 64 + lane] : -1;
       // ---- gather: lane (h, s) builds its half of the input column of sample s of both blocks
+      int32_t ray[2], valid[2];
+      swap_halves(ko >= 0 ? ko / W.cpr : 0, ray[0], ray[1]);
+      swap_halves(ko >= 0 ? 1 : 0, valid[0], valid[1]);
       vec xin[2][NKC0];
-      float wgt[2];
-      int32_t rec[2], ray[2];
       if constexpr (HALF) {
-        // branch-free for both column blocks (padding slots read valid memory with zero weights), so
-        // the brick-table and feature loads of the two blocks are in flight together
+        // branch-free for both column blocks (padding records read slot 0 with zero weights), so the
+        // feature loads of the two blocks are in flight together
+        Corners co;
+        find_corners(S.brick_map + tile_base, tile_res, v3(ro.x, ro.y, ro.z), co);
+        h2 wo[8];
+        corner_weights_h2(co, wo);
+        const _Float16 keep = ko >= 0 ? (_Float16)1.0f : (_Float16)0.0f;
         Corners cn[2];
         h2 w2[2][8];
 #pragma unroll
-        for (int cbk = 0; cbk < 2; ++cbk) {
-          const int32_t k = rk[cbk];
-          rec[cbk] = k;
-          ray[cbk] = k >= 0 ? k / W.cpr : 0;
-          wgt[cbk] = k >= 0 ? rr[cbk].w : 0.0f;
-          find_corners(S.brick_map + tile_base, tile_res, v3(rr[cbk].x, rr[cbk].y, rr[cbk].z), cn[cbk]);
+        for (int q = 0; q < 8; ++q) {
+          swap_halves(co.slot[q], cn[0].slot[q], cn[1].slot[q]);
+          const h2 wq = wo[q] * h2{keep, keep};
+          uint32_t b0, b1;
+          swap_halves(__builtin_bit_cast(uint32_t, wq), b0, b1);
+          w2[0][q] = __builtin_bit_cast(h2, b0);
+          w2[1][q] = __builtin_bit_cast(h2, b1);
         }
+        // one 64-bit row address per corner (this lane's 8 features of k-chunk 0); the k-chunks
+        // follow at constant offsets of 32 bytes
+        const uint8_t* cp[2][8];
 #pragma unroll
-        for (int cbk = 0; cbk < 2; ++cbk) {
-          corner_weights_h2(cn[cbk], w2[cbk]);
-          const _Float16 keep = rk[cbk] >= 0 ? (_Float16)1.0f : (_Float16)0.0f;  // padding slots: zero weights
+        for (int cbk = 0; cbk < 2; ++cbk)
 #pragma unroll
-          for (int q = 0; q < 8; ++q) w2[cbk][q] = w2[cbk][q] * h2{keep, keep};
-        }
+          for (int q = 0; q < 8; ++q)
+            cp[cbk][q] = reinterpret_cast<const uint8_t*>(S.feats) + ((int64_t)cn[cbk].slot[q] * F + 8 * h) * 2;
 #pragma unroll
         for (int kc = 0; kc < NFC; ++kc)
 #pragma unroll
-          for (int cbk = 0; cbk < 2; ++cbk)
-            xin[cbk][kc] = interp8_h2<M>(reinterpret_cast<const _Float16*>(S.feats), cn[cbk], w2[cbk], F, 16 * kc + 8 * h);
+          for (int cbk = 0; cbk < 2; ++cbk) xin[cbk][kc] = interp8_h2<M>(cp[cbk], w2[cbk], 32 * kc);
+      } else {
+        // FP32 features: both records in every lane, each block's corners resolved in turn
+        float4 rr[2];
+        swap_halves(ro.x, rr[0].x, rr[1].x);
+        swap_halves(ro.y, rr[0].y, rr[1].y);
+        swap_halves(ro.z, rr[0].z, rr[1].z);
 #pragma unroll
         for (int cbk = 0; cbk < 2; ++cbk) {
-          const vec e = load_frag<vec>(reinterpret_cast<const uint8_t*>(W.enc + (int64_t)ray[cbk] * 16 + 8 * h));
-          float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          xin[cbk][NFC] = rec[cbk] >= 0 ? e : M::cvt8(z);
-        }
-      } else {
+          float xv[NFC][8];
 #pragma unroll
-      for (int cbk = 0; cbk < 2; ++cbk) {
-        const int32_t k = rk[cbk];
-        rec[cbk] = k;
-        ray[cbk] = k >= 0 ? k / W.cpr : 0;
-        wgt[cbk] = 0.0f;
-        float xv[NKC0][8];
+          for (int kc = 0; kc < NFC; ++kc)
 #pragma unroll
-        for (int kc = 0; kc < NKC0; ++kc)
+            for (int e = 0; e < 8; ++e) xv[kc][e] = 0.0f;
+          if (valid[cbk]) {
+            Corners cn;
+            find_corners(S.brick_map + tile_base, tile_res, v3(rr[cbk].x, rr[cbk].y, rr[cbk].z), cn);
+            uint32_t cmask[8];  // ahead of the loads: the masking statements bound scheduling regions
 #pragma unroll
-          for (int e = 0; e < 8; ++e) xv[kc][e] = 0.0f;
-        if (k >= 0) {
-          const float4 rf = rr[cbk];
-          wgt[cbk] = rf.w;
-          Corners cn;
-          find_corners(S.brick_map + tile_base, tile_res, v3(rf.x, rf.y, rf.z), cn);
-          if constexpr (HALF) {
-            h2 w2[8];
-            corner_weights_h2(cn, w2);
+            for (int q = 0; q < 8; ++q) cmask[q] = corner_mask(cn.slot[q] >= 0);
 #pragma unroll
-            for (int kc = 0; kc < NFC; ++kc)
-              xin[cbk][kc] = interp8_h2<M>(reinterpret_cast<const _Float16*>(S.feats), cn, w2, F, 16 * kc + 8 * h);
-          }
-          uint32_t cmask[8];  // ahead of the loads: the masking statements bound scheduling regions
+            for (int kc = 0; kc < NFC; ++kc) {
+              float v[8][8];
 #pragma unroll
-          for (int q = 0; q < 8; ++q) cmask[q] = corner_mask(cn.slot[q] >= 0);
+              for (int q = 0; q < 8; ++q)  // unconditional loads, empty corners masked to zero
+                load8<false>(S.feats, (int64_t)(cn.slot[q] >= 0 ? cn.slot[q] : 0) * F + 16 * kc + 8 * h, cmask[q], v[q]);
 #pragma unroll
-          for (int kc = 0; kc < (HALF ? 0 : NFC); ++kc) {
-            float v[8][8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q)  // unconditional loads, empty corners masked to zero
-              load8<HALF>(S.feats, (int64_t)(cn.slot[q] >= 0 ? cn.slot[q] : 0) * F + 16 * kc + 8 * h, cmask[q], v[q]);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-              const float qv[8] = {v[0][e], v[1][e], v[2][e], v[3][e], v[4][e], v[5][e], v[6][e], v[7][e]};
-              xv[kc][e] = tri_combine(qv, cn.wx, cn.wy, cn.wz);
+              for (int e = 0; e < 8; ++e) {
+                const float qv[8] = {v[0][e], v[1][e], v[2][e], v[3][e], v[4][e], v[5][e], v[6][e], v[7][e]};
+                xv[kc][e] = tri_combine(qv, cn.wx, cn.wy, cn.wz);
+              }
             }
           }
-        }
-        if (!HALF || k < 0) {
 #pragma unroll
           for (int kc = 0; kc < NFC; ++kc) xin[cbk][kc] = M::cvt8(xv[kc]);
+          __builtin_amdgcn_sched_barrier(0);
         }
-        // view encoding: precomputed per ray in the operand type by the count pass
-        if (k >= 0) {
-          xin[cbk][NFC] = load_frag<vec>(reinterpret_cast<const uint8_t*>(W.enc + (int64_t)ray[cbk] * 16 + 8 * h));
-        } else {
-          float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          xin[cbk][NFC] = M::cvt8(z);
-        }
-        __builtin_amdgcn_sched_barrier(0);
       }
-      }
-      float4 nr[2] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+      // view encoding: precomputed per ray in the operand type by the count pass
 #pragma unroll
-      for (int cbk = 0; cbk < 2; ++cbk)
-        if (nk[cbk] >= 0) nr[cbk] = W.rec_f[nk[cbk]];
+      for (int cbk = 0; cbk < 2; ++cbk) {
+        const vec e = load_frag<vec>(reinterpret_cast<const uint8_t*>(W.enc + (int64_t)ray[cbk] * 16 + 8 * h));
+        float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        xin[cbk][NFC] = valid[cbk] ? e : M::cvt8(z);
+      }
+      const float4 nro = nko >= 0 ? W.rec_f[nko] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       __builtin_amdgcn_sched_barrier(0);
 
       // ---- MLP, A fragments from LDS
@@ -1000,21 +1004,19 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
         const float* bias = reinterpret_cast<const float*>(Lp + NKH * 1024);
         f32x16 acc0 = bias_block(bias, 0, h), acc1 = acc0;
         mma_block<M, NKH>(Lp, lane, src[0], src[1], acc0, acc1);
-        // rows 0..2 of the output block sit in elements 0..2 of the lanes with h == 0
-        if (h == 0) {
-          if (rec[0] >= 0)
-            W.rec_f[rec[0]] = make_float4(wgt[0] * sigmoid_cw(acc0[0]), wgt[0] * sigmoid_cw(acc0[1]),
-                                          wgt[0] * sigmoid_cw(acc0[2]), 0.0f);
-          if (rec[1] >= 0)
-            W.rec_f[rec[1]] = make_float4(wgt[1] * sigmoid_cw(acc1[0]), wgt[1] * sigmoid_cw(acc1[1]),
-                                          wgt[1] * sigmoid_cw(acc1[2]), 0.0f);
-        }
-      }
+        // rows 0..2 of the output block sit in elements 0..2 of the lanes with h == 0, for both blocks;
+        // one swap per row hands block 1's to the lanes with h == 1, so every lane finishes its own record
+        float o[3];
 #pragma unroll
-      for (int cbk = 0; cbk < 2; ++cbk) {
-        rk[cbk] = nk[cbk];
-        rr[cbk] = nr[cbk];
+        for (int i = 0; i < 3; ++i) {
+          float unused;
+          swap_lanes(acc0[i], acc1[i], o[i], unused);
+        }
+        if (ko >= 0)
+          W.rec_f[ko] = make_float4(ro.w * sigmoid_cw(o[0]), ro.w * sigmoid_cw(o[1]), ro.w * sigmoid_cw(o[2]), 0.0f);
       }
+      ko = nko;
+      ro = nro;
     }
     cb0 = seg_end;
   }
