@@ -817,6 +817,14 @@ __device__ __forceinline__ f16x8 interp8_h2<Mfma<SNISR_F16>>(const uint8_t* cons
 }
 
 
+// Sigmoid on the hardware exp2 and reciprocal (about 1 ulp each): the colour of the MFMA path, whose
+// FP16 layers already set its error; the FP32 path keeps sigmoid_cw.  exp2 of a large argument
+// gives +inf and 1 / inf = 0, so both tails saturate correctly.
+__device__ __forceinline__ float sigmoid_hw(float z) {
+  const float e = __builtin_amdgcn_exp2f(z * -1.44269504f);
+  return __builtin_amdgcn_rcpf(1.0f + e);
+}
+
 // v_permlane32_swap on one value held by every lane: lo receives the value of lanes 0..31 and hi
 // that of lanes 32..63, both in all 64 lanes (lane l and lane l ^ 32 end up with the same pair).
 template <class T>
@@ -926,12 +934,13 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
         }
         // one 64-bit row address per corner (this lane's 8 features of k-chunk 0); the k-chunks
         // follow at constant offsets of 32 bytes
+        // (slots are non-negative here: corner_weights_h2 redirected the empty ones to slot 0)
         const uint8_t* cp[2][8];
+        const uint8_t* base_h = reinterpret_cast<const uint8_t*>(S.feats) + 16 * h;
 #pragma unroll
         for (int cbk = 0; cbk < 2; ++cbk)
 #pragma unroll
-          for (int q = 0; q < 8; ++q)
-            cp[cbk][q] = reinterpret_cast<const uint8_t*>(S.feats) + ((int64_t)cn[cbk].slot[q] * F + 8 * h) * 2;
+          for (int q = 0; q < 8; ++q) cp[cbk][q] = base_h + (uint64_t)(uint32_t)cn[cbk].slot[q] * (uint64_t)(2 * F);
 #pragma unroll
         for (int kc = 0; kc < NFC; ++kc)
 #pragma unroll
@@ -1013,7 +1022,7 @@ __global__ __launch_bounds__(512) void k_mlp_mfma(snisr_scene S, snisr_rays R, R
           swap_lanes(acc0[i], acc1[i], o[i], unused);
         }
         if (ko >= 0)
-          W.rec_f[ko] = make_float4(ro.w * sigmoid_cw(o[0]), ro.w * sigmoid_cw(o[1]), ro.w * sigmoid_cw(o[2]), 0.0f);
+          W.rec_f[ko] = make_float4(ro.w * sigmoid_hw(o[0]), ro.w * sigmoid_hw(o[1]), ro.w * sigmoid_hw(o[2]), 0.0f);
       }
       ko = nko;
       ro = nro;
