@@ -15,6 +15,10 @@
 
 #include "snisr.h"
 
+#ifndef SNISR_MLP_PREFETCH
+#define SNISR_MLP_PREFETCH 2  // A-fragment LDS reads in flight ahead of the MFMA that uses them
+#endif
+
 namespace snisr {
 
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
@@ -117,27 +121,31 @@ __device__ __forceinline__ void relu_cvt<Mfma<SNISR_F16>>(const f32x16& acc, f16
 // for the NOB 32-row output blocks.  Software-pipelined over the output blocks: the ReLU and
 // conversion of block ob - 1 are issued between the first MFMAs of block ob (vector work under
 // the running matrix pipe instead of after it), and the bias of block ob + 1 is read from LDS
-// before the last MFMA pair of block ob.  A fragments come from LDS one step ahead.
-template <class M, int NK, int NOB>
+// before the last MFMA pair of block ob.  The layer's A fragments are consecutive 1 KB pieces
+// of LDS, read PD steps ahead through a ring that runs across the output blocks.
+template <class M, int NK, int NOB, int PD = SNISR_MLP_PREFETCH>
 __device__ __forceinline__ void mma_layer(const uint8_t* A, const float* bias, int lane, int h,
                                           const typename M::vec (&s0)[NK], const typename M::vec (&s1)[NK],
                                           typename M::vec (&d0)[2 * NOB], typename M::vec (&d1)[2 * NOB]) {
   using vec = typename M::vec;
   static_assert(NK >= 2, "two k-steps at least");
   constexpr int K0 = 1, K1 = NK > 2 ? 2 : 1;  // k-steps that carry the pending activations
+  constexpr int NG = NK * NOB, R = PD + 1;
+  const uint8_t* Al = A + lane * 16;
+  vec ring[R];
+#pragma unroll
+  for (int g = 0; g < PD && g < NG; ++g) ring[g] = load_frag<vec>(Al + g * 1024);
   f32x16 p0, p1;
   f32x16 bn = bias_block(bias, 0, h);
 #pragma unroll
   for (int ob = 0; ob < NOB; ++ob) {
-    const uint8_t* Ab = A + ob * NK * 1024 + lane * 16;
-    vec ring[2];
-    ring[0] = load_frag<vec>(Ab);
     f32x16 acc0 = bn, acc1 = bn;
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
-      if (kc + 1 < NK) ring[(kc + 1) & 1] = load_frag<vec>(Ab + (kc + 1) * 1024);
+      const int g = ob * NK + kc;
+      if (g + PD < NG) ring[(g + PD) % R] = load_frag<vec>(Al + (g + PD) * 1024);
       if (kc == NK - 1 && ob + 1 < NOB) bn = bias_block(bias, ob + 1, h);
-      const vec a = ring[kc & 1];
+      const vec a = ring[g % R];
       acc0 = M::mma(a, s0[kc], acc0);
       acc1 = M::mma(a, s1[kc], acc1);
       if (ob > 0 && kc == K0) relu_cvt<M>(p0, d0[2 * ob - 2], d0[2 * ob - 1]);
