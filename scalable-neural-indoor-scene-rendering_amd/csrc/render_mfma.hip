@@ -739,26 +739,38 @@ __global__ __launch_bounds__(kSortBlock) void k_rtile_scatter(RenderWs W) {
   }
 }
 
-// ordered per-ray sum of the weighted colours (NaN for a ray whose samples exceeded its slots)
+// per-ray sum of the weighted colours (NaN for a ray whose samples exceeded its slots): four lanes
+// per ray, lane j summing records j, j + 4, ... in order, so that a wavefront reads 16 rays' records
+// as 64-byte pieces; the four partial sums are then added as (s0 + s1) + (s2 + s3).  The order is
+// fixed, so the result is deterministic.
 __global__ __launch_bounds__(256) void k_render_reduce(RenderWs W, float* __restrict__ rgb) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= W.n) return;
-  const int c = W.counts[i];
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+  const int j = threadIdx.x & 3;
+  const bool live = i < W.n;  // every lane takes part in the shuffles below
+  const int c = live ? W.counts[i] : 0;
+  const bool bad = live && (c > W.cpr || *W.skip);
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-  if (c > W.cpr || *W.skip) {
-    c0 = c1 = c2 = __builtin_nanf("");
-  } else {
+  if (live && !bad) {
     const float4* v = W.rec_f + i * (int64_t)W.cpr;
-    for (int k = 0; k < c; ++k) {
+    for (int k = j; k < c; k += 4) {
       const float4 q = v[k];
       c0 = c0 + q.x;
       c1 = c1 + q.y;
       c2 = c2 + q.z;
     }
   }
-  rgb[3 * i] = c0;
-  rgb[3 * i + 1] = c1;
-  rgb[3 * i + 2] = c2;
+#pragma unroll
+  for (int m = 1; m <= 2; m <<= 1) {
+    c0 = c0 + __shfl_xor(c0, m);
+    c1 = c1 + __shfl_xor(c1, m);
+    c2 = c2 + __shfl_xor(c2, m);
+  }
+  if (live && j == 0) {
+    const float nan = __builtin_nanf("");
+    rgb[3 * i] = bad ? nan : c0;
+    rgb[3 * i + 1] = bad ? nan : c1;
+    rgb[3 * i + 2] = bad ? nan : c2;
+  }
 }
 
 // Eight trilinear corner weights (corner order x0y0z0, x1y0z0, ... as in tri_combine) as half2
@@ -1722,7 +1734,7 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
   }
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 5, st));
-  hipLaunchKernelGGL(k_render_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, W, out->rgb);
+  hipLaunchKernelGGL(k_render_reduce, dim3((unsigned)((4 * n + 255) / 256)), dim3(256), 0, st, W, out->rgb);
   SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 6, st));
   return SNISR_OK;
