@@ -433,6 +433,21 @@ def main():
 
     rgb = out["rgb"].cpu().numpy()
     extra = {}
+    # the FP16 MFMA rate a vendor GEMM (torch.matmul, 8192^3) sustains on the same box, beside the
+    # published peak: the MLP kernel's fraction of it says how close it runs to what this part sustains
+    ga = torch.randn(8192, 8192, device=dev, dtype=torch.float16)
+    gb = torch.randn(8192, 8192, device=dev, dtype=torch.float16)
+    for _ in range(3):
+        gc = ga @ gb
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for _ in range(10):
+        gc = ga @ gb
+    torch.cuda.synchronize()
+    gemm_tflops = 2 * 8192 ** 3 / ((time.perf_counter() - a) / 10) / 1e12
+    del ga, gb, gc
+    extra["gemm_fp16_8192_tflops"] = round(gemm_tflops, 1)
+    extra["mlp_vs_gemm_fp16"] = round(achieved / gemm_tflops, 3)
     # plain FP32 path (one thread per ray) on the same frame, for the MFMA/FP32 speed ratio
     if args.fp32_frames > 0:
         D.render_fp32(ds, origin, rays_d, t_hit=t_hit, image_width=W)
