@@ -341,6 +341,24 @@ int snisr_sample_bg(const float* rays_o, const float* rays_d, const float* bg_zv
 int64_t snisr_pack_mlp_size(const snisr_scene* scene, int32_t dtype);
 int snisr_pack_mlp(const snisr_scene* scene, int32_t dtype, void* packed, void* stream);
 
+/* Screen-space refinement U-Net (rendering/scripts/networks.py:5-73: CNN / Unet with 5 input
+ * channels; applied in FRender.render at rendering/src/__init__.py:336-339, where the reference runs
+ * it through torch2trt in FP16, __init__.py:85-88).  Replaces that TensorRT engine with FP16 MFMA
+ * convolutions, FP32 accumulation.
+ * snisr_unet_pack: weights[l] / biases[l] are device pointers to conv{l+1}.weight f32[cout, cin, 3, 3]
+ *   and conv{l+1}.bias f32[cout] of the module (cin/cout of conv1..conv10: 5/64, 64/32, 32/64, 64/64,
+ *   64/128, 128/128, 192/64, 64/64, 96/32, 32/3); packed (device, 256-byte aligned) receives
+ *   snisr_unet_packed_size() bytes.
+ * snisr_unet_refine: out f32[H, W, 3] = min(ReLU(Unet(x)), 1) for x = (frame rgb f32[H, W, 3],
+ *   inverse_near f32[H, W], boundary u8[H, W] as 0/1); the reference clamps the network output to
+ *   [0, 1].  H and W are multiples of 4; the workspace (256-byte aligned) holds
+ *   snisr_unet_workspace_size(H, W) bytes (-1 with an error for bad sizes). */
+int64_t snisr_unet_packed_size(void);
+int snisr_unet_pack(const float* const* weights, const float* const* biases, void* packed, void* stream);
+int64_t snisr_unet_workspace_size(int32_t H, int32_t W);
+int snisr_unet_refine(const void* packed, const float* frame, const float* inverse_near, const uint8_t* boundary,
+                      int32_t H, int32_t W, void* workspace, int64_t workspace_bytes, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

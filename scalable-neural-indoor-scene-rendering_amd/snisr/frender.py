@@ -59,14 +59,20 @@ class FRender:
         self.cnn = None
         self.frame = None
 
-    def load_cnn(self, state_dict=None, half: bool = True):
+    def load_cnn(self, state_dict=None, half: bool = True, backend: str = "hip"):
         """Enables the screen-space refinement network (the reference's CNN / Unet,
         rendering/src/__init__.py:79-90); ``state_dict`` as saved by the reference (None keeps the
-        random initialisation)."""
-        from .cnn import CNN
+        random initialisation).  backend "hip": the library's FP16 MFMA convolutions (HipUnet);
+        "torch": the module through torch (FP16 if ``half``)."""
+        from .cnn import CNN, HipUnet
         net = CNN()
         if state_dict is not None:
             net.load_state_dict(state_dict)
+        if backend == "hip":
+            self.cnn = HipUnet(net, device=self.device)
+            return
+        if backend != "torch":
+            raise ValueError(f"backend must be 'hip' or 'torch', got {backend!r}")
         net = net.to(self.device).eval()
         self.cnn = net.half() if half else net
 
@@ -143,7 +149,11 @@ class FRender:
                                              align_corners=True)[0].permute(1, 2, 0)
         self.frame = torch.clamp(self.frame_diffuse + self.frame_speculars, 0, 1)
         if self.cnn is not None:
-            from .cnn import refine
-            self.frame = refine(self.cnn, self.frame, self.large_inverse_near, self.boundary)
+            from .cnn import HipUnet, refine
+            if isinstance(self.cnn, HipUnet):
+                self.frame = self.cnn(self.frame.contiguous(), self.large_inverse_near.contiguous(),
+                                      self.boundary.contiguous())
+            else:
+                self.frame = refine(self.cnn, self.frame, self.large_inverse_near, self.boundary)
         self.n_reflection_blocks = n_blocks
         return self.frame

@@ -95,3 +95,16 @@ def test_python_layer_fails_loudly_without_device_tensors():
     from snisr import device as D
     with pytest.raises(ValueError):
         D.compute_rays(torch.zeros(3), torch.zeros(9), 2, 2)
+
+
+def test_unet_entry_points_validate_before_device_work():
+    L = N.lib()
+    assert L.snisr_unet_packed_size() > 0
+    assert L.snisr_unet_workspace_size(720, 1280) > L.snisr_unet_workspace_size(64, 96) > 0
+    assert L.snisr_unet_workspace_size(7, 8) == -1 and b"multiples of 4" in L.snisr_last_error()
+    rc = L.snisr_unet_refine(None, None, None, None, 8, 8, None, 0, None, None)
+    assert rc != 0 and b"NULL" in L.snisr_last_error()
+    rc = L.snisr_unet_refine(None, None, None, None, 6, 8, None, 0, None, None)
+    assert rc != 0 and b"multiples of 4" in L.snisr_last_error()
+    nul = (C.c_void_p * 10)()
+    assert L.snisr_unet_pack(nul, nul, C.c_void_p(256), None) != 0
