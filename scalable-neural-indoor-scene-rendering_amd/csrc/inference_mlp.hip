@@ -18,7 +18,7 @@
 // bands 5..9 (weights of layer 0 are packed in that order, so k slot s >= 2 is band slot s - 2 in
 // both halves).  Each half evaluates the hardware sin / cos once per coordinate, at 2^(5h) x (the
 // reference uses the same fast intrinsics, __sinf / __cosf), then four double-angle steps
-// (sin 2a = 2 sin a cos a, cos 2a = cos^2 a - sin^2 a); the error stays near 1e-5, below the FP16
+// (sin 2a = 2 sin a cos a, cos 2a = 1 - 2 sin^2 a); the error stays near 1e-5, below the FP16
 // rounding of the inputs.  LeakyReLU runs on the FP16 values, as in the reference (x < 0 -> 0.01 x
 // in half precision), as packed-half multiply and max.
 #include "mfma_ops.hpp"
@@ -96,8 +96,9 @@ __device__ __forceinline__ void pe_half(const float (&p)[3], int h, float (&v)[3
     cs[0][r] = __cosf(a);
 #pragma unroll
     for (int k = 1; k < 5; ++k) {
-      sn[k][r] = 2.0f * sn[k - 1][r] * cs[k - 1][r];
-      cs[k][r] = cs[k - 1][r] * cs[k - 1][r] - sn[k - 1][r] * sn[k - 1][r];
+      const float t = sn[k - 1][r] + sn[k - 1][r];
+      sn[k][r] = t * cs[k - 1][r];                         // sin 2a = 2 sin a cos a
+      cs[k][r] = __builtin_fmaf(-t, sn[k - 1][r], 1.0f);  // cos 2a = 1 - 2 sin^2 a
     }
   }
   v[0] = h ? p[2] : p[0];
@@ -234,25 +235,39 @@ __global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
         acc[0] = bias_block(bias, 0, h);
         acc[1] = acc[0];
         mma_block<M, 4>(Lp, lane, xb[0], xb[1], acc[0], acc[1]);
+        // lane (h, s) finishes sample s of column block h: it evaluates that sample's SH basis, which
+        // v_permlane32_swap hands to the other half, weights the rows of both blocks it holds, and one
+        // swap per channel adds the two halves' partial sums of block h (sigma, row 27, is held by
+        // h = 0 and reaches the block-1 lanes the same way)
+        const int64_t tk = h ? task[1] : task[0];
+        float bo[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (tk >= 0) {
+          const int ori = h ? orip[1] : orip[0];
+          const int py = div_small(ori, Q.W, inv_w), px = ori - py * Q.W;
+          const float* d = Q.rays_d + (int64_t)py * Q.d_row_stride + (int64_t)px * Q.d_col_stride;
+          const V3 u = normalize_dir(v3(d[0], d[1], d[2]));
+          sh2_basis(u.x, u.y, u.z, bo);
+        }
+        float bs[2][9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) swap_halves(bo[j], bs[0][j], bs[1][j]);
+        float pr[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) {
-          float pr[3] = {0.0f, 0.0f, 0.0f};
-          if (task[cb] >= 0) {
-            const int py = div_small(orip[cb], Q.W, inv_w), px = orip[cb] - py * Q.W;
-            const float* d = Q.rays_d + (int64_t)py * Q.d_row_stride + (int64_t)px * Q.d_col_stride;
-            const V3 u = normalize_dir(v3(d[0], d[1], d[2]));
-            float bs[9];
-            sh2_basis(u.x, u.y, u.z, bs);
-            if (h == 0) sh_partial<0>(acc[cb], bs, pr);
-            else sh_partial<1>(acc[cb], bs, pr);
-          }
-          float raw[3];
-#pragma unroll
-          for (int k = 0; k < 3; ++k) raw[k] = pr[k] + __shfl_xor(pr[k], 32);
-          if (h == 0 && task[cb] >= 0)
-            reinterpret_cast<float4*>(Q.sample_results)[task[cb]] =
-                make_float4(sigmoid_cw(raw[0]), sigmoid_cw(raw[1]), sigmoid_cw(raw[2]), fmaxf(acc[cb][15], 0.0f));
+          if (h == 0) sh_partial<0>(acc[cb], bs[cb], pr[cb]);
+          else sh_partial<1>(acc[cb], bs[cb], pr[cb]);
         }
+        float raw[3], sg, unused;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          float lo, hi;
+          swap_lanes(pr[0][k], pr[1][k], lo, hi);
+          raw[k] = lo + hi;
+        }
+        swap_lanes(acc[0][15], acc[1][15], sg, unused);
+        if (tk >= 0)
+          reinterpret_cast<float4*>(Q.sample_results)[tk] =
+              make_float4(sigmoid_hw(raw[0]), sigmoid_hw(raw[1]), sigmoid_hw(raw[2]), fmaxf(sg, 0.0f));
       }
     }
   }

@@ -829,33 +829,6 @@ __device__ __forceinline__ f16x8 interp8_h2<Mfma<SNISR_F16>>(const uint8_t* cons
 }
 
 
-// Sigmoid on the hardware exp2 and reciprocal (about 1 ulp each): the colour of the MFMA path, whose
-// FP16 layers already set its error; the FP32 path keeps sigmoid_cw.  exp2 of a large argument
-// gives +inf and 1 / inf = 0, so both tails saturate correctly.
-__device__ __forceinline__ float sigmoid_hw(float z) {
-  const float e = __builtin_amdgcn_exp2f(z * -1.44269504f);
-  return __builtin_amdgcn_rcpf(1.0f + e);
-}
-
-// v_permlane32_swap on one value held by every lane: lo receives the value of lanes 0..31 and hi
-// that of lanes 32..63, both in all 64 lanes (lane l and lane l ^ 32 end up with the same pair).
-template <class T>
-__device__ __forceinline__ void swap_halves(T own, T& lo, T& hi) {
-  const uint32_t v = __builtin_bit_cast(uint32_t, own);
-  const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-  lo = __builtin_bit_cast(T, (uint32_t)r[0]);
-  hi = __builtin_bit_cast(T, (uint32_t)r[1]);
-}
-
-// v_permlane32_swap of two values: lanes 32..63 of a are exchanged with lanes 0..31 of b, so
-// a_out = (a of lanes 0..31, b of lanes 0..31) and b_out = (a of lanes 32..63, b of lanes 32..63).
-__device__ __forceinline__ void swap_lanes(float a, float b, float& a_out, float& b_out) {
-  const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
-                                                  false, false);
-  a_out = __builtin_bit_cast(float, (uint32_t)r[0]);
-  b_out = __builtin_bit_cast(float, (uint32_t)r[1]);
-}
-
 // ------------------------------------------------------------------------- MFMA MLP
 // Persistent: one 512-thread workgroup per CU; workgroup g owns an equal contiguous span of the
 // 64-record chunks of the tile-sorted order.  For every tile inside its span it stages that tile's
