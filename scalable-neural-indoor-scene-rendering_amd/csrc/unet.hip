@@ -184,6 +184,104 @@ __global__ __launch_bounds__(kConvThreads) void k_unet_conv(ConvArgs a) {
   }
 }
 
+// Layers whose workgroups hold a single 32-channel output block are bound by their B loads when
+// every tap reloads its pixels (one 16-byte load per MFMA).  Used for conv2 (64 -> 32 at full
+// resolution: 162 -> 119 us at 720p); conv9 (96 -> 32) fits only 4 rows next to its weights and
+// is faster on k_unet_conv (220 vs 259 us).  This variant stages the ROWS + 2 input
+// rows of a 66-pixel span (a tile of ROWS x 64 output pixels and its halo; zero outside the image,
+// read from the zero row) in LDS once per tile, and the nine taps of its ROWS wavefronts (one
+// output row each) take their B operands from there.  The next tile's halo is loaded into
+// registers while the current tile is computed, and written to LDS after it.
+template <int NKC, int C0, int ROWS>
+__global__ __launch_bounds__(64 * ROWS) void k_unet_conv_rows(ConvArgs a) {
+  using M = Mfma<SNISR_F16>;
+  using vec = f16x8;
+  constexpr int NS = 9 * NKC;
+  constexpr int SPAN = 66;                   // pixels per staged row
+  constexpr int CHK = 2 * NKC;               // 16-byte chunks per pixel
+  constexpr int PITCH = NKC * 32 + 16;       // bytes per staged pixel (the 16 extra bytes spread the banks)
+  constexpr int TOTAL = (ROWS + 2) * SPAN * CHK;
+  constexpr int PER = (TOTAL + 64 * ROWS - 1) / (64 * ROWS);
+  extern __shared__ __align__(16) uint8_t lds[];
+  uint8_t* halo = lds + NS * 1024 + 128;
+  const int ob = blockIdx.y;
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(a.layer + (int64_t)ob * NS * 1024);
+    uint4* dst = reinterpret_cast<uint4*>(lds);
+    for (int k = threadIdx.x; k < NS * 64; k += blockDim.x) dst[k] = src[k];
+    const float4* bs = reinterpret_cast<const float4*>(a.layer + (int64_t)gridDim.y * NS * 1024) + ob * 8;
+    float4* bd = reinterpret_cast<float4*>(lds + NS * 1024);
+    if (threadIdx.x < 8) bd[threadIdx.x] = bs[threadIdx.x];
+  }
+  const float* bias = reinterpret_cast<const float*>(lds + NS * 1024);
+  const int lane = threadIdx.x & 63, h = lane >> 5, s = lane & 31, wv = threadIdx.x >> 6;
+  const int H = a.H, W = a.W;
+  const int tpr = (W + 63) >> 6;
+  const int64_t ntiles = (int64_t)((H + ROWS - 1) / ROWS) * tpr;
+  // this thread's chunks of a tile's halo, loaded into registers
+  uint4 hv[PER];
+  const auto fetch = [&](int64_t tile) {
+    const int y0 = (int)(tile / tpr) * ROWS, x0 = (int)(tile % tpr) * 64;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int k = threadIdx.x + j * 64 * ROWS;
+      const int c = k % CHK, px = (k / CHK) % SPAN, r = k / (CHK * SPAN);
+      const int yy = y0 + r - 1, xx = x0 + px - 1, ch = 8 * c;
+      const bool ok = k < TOTAL && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W;
+      const int64_t pix = ok ? (int64_t)yy * W + xx : 0;
+      const _Float16* p = ch < C0 ? a.src0 + pix * C0 + ch : a.src1 + pix * (NKC * 16 - C0) + (ch - C0);
+      hv[j] = *reinterpret_cast<const uint4*>(ok ? p : a.zero + 8 * (c & 1));
+    }
+  };
+  int64_t tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  for (; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // the weights are staged; the previous tile's halo reads are done
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int k = threadIdx.x + j * 64 * ROWS;
+      if (k < TOTAL) {
+        const int c = k % CHK, px = (k / CHK) % SPAN, r = k / (CHK * SPAN);
+        *reinterpret_cast<uint4*>(halo + (r * SPAN + px) * PITCH + c * 16) = hv[j];
+      }
+    }
+    __syncthreads();
+    if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);  // in flight during the MFMAs below
+    const int y = (int)(tile / tpr) * ROWS + wv, x0 = (int)(tile % tpr) * 64;
+    if (y >= H) continue;
+    f32x16 acc[2];
+    acc[0] = bias_block(bias, 0, h);
+    acc[1] = acc[0];
+#pragma unroll
+    for (int g = 0; g < NS; ++g) {
+      const int tap = g / NKC, kc = g % NKC, dy = tap / 3, dx = tap % 3;
+      const vec af = load_frag<vec>(lds + g * 1024 + lane * 16);
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        const vec bf = load_frag<vec>(halo + ((wv + dy) * SPAN + 32 * cb + s + dx) * PITCH + (16 * kc + 8 * h) * 2);
+        acc[cb] = M::mma(af, bf, acc[cb]);
+      }
+    }
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int x = x0 + 32 * cb + s;
+      if (x >= W) continue;
+      const int64_t pix = (int64_t)y * W + x;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        _Float16 v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float t = acc[cb][4 * j + r];
+          v[r] = (_Float16)(t > 0.0f ? t : 0.0f);
+        }
+        const int co = 32 * ob + 8 * j + 4 * h;
+        *reinterpret_cast<uint2*>(a.dst + pix * a.cout + co) = *reinterpret_cast<const uint2*>(v);
+      }
+    }
+  }
+}
+
 // 2x2 average pooling of an NHWC FP16 tensor (nn.AvgPool2d(2, 2)), FP32 sum of the window in row order
 __global__ void k_unet_pool(const _Float16* __restrict__ in, int H, int W, int C, _Float16* __restrict__ out) {
   const int G = C / 8, Ho = H / 2, Wo = W / 2;
@@ -312,6 +410,38 @@ int conv_launch(int l, const _Float16* s0, const _Float16* s1, int H, int W, con
   return SNISR_OK;
 }
 
+template <int NKC, int C0, int ROWS>
+int conv_rows_launch(int l, const _Float16* s0, const _Float16* s1, int H, int W, const UnetWs& ws,
+                     const uint8_t* packed, _Float16* dst, hipStream_t st) {
+  static_assert(NKC * 16 <= 192 && C0 % 16 == 0 && C0 <= NKC * 16, "chunks within the zero row and the sources");
+  const UnetLayer L = kUnet[l];
+  const size_t lds = (size_t)9 * NKC * 1024 + 128 + (size_t)(ROWS + 2) * 66 * (NKC * 32 + 16);
+  auto kern = k_unet_conv_rows<NKC, C0, ROWS>;
+  static bool lds_set = false;
+  if (!lds_set) {
+    SNISR_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    lds_set = true;
+  }
+  ConvArgs a;
+  a.src0 = s0;
+  a.src1 = s1;
+  a.H = H;
+  a.W = W;
+  a.zero = ws.zero;
+  a.layer = packed + unet_layer_off(l);
+  a.dst = dst;
+  a.out = nullptr;
+  a.cout = L.cout;
+  const int64_t tiles = (int64_t)((H + ROWS - 1) / ROWS) * ((W + 63) / 64);
+  const int per_cu = lds > 80 * 1024 ? 1 : 2;
+  int64_t gx = ((int64_t)per_cu * device_cus() + L.nob - 1) / L.nob;
+  if (gx > tiles) gx = tiles;
+  if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)gx, (unsigned)L.nob), dim3(64 * ROWS), lds, st, a);
+  SNISR_LAUNCHED();
+  return SNISR_OK;
+}
+
 int pool_launch(const _Float16* in, int H, int W, int C, _Float16* out, hipStream_t st) {
   const int64_t n = (int64_t)(H / 2) * (W / 2) * (C / 8);
   hipLaunchKernelGGL(k_unet_pool, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, H, W, C, out);
@@ -378,7 +508,7 @@ int snisr_unet_refine(const void* packed, const float* frame, const float* inver
   int e;
   // o2 = relu(conv2(relu(conv1(x))))
   if ((e = conv_launch<1, 16, 2, 0>(0, w.x16, nullptr, H, W, w, pk, w.a1, nullptr, st))) return e;
-  if ((e = conv_launch<4, 64, 1, 0>(1, w.a1, nullptr, H, W, w, pk, w.o2, nullptr, st))) return e;
+  if ((e = conv_rows_launch<4, 64, 8>(1, w.a1, nullptr, H, W, w, pk, w.o2, st))) return e;
   // o5 = relu(conv4(relu(conv3(down(o2)))))
   if ((e = pool_launch(w.o2, H, W, 32, w.p1, st))) return e;
   if ((e = conv_launch<2, 32, 2, 0>(2, w.p1, nullptr, H2, W2, w, pk, w.a3, nullptr, st))) return e;
