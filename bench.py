@@ -92,7 +92,7 @@ def train_bench(args, dev):
     sigma = ds.sigma.clone().requires_grad_(True)
     feats = ds.feats.clone().requires_grad_(True)
     mlp = ds.mlp_params.clone().requires_grad_(True)
-    opt = torch.optim.Adam([sigma, feats, mlp], lr=1e-4)
+    opt = TR.Adam([sigma, feats, mlp], lr=1e-4)   # one-pass HIP Adam (torch.optim.Adam semantics)
     ws = TR.TrainWorkspace()
     losses = []
 

@@ -341,6 +341,14 @@ int snisr_sample_bg(const float* rays_o, const float* rays_d, const float* bg_zv
 int64_t snisr_pack_mlp_size(const snisr_scene* scene, int32_t dtype);
 int snisr_pack_mlp(const snisr_scene* scene, int32_t dtype, void* packed, void* stream);
 
+/* One Adam step on a flat FP32 tensor (the training chain's optimizer: torch.optim.Adam(params,
+ * weight_decay=1e-5), training/scripts/tile/tileGroup.py:342, stepped at tile.py:780-785), in one
+ * pass over param, grad and the two moment buffers (all f32[n], updated in place), with torch's
+ * formulas and operation order; step counts from 1, the scalars are formed in double precision as
+ * torch forms them. */
+int snisr_adam_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n, double lr,
+                    double beta1, double beta2, double eps, double weight_decay, int64_t step, void* stream);
+
 /* Screen-space refinement U-Net (rendering/scripts/networks.py:5-73: CNN / Unet with 5 input
  * channels; applied in FRender.render at rendering/src/__init__.py:336-339, where the reference runs
  * it through torch2trt in FP16, __init__.py:85-88).  Replaces that TensorRT engine with FP16 MFMA

@@ -106,6 +106,8 @@ SIGNATURES = [
                                   C.c_int32, C.c_void_p, C.c_void_p]),
     ("snisr_compute_group_boundary", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                                C.c_void_p]),
+    ("snisr_adam_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, C.c_int64, C.c_void_p]),
     ("snisr_unet_packed_size", C.c_int64, []),
     ("snisr_unet_pack", C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p]),
     ("snisr_unet_workspace_size", C.c_int64, [C.c_int32, C.c_int32]),

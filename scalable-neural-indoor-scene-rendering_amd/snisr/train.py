@@ -92,3 +92,40 @@ class VoxelRender(torch.autograd.Function):
 def render(sigma, feats, mlp_params, origins, dirs, ds: DeviceScene, t_hit=None, workspace=None, **march_kw):
     """Differentiable rgb [N,3] of the rays through the scene held by ``ds`` with the given parameters."""
     return VoxelRender.apply(sigma, feats, mlp_params, origins, dirs, t_hit, ds, workspace, march_kw)
+
+
+class Adam(torch.optim.Optimizer):
+    """torch.optim.Adam (no amsgrad) on the library's one-pass kernel (snisr_adam_step): the
+    reference's optimizer, torch.optim.Adam(params, weight_decay=1e-5) (training/scripts/tile/
+    tileGroup.py:342).  Parameters must be contiguous FP32 CUDA tensors; the state holds ``step``,
+    ``exp_avg`` and ``exp_avg_sq`` like torch's."""
+
+    def __init__(self, params, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, weight_decay=0.0):
+        if not 0.0 <= betas[0] < 1.0 or not 0.0 <= betas[1] < 1.0:
+            raise ValueError(f"betas must lie in [0, 1), got {betas}")
+        super().__init__(params, dict(lr=lr, betas=betas, eps=eps, weight_decay=weight_decay))
+
+    @torch.no_grad()
+    def step(self, closure=None):
+        loss = None
+        if closure is not None:
+            with torch.enable_grad():
+                loss = closure()
+        for group in self.param_groups:
+            b1, b2 = group["betas"]
+            for p in group["params"]:
+                if p.grad is None:
+                    continue
+                _require_cuda(p, "param", torch.float32)
+                _require_cuda(p.grad, "grad", torch.float32)
+                st = self.state[p]
+                if not st:
+                    st["step"] = 0
+                    st["exp_avg"] = torch.zeros_like(p)
+                    st["exp_avg_sq"] = torch.zeros_like(p)
+                st["step"] += 1
+                N.check(N.lib().snisr_adam_step(_ptr(p), _ptr(p.grad), _ptr(st["exp_avg"]), _ptr(st["exp_avg_sq"]),
+                                                p.numel(), float(group["lr"]), float(b1), float(b2),
+                                                float(group["eps"]), float(group["weight_decay"]), st["step"],
+                                                _stream()), "adam_step")
+        return loss
