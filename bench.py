@@ -97,7 +97,7 @@ def train_bench(args, dev):
     losses = []
 
     def step():
-        opt.zero_grad(set_to_none=False)
+        opt.zero_grad(set_to_none=True)
         rgb = TR.render(sigma, feats, mlp, o, d, ds, workspace=ws)
         loss = torch.mean((rgb - target) ** 2)
         loss.backward()
@@ -141,6 +141,22 @@ def ref_pipeline_bench(args, dev):
         fr.render()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.ref_frames
+
+    def graphed_ms():
+        fr._graph = None
+        ref = fr.render().clone()
+        for _ in range(2):
+            out = fr.render_graphed()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref)  # the replayed frame is the reference-form frame
+        t = time.perf_counter()
+        for _ in range(args.ref_frames):
+            fr.render_graphed()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / args.ref_frames
+
+    # the same frame without host synchronisation (device-side regrouping), captured in a HIP graph
+    ms_graph = graphed_ms()
     # the same frame with the U-Net refinement enabled (FP16, random weights of the reference's shape)
     torch.manual_seed(0)
     fr.load_cnn()
@@ -152,13 +168,19 @@ def ref_pipeline_bench(args, dev):
         fr.render()
     torch.cuda.synchronize()
     ms_cnn = (time.perf_counter() - t0) * 1e3 / args.ref_frames
+    ms_cnn_graph = graphed_ms()
     return {"ref_pipeline_ms_per_frame": round(ms, 3), "ref_pipeline_mrays_s": round(720 * 1280 / ms / 1e3, 2),
             "ref_pipeline_with_unet_ms_per_frame": round(ms_cnn, 3),
+            "ref_pipeline_graphed_ms_per_frame": round(ms_graph, 3),
+            "ref_pipeline_graphed_mrays_s": round(720 * 1280 / ms_graph / 1e3, 2),
+            "ref_pipeline_graphed_with_unet_ms_per_frame": round(ms_cnn_graph, 3),
             "ref_pipeline_reflection_blocks": int(fr.n_reflection_blocks),
             "ref_pipeline_config": "FRender.render form at 1280x720: compute_rays, tracing_tiles (K=30), octree "
                                    "diffuse pass, sort_by_key/assign_pixels, group boundary, 4 x (inverse_z_sampling, "
                                    "inference_mlp FP16 MFMA, integrate_points) at 640x360, bilinear upsample; "
-                                   "with_unet adds the 5-channel U-Net refinement in FP16; seeds 12/13/2"}
+                                   "with_unet adds the 5-channel U-Net refinement in FP16; graphed: the same frame "
+                                   "with the regrouping on the device (no host synchronisation) between the replays of "
+                                   "two HIP graphs (diffuse pass; reflection and compose); seeds 12/13/2"}
 
 
 def c4_bench(args, dev):

@@ -287,11 +287,27 @@ typedef struct snisr_mlp_query {
   int32_t H, W;
   const void* packed_params;          /* snisr_pack_inference_mlp output, 16-byte aligned */
   float* sample_results;              /* [H*W, samples_per_ray, 4], 16-byte aligned; skipped samples untouched */
+  const int32_t* n_blocks_device;     /* NULL, or a device count of the blocks to process (at most n_blocks),
+                                         written by snisr_regroup_pixels: no host round trip for it */
 } snisr_mlp_query;
 
 int64_t snisr_inference_mlp_packed_size(int64_t n_groups);
 int snisr_pack_inference_mlp(const void* params_f16, int64_t n_groups, void* packed, void* stream);
 int snisr_inference_mlp(const snisr_mlp_query* query, void* stream);
+
+/* The per-frame pixel regrouping of FRender.render (sort_by_key, the host-side block_starts
+ * and assign_pixels: rendering/src/__init__.py:272-291, reorder.cu:11-94) as one device-side pass
+ * with no host synchronisation (the reference makes two per frame):  query i32[n] = pixel indices
+ * stably sorted by net_idx i16[n]; the blocks of at most pixels_per_block consecutive sorted pixels
+ * of one group, in group order: block_net i16[max_blocks] (-1 past the last block),
+ * block_starts i32[max_blocks + 1] (positions in query, n past the last block), and the block count
+ * in n_blocks (device).  Blocks of group -1 are listed like the reference's (inference_mlp skips
+ * them).  max_blocks >= snisr_regroup_max_blocks(n, pixels_per_block) always suffices. */
+int64_t snisr_regroup_max_blocks(int64_t n, int32_t pixels_per_block);
+int64_t snisr_regroup_workspace_size(int64_t n);
+int snisr_regroup_pixels(const int16_t* net_idx, int64_t n, int32_t pixels_per_block, int64_t max_blocks,
+                         int32_t* query, int16_t* block_net, int32_t* block_starts, int32_t* n_blocks,
+                         void* workspace, int64_t workspace_bytes, void* stream);
 
 /* A11 — training (FP32 features and MLP).  snisr_train_forward renders like snisr_render_fp32
  * (identical traversal and samples) and keeps per-sample records in the workspace;

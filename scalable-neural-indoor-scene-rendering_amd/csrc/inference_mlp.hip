@@ -155,9 +155,14 @@ __global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
   const int spr = Q.samples_per_ray;
   const float inv_spr = 1.0f / (float)spr, inv_w = 1.0f / (float)Q.W;
   const int64_t npix = (int64_t)Q.H * Q.W;
-  const int64_t per = (Q.n_blocks + gridDim.x - 1) / gridDim.x;
+  int64_t nb = Q.n_blocks;  // or the device-side count of a host-sync-free regrouping
+  if (Q.n_blocks_device) {
+    const int64_t d = *Q.n_blocks_device;
+    nb = d < nb ? (d > 0 ? d : 0) : nb;
+  }
+  const int64_t per = (nb + gridDim.x - 1) / gridDim.x;
   const int64_t b0 = (int64_t)blockIdx.x * per;
-  const int64_t b1 = b0 + per < Q.n_blocks ? b0 + per : Q.n_blocks;
+  const int64_t b1 = b0 + per < nb ? b0 + per : nb;
   const uint8_t* __restrict__ gw = reinterpret_cast<const uint8_t*>(Q.packed_params);
   int loaded = -1;
   for (int64_t b = b0; b < b1; ++b) {

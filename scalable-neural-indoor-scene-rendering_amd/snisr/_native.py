@@ -47,7 +47,8 @@ class MlpQuery(C.Structure):
                 ("pixel_starts", C.c_void_p), ("query_pixel_indices", C.c_void_p), ("n_query", C.c_int64),
                 ("transparency", C.c_void_p), ("group_centers", C.c_void_p), ("samples", C.c_void_p),
                 ("rays_d", C.c_void_p), ("d_row_stride", C.c_int64), ("d_col_stride", C.c_int64),
-                ("H", C.c_int32), ("W", C.c_int32), ("packed_params", C.c_void_p), ("sample_results", C.c_void_p)]
+                ("H", C.c_int32), ("W", C.c_int32), ("packed_params", C.c_void_p), ("sample_results", C.c_void_p),
+                ("n_blocks_device", C.c_void_p)]
 
 
 class OctreeDesc(C.Structure):
@@ -106,6 +107,10 @@ SIGNATURES = [
                                   C.c_int32, C.c_void_p, C.c_void_p]),
     ("snisr_compute_group_boundary", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                                C.c_void_p]),
+    ("snisr_regroup_max_blocks", C.c_int64, [C.c_int64, C.c_int32]),
+    ("snisr_regroup_workspace_size", C.c_int64, [C.c_int64]),
+    ("snisr_regroup_pixels", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     ("snisr_adam_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_double,
                                   C.c_double, C.c_double, C.c_double, C.c_int64, C.c_void_p]),
     ("snisr_unet_packed_size", C.c_int64, []),

@@ -208,7 +208,7 @@ def pack_inference_params(params: torch.Tensor) -> torch.Tensor:
 def inference_mlp(max_samples_per_ray: int, num_block: int, num_thread: int, inverse_far: float, trans_stop: float,
                   netIdxs: torch.Tensor, transparency: torch.Tensor, query_pixel_indices: torch.Tensor,
                   pixel_starts: torch.Tensor, group_centers: torch.Tensor, samples: torch.Tensor,
-                  rays_d: torch.Tensor, params: torch.Tensor, sample_results: torch.Tensor) -> None:
+                  rays_d: torch.Tensor, params: torch.Tensor, sample_results: torch.Tensor, n_blocks_device: torch.Tensor | None = None) -> None:
     """rendering/src/rayreflection_render.cu:303-335: the reflection MLP of every live sample of the
     num_block pixel blocks; writes (rgb, sigma) into sample_results [H, W, spr, 4] at the samples'
     original pixels.  rays_d may be a strided [H, W, 3] view such as ``rays_d[::2, ::2]``."""
@@ -245,6 +245,9 @@ def inference_mlp(max_samples_per_ray: int, num_block: int, num_thread: int, inv
     Q.rays_d, Q.d_row_stride, Q.d_col_stride = rays_d.data_ptr(), rays_d.stride(0), rays_d.stride(1)
     Q.H, Q.W = H, W
     Q.packed_params, Q.sample_results = packed.data_ptr(), sample_results.data_ptr()
+    if n_blocks_device is not None:  # the block count stays on the device (regroup_pixels)
+        _require_cuda(n_blocks_device, "n_blocks_device", torch.int32)
+        Q.n_blocks_device = n_blocks_device.data_ptr()
     N.check(N.lib().snisr_inference_mlp(C.byref(Q), _stream()), "inference_mlp")
 
 
@@ -295,3 +298,29 @@ def rendering_diffuse_octree_fp16(rays_d: torch.Tensor, visitedTiles: torch.Tens
                                         float(early_stop), 1 if soft_hit else 0, _ptr(frame_diffuse),
                                         _ptr(inverse_near), _ptr(netIdxs), _ptr(n_samples), _stream()),
             "rendering_diffuse_octree_fp16")
+
+
+def regroup_max_blocks(n: int, pixels_per_block: int) -> int:
+    return int(N.lib().snisr_regroup_max_blocks(int(n), int(pixels_per_block)))
+
+
+def regroup_pixels(netIdxs: torch.Tensor, pixels_per_block: int, query: torch.Tensor, block_net: torch.Tensor,
+                   block_starts: torch.Tensor, n_blocks: torch.Tensor, workspace: torch.Tensor) -> None:
+    """The regrouping of FRender.render (sort_by_key, block_starts, assign_pixels: rendering/src/
+    __init__.py:272-291) on the device without host synchronisation (snisr_regroup_pixels): query =
+    pixel indices sorted by group, block_net / block_starts the <= pixels_per_block blocks of each group
+    (-1 / n past the last block, block_net sized for regroup_max_blocks), n_blocks [1] the block count.
+    workspace: uint8, snisr_regroup_workspace_size bytes."""
+    net = netIdxs.reshape(-1)
+    _require_cuda(net, "netIdxs", torch.int16)
+    for t, name, dt in ((query, "query", torch.int32), (block_net, "block_net", torch.int16),
+                        (block_starts, "block_starts", torch.int32), (n_blocks, "n_blocks", torch.int32),
+                        (workspace, "workspace", torch.uint8)):
+        _require_cuda(t, name, dt)
+    n = net.numel()
+    mb = block_net.numel()
+    if query.numel() < n or block_starts.numel() < mb + 1 or n_blocks.numel() < 1:
+        raise ValueError("query needs n entries, block_starts len(block_net) + 1, n_blocks 1")
+    N.check(N.lib().snisr_regroup_pixels(_ptr(net), n, int(pixels_per_block), mb, _ptr(query), _ptr(block_net),
+                                         _ptr(block_starts), _ptr(n_blocks), _ptr(workspace), workspace.numel(),
+                                         _stream()), "regroup_pixels")

@@ -87,12 +87,100 @@ class FRender:
         return fr
 
     def set_camera(self, cam: Camera):
-        self.intrinsic = torch.from_numpy(cam.intrinsic).to(self.device)
-        self.c2w = torch.from_numpy(cam.c2w).to(self.device)
-        self.origin = torch.from_numpy(cam.origin).to(self.device)
+        """New camera; copied into the existing device tensors, so a captured frame (render_graphed)
+        renders the new view on its next replay."""
+        for name, v in (("intrinsic", cam.intrinsic), ("c2w", cam.c2w), ("origin", cam.origin)):
+            t = torch.from_numpy(np.ascontiguousarray(v))
+            cur = getattr(self, name, None)
+            if cur is not None and cur.shape == t.shape and cur.dtype == t.dtype:
+                cur.copy_(t.to(self.device))
+            else:
+                setattr(self, name, t.to(self.device))
 
     @torch.no_grad()
     def render(self):
+        """One frame in the reference's call sequence, including its regrouping through sort_by_key /
+        assign_pixels with the two host synchronisations (the returned group count and block count)."""
+        rays_d, inverse_near, netIdxs = self._diffuse()
+        s = self.reflection_scale
+        npx = self.downscale_numPixel
+        dev = self.device
+        query = torch.arange(0, npx, 1, device=dev, dtype=torch.int32)
+        pixel_starts = torch.arange(0, npx, 1, device=dev, dtype=torch.int32)
+        numNet = FR.sort_by_key(netIdxs, query, pixel_starts)
+        tail = torch.full((1,), npx, dtype=torch.int32, device=dev)
+        per_net = torch.cat([pixel_starts[1:numNet], tail]) - pixel_starts[:numNet]
+        pixel_starts = torch.cat([pixel_starts[:numNet], tail])
+        block_starts = torch.cat([torch.zeros(1, dtype=torch.int32, device=dev),
+                                  torch.cumsum(torch.ceil(per_net / self.max_pixels_per_block), dim=0).int()])
+        netIdxs = netIdxs[:numNet]
+        n_blocks = int(block_starts[-1])
+        new_net = torch.empty(n_blocks, device=dev, dtype=torch.int16)
+        new_starts = torch.empty(n_blocks, device=dev, dtype=torch.int32)
+        FR.assign_pixels(netIdxs, pixel_starts, block_starts, self.num_thread, new_net, new_starts)
+        new_starts = torch.cat([new_starts, tail])
+        self.n_reflection_blocks = n_blocks
+        return self._reflection_and_compose(rays_d, inverse_near, n_blocks, new_net, query, new_starts, None)
+
+    @torch.no_grad()
+    def render_device(self):
+        """The same frame with no host synchronisation: the regrouping runs on the device
+        (regroup_pixels) and inference_mlp reads the block count from device memory.  The output
+        equals render()'s."""
+        rays_d, inverse_near, netIdxs = self._diffuse()
+        mb, query, block_net, block_starts, nblk = self._regroup(netIdxs)
+        return self._reflection_and_compose(rays_d, inverse_near, mb, block_net, query, block_starts, nblk)
+
+    def _regroup(self, netIdxs):
+        """Device-side regrouping into buffers kept across frames (so captured kernels can read them)."""
+        npx, P = self.downscale_numPixel, self.max_pixels_per_block
+        dev = self.device
+        if getattr(self, "_rg", None) is None:
+            from . import _native as N
+            mb = FR.regroup_max_blocks(npx, P)
+            self._rg = (mb, torch.empty(npx, dtype=torch.int32, device=dev),
+                        torch.empty(mb, dtype=torch.int16, device=dev),
+                        torch.empty(mb + 1, dtype=torch.int32, device=dev),
+                        torch.empty(1, dtype=torch.int32, device=dev),
+                        torch.empty(int(N.lib().snisr_regroup_workspace_size(npx)), dtype=torch.uint8, device=dev))
+        mb, query, block_net, block_starts, nblk, ws = self._rg
+        FR.regroup_pixels(netIdxs, P, query, block_net, block_starts, nblk, ws)
+        self.n_blocks_device = nblk
+        return mb, query, block_net, block_starts, nblk
+
+    def render_graphed(self):
+        """render_device with its two kernel sequences captured in HIP graphs: the diffuse pass and the
+        reflection / composition pass each replay as one graph; the regrouping between them runs
+        eagerly (its radix sort clears device memory with memset operations, which are kept out of
+        graphs), still without a host synchronisation.  The returned frame tensor is the second
+        graph's output buffer, overwritten by the next replay."""
+        if getattr(self, "_graph", None) is None:
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):  # warm-up: packs the MLP parameters, sizes the workspaces
+                self.render_device()
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            with torch.no_grad():
+                ga = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(ga):
+                    a_out = self._diffuse()
+                a_keep = (a_out, self.frame_diffuse, self.large_inverse_near, self.large_netIdx)
+                mb, query, block_net, block_starts, nblk = self._regroup(a_out[2])
+                torch.cuda.synchronize()
+                gc = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(gc):
+                    out = self._reflection_and_compose(a_out[0], a_out[1], mb, block_net, query, block_starts, nblk)
+            self._graph = (ga, gc, a_keep, out)
+        ga, gc, a_keep, out = self._graph
+        ga.replay()
+        with torch.no_grad():
+            self._regroup(a_keep[0][2])
+        gc.replay()
+        self.frame = out
+        return out
+
+    def _diffuse(self):
         dev = self.device
         H, W, s = self.H, self.W, self.reflection_scale
         rays_d = torch.full((H, W, 3), -1, device=dev, dtype=torch.float32)
@@ -113,21 +201,11 @@ class FRender:
         self.large_netIdx = netIdxs.clone()
         inverse_near = inverse_near[::s, ::s].contiguous()
         netIdxs = netIdxs[::s, ::s].flatten()
-        npx = self.downscale_numPixel
-        query = torch.arange(0, npx, 1, device=dev, dtype=torch.int32)
-        pixel_starts = torch.arange(0, npx, 1, device=dev, dtype=torch.int32)
-        numNet = FR.sort_by_key(netIdxs, query, pixel_starts)
-        tail = torch.full((1,), npx, dtype=torch.int32, device=dev)
-        per_net = torch.cat([pixel_starts[1:numNet], tail]) - pixel_starts[:numNet]
-        pixel_starts = torch.cat([pixel_starts[:numNet], tail])
-        block_starts = torch.cat([torch.zeros(1, dtype=torch.int32, device=dev),
-                                  torch.cumsum(torch.ceil(per_net / self.max_pixels_per_block), dim=0).int()])
-        netIdxs = netIdxs[:numNet]
-        n_blocks = int(block_starts[-1])
-        new_net = torch.empty(n_blocks, device=dev, dtype=torch.int16)
-        new_starts = torch.empty(n_blocks, device=dev, dtype=torch.int32)
-        FR.assign_pixels(netIdxs, pixel_starts, block_starts, self.num_thread, new_net, new_starts)
-        new_starts = torch.cat([new_starts, tail])
+        return rays_d, inverse_near, netIdxs
+
+    def _reflection_and_compose(self, rays_d, inverse_near, n_blocks, new_net, query, new_starts, n_blocks_device):
+        dev = self.device
+        H, W, s = self.H, self.W, self.reflection_scale
         self.boundary = torch.zeros((H, W, 1), dtype=torch.bool, device=dev)
         FR.compute_group_boundary(self.large_netIdx, self.boundary, self.ksize_boundary, self.num_thread * 2)
         sample_far = 1.0 / inverse_near + self.far
@@ -143,7 +221,7 @@ class FRender:
                                   samples, steps)
             FR.inference_mlp(16, n_blocks, self.num_thread, self.inverse_far, self.early_stop_R, new_net,
                              transparency, query, new_starts, self.group_centers, samples, rays_ds, self.net_params,
-                             sample_results)
+                             sample_results, n_blocks_device=n_blocks_device)
             FR.integrate_points(sample_results, steps, self.num_thread * 2, self.early_stop_R, transparency, spec)
         self.frame_speculars = F.interpolate(spec[None].permute(0, 3, 1, 2), scale_factor=s, mode="bilinear",
                                              align_corners=True)[0].permute(1, 2, 0)
@@ -155,5 +233,4 @@ class FRender:
                                       self.boundary.contiguous())
             else:
                 self.frame = refine(self.cnn, self.frame, self.large_inverse_near, self.boundary)
-        self.n_reflection_blocks = n_blocks
         return self.frame

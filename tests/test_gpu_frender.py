@@ -102,3 +102,53 @@ def test_frender_cnn_refinement_matches_fp32_module():
     assert err.max() < 2e-2 and err.mean() < 2e-3, (err.max(), err.mean())
     net = fr.large_netIdx
     assert bool(fr.boundary.any()) == bool((net != net.flatten()[0]).any())   # flags iff the groups vary
+
+
+@pytest.mark.parametrize("n,groups,seed", [(230400, 9, 0), (4097, 3, 1), (1, 1, 2), (640, 40, 3)])
+def test_regroup_pixels_matches_host_regrouping(n, groups, seed):
+    """snisr_regroup_pixels against the reference's regrouping in numpy (stable sort, balanced
+    <= 64-pixel blocks per group, group -1 included): bit-exact query, block groups and starts."""
+    from snisr import fastrendering as FR
+    rng = np.random.default_rng(seed)
+    net = rng.integers(-1, groups - 1, size=n, endpoint=True).astype(np.int16)
+    q_ref, bnet_ref, bps_ref = S.reflection_blocks(net)
+    dev = torch.device("cuda:0")
+    from snisr import _native as N
+    mb = FR.regroup_max_blocks(n, 64)
+    query = torch.empty(n, dtype=torch.int32, device=dev)
+    bnet = torch.full((mb,), 7, dtype=torch.int16, device=dev)
+    bps = torch.full((mb + 1,), 7, dtype=torch.int32, device=dev)
+    nbl = torch.zeros(1, dtype=torch.int32, device=dev)
+    ws = torch.empty(int(N.lib().snisr_regroup_workspace_size(n)), dtype=torch.uint8, device=dev)
+    FR.regroup_pixels(torch.from_numpy(net).to(dev), 64, query, bnet, bps, nbl, ws)
+    k = int(nbl)
+    assert k == bnet_ref.size
+    assert np.array_equal(query.cpu().numpy(), q_ref)
+    assert np.array_equal(bnet[:k].cpu().numpy(), bnet_ref)
+    assert np.array_equal(bps[:k + 1].cpu().numpy(), bps_ref)
+    assert bool((bnet[k:] == -1).all()) and bool((bps[k:] == n).all())
+
+
+@pytest.mark.parametrize("H,W", [(96, 160), (720, 1280)])
+def test_frender_device_and_graphed_frames_equal_reference_form(H, W):
+    """render_device (no host synchronisation) and its HIP-graph replay give the frame of render(),
+    bit for bit, across camera changes; the replays interleave with eager work.  At 720p the
+    regrouping sorts 230,400 pixels, where the radix sort takes its multi-pass path."""
+    print(f"frender graphed {H}x{W}", flush=True)
+    osc = S.make_octree_scene(9, tile_shape=(4, 4, 1), block_occupancy=0.3, n_groups=6)
+    params, centers = S.make_reflection_groups(19, 6, (0, 0, 0), (4, 4, 1))
+    cams = [S.room_camera(sd, (0, 0, 0), (4, 4, 1), H, W, focal=95.0 * W / 160) for sd in (9, 10)]
+    fr = frender.FRender(osc, params, centers, H=H, W=W)
+    fr.load_cnn()
+    eager = []
+    for cam in cams:
+        fr.set_camera(cam)
+        eager.append(fr.render().clone())
+        assert torch.equal(fr.render_device(), eager[-1])
+    for rep in range(3):
+        for k, cam in enumerate(cams):
+            fr.set_camera(cam)
+            out = fr.render_graphed()
+            torch.cuda.synchronize()
+            assert torch.equal(out, eager[k]), (rep, k)
+            _ = torch.randn(1000, device="cuda").sum().item()  # eager work between replays
