@@ -48,6 +48,17 @@ __host__ __device__ constexpr int pe_input_index(int h, int s) {
   return h == 0 ? (s < 2 ? s : s + 1) : (s == 0 ? 2 : (s == 1 ? kRefIn : s + 31));
 }
 
+// Reference output held by MFMA row p of layer l (-1: a zero row).  The hidden layers keep their
+// order.  The output layer's 32 rows are permuted so that slot i of both lane halves (row
+// 8(i/4) + 4h + i%4) holds the same colour channel i % 3: half 0 holds the coefficients
+// 3 b + c of bases b = 0..4 in slots 0..14 and sigma (output 27) in slot 15; half 1 holds bases
+// 5..8 in slots 0..11 and zero rows in slots 12..15.  The SH sum then needs no per-half indexing.
+__host__ __device__ constexpr int ref_out_row(int l, int p) {
+  if (l != kRefLayers - 1) return p;
+  const int h = (p / 4) % 2, i = 4 * (p / 8) + p % 4;
+  return h == 0 ? (i < 15 ? i : kRefOut - 1) : (i < 12 ? 15 + i : -1);
+}
+
 // a / b for 0 <= a < 2^24, b > 0, from the FP32 quotient corrected by one step
 __device__ __forceinline__ int div_small(int a, int b, float inv_b) {
   int q = (int)((float)a * inv_b);
@@ -75,14 +86,14 @@ __global__ void k_pack_ref_layer(const uint16_t* __restrict__ src, uint8_t* __re
   if (r < nw) {
     const int frag = (int)(r / 512), within = (int)(r % 512);
     const int ob = frag / 4, kc = frag % 4, lane = within / 8, e = within % 8;
-    const int h = lane / 32, row = 32 * ob + lane % 32;
+    const int h = lane / 32, row = ref_out_row(l, 32 * ob + lane % 32);
     const int x = l == 0 ? pe_input_index(h, 8 * kc + e) : pack_input_index(l, kc, h, e);
-    reinterpret_cast<uint16_t*>(D)[r] = (row < out && x < in) ? Wt[x * out + row] : (uint16_t)0;
+    reinterpret_cast<uint16_t*>(D)[r] = (row >= 0 && row < out && x < in) ? Wt[x * out + row] : (uint16_t)0;
   } else {
     const int q = (int)(r - nw);
     const int ob = q / 32, h = (q / 16) % 2, i = q % 16;
-    const int row = 32 * ob + 8 * (i / 4) + 4 * h + (i % 4);
-    reinterpret_cast<float*>(D + nob * 4 * 1024)[q] = row < out ? half_bits_to_float(bias[row]) : 0.0f;
+    const int row = ref_out_row(l, 32 * ob + 8 * (i / 4) + 4 * h + (i % 4));
+    reinterpret_cast<float*>(D + nob * 4 * 1024)[q] = row >= 0 && row < out ? half_bits_to_float(bias[row]) : 0.0f;
   }
 }
 
@@ -133,18 +144,55 @@ __device__ __forceinline__ void sh2_basis(float x, float y, float z, float (&b)[
   b[7] = -c2 * (x * z); b[8] = c24 * (x * x - y * y);
 }
 
-// partial SH colour from the 16 output rows this lane holds (rows 8(i/4) + 4h + i%4; r < 27 are
-// coefficients 3 * basis + channel)
-template <int H>
-__device__ __forceinline__ void sh_partial(const f32x16& acc, const float (&b)[9], float (&pr)[3]) {
+// partial SH colour from the output rows this lane holds: slot i < 15 is coefficient
+// 3 (5h + i/3) + i%3 (zero rows in slots 12..14 of half 1), w[j] the basis value of 5h + j
+__device__ __forceinline__ void sh_partial(const f32x16& acc, const float (&w)[5], float (&pr)[3]) {
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int r = 8 * (i / 4) + 4 * H + (i % 4);
-    if (r < 27) pr[r % 3] += b[r / 3] * acc[i];
-  }
+  for (int i = 0; i < 15; ++i) pr[i % 3] += w[i / 3] * acc[i];
 }
 
-__global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
+// The inputs of one 64-column chunk as lane (h, s) needs them: the positions of columns s and
+// 32 + s, and the record it finishes (column 32h + s): its task index (-1: no sample) and the ray
+// direction of its pixel.  The loads are unconditional (a column past ncol reads sample 0 of
+// pixel 0, which exists whenever the kernel runs), so the number of loads in flight is the same on
+// every path and the wait for them can stay at their first use, one chunk later.
+struct ChunkIn {
+  float p[2][3];
+  float d[3];
+  int64_t task;
+};
+
+__device__ __forceinline__ void fetch_chunk(const snisr_mlp_query& Q, const int* act, int c0, int ncol, float inv_spr,
+                                            float inv_w, int h, int s, ChunkIn& in) {
+  const int spr = Q.samples_per_ray;
+  int64_t tk = -1;
+  int own = 0;
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb) {
+    const int c = c0 + cb * 32 + s;
+    const bool ok = c < ncol;
+    const int cc = ok ? c : 0;
+    const int q = div_small(cc, spr, inv_spr);  // < 64 for cc < ncol = 64 * samples_per_ray at most
+    const int ori = ok ? act[q] : 0;
+    const int64_t t = (int64_t)ori * spr + (cc - q * spr);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) in.p[cb][i] = Q.samples[t * 3 + i];
+    if (cb == h) {
+      tk = ok ? t : -1;
+      own = ori;
+    }
+  }
+  in.task = tk;
+  const int py = div_small(own, Q.W, inv_w), px = own - py * Q.W;
+  const float* d = Q.rays_d + (int64_t)py * Q.d_row_stride + (int64_t)px * Q.d_col_stride;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) in.d[i] = d[i];
+}
+
+// At least two waves per SIMD caps the kernel at 256 registers, so the compiler keeps the MFMA
+// accumulators in VGPRs (with no cap it placed them in AGPRs and read each result back with a
+// v_accvgpr_read: 352 extra instructions per 64 samples).
+__global__ __launch_bounds__(256, 2) void k_inference_mlp(snisr_mlp_query Q) {
   __shared__ __align__(16) uint8_t lw[kRefGroupBytes];
   __shared__ int act[64];
   __shared__ float ctr[3];
@@ -194,30 +242,23 @@ __global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
       }
       __syncthreads();
       const int ncol = nact_s * spr;
+      // a chunk's inputs are loaded one chunk ahead, so their latency hides behind the layers
+      ChunkIn cur, nxt;
+      fetch_chunk(Q, act, wv * 64, ncol, inv_spr, inv_w, h, s, cur);
       for (int c0 = wv * 64; c0 < ncol; c0 += nwv * 64) {
         f16x8 xa[2][4], xb[2][4];
-        int64_t task[2];
-        int orip[2];
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) {
-          const int c = c0 + cb * 32 + s;
-          float pt[3] = {0.0f, 0.0f, 0.0f};
-          task[cb] = -1;
-          orip[cb] = 0;
-          if (c < ncol) {
-            const int q = div_small(c, spr, inv_spr);
-            const int ori = act[q];
-            orip[cb] = ori;
-            task[cb] = (int64_t)ori * spr + (c - q * spr);
+          float pt[3];
 #pragma unroll
-            for (int i = 0; i < 3; ++i) pt[i] = (Q.samples[task[cb] * 3 + i] - ctr[i]) * Q.inverse_far;
-          }
+          for (int i = 0; i < 3; ++i) pt[i] = (cur.p[cb][i] - ctr[i]) * Q.inverse_far;
           float v[32];
           pe_half(pt, h, v);
 #pragma unroll
           for (int kc = 0; kc < 4; ++kc) xa[cb][kc] = M::cvt8(v + 8 * kc);
           __builtin_amdgcn_sched_barrier(0);
         }
+        fetch_chunk(Q, act, c0 + nwv * 64, ncol, inv_spr, inv_w, h, s, nxt);
         // five hidden layers (A fragments from LDS), activations ping-pong xa <-> xb
 #pragma unroll
         for (int l = 0; l < kRefLayers - 1; ++l) {
@@ -242,15 +283,12 @@ __global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
         mma_block<M, 4>(Lp, lane, xb[0], xb[1], acc[0], acc[1]);
         // lane (h, s) finishes sample s of column block h: it evaluates that sample's SH basis, which
         // v_permlane32_swap hands to the other half, weights the rows of both blocks it holds, and one
-        // swap per channel adds the two halves' partial sums of block h (sigma, row 27, is held by
-        // h = 0 and reaches the block-1 lanes the same way)
-        const int64_t tk = h ? task[1] : task[0];
+        // swap per channel adds the two halves' partial sums of block h (sigma, slot 15 of lane half
+        // 0, reaches the block-1 lanes the same way)
+        const int64_t tk = cur.task;
         float bo[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (tk >= 0) {
-          const int ori = h ? orip[1] : orip[0];
-          const int py = div_small(ori, Q.W, inv_w), px = ori - py * Q.W;
-          const float* d = Q.rays_d + (int64_t)py * Q.d_row_stride + (int64_t)px * Q.d_col_stride;
-          const V3 u = normalize_dir(v3(d[0], d[1], d[2]));
+          const V3 u = normalize_dir(v3(cur.d[0], cur.d[1], cur.d[2]));
           sh2_basis(u.x, u.y, u.z, bo);
         }
         float bs[2][9];
@@ -259,8 +297,10 @@ __global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
         float pr[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) {
-          if (h == 0) sh_partial<0>(acc[cb], bs[cb], pr[cb]);
-          else sh_partial<1>(acc[cb], bs[cb], pr[cb]);
+          float w[5];
+#pragma unroll
+          for (int j = 0; j < 5; ++j) w[j] = h ? (j < 4 ? bs[cb][5 + j] : 0.0f) : bs[cb][j];
+          sh_partial(acc[cb], w, pr[cb]);
         }
         float raw[3], sg, unused;
 #pragma unroll
@@ -273,6 +313,7 @@ __global__ __launch_bounds__(256) void k_inference_mlp(snisr_mlp_query Q) {
         if (tk >= 0)
           reinterpret_cast<float4*>(Q.sample_results)[tk] =
               make_float4(sigmoid_hw(raw[0]), sigmoid_hw(raw[1]), sigmoid_hw(raw[2]), fmaxf(sg, 0.0f));
+        cur = nxt;
       }
     }
   }

@@ -1180,7 +1180,8 @@ __device__ __forceinline__ void dw_layer(const float* __restrict__ A, const floa
 }
 
 template <int IN, int WD, int L>
-__global__ __launch_bounds__(256) void k_train_dw(Workspace W, int64_t mlp_stride, float* __restrict__ dmlp) {
+// two waves per SIMD at least: the 256-register cap keeps the accumulators out of AGPRs
+__global__ __launch_bounds__(256, 2) void k_train_dw(Workspace W, int64_t mlp_stride, float* __restrict__ dmlp) {
   static_assert(WD <= 128 && (IN + 3) / 4 * 4 <= 128, "rows fit the LDS pitch");
   __shared__ __align__(16) float Ds[kDwRows * kDwPitch];
   __shared__ __align__(16) float As[kDwRows * kDwPitch];
