@@ -121,16 +121,6 @@ __device__ __forceinline__ void pe_half(const float (&p)[3], int h, float (&v)[3
   }
 }
 
-__device__ __forceinline__ void leaky_cvt(const f32x16& acc, f16x8& lo, f16x8& hi) {
-  float v[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = acc[i];
-  const f16x8 slope = {0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16};
-  const f16x8 a = Mfma<SNISR_F16>::cvt8(v), b = Mfma<SNISR_F16>::cvt8(v + 8);
-  lo = __builtin_elementwise_max(a, a * slope);
-  hi = __builtin_elementwise_max(b, b * slope);
-}
-
 // real SH of degree <= 2 (orthonormal on the sphere; the reference's sign convention)
 __device__ __forceinline__ void sh2_basis(float x, float y, float z, float (&b)[9]) {
   constexpr float c0 = 0.28209479177387814f;   // 1 / (2 sqrt(pi))
@@ -259,20 +249,15 @@ __global__ __launch_bounds__(256, 2) void k_inference_mlp(snisr_mlp_query Q) {
           __builtin_amdgcn_sched_barrier(0);
         }
         fetch_chunk(Q, act, c0 + nwv * 64, ncol, inv_spr, inv_w, h, s, nxt);
-        // five hidden layers (A fragments from LDS), activations ping-pong xa <-> xb
+        // five hidden layers (A fragments from LDS), activations ping-pong xa <-> xb; the LeakyReLU
+        // of one output block is issued under the MFMAs of the next
 #pragma unroll
         for (int l = 0; l < kRefLayers - 1; ++l) {
           auto& src = (l & 1) ? xb : xa;
           auto& dst = (l & 1) ? xa : xb;
           const uint8_t* Lp = lw + l * kRefHiddenBytes;
           const float* bias = reinterpret_cast<const float*>(Lp + 2 * 4 * 1024);
-#pragma unroll
-          for (int ob = 0; ob < 2; ++ob) {
-            f32x16 acc0 = bias_block(bias, ob, h), acc1 = acc0;
-            mma_block<M, 4>(Lp + ob * 4 * 1024, lane, src[0], src[1], acc0, acc1);
-            leaky_cvt(acc0, dst[0][2 * ob], dst[0][2 * ob + 1]);
-            leaky_cvt(acc1, dst[1][2 * ob], dst[1][2 * ob + 1]);
-          }
+          mma_layer<M, 4, 2, SNISR_MLP_PREFETCH, true>(Lp, bias, lane, h, src[0], src[1], dst[0], dst[1]);
         }
         // output layer: 28 rows in one 32-row block; the input is the output of layer 4 (in xb)
         const uint8_t* Lp = lw + (kRefLayers - 1) * kRefHiddenBytes;

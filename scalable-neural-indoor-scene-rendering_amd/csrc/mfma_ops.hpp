@@ -117,13 +117,36 @@ __device__ __forceinline__ void relu_cvt<Mfma<SNISR_F16>>(const f32x16& acc, f16
   hi = __builtin_elementwise_max(Mfma<SNISR_F16>::cvt8(v + 8), z);
 }
 
-// One layer over two 32-column blocks: d{0,1}[2ob], d{0,1}[2ob + 1] = ReLU(W(ob) s{0,1} + b(ob))
+// LeakyReLU(0.01) on FP16 values: convert, then max(x, 0.01 x) on packed halves (x < 0 -> 0.01 x
+// in half precision)
+__device__ __forceinline__ void leaky_cvt_f16(const f32x16& acc, f16x8& lo, f16x8& hi) {
+  float v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = acc[i];
+  const f16x8 slope = {0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16, 0.01f16};
+  const f16x8 a = Mfma<SNISR_F16>::cvt8(v), b = Mfma<SNISR_F16>::cvt8(v + 8);
+  lo = __builtin_elementwise_max(a, a * slope);
+  hi = __builtin_elementwise_max(b, b * slope);
+}
+
+template <class M, bool LEAKY>
+__device__ __forceinline__ void act_cvt(const f32x16& acc, typename M::vec& lo, typename M::vec& hi) {
+  if constexpr (LEAKY) {
+    static_assert(sizeof(typename M::vec) == sizeof(f16x8), "LeakyReLU on FP16 operands");
+    leaky_cvt_f16(acc, lo, hi);
+  } else {
+    relu_cvt<M>(acc, lo, hi);
+  }
+}
+
+// One layer over two 32-column blocks: d{0,1}[2ob], d{0,1}[2ob + 1] = act(W(ob) s{0,1} + b(ob))
+// (ReLU, or LeakyReLU(0.01) in FP16 when LEAKY)
 // for the NOB 32-row output blocks.  Software-pipelined over the output blocks: the ReLU and
 // conversion of block ob - 1 are issued between the first MFMAs of block ob (vector work under
 // the running matrix pipe instead of after it), and the bias of block ob + 1 is read from LDS
 // before the last MFMA pair of block ob.  The layer's A fragments are consecutive 1 KB pieces
 // of LDS, read PD steps ahead through a ring that runs across the output blocks.
-template <class M, int NK, int NOB, int PD = SNISR_MLP_PREFETCH>
+template <class M, int NK, int NOB, int PD = SNISR_MLP_PREFETCH, bool LEAKY = false>
 __device__ __forceinline__ void mma_layer(const uint8_t* A, const float* bias, int lane, int h,
                                           const typename M::vec (&s0)[NK], const typename M::vec (&s1)[NK],
                                           typename M::vec (&d0)[2 * NOB], typename M::vec (&d1)[2 * NOB]) {
@@ -148,15 +171,15 @@ __device__ __forceinline__ void mma_layer(const uint8_t* A, const float* bias, i
       const vec a = ring[g % R];
       acc0 = M::mma(a, s0[kc], acc0);
       acc1 = M::mma(a, s1[kc], acc1);
-      if (ob > 0 && kc == K0) relu_cvt<M>(p0, d0[2 * ob - 2], d0[2 * ob - 1]);
-      if (ob > 0 && kc == K1) relu_cvt<M>(p1, d1[2 * ob - 2], d1[2 * ob - 1]);
+      if (ob > 0 && kc == K0) act_cvt<M, LEAKY>(p0, d0[2 * ob - 2], d0[2 * ob - 1]);
+      if (ob > 0 && kc == K1) act_cvt<M, LEAKY>(p1, d1[2 * ob - 2], d1[2 * ob - 1]);
       __builtin_amdgcn_sched_barrier(0);
     }
     p0 = acc0;
     p1 = acc1;
   }
-  relu_cvt<M>(p0, d0[2 * NOB - 2], d0[2 * NOB - 1]);
-  relu_cvt<M>(p1, d1[2 * NOB - 2], d1[2 * NOB - 1]);
+  act_cvt<M, LEAKY>(p0, d0[2 * NOB - 2], d0[2 * NOB - 1]);
+  act_cvt<M, LEAKY>(p1, d1[2 * NOB - 2], d1[2 * NOB - 1]);
 }
 
 // Sigmoid on the hardware exp2 and reciprocal (about 1 ulp each): the colour of the MFMA path, whose
