@@ -1109,6 +1109,7 @@ __device__ __forceinline__ void dw_layer(const float* __restrict__ A, const floa
   constexpr int ND = (kDwRows * OP4 + 255) / 256, NA = (kDwRows * IP4 + 255) / 256;
   constexpr int NIB = (INL + 31) / 32, NOB = (OUTL + 31) / 32, NPAIR = NIB * NOB;
   static_assert(NPAIR <= 16, "at most 4 pairs per wave");
+  static_assert(2 * OP <= 256, "two threads per output column");
   const int lane = threadIdx.x & 63, c = lane & 31, h = lane >> 5, wv = threadIdx.x >> 6;
   f32x16 acc[4];
 #pragma unroll
@@ -1144,22 +1145,35 @@ __device__ __forceinline__ void dw_layer(const float* __restrict__ A, const floa
     }
     __syncthreads();
     if (r0 + kDwRows < je) fetch(r0 + kDwRows);
-    if (threadIdx.x < OP) {
-#pragma unroll 8
-      for (int rr = 0; rr < kDwRows; ++rr) accb += Ds[rr * kDwPitch + threadIdx.x];
+    // db: thread t < 2 OP sums rows (t / OP) * 16 .. + 15 of column t % OP, four independent
+    // partial sums so that the LDS reads do not wait on each other
+    if (threadIdx.x < 2 * OP) {
+      const int col = threadIdx.x % OP, r0h = (threadIdx.x / OP) * (kDwRows / 2);
+      float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int rr = 0; rr < kDwRows / 2; ++rr) part[rr & 3] += Ds[(r0h + rr) * kDwPitch + col];
+      accb += (part[0] + part[1]) + (part[2] + part[3]);
     }
+    // the 16 steps' operands of the wave's next pair are read from LDS while the current pair's
+    // products run (a ring of two operand sets), so no product waits on its own LDS read
+    float xs[2][kDwRows / 2], ys[2][kDwRows / 2];
+    auto load_pair = [&](int p, float (&x)[kDwRows / 2], float (&y)[kDwRows / 2]) {
+      const int ob = p / NIB, ib = p % NIB;
+#pragma unroll
+      for (int st = 0; st < kDwRows / 2; ++st) {
+        x[st] = Ds[(2 * st + h) * kDwPitch + ob * 32 + c];
+        y[st] = As[(2 * st + h) * kDwPitch + ib * 32 + c];
+      }
+    };
+    if (wv < NPAIR) load_pair(wv, xs[0], ys[0]);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int p = wv + 4 * q;
       if (p < NPAIR) {  // wave-uniform
-        const int ob = p / NIB, ib = p % NIB;
+        if (q + 1 < 4 && p + 4 < NPAIR) load_pair(p + 4, xs[(q + 1) & 1], ys[(q + 1) & 1]);
         f32x16 a = acc[q];
 #pragma unroll
-        for (int st = 0; st < kDwRows / 2; ++st) {
-          const float x = Ds[(2 * st + h) * kDwPitch + ob * 32 + c];
-          const float y = As[(2 * st + h) * kDwPitch + ib * 32 + c];
-          a = __builtin_amdgcn_mfma_f32_32x32x2f32(x, y, a, 0, 0, 0);
-        }
+        for (int st = 0; st < kDwRows / 2; ++st) a = __builtin_amdgcn_mfma_f32_32x32x2f32(xs[q & 1][st], ys[q & 1][st], a, 0, 0, 0);
         acc[q] = a;
       }
     }
@@ -1176,11 +1190,13 @@ __device__ __forceinline__ void dw_layer(const float* __restrict__ A, const floa
       if (o < OUTL && i < INL && acc[q][r] != 0.0f) atomicAdd(&G[(int64_t)o * INL + i], acc[q][r]);
     }
   }
-  if (threadIdx.x < OUTL && accb != 0.0f) atomicAdd(&G[(int64_t)OUTL * INL + threadIdx.x], accb);
+  if (threadIdx.x < 2 * OP && threadIdx.x % OP < OUTL && accb != 0.0f)
+    atomicAdd(&G[(int64_t)OUTL * INL + threadIdx.x % OP], accb);
 }
 
 template <int IN, int WD, int L>
-// two waves per SIMD at least: the 256-register cap keeps the accumulators out of AGPRs
+// two waves per SIMD at least: the 256-register cap keeps the accumulators in VGPRs (AGPR
+// accumulators were copied in and out for every pair and row tile)
 __global__ __launch_bounds__(256, 2) void k_train_dw(Workspace W, int64_t mlp_stride, float* __restrict__ dmlp) {
   static_assert(WD <= 128 && (IN + 3) / 4 * 4 <= 128, "rows fit the LDS pitch");
   __shared__ __align__(16) float Ds[kDwRows * kDwPitch];
