@@ -111,9 +111,12 @@ __global__ __launch_bounds__(64) void k_render_octree(snisr_octree O, const floa
             cell[b] = vl[0] * MM + vl[1] * M + vl[2];
             z += step;
           }
-          bool take[kOctBatch];
+          // the flag and voxel-id loads depend only on the sample positions (cell and base are
+          // clamped into the block), so they are issued together, unconditionally, and the flag is
+          // applied afterwards: two dependent load levels per batch instead of three
+          uint8_t flg[kOctBatch];
 #pragma unroll
-          for (int b = 0; b < kOctBatch; ++b) take[b] = live[b] && flags[cell[b]] != 0;
+          for (int b = 0; b < kOctBatch; ++b) flg[b] = flags[cell[b]];
           // trilinear corners over the dilated grid (x, then y, then z)
           int base[kOctBatch];
           float w[kOctBatch][3];
@@ -133,8 +136,14 @@ __global__ __launch_bounds__(64) void k_render_octree(snisr_octree O, const floa
 #pragma unroll
           for (int b = 0; b < kOctBatch; ++b)
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-              vid[b][q] = take[b] ? nodes[base[b] + (q & 1) * MM + ((q >> 1) & 1) * M + (q >> 2)] : (int16_t)-1;
+            for (int q = 0; q < 8; ++q) vid[b][q] = nodes[base[b] + (q & 1) * MM + ((q >> 1) & 1) * M + (q >> 2)];
+          bool take[kOctBatch];
+#pragma unroll
+          for (int b = 0; b < kOctBatch; ++b) {
+            take[b] = live[b] && flg[b] != 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) vid[b][q] = take[b] ? vid[b][q] : (int16_t)-1;
+          }
           uint2 raw[kOctBatch][8];
 #pragma unroll
           for (int b = 0; b < kOctBatch; ++b)
