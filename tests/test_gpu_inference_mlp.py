@@ -165,3 +165,27 @@ def test_reflection_pass_matches_oracle(oracle_mod):
     cg = col.cpu().numpy().reshape(-1, 3)
     err = np.abs(cg - col_ref)
     assert err.max() <= 2e-2 and err.mean() <= 1e-3, (err.max(), err.mean())
+
+
+def test_inference_mlp_blocks_with_every_pixel_stopped(oracle_mod):
+    """Pixels of the first half of the query order (whole blocks) are below trans_stop, so those
+    blocks have no live column at all; the live blocks match the oracle and no stopped sample is
+    written."""
+    fr = S.reflection_frame(seed=77, H=64, W=96, n_groups=4)
+    DH, DW = fr.net_idx.shape
+    spr = 7
+    rng = np.random.default_rng(77)
+    qpi, bnet, bps = S.reflection_blocks(fr.net_idx)
+    T = rng.uniform(0.5, 1.0, DH * DW).astype(np.float32)
+    stopped = np.asarray(qpi[: len(qpi) // 2])
+    T[stopped] = 0.0                                            # every pixel of these blocks stops
+    samples = (fr.origin + rng.uniform(-4, 4, (DH * DW * spr, 3))).astype(np.float32)
+    got = run_mlp(fr, qpi, bnet, bps, T, samples, spr, to_dev(fr.rays_d)[::2, ::2])
+    ref = np.full_like(got, SENTINEL)
+    oracle_mod.inference_mlp(spr, bnet, bps, qpi, T, fr.centers, samples, fr.rays_d[::2, ::2], fr.params,
+                             1.0 / fr.far, fr.trans_stop, ref, quant16=True)
+    live = live_mask(fr, bnet, bps, qpi, T, spr)
+    assert live.any() and not live.all()
+    check_close(got, ref, live)
+    dead = (T.reshape(-1)[:, None] < fr.trans_stop).repeat(spr, 1).reshape(-1)
+    assert np.all(got.reshape(-1, 4)[dead] == SENTINEL)
