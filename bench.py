@@ -22,6 +22,7 @@ sys.path.insert(0, os.path.join(REPO, "scalable-neural-indoor-scene-rendering_am
 
 METRIC = "Mrays/s render (1280x720) & train fwd+bwd at PSNR>=50dB vs FP64, 1 MI355X"
 MFMA_PEAK_F16_TFLOPS = 2500.0   # MI355X dense FP16 MFMA
+MFMA_PEAK_F32_TFLOPS = 157.3    # MI355X dense FP32 MFMA
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_mlp_mfma launch on this workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (factors from a 256 MiB device copy), see profiles/c2_render_kernels.md
@@ -113,7 +114,15 @@ def train_bench(args, dev):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.train_steps
     n = o.shape[0]
+    samples = int(ws.samples_last())
+    # forward + backward of the MLP = 3 x the forward FLOPs per sample (FP32 MFMA)
+    tflops = 3 * samples * mlp_flops_per_sample(sc.mlp_in, sc.mlp_width, sc.mlp_layers) / (ms * 1e-3) / 1e12
     return {"train_mrays_s": round(n / (ms * 1e-3) / 1e6, 3), "train_ms_per_step": round(ms, 3),
+            "train_samples_per_step": samples,
+            "train_roofline": dict(bound="mfma", dtype="f32", achieved=round(tflops, 1),
+                                   peak=MFMA_PEAK_F32_TFLOPS, unit="TFLOP/s",
+                                   frac=round(tflops / MFMA_PEAK_F32_TFLOPS, 4),
+                                   note="3 x 78,336 FLOP per sample over the whole step (fwd, adjoint, Adam)"),
             "train_rays_per_step": int(n), "train_loss_first_last": [float(losses[0]), float(losses[-1])],
             "train_config": "C3: C2 scene with FP32 features/MLP, 16 tiles x 8192 rays, MSE vs a perturbed "
                             "copy, fwd + hand-written adjoint + Adam (lr 1e-4)"}
@@ -169,7 +178,24 @@ def ref_pipeline_bench(args, dev):
     torch.cuda.synchronize()
     ms_cnn = (time.perf_counter() - t0) * 1e3 / args.ref_frames
     ms_cnn_graph = graphed_ms()
-    return {"ref_pipeline_ms_per_frame": round(ms, 3), "ref_pipeline_mrays_s": round(720 * 1280 / ms / 1e3, 2),
+    psnr = {}
+    if not args.no_cpu:
+        # the plain frame (no U-Net: the refinement network is random here) on 600 seeded pixels
+        # against the oracle chain with the reflection MLPs in FP64
+        import numpy as np
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        fr.cnn = None
+        frame = fr.render().cpu().numpy().reshape(-1, 3)
+        hi = np.sort(np.random.default_rng(31).choice(720 * 1280, 600, replace=False))
+        grid_sc = S.make_scene(0, tile_shape=osc.tile_shape, tile_size=osc.tile_size, scene_min=tuple(osc.scene_min),
+                               res=8, occupancy=0.0)
+        grid_sc.tile_index_map = osc.tile_index_map.copy()
+        ref = O.frender_subset(osc, O.OracleScene(grid_sc), params, centers, cam.origin,
+                               O.compute_rays(720, 1280, cam.intrinsic, cam.c2w), 720, 1280, fr.far, hi)
+        mse = float(np.mean((frame[hi].astype(np.float64) - ref["frame"]) ** 2))
+        psnr["ref_pipeline_psnr_vs_fp64_db"] = round(10 * np.log10(1.0 / max(mse, 1e-30)), 2)
+    return {**psnr, "ref_pipeline_ms_per_frame": round(ms, 3), "ref_pipeline_mrays_s": round(720 * 1280 / ms / 1e3, 2),
             "ref_pipeline_with_unet_ms_per_frame": round(ms_cnn, 3),
             "ref_pipeline_graphed_ms_per_frame": round(ms_graph, 3),
             "ref_pipeline_graphed_mrays_s": round(720 * 1280 / ms_graph / 1e3, 2),
@@ -179,8 +205,9 @@ def ref_pipeline_bench(args, dev):
                                    "diffuse pass, sort_by_key/assign_pixels, group boundary, 4 x (inverse_z_sampling, "
                                    "inference_mlp FP16 MFMA, integrate_points) at 640x360, bilinear upsample; "
                                    "with_unet adds the 5-channel U-Net refinement in FP16; graphed: the same frame "
-                                   "with the regrouping on the device (no host synchronisation) between the replays of "
-                                   "two HIP graphs (diffuse pass; reflection and compose); seeds 12/13/2"}
+                                   "with the regrouping on the device (no host synchronisation), all captured in one HIP "
+                                   "graph; PSNR on 600 seeded pixels of the plain frame vs the oracle chain with "
+                                   "FP64 reflection MLPs; seeds 12/13/2"}
 
 
 def c4_bench(args, dev):
@@ -253,69 +280,45 @@ def c5_bench(args, dev):
     import torch
     from snisr import _native as N
     from snisr import device as D
+    from snisr import graph as G
     from snisr import scene as S
 
     sc, cams, mesh = S.config_c5()
     ds = D.DeviceScene(sc, mfma_dtype=N.SNISR_F16)
     dm = D.DeviceMesh(mesh.vertices, mesh.faces)
-    V, H, W = len(cams), cams[0].H, cams[0].W
-    n = V * H * W
-    intr = [torch.from_numpy(c.intrinsic).to(dev) for c in cams]
-    c2w = [torch.from_numpy(c.c2w).to(dev) for c in cams]
-    origins = torch.from_numpy(np.repeat(np.stack([c.origin for c in cams]), H * W, axis=0)).to(dev)
-    rays = torch.empty((V, H, W, 3), dtype=torch.float32, device=dev)
-    t_hit = torch.empty(n, dtype=torch.float32, device=dev)
-    tri = torch.empty(n, dtype=torch.int32, device=dev)
-    ws = D.MfmaWorkspace()
-
-    def trace():
-        for v in range(V):
-            D.compute_rays(intr[v], c2w[v], H, W, out=rays[v])
-        dm.first_hit(origins, rays, image_width=W, t_out=t_hit, id_out=tri)
-
-    trace()
-    first = D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, check=True)
-    outs = {k: first[k] for k in ("rgb", "T", "n_samples", "depth")}
-    cap = D.records_for(first["max_samples_per_ray"])
-    scap = int(first["total_samples"] * 1.05) + 64
-
-    def step():
-        trace()
-        D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap,
-                      sample_capacity=scap, check=False, outputs=outs)
-
+    n = len(cams) * cams[0].H * cams[0].W
+    fg = G.FrameGraph(ds, dm, cams)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
     for e in ev:
         e.record()
-    trace()
-    D.render_mfma(ds, origins, rays, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap,
-                  sample_capacity=scap, check=False, outputs=outs, events=ev)
+    fg.eager(ev)
     torch.cuda.synchronize()
     stage = [ev[j].elapsed_time(ev[j + 1]) for j in range(6)]
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        step()
-    torch.cuda.current_stream().wait_stream(side)
-    torch.cuda.synchronize()
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
-        step()
-    g.replay()
+    fg.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.c5_steps):
-        g.replay()
+        fg.replay()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.c5_steps
-    total, over = D.mfma_stats(ws.buf)
+    total, over = D.mfma_stats(fg.ws.buf)
     assert over == 0
+    outs, origins, rays, t_hit = fg.outputs, fg.origin, fg.rays, fg.t_hit
     flops = total * mlp_flops_per_sample(sc.mlp_in, sc.mlp_width, sc.mlp_layers)
+    render_ms = float(sum(stage))
+    # algorithmic gather + I/O bytes of the render (8 corners x (32 FP16 features + FP32 sigma) per
+    # sample, direction in and colour out per ray) over the render's kernel time
+    alg_bytes = total * 8 * (32 * 2 + 4) + n * 24
+    mlp_tflops = flops / (stage[4] * 1e-3) / 1e12
     res = {"c5_mrays_s": round(n / ms / 1e3, 2), "c5_ms_per_step": round(ms, 3), "c5_rays": int(n),
-           "c5_samples": int(total),
+           "c5_samples": int(total), "c5_msamples_s": round(total / ms / 1e3, 1),
            "c5_stage_ms": {k: round(float(v), 3) for k, v in zip(
                ["schedule", "march", "tile_hist", "tile_scatter", "mlp_mfma", "reduce"], stage)},
-           "c5_mlp_tflops": round(flops / (stage[4] * 1e-3) / 1e12, 1),
+           "c5_mlp_tflops": round(mlp_tflops, 1),
+           "c5_mfma_frac": round(mlp_tflops / MFMA_PEAK_F16_TFLOPS, 4),
+           "c5_alg_gbs": round(alg_bytes / (render_ms * 1e-3) / 1e9, 1),
+           "c5_hbm_frac_alg": round(alg_bytes / (render_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "c5_march_gbs": round((total * (8 * 4 + 16) + n * 16) / (stage[1] * 1e-3) / 1e9, 1),
            "c5_config": "256 tiles (16x16x1) of 128^3 at 4%, F=32 FP16, MLP 47-128x3-3 FP16 MFMA FP32 acc, "
                         "2M-triangle BVH first hit, 4 seeded 1920x1080 views in one batch, HIP-graph replay, seed 5"}
     if not args.no_cpu:
@@ -354,6 +357,7 @@ def main():
     import torch
     from snisr import _native as N
     from snisr import device as D
+    from snisr import graph as G
     from snisr import scene as S
 
     dev = torch.device("cuda:0")
@@ -369,32 +373,19 @@ def main():
     sc, cam, mesh = S.config_c2()
     ds = D.DeviceScene(sc, mfma_dtype=N.SNISR_F16)
     dm = D.DeviceMesh(mesh.vertices, mesh.faces)
-    intr = torch.from_numpy(cam.intrinsic).to(dev)
-    c2w = torch.from_numpy(cam.c2w).to(dev)
-    origin = torch.from_numpy(cam.origin).to(dev)
     H, W = cam.H, cam.W
     n_rays = H * W
-    rays_d = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
-    t_hit = torch.empty(n_rays, dtype=torch.float32, device=dev)
-    tri_id = torch.empty(n_rays, dtype=torch.int32, device=dev)
-    ws = D.MfmaWorkspace()
     flops_per_sample = mlp_flops_per_sample(sc.mlp_in, sc.mlp_width, sc.mlp_layers)
-
-    def step(events=None):
-        D.compute_rays(intr, c2w, H, W, out=rays_d)
-        dm.first_hit(origin, rays_d, image_width=W, t_out=t_hit, id_out=tri_id)
-        return D.render_mfma(ds, origin, rays_d, t_hit=t_hit, image_width=W, workspace=ws, check=False,
-                             events=events)
-
-    # size the workspace once (exact sample count of this view), then warm up
-    first = D.render_mfma(ds, origin, D.compute_rays(intr, c2w, H, W), image_width=W, workspace=ws, check=True)
-    max_per_ray, frame_samples = first["max_samples_per_ray"], first["total_samples"]
-    del first
+    # the step (camera rays, BVH nearest hit, MFMA render) captured once in a HIP graph; the workspace
+    # is sized from an eager render of the same view
+    fg = G.FrameGraph(ds, dm, [cam])
+    origin, rays_d, t_hit = fg.origin, fg.rays.reshape(-1, 3), fg.t_hit
     for _ in range(args.warmup):
-        out = step()
+        out = fg.eager()
     torch.cuda.synchronize()
-    total_samples, overflow = D.mfma_stats(ws.buf)
+    total_samples, overflow = D.mfma_stats(fg.ws.buf)
     assert overflow == 0
+    eager_rgb = out["rgb"].clone()
 
     # per-stage kernel times from eager steps with HIP events between the stages
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(args.steps)]
@@ -404,42 +395,22 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        out = step(ev[k])
+        fg.eager(ev[k])
     torch.cuda.synchronize()
     eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    # the timed steps: the whole step (rays, mesh hit, render) captured once in a HIP graph
-    outs = {k: out[k] for k in ("rgb", "T", "n_samples", "depth")}
-    cap = D.records_for(max_per_ray)
-    scap = int(frame_samples * 1.05) + 64
-    ws.get(ds, n_rays, cap, scap)
-
-    def gstep():
-        D.compute_rays(intr, c2w, H, W, out=rays_d)
-        dm.first_hit(origin, rays_d, image_width=W, t_out=t_hit, id_out=tri_id)
-        D.render_mfma(ds, origin, rays_d, t_hit=t_hit, image_width=W, workspace=ws, capacity=cap,
-                      sample_capacity=scap, check=False, outputs=outs)
-
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        gstep()
-    torch.cuda.current_stream().wait_stream(side)
-    torch.cuda.synchronize()
-    graph = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(graph):
-        gstep()
+    # the timed steps: graph replays
     for _ in range(max(1, args.warmup)):
-        graph.replay()
+        fg.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        graph.replay()
+        fg.replay()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    assert D.mfma_stats(ws.buf)[1] == 0
-    assert torch.equal(outs["rgb"], out["rgb"])   # the replay renders the same frame
-    out = outs
+    assert not fg.overflowed()
+    out = fg.outputs
+    assert torch.equal(out["rgb"], eager_rgb)   # the replay renders the same frame
     stage = np.array([[ev[k][j].elapsed_time(ev[k][j + 1]) for j in range(6)] for k in range(args.steps)])
     stage_ms = stage.mean(0)
     mlp_ms = float(stage_ms[4])
@@ -452,6 +423,16 @@ def main():
                     traffic_unit="bytes/launch",
                     kernel="k_mlp_mfma<32,128,4,f16 feats,f16 MFMA>",
                     flop_per_launch=flops, samples_per_launch=total_samples)
+    # the traversal / compositing half on the HBM roofline: algorithmic bytes of the sigma-only march
+    # (8 sigma corners x 4 B gathered and one 16 B record written per sample, direction + mesh bound
+    # 16 B read per ray) and of the ordered reduction (16 B read per sample, 12 B written per ray)
+    march_bytes = total_samples * (8 * 4 + 16) + n_rays * 16
+    reduce_bytes = total_samples * 16 + n_rays * 12
+    march_gbs = march_bytes / (float(stage_ms[1]) * 1e-3) / 1e9
+    reduce_gbs = reduce_bytes / (float(stage_ms[5]) * 1e-3) / 1e9
+    roofline_march = dict(bound="hbm", kernel="k_march_render", achieved=round(march_gbs, 1), peak=HBM_PEAK_GBS,
+                          unit="GB/s", frac=round(march_gbs / HBM_PEAK_GBS, 4), bytes_per_launch=int(march_bytes),
+                          reduce_achieved=round(reduce_gbs, 1), reduce_frac=round(reduce_gbs / HBM_PEAK_GBS, 4))
 
     rgb = out["rgb"].cpu().numpy()
     extra = {}
@@ -527,11 +508,15 @@ def main():
             "rays": n_rays,
             "samples_per_frame": int(total_samples),
             "mesh_triangles": int(mesh.faces.shape[0]),
+            "density_unit": "stored sigma = exp(N(0,1)) / voxel_size (optical depth per voxel length); "
+                            "sample step 0.2 voxel; the march stops at T < 1e-3 or the mesh hit",
         },
+        "msamples_s": round(total_samples / (ms_per_step * 1e-3) / 1e6, 1),
         "eager_ms_per_step": round(eager_ms, 4),
         "stage_ms": {k: round(float(v), 4) for k, v in zip(
             ["schedule", "march", "tile_hist", "tile_scatter", "mlp_mfma", "reduce"], stage_ms)},
         "roofline": roofline,
+        "roofline_march": roofline_march,
         "cpu_baseline": cpu,
         **extra,
     }

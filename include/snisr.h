@@ -117,7 +117,8 @@ typedef struct snisr_render_out {
 
 const char* snisr_last_error(void);
 int snisr_version(void);
-/* sizeof of the descriptors (0 scene, 1 march params, 2 rays, 3 render out, 4 mlp query, 5 octree, 6 tuning) so that foreign
+/* sizeof of the descriptors (0 scene, 1 march params, 2 rays, 3 render out, 4 mlp query, 5 octree, 6 tuning,
+ * 7 dense grid, 8 tile mesh, 9 graph desc) so that foreign
  * bindings can check their struct layouts; no reference counterpart. */
 int64_t snisr_abi_sizeof(int32_t which);
 
@@ -203,6 +204,39 @@ int snisr_bvh_build(const float* vertices, int64_t n_vertices, const int32_t* fa
 int snisr_mesh_first_hit(const void* nodes, int32_t n_nodes, const void* tris, int64_t n_tris, int32_t depth,
                          const snisr_rays* rays, float* t_out, int32_t* id_out, void* stream);
 
+/* A9, per-tile form — ray_firstHit_inTile_kernel / ray_doubleHit_inTile_kernel
+ * (training/src/cuda_func.cu:130-444) with the per-tile per-block face lists BConFaceIdx / BConFaceNum
+ * (training/scripts/tools.py:364-379).  Each tile's num_block^3 blocks (block (x*nb + y)*nb + z) list
+ * the faces that may be hit inside them: face_num int2[n_tiles * nb^3] = (start, count) into
+ * face_idx i32[n_face_idx].  One result per (ray, visited tile): visited i32[n_rays, K] (-1 = none);
+ * rays_o / rays_d f32[n_rays, 3] are used as given.  The ray walks the tile's blocks from its AABB
+ * entry; hits need t > 0 and a hit point inside the block within 1e-5.
+ *   mode 0 (first): hits f32[n_rays, K] = nearest t of the first block with a hit (left unwritten
+ *          without a hit, as the reference leaves it), hit_face i32[n_rays, K] its face or -1;
+ *   mode 1 (double): hits f32[n_rays, K, 2] = the two nearest t (-1 where absent); hit_face unused.
+ * snisr_tile_face_lists (host, synchronous) builds the lists for a mesh: a face is listed in every
+ * block whose box, widened by max(2e-5, 1e-4 block edges), it intersects (separating-axis test), so
+ * every hit the kernel can accept is listed.  Call it with face_idx NULL to get *total, then again
+ * with room for *total entries. */
+typedef struct snisr_tile_mesh {
+  const float* vertices;     /* [V, 3] */
+  const int32_t* faces;      /* [n_faces, 3] */
+  int64_t n_faces;
+  const float* centers;      /* [n_tiles, 3] tile centres */
+  const int32_t* face_idx;   /* [n_face_idx] */
+  int64_t n_face_idx;
+  const int32_t* face_num;   /* [n_tiles * num_block^3, 2], 8-byte aligned */
+  float tile_size;
+  int32_t num_block;
+} snisr_tile_mesh;
+
+int snisr_mesh_hit_tiles(const snisr_tile_mesh* mesh, const float* rays_o, const float* rays_d, int64_t n_rays,
+                         int32_t K, const int32_t* visited, int32_t mode, float* hits, int32_t* hit_face,
+                         void* stream);
+int snisr_tile_face_lists(const float* vertices, int64_t n_vertices, const int32_t* faces, int64_t n_faces,
+                          const float* centers, int32_t n_tiles, float tile_size, int32_t num_block, int32_t* face_num,
+                          int32_t* face_idx, int64_t capacity, int64_t* total);
+
 /* A3 — rendering_diffuse_octree_fp16 (rendering/src/raydiffuse_render_octree_fp16.cu:238-287,
  * include/raydiffuse_render_octree.h:7-21) on the reference's octree scene format.  Tiles hold 4^3
  * blocks; an occupied block is a node of num_voxel^3 dilated cells (reference: 18). */
@@ -227,6 +261,27 @@ int snisr_render_octree(const snisr_octree* octree, const float* origin, const f
                         int64_t n, int32_t K, int32_t image_width, float trans_th, float early_stop, int32_t soft_hit,
                         float* frame_diffuse, float* inverse_near, int16_t* net_idx, int32_t* n_samples,
                         void* stream);
+
+/* A10 — render_diffuse_kernel (training/src/cuda_func.cu:1687-1802): the reference's plain FP32
+ * marcher over dense per-tile grids (the training tiles' Voxel-T / Node-T grids, training/scripts/tile/
+ * tile.py:415-432).  A grid slot holds num_voxel^3 dilated cells (NUM_VOXEL + 2 per side), cell
+ * (x*num_voxel + y)*num_voxel + z, of float4 (r, g, b, sigma) and an int16 flag (only 1 is sampled);
+ * sparse_to_group maps a tile id to its slot (-1: the tile is outside the group, which ends the ray
+ * as invalid).  Per ray (origin and direction f32[n,3], used as given) over visited i32[n,K]: the
+ * tile's AABB chord is marched from its entry in sample_step; flagged cells are interpolated with
+ * the OVERLAP rule and composited in FP32 until T < 1e-5.  Outputs: diffuse f32[n,3]; depth f32[n] =
+ * the first sample reached with T <= 0.5 (-1 when invalid); valid i32[n]. */
+typedef struct snisr_dense_grid {
+  const float* voxels;             /* [slots, num_voxel^3, 4], 16-byte aligned */
+  const int16_t* nodes;            /* [slots, num_voxel^3] */
+  const int32_t* sparse_to_group;  /* [n_tiles] tile -> slot or -1 */
+  const float* centers;            /* [n_tiles, 3] tile centres */
+  float tile_size, voxel_size, sample_step;
+  int32_t num_voxel;
+} snisr_dense_grid;
+
+int snisr_render_dense(const snisr_dense_grid* grid, const float* rays_o, const float* rays_d, const int32_t* visited,
+                       int64_t n, int32_t K, float* diffuse, float* depth, int32_t* valid, void* stream);
 
 /* A6 — inverse_z_sampling (rendering/src/sampling.cu:48-81, include/sampling.h:7-15).  For pixel p
  * (row-major H x W) and point j < sample_per_ray: z = 1/((offset+j) ib[p] + in[p]), step =
@@ -382,6 +437,57 @@ int snisr_unet_pack(const float* const* weights, const float* const* biases, voi
 int64_t snisr_unet_workspace_size(int32_t H, int32_t W);
 int snisr_unet_refine(const void* packed, const float* frame, const float* inverse_near, const uint8_t* boundary,
                       int32_t H, int32_t W, void* workspace, int64_t workspace_bytes, float* out, void* stream);
+
+/* Scene handle and captured frames: the native host side of the renderer.
+ *
+ * snisr_scene_create uploads a scene whose pointers are HOST memory (same descriptor, same layout and
+ * sizes as the device form: tile tables, brick_map_len brick entries, n_bricks bricks of sigma and
+ * feats, n_tiles x mlp_stride MLP parameters) into device buffers owned by the handle, packs the FP32
+ * MLP and, with mfma_dtype SNISR_F16 / SNISR_BF16, the MFMA fragments (the role of FRender.
+ * render_prepare, rendering/src/__init__.py:98-135).  snisr_scene_get_desc returns the device
+ * descriptor for the stateless operators above.
+ *
+ * snisr_graph_create builds a frame driver for H x W camera frames of that scene (the role of
+ * FRender.render, rendering/src/__init__.py:247-341): device buffers for the camera, rays, mesh
+ * bound and outputs; for the MFMA path a workspace sized from an eager render of the given camera;
+ * then the step (snisr_compute_rays, snisr_mesh_first_hit when a BVH is given, snisr_render_fp32 or
+ * snisr_render_mfma_tuned) captured once into a HIP graph on the driver's own stream.
+ * snisr_graph_render copies a new camera (host intrinsic[3], c2w[9], origin[3]; all NULL keeps the
+ * last one) and replays the graph, ordered after and before `stream` (NULL: the driver's stream).
+ * snisr_graph_outputs returns the device outputs (rgb f32[H*W,3], T, n_samples, depth, t_hit or NULL),
+ * overwritten by every replay.  snisr_graph_stats reports the last replay's samples and overflow
+ * (a view with more samples than the sized workspace renders NaN; create a new graph for it) and
+ * synchronises.  No reference counterpart for the handle types themselves. */
+typedef struct snisr_scene_handle snisr_scene_handle;
+typedef struct snisr_graph snisr_graph;
+
+typedef struct snisr_graph_desc {
+  int32_t H, W;
+  int32_t mlp_dtype;          /* SNISR_F32: the plain FP32 path; the scene's MFMA dtype: the MFMA render */
+  int32_t bvh_n_nodes;
+  const void* bvh_nodes;      /* device BVH from snisr_bvh_build, or NULL (no mesh bound) */
+  const void* bvh_tris;
+  int64_t bvh_n_tris;
+  int32_t bvh_depth;
+  int32_t records_per_ray;    /* 0: sized by the first render */
+  int64_t sample_capacity;    /* 0: sized by the first render */
+  float sample_headroom;      /* sorted-sample room over the first render's samples (e.g. 0.05) */
+  snisr_march_params params;
+} snisr_graph_desc;
+
+int snisr_scene_create(const snisr_scene* host, int32_t mfma_dtype, snisr_scene_handle** out);
+int snisr_scene_destroy(snisr_scene_handle* scene);
+const snisr_scene* snisr_scene_get_desc(const snisr_scene_handle* scene);
+int snisr_graph_create(const snisr_scene_handle* scene, const snisr_graph_desc* desc, const float* intrinsic,
+                       const float* c2w, const float* origin, snisr_graph** out);
+int snisr_graph_render(snisr_graph* graph, const float* intrinsic, const float* c2w, const float* origin,
+                       void* stream);
+int snisr_graph_outputs(const snisr_graph* graph, float** rgb, float** T, int32_t** n_samples, float** depth,
+                        float** t_hit);
+int snisr_graph_stats(snisr_graph* graph, int64_t* total, int32_t* overflow, int32_t* records_per_ray,
+                      int64_t* sample_capacity);
+int snisr_graph_synchronize(snisr_graph* graph);
+int snisr_graph_destroy(snisr_graph* graph);
 
 #ifdef __cplusplus
 }

@@ -37,11 +37,13 @@ class _ParamsC(C.Structure):
 
 
 def lib():
+    """The oracle library; SNISR_ORACLE_LIB names another build of it (the sanitizer build)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+        path = os.environ.get("SNISR_ORACLE_LIB") or _LIB_PATH
+        if path == _LIB_PATH and not os.path.exists(_LIB_PATH):
             build()
-        _lib = C.CDLL(_LIB_PATH)
+        _lib = C.CDLL(path)
         _lib.orc_expf_c.restype = C.c_float
         _lib.orc_expf_c.argtypes = [C.c_float]
         _lib.orc_num_threads.restype = C.c_int
@@ -372,3 +374,127 @@ def group_boundary(net_idx, ksize, boundary=None):
     out = np.zeros((H, W), np.uint8) if boundary is None else boundary
     lib().orc_group_boundary(_p(net), C.c_int(H), C.c_int(W), C.c_int(ksize), _p(out))
     return out
+
+
+def _region_blocks(net, max_px):
+    """Stable sort of the pixels by group and <= max_px-pixel blocks per group (sizes differing by at
+    most one, larger first), the reference's regrouping (rendering/src/__init__.py:272-291)."""
+    order = np.argsort(net, kind="stable").astype(np.int32)
+    ks = net[order]
+    heads = np.flatnonzero(np.concatenate([[True], ks[1:] != ks[:-1]])) if net.size else np.zeros(0, np.int64)
+    ends = np.concatenate([heads[1:], [net.size]])
+    bnet, bps = [], []
+    for a, b in zip(heads, ends):
+        nb = -(-(b - a) // max_px)
+        base, add = divmod(b - a, nb)
+        p = a
+        for k in range(nb):
+            bnet.append(ks[a])
+            bps.append(p)
+            p += base + (1 if k < add else 0)
+    bps.append(net.size)
+    return order, np.asarray(bnet, np.int16), np.asarray(bps, np.int32)
+
+
+def frender_subset(osc, grid: OracleScene, params_f16, centers, origin, dirs, H, W, far, hi_idx, K=30, scale=2,
+                   trans_th=0.8, early_stop_D=0.01, early_stop_R=0.01, max_px=64, quant16=False):
+    """The FRender.render frame (rendering/src/__init__.py:247-334) at a subset of its full-resolution
+    pixels ``hi_idx`` (flat y*W + x), without rendering the whole frame: the octree diffuse pass of
+    those pixels, the reflection pass (4 x 16 inverse-depth samples through the group MLP in FP64,
+    FP16-rounded activations only with ``quant16``) of the downscaled pixels their bilinear x2
+    upsampling reads (align_corners=True), and diffuse + reflection clamped to [0, 1].  ``osc`` is an
+    octree scene (snisr.scene.OctreeScene fields), ``grid`` an OracleScene with its tile grid for the
+    tile tracer, ``dirs`` the frame's f32[H*W, 3] camera directions.
+    Returns dict(frame f64[M,3], diffuse f32[M,3], spec_lo f32[L,3], lo_idx (flat DH*DW indices))."""
+    DH, DW = H // scale, W // scale
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(H * W, 3)
+    hi_idx = np.asarray(hi_idx, np.int64)
+    hy, hx = hi_idx // W, hi_idx % W
+    # torch's align_corners=True source coordinate: (in - 1) / (out - 1) * dst, in float32
+    sy = (np.float32(DH - 1) / np.float32(H - 1)) * hy.astype(np.float32)
+    sx = (np.float32(DW - 1) / np.float32(W - 1)) * hx.astype(np.float32)
+    y0 = np.floor(sy).astype(np.int64)
+    x0 = np.floor(sx).astype(np.int64)
+    y1, x1 = np.minimum(y0 + 1, DH - 1), np.minimum(x0 + 1, DW - 1)
+    ly, lx = (sy - y0).astype(np.float64), (sx - x0).astype(np.float64)
+    corners = [y0 * DW + x0, y0 * DW + x1, y1 * DW + x0, y1 * DW + x1]
+    lo_idx = np.unique(np.concatenate(corners))
+    n_lo = lo_idx.size
+    lo_full = (lo_idx // DW) * scale * W + (lo_idx % DW) * scale      # ray of the downscaled pixel
+    d_lo = np.ascontiguousarray(dirs[lo_full])
+    org = np.ascontiguousarray(origin, np.float32)
+    # diffuse pass of the downscaled pixels (first-hit inverse depth and group) and of the subset
+    _, inv_near, net, _ = render_octree(osc, org, d_lo, trace_tiles(grid, org, d_lo, K), trans_th, early_stop_D)
+    d_hi = np.ascontiguousarray(dirs[hi_idx])
+    diff_hi, _, _, _ = render_octree(osc, org, d_hi, trace_tiles(grid, org, d_hi, K), trans_th, early_stop_D)
+    # reflection pass on the downscaled pixels as a 1 x n_lo image (FRender._reflection_and_compose)
+    sample_far = (np.float32(1.0) / inv_near + np.float32(far)).astype(np.float32)
+    inv_bound = ((np.float32(1.0) / sample_far - inv_near) / np.float32(63)).astype(np.float32)
+    order, bnet, bps = _region_blocks(net, max_px)
+    T = np.ones(n_lo, np.float32)
+    col = np.zeros((n_lo, 3), np.float32)
+    for step in range(0, 64, 16):
+        pts, st = inverse_z_sampling(org, d_lo, inv_near, inv_bound, step, 16)
+        res = np.zeros((n_lo * 16, 4), np.float32)
+        inference_mlp(16, bnet, bps, order, T, centers, pts, d_lo.reshape(1, n_lo, 3), params_f16, 1.0 / far,
+                      early_stop_R, res, quant16=quant16)
+        integrate_points(res, st, np.float32(early_stop_R), T, col)
+    pos = {int(v): k for k, v in enumerate(lo_idx)}
+    s = [col[np.fromiter((pos[int(v)] for v in c), np.int64, c.size)].astype(np.float64) for c in corners]
+    lxe, lye = lx[:, None], ly[:, None]
+    up = (1 - lye) * ((1 - lxe) * s[0] + lxe * s[1]) + lye * ((1 - lxe) * s[2] + lxe * s[3])
+    frame = np.clip(diff_hi.astype(np.float64) + up, 0.0, 1.0)
+    return dict(frame=frame, diffuse=diff_hi, spec_lo=col, lo_idx=lo_idx, net_lo=net, inv_near_lo=inv_near)
+
+
+def dense_overlap(voxels, center, tile_size, voxel_size, points):
+    """OVERLAP interpolation (training/src/include/interpolation.h:120-134) of one dense grid
+    f32[nv, nv, nv, 4] at world points f32[M, 3] -> f32[M, 4]."""
+    vox = np.ascontiguousarray(voxels, np.float32)
+    M = vox.shape[0]
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    out = np.empty((p.shape[0], 4), np.float32)
+    lib().orc_dense_overlap_f32(_p(vox), C.c_int(M), _p(np.ascontiguousarray(center, np.float32)),
+                                C.c_float(tile_size), C.c_float(voxel_size), _p(p), C.c_longlong(p.shape[0]), _p(out))
+    return out
+
+
+def render_dense(voxels, nodes, sparse_to_group, centers, tile_size, voxel_size, sample_step, rays_o, rays_d,
+                 visited):
+    """FP32 restatement of render_diffuse_kernel (training/src/cuda_func.cu:1687-1802): returns
+    diffuse f32[n, 3], depth f32[n], valid i32[n]."""
+    vox = np.ascontiguousarray(voxels, np.float32)
+    M = vox.shape[1]
+    nod = np.ascontiguousarray(nodes, np.int16)
+    o = np.ascontiguousarray(rays_o, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(rays_d, np.float32).reshape(-1, 3)
+    n = d.shape[0]
+    vis = np.ascontiguousarray(visited, np.int32).reshape(n, -1)
+    dif = np.empty((n, 3), np.float32)
+    dep = np.empty(n, np.float32)
+    val = np.empty(n, np.int32)
+    lib().orc_render_dense_f32(_p(vox), _p(nod), _p(np.ascontiguousarray(sparse_to_group, np.int32)),
+                               _p(np.ascontiguousarray(centers, np.float32)), C.c_float(tile_size),
+                               C.c_float(voxel_size), C.c_float(sample_step), C.c_int(M), _p(o), _p(d), _p(vis),
+                               C.c_longlong(n), C.c_int(vis.shape[1]), _p(dif), _p(dep), _p(val))
+    return dif, dep, val
+
+
+def mesh_hit_tiles(vertices, faces, centers, tile_size, num_block, rays_o, rays_d, visited, mode=0):
+    """Restatement of ray_firstHit_inTile_kernel (mode 0) / ray_doubleHit_inTile_kernel (mode 1),
+    training/src/cuda_func.cu:130-444, testing every face near each walked block.  Returns
+    (hits f32[n, K] with NaN where unwritten, faces i32[n, K]) for mode 0, hits f32[n, K, 2] for mode 1."""
+    V = np.ascontiguousarray(vertices, np.float32)
+    Fc = np.ascontiguousarray(faces, np.int32)
+    o = np.ascontiguousarray(rays_o, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(rays_d, np.float32).reshape(-1, 3)
+    n = d.shape[0]
+    vis = np.ascontiguousarray(visited, np.int32).reshape(n, -1)
+    K = vis.shape[1]
+    hits = np.full((n, K, 2) if mode == 1 else (n, K), np.nan, np.float32)
+    face = np.full((n, K), -7, np.int32)
+    lib().orc_mesh_hit_tiles_f32(C.c_int(mode), _p(V), _p(Fc), C.c_int(Fc.shape[0]),
+                                 _p(np.ascontiguousarray(centers, np.float32)), C.c_float(tile_size),
+                                 C.c_int(num_block), _p(o), _p(d), C.c_longlong(n), C.c_int(K), _p(vis), _p(hits),
+                                 _p(face))
+    return hits if mode == 1 else (hits, face)

@@ -1402,3 +1402,233 @@ void orc_group_boundary(const int16_t* net, int H, int W, int ksize, uint8_t* bo
   }
 }
 }  // extern "C"
+
+// ------------------------------------------------------------------ A10 dense FP32 marcher
+// Restatement of render_diffuse_kernel (training/src/cuda_func.cu:1687-1802) with the OVERLAP
+// interpolation (training/src/include/interpolation.h:120-134) and trilinear_base's upper-edge
+// degradation (interpolation.h:41-116).  Grids: nv^3 cells of float4 (r, g, b, sigma) per group slot,
+// cell (x*nv + y)*nv + z; flags int16 per cell, only 1 is sampled.  Out-of-grid cells (not reachable
+// from inside a tile) read as unflagged / clamped, the device's guard.
+namespace orc {
+
+struct RGBA {
+  float v[4];
+};
+
+static inline RGBA lerp_rgba(const RGBA& a, const RGBA& b, float t) {   // interpolate1D
+  RGBA r;
+  for (int c = 0; c < 4; ++c) r.v[c] = a.v[c] * (1.0f - t) + b.v[c] * t;
+  return r;
+}
+
+static RGBA dense_overlap(const float* vox, int M, float px, float py, float pz) {
+  int i0[3] = {(int)px, (int)py, (int)pz};
+  float f[3] = {px - (float)i0[0], py - (float)i0[1], pz - (float)i0[2]};
+  bool edge[3];
+  int c0[3], c1[3];
+  for (int a = 0; a < 3; ++a) {
+    if (f[a] < 0.0f) f[a] = 0.0f;
+    edge[a] = i0[a] + 1 >= M;
+    c0[a] = std::min(std::max(i0[a], 0), M - 1);
+    c1[a] = std::min(std::max(i0[a] + 1, 0), M - 1);
+  }
+  auto cell = [&](int x, int y, int z) {
+    RGBA r;
+    const float* p = vox + 4 * (((long long)x * M + y) * M + z);
+    for (int c = 0; c < 4; ++c) r.v[c] = p[c];
+    return r;
+  };
+  // axes that stay: interpolate along x, then y, then z over the non-degenerate axes only
+  const int xs[2] = {c0[0], c1[0]}, ys[2] = {c0[1], c1[1]}, zs[2] = {c0[2], c1[2]};
+  const int nx = edge[0] ? 1 : 2, ny = edge[1] ? 1 : 2, nz = edge[2] ? 1 : 2;
+  RGBA plane[2][2];
+  for (int zi = 0; zi < nz; ++zi)
+    for (int yi = 0; yi < ny; ++yi)
+      plane[zi][yi] = nx == 2 ? lerp_rgba(cell(xs[0], ys[yi], zs[zi]), cell(xs[1], ys[yi], zs[zi]), f[0])
+                              : cell(xs[0], ys[yi], zs[zi]);
+  RGBA line[2];
+  for (int zi = 0; zi < nz; ++zi) line[zi] = ny == 2 ? lerp_rgba(plane[zi][0], plane[zi][1], f[1]) : plane[zi][0];
+  return nz == 2 ? lerp_rgba(line[0], line[1], f[2]) : line[0];
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// OVERLAP interpolation at world points p (tile centre c, edge tile_size, cell size vs)
+void orc_dense_overlap_f32(const float* vox, int M, const float* c, float tile_size, float vs, const float* p,
+                           long long n, float* out) {
+  const float half = tile_size / 2.0f, inv = 1.0f / vs;
+  for (long long i = 0; i < n; ++i) {
+    float q[3];
+    for (int a = 0; a < 3; ++a) q[a] = (p[3 * i + a] - (c[a] - half)) * inv + 0.5f;
+    const orc::RGBA r = orc::dense_overlap(vox, M, q[0], q[1], q[2]);
+    for (int k = 0; k < 4; ++k) out[4 * i + k] = r.v[k];
+  }
+}
+
+void orc_render_dense_f32(const float* voxels, const int16_t* nodes, const int32_t* sparse_to_group,
+                          const float* centers, float tile_size, float voxel_size, float sample_step, int M,
+                          const float* rays_o, const float* rays_d, const int32_t* visited, long long n, int K,
+                          float* diffuse, float* depth, int32_t* valid) {
+  const long long cells = (long long)M * M * M;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long r = 0; r < n; ++r) {
+    const orc::V3<float> o{rays_o[3 * r], rays_o[3 * r + 1], rays_o[3 * r + 2]};
+    const orc::V3<float> d{rays_d[3 * r], rays_d[3 * r + 1], rays_d[3 * r + 2]};
+    float col[3] = {0.0f, 0.0f, 0.0f}, T = 1.0f, dep = 0.0f;
+    int hit_tile = -1;
+    const float half = tile_size / 2.0f, inv = 1.0f / voxel_size;
+    for (int j = 0; j < K; ++j) {
+      const int tile = visited[r * K + j];
+      if (tile == -1 || T < 0.00001f) break;
+      const int slot = sparse_to_group[tile];
+      if (slot == -1) {
+        hit_tile = -1;
+        break;
+      }
+      const float* vox = voxels + 4 * cells * slot;
+      const int16_t* fl = nodes + cells * slot;
+      const float cc[3] = {centers[3 * tile], centers[3 * tile + 1], centers[3 * tile + 2]};
+      float near, far;
+      orc::ray_aabb(o, d, orc::V3<float>{cc[0], cc[1], cc[2]}, orc::V3<float>{half, half, half}, near, far);
+      float z = near;
+      while (z < far) {
+        const float p[3] = {o.x + z * d.x, o.y + z * d.y, o.z + z * d.z};
+        int idx[3];
+        bool inside = true;
+        for (int a = 0; a < 3; ++a) {
+          idx[a] = (int)((p[a] - (cc[a] - half - voxel_size)) * inv);
+          inside = inside && idx[a] >= 0 && idx[a] < M;
+        }
+        if (!inside || fl[((long long)idx[0] * M + idx[1]) * M + idx[2]] != 1) {
+          z = z + sample_step;
+          continue;
+        }
+        if (hit_tile == -1 && T <= 0.5f) {
+          dep = z;
+          hit_tile = tile;
+        }
+        float q[3];
+        for (int a = 0; a < 3; ++a) q[a] = (p[a] - (cc[a] - half)) * inv + 0.5f;
+        const orc::RGBA v = orc::dense_overlap(vox, M, q[0], q[1], q[2]);
+        const float interval = std::min(sample_step, far - z);
+        const float alpha = 1.0f - orc_expf(-1.0f * v.v[3] * interval);
+        const float w = alpha * T;
+        for (int c = 0; c < 3; ++c) col[c] = col[c] + w * v.v[c];
+        T = T * (1.0f - alpha);
+        if (T < 0.00001f) break;
+        z = z + sample_step;
+      }
+    }
+    valid[r] = hit_tile != -1 ? 1 : 0;
+    depth[r] = hit_tile != -1 ? dep : -1.0f;
+    for (int c = 0; c < 3; ++c) diffuse[3 * r + c] = col[c];
+  }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ A9 per-tile first / double hit
+// Restatement of ray_firstHit_inTile_kernel / ray_doubleHit_inTile_kernel (training/src/cuda_func.cu:
+// 130-444) without face lists: in every block of the walk, every face whose bounding box comes
+// within 2e-5 of the block is tested (a face farther away cannot produce a hit point within the
+// kernel's 1e-5 acceptance band), in face order, with the reference's acceptance and update rules.
+extern "C" {
+
+void orc_mesh_hit_tiles_f32(int mode, const float* V, const int* Fc, int nf, const float* centers, float tile_size,
+                            int nb, const float* rays_o, const float* rays_d, long long n_rays, int K,
+                            const int32_t* visited, float* hits, int32_t* hit_face) {
+  std::vector<float> fb(6 * (size_t)nf);   // face bounding boxes
+  for (int f = 0; f < nf; ++f)
+    for (int a = 0; a < 3; ++a) {
+      float lo = V[3 * Fc[3 * f] + a], hi = lo;
+      for (int k = 1; k < 3; ++k) {
+        lo = std::min(lo, V[3 * Fc[3 * f + k] + a]);
+        hi = std::max(hi, V[3 * Fc[3 * f + k] + a]);
+      }
+      fb[6 * f + a] = lo;
+      fb[6 * f + 3 + a] = hi;
+    }
+  const float BIG = 100000000.0f;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long it = 0; it < n_rays * K; ++it) {
+    if (mode == 1) {
+      hits[2 * it] = -1.0f;
+      hits[2 * it + 1] = -1.0f;
+    } else {
+      hit_face[it] = -1;
+    }
+    const int tile = visited[it];
+    if (tile == -1) continue;
+    const long long r = it / K;
+    const orc::V3<float> o{rays_o[3 * r], rays_o[3 * r + 1], rays_o[3 * r + 2]};
+    const orc::V3<float> d{rays_d[3 * r], rays_d[3 * r + 1], rays_d[3 * r + 2]};
+    const float bs = tile_size / (float)nb, half = tile_size / 2.0f, hb = bs / 2.0f;
+    const float c[3] = {centers[3 * tile], centers[3 * tile + 1], centers[3 * tile + 2]};
+    const float mc[3] = {c[0] - half, c[1] - half, c[2] - half};
+    float bl, bh;
+    orc::ray_aabb(o, d, orc::V3<float>{c[0], c[1], c[2]}, orc::V3<float>{half, half, half}, bl, bh);
+    const float od[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    int cur[3], step[3];
+    float tmax[3], tdel[3];
+    const float inv_bs = 1.0f / bs;
+    for (int a = 0; a < 3; ++a) {
+      const float no = orc::clampr(od[a] + bl * dd[a] - mc[a], 0.0f, tile_size - 0.000001f);
+      cur[a] = std::min((int)(no * inv_bs), nb - 1);
+      step[a] = dd[a] >= 0.0f ? 1 : -1;
+      float nbd = (float)(cur[a] + step[a]) * bs;
+      if (step[a] < 0) nbd += bs;
+      tmax[a] = orc::safe_div(nbd - no, dd[a]);
+      tdel[a] = orc::safe_div((float)step[a] * bs, dd[a]);
+    }
+    float near = BIG, second = BIG;
+    int face = -1;
+    for (;;) {
+      float bc[3];
+      for (int a = 0; a < 3; ++a) bc[a] = mc[a] + (float)cur[a] * bs + hb;
+      for (int f = 0; f < nf; ++f) {
+        bool close = true;
+        for (int a = 0; a < 3; ++a)
+          close = close && fb[6 * f + a] <= bc[a] + hb + 2e-5f && fb[6 * f + 3 + a] >= bc[a] - hb - 2e-5f;
+        if (!close) continue;
+        const int* ix = Fc + 3 * f;
+        const orc::V3<float> A{V[3 * ix[0]], V[3 * ix[0] + 1], V[3 * ix[0] + 2]};
+        const orc::V3<float> B{V[3 * ix[1]], V[3 * ix[1] + 1], V[3 * ix[1] + 2]};
+        const orc::V3<float> Cc{V[3 * ix[2]], V[3 * ix[2] + 1], V[3 * ix[2] + 2]};
+        const float t = orc::ray_tri(o, d, A, B, Cc);
+        if (!(t > 0.0f && t < (mode == 1 ? second : near))) continue;
+        bool inside = true;
+        for (int a = 0; a < 3; ++a) inside = inside && std::fabs(od[a] + t * dd[a] - bc[a]) - hb <= 0.00001f;
+        if (!inside) continue;
+        if (mode == 1) {
+          if (t < near) {
+            second = near;
+            near = t;
+          } else {
+            second = t;
+          }
+        } else {
+          near = t;
+          face = f;
+        }
+      }
+      if (mode == 1 ? second != BIG : near != BIG) break;
+      const int ax = tmax[0] < tmax[1] ? (tmax[0] < tmax[2] ? 0 : 2) : (tmax[1] < tmax[2] ? 1 : 2);
+      cur[ax] += step[ax];
+      tmax[ax] += tdel[ax];
+      if (cur[ax] < 0 || cur[ax] >= nb) break;
+    }
+    if (mode == 1) {
+      if (near != BIG) {
+        hits[2 * it] = near;
+        if (second != BIG) hits[2 * it + 1] = second;
+      }
+    } else if (near != BIG) {
+      hits[it] = near;
+      hit_face[it] = face;
+    }
+  }
+}
+
+}  // extern "C"

@@ -87,6 +87,9 @@ int64_t snisr_abi_sizeof(int32_t which) {
     case 4: return (int64_t)sizeof(snisr_mlp_query);
     case 5: return (int64_t)sizeof(snisr_octree);
     case 6: return (int64_t)sizeof(snisr_tuning);
+    case 7: return (int64_t)sizeof(snisr_dense_grid);
+    case 8: return (int64_t)sizeof(snisr_tile_mesh);
+    case 9: return (int64_t)sizeof(snisr_graph_desc);
     default: return -1;
   }
 }

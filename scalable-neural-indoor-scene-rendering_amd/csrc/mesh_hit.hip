@@ -206,6 +206,110 @@ __global__ __launch_bounds__(64) void k_mesh_first_hit_packet(const float4* __re
   }
 }
 
+// ---------------------------------------------------------------------------- per-tile form
+// ray_firstHit_inTile_kernel / ray_doubleHit_inTile_kernel (training/src/cuda_func.cu:130-444): one
+// thread per (ray, visited tile).  The ray enters the tile's num_block^3 block grid at its AABB entry
+// (clamped into the tile) and walks it with the Amanatides-Woo tie rules; in each block it tests the
+// block's face list (face_num[block] = (start, count) into face_idx) and accepts hits with t > 0 whose
+// point lies inside the block within 1e-5.  First-hit mode stops at the first block with a hit and
+// reports the nearest t and its face; double-hit mode keeps the two nearest accepted t and stops at
+// the first block after which both are known.
+template <bool DOUBLE>
+__global__ __launch_bounds__(256) void k_mesh_hit_tiles(snisr_tile_mesh Mh, const float* __restrict__ rays_o,
+                                                        const float* __restrict__ rays_d,
+                                                        const int32_t* __restrict__ visited, int64_t n_items, int K,
+                                                        float* __restrict__ hits, int32_t* __restrict__ hit_face) {
+  const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= n_items) return;
+  if (DOUBLE) {
+    hits[2 * item] = -1.0f;
+    hits[2 * item + 1] = -1.0f;
+  } else {
+    hit_face[item] = -1;
+  }
+  const int tile = visited[item];
+  if (tile == -1) return;
+  const int64_t ray = item / K;
+  const int nb = Mh.num_block;
+  const int2* fnum = reinterpret_cast<const int2*>(Mh.face_num) + (int64_t)tile * nb * nb * nb;
+  const V3 o = v3(rays_o[3 * ray], rays_o[3 * ray + 1], rays_o[3 * ray + 2]);
+  const V3 d = v3(rays_d[3 * ray], rays_d[3 * ray + 1], rays_d[3 * ray + 2]);
+  const float ts = Mh.tile_size;
+  const float bs = ts / (float)nb;
+  const float half = ts / 2.0f;
+  const V3 tc = v3(Mh.centers[3 * tile], Mh.centers[3 * tile + 1], Mh.centers[3 * tile + 2]);
+  const V3 mc = v3(tc.x - half, tc.y - half, tc.z - half);
+  float bl, bh;
+  ray_aabb(o, d, tc, v3(half, half, half), bl, bh);
+  V3 no = v3(o.x + bl * d.x - mc.x, o.y + bl * d.y - mc.y, o.z + bl * d.z - mc.z);
+  no = v3(clampx(no.x, 0.0f, ts - 0.000001f), clampx(no.y, 0.0f, ts - 0.000001f), clampx(no.z, 0.0f, ts - 0.000001f));
+  const float inv_bs = 1.0f / bs;
+  // entry block; the clamp into the grid only matters where rounding of no / bs could reach num_block
+  int cur[3] = {min((int)(no.x * inv_bs), nb - 1), min((int)(no.y * inv_bs), nb - 1), min((int)(no.z * inv_bs), nb - 1)};
+  int step[3];
+  float tmax[3], tdel[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float da = get(d, a);
+    step[a] = da >= 0.0f ? 1 : -1;
+    float nbd = (float)(cur[a] + step[a]) * bs;
+    if (step[a] < 0) nbd += bs;
+    tmax[a] = safe_div(nbd - get(no, a), da);
+    tdel[a] = safe_div((float)step[a] * bs, da);
+  }
+  const float hb = bs / 2.0f;
+  float near = 100000000.0f, second = 100000000.0f;
+  int face = -1;
+  while (true) {
+    // cur stays inside the block grid: the walk starts inside and ends as soon as it leaves
+    const int bidx = cur[2] + cur[1] * nb + cur[0] * nb * nb;
+    const V3 bc = v3(mc.x + (float)cur[0] * bs + hb, mc.y + (float)cur[1] * bs + hb, mc.z + (float)cur[2] * bs + hb);
+    const int2 fl = fnum[bidx];
+    for (int j = 0; j < fl.y; ++j) {
+      const int f = Mh.face_idx[fl.x + j];
+      const int32_t* fv = Mh.faces + 3 * (int64_t)f;
+      const float* a = Mh.vertices + 3 * (int64_t)fv[0];
+      const float* b = Mh.vertices + 3 * (int64_t)fv[1];
+      const float* c = Mh.vertices + 3 * (int64_t)fv[2];
+      const float t = ray_triangle(o, d, v3(a[0], a[1], a[2]), v3(b[0], b[1], b[2]), v3(c[0], c[1], c[2]));
+      if (t > 0.0f && t < (DOUBLE ? second : near)) {
+        const float dx = fabsf(o.x + t * d.x - bc.x) - hb;
+        const float dy = fabsf(o.y + t * d.y - bc.y) - hb;
+        const float dz = fabsf(o.z + t * d.z - bc.z) - hb;
+        if (dx <= 0.00001f && dy <= 0.00001f && dz <= 0.00001f) {
+          if (DOUBLE) {
+            if (t < near) {
+              second = near;
+              near = t;
+            } else {
+              second = t;
+            }
+          } else {
+            near = t;
+            face = f;
+          }
+        }
+      }
+    }
+    if (DOUBLE ? second != 100000000.0f : near != 100000000.0f) break;
+    int ax;
+    if (tmax[0] < tmax[1]) ax = tmax[0] < tmax[2] ? 0 : 2;
+    else ax = tmax[1] < tmax[2] ? 1 : 2;
+    cur[ax] += step[ax];
+    tmax[ax] += tdel[ax];
+    if (cur[0] < 0 || cur[1] < 0 || cur[2] < 0 || cur[0] >= nb || cur[1] >= nb || cur[2] >= nb) break;
+  }
+  if (DOUBLE) {
+    if (near != 100000000.0f) {
+      hits[2 * item] = near;
+      if (second != 100000000.0f) hits[2 * item + 1] = second;
+    }
+  } else if (near != 100000000.0f) {
+    hits[item] = near;
+    hit_face[item] = face;
+  }
+}
+
 int packet_width(const snisr_rays& R);  // render_fp32.hip
 
 }  // namespace snisr
@@ -231,6 +335,33 @@ int snisr_mesh_first_hit(const void* nodes, int32_t n_nodes, const void* tris, i
   else
     hipLaunchKernelGGL(k_mesh_first_hit, dim3((unsigned)((rays->n + 63) / 64)), dim3(64), 0, (hipStream_t)stream,
                        (const float4*)nodes, (const float4*)tris, *rays, t_out, id_out, pw);
+  SNISR_LAUNCHED();
+  return SNISR_OK;
+}
+
+int snisr_mesh_hit_tiles(const snisr_tile_mesh* mesh, const float* rays_o, const float* rays_d, int64_t n_rays,
+                         int32_t K, const int32_t* visited, int32_t mode, float* hits, int32_t* hit_face,
+                         void* stream) {
+  SNISR_REQUIRE(mesh, SNISR_ERR_ARG, "mesh_hit_tiles: mesh descriptor is NULL");
+  SNISR_REQUIRE(mode == 0 || mode == 1, SNISR_ERR_ARG, "mesh_hit_tiles: mode must be 0 (first) or 1 (double)");
+  SNISR_REQUIRE(n_rays >= 0 && K >= 0, SNISR_ERR_ARG, "mesh_hit_tiles: negative size");
+  const int64_t items = n_rays * K;
+  if (items == 0) return SNISR_OK;
+  SNISR_REQUIRE(rays_o && rays_d && visited && hits && (mode == 1 || hit_face), SNISR_ERR_ARG,
+                "mesh_hit_tiles: NULL pointer");
+  SNISR_REQUIRE(mesh->centers && mesh->face_num && (mesh->n_face_idx == 0 || mesh->face_idx) &&
+                    (mesh->n_faces == 0 || (mesh->faces && mesh->vertices)),
+                SNISR_ERR_ARG, "mesh_hit_tiles: mesh tables must be non-NULL");
+  SNISR_REQUIRE(mesh->num_block >= 1 && mesh->num_block <= 1024 && mesh->tile_size > 0.0f, SNISR_ERR_ARG,
+                "mesh_hit_tiles: num_block in [1, 1024] and a positive tile_size");
+  SNISR_REQUIRE(aligned(mesh->face_num, 8), SNISR_ERR_ALIGN, "mesh_hit_tiles: face_num must be 8-byte aligned");
+  const dim3 g((unsigned)((items + 255) / 256)), b(256);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_mesh_hit_tiles<true>, g, b, 0, (hipStream_t)stream, *mesh, rays_o, rays_d, visited, items, K,
+                       hits, hit_face);
+  else
+    hipLaunchKernelGGL(k_mesh_hit_tiles<false>, g, b, 0, (hipStream_t)stream, *mesh, rays_o, rays_d, visited, items,
+                       K, hits, hit_face);
   SNISR_LAUNCHED();
   return SNISR_OK;
 }

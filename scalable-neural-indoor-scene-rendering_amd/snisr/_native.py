@@ -58,6 +58,25 @@ class OctreeDesc(C.Structure):
                 ("sample_step", C.c_float), ("num_voxel", C.c_int32)]
 
 
+class DenseGridDesc(C.Structure):
+    _fields_ = [("voxels", C.c_void_p), ("nodes", C.c_void_p), ("sparse_to_group", C.c_void_p),
+                ("centers", C.c_void_p), ("tile_size", C.c_float), ("voxel_size", C.c_float),
+                ("sample_step", C.c_float), ("num_voxel", C.c_int32)]
+
+
+class TileMeshDesc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("faces", C.c_void_p), ("n_faces", C.c_int64), ("centers", C.c_void_p),
+                ("face_idx", C.c_void_p), ("n_face_idx", C.c_int64), ("face_num", C.c_void_p),
+                ("tile_size", C.c_float), ("num_block", C.c_int32)]
+
+
+class GraphDesc(C.Structure):
+    _fields_ = [("H", C.c_int32), ("W", C.c_int32), ("mlp_dtype", C.c_int32), ("bvh_n_nodes", C.c_int32),
+                ("bvh_nodes", C.c_void_p), ("bvh_tris", C.c_void_p), ("bvh_n_tris", C.c_int64),
+                ("bvh_depth", C.c_int32), ("records_per_ray", C.c_int32), ("sample_capacity", C.c_int64),
+                ("sample_headroom", C.c_float), ("params", MarchParams)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("ray_order", C.c_int32), ("mlp_threads", C.c_int32), ("sort_grid", C.c_int32),
                 ("records_per_ray", C.c_int32)]
@@ -132,6 +151,25 @@ SIGNATURES = [
     ("snisr_train_backward", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                        C.c_void_p, C.c_int32, C.c_void_p]),
+    ("snisr_render_dense", C.c_int, [C.POINTER(DenseGridDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                     C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("snisr_mesh_hit_tiles", C.c_int, [C.POINTER(TileMeshDesc), C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                       C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("snisr_tile_face_lists", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
+                                        C.c_float, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                                        C.POINTER(C.c_int64)]),
+    ("snisr_scene_create", C.c_int, [C.POINTER(SceneDesc), C.c_int32, C.POINTER(C.c_void_p)]),
+    ("snisr_scene_destroy", C.c_int, [C.c_void_p]),
+    ("snisr_scene_get_desc", C.POINTER(SceneDesc), [C.c_void_p]),
+    ("snisr_graph_create", C.c_int, [C.c_void_p, C.POINTER(GraphDesc), C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_void_p)]),
+    ("snisr_graph_render", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("snisr_graph_outputs", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    ("snisr_graph_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int64)]),
+    ("snisr_graph_synchronize", C.c_int, [C.c_void_p]),
+    ("snisr_graph_destroy", C.c_int, [C.c_void_p]),
     ("snisr_pack_mlp_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int32]),
     ("snisr_pack_mlp", C.c_int, [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p]),
 ]
@@ -149,7 +187,8 @@ def lib():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    for which, st in enumerate((SceneDesc, MarchParams, RaysDesc, RenderOut, MlpQuery, OctreeDesc, Tuning)):
+    for which, st in enumerate((SceneDesc, MarchParams, RaysDesc, RenderOut, MlpQuery, OctreeDesc, Tuning,
+                                   DenseGridDesc, TileMeshDesc, GraphDesc)):
         got = L.snisr_abi_sizeof(which)
         if got != C.sizeof(st):
             raise RuntimeError(f"snisr: ABI mismatch for {st.__name__}: C {got} vs ctypes {C.sizeof(st)}")
@@ -157,7 +196,18 @@ def lib():
     return L
 
 
+STATUS_NAMES = {-1: "SNISR_ERR_ARG", -2: "SNISR_ERR_SHAPE", -3: "SNISR_ERR_ALIGN", -4: "SNISR_ERR_DEVICE",
+                -5: "SNISR_ERR_CAPACITY", -6: "SNISR_ERR_UNSUPPORTED"}
+
+
+class SnisrError(RuntimeError):
+    """A failed C-ABI call; ``status`` is the snisr_status code."""
+
+    def __init__(self, what: str, status: int, msg: str):
+        super().__init__(f"{what} failed ({STATUS_NAMES.get(status, status)}): {msg}")
+        self.status = status
+
+
 def check(rc: int, what: str):
     if rc != 0:
-        msg = lib().snisr_last_error().decode()
-        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+        raise SnisrError(what, rc, lib().snisr_last_error().decode())

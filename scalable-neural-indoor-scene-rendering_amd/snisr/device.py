@@ -322,3 +322,188 @@ class DeviceMesh:
         N.check(N.lib().snisr_mesh_first_hit(_ptr(self.nodes), self.n_nodes, _ptr(self.tris), self.n_tris, self.depth,
                                              C.byref(r), _ptr(t), _ptr(ids), _stream()), "mesh_first_hit")
         return t, ids
+
+
+class TileFaceMesh:
+    """A proxy mesh with per-tile per-block face lists (the reference's BConFaceIdx / BConFaceNum,
+    training/scripts/tools.py:364-379), built on the host by snisr_tile_face_lists, for the per-tile
+    first / double hits of ray_firstHit_inTile_kernel / ray_doubleHit_inTile_kernel
+    (training/src/cuda_func.cu:130-444)."""
+
+    def __init__(self, vertices: np.ndarray, faces: np.ndarray, centers: np.ndarray, tile_size: float,
+                 num_block: int = 32, device="cuda", face_idx=None, face_num=None):
+        L = N.lib()
+        V = np.ascontiguousarray(vertices, np.float32)
+        F = np.ascontiguousarray(faces, np.int32)
+        ctr = np.ascontiguousarray(centers, np.float32).reshape(-1, 3)
+        nb = int(num_block)
+        if face_num is None:
+            fnum = np.empty((ctr.shape[0] * nb ** 3, 2), np.int32)
+            total = C.c_int64(0)
+            args = (V.ctypes.data_as(C.c_void_p), V.shape[0], F.ctypes.data_as(C.c_void_p), F.shape[0],
+                    ctr.ctypes.data_as(C.c_void_p), ctr.shape[0], float(tile_size), nb,
+                    fnum.ctypes.data_as(C.c_void_p))
+            N.check(L.snisr_tile_face_lists(*args, None, 0, C.byref(total)), "tile_face_lists")
+            fidx = np.empty(max(total.value, 1), np.int32)
+            N.check(L.snisr_tile_face_lists(*args, fidx.ctypes.data_as(C.c_void_p), fidx.size, C.byref(total)),
+                    "tile_face_lists")
+            fidx = fidx[:total.value]
+        else:   # lists loaded from files, as the reference's load_preprocess reads them
+            fidx = np.ascontiguousarray(face_idx, np.int32).reshape(-1)
+            fnum = np.ascontiguousarray(face_num, np.int32).reshape(-1, 2)
+        self.face_idx_host, self.face_num_host = fidx, fnum
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
+        self.vertices, self.faces, self.centers = t(V), t(F), t(ctr)
+        self.face_idx = t(fidx if fidx.size else np.zeros(1, np.int32))
+        self.face_num = t(fnum)
+        d = N.TileMeshDesc()
+        d.vertices, d.faces, d.n_faces = self.vertices.data_ptr(), self.faces.data_ptr(), F.shape[0]
+        d.centers, d.face_idx, d.n_face_idx = self.centers.data_ptr(), self.face_idx.data_ptr(), fidx.size
+        d.face_num, d.tile_size, d.num_block = self.face_num.data_ptr(), float(tile_size), nb
+        self.desc = d
+
+    def hits(self, rays_o: torch.Tensor, rays_d: torch.Tensor, visited: torch.Tensor, mode: int = 0,
+             hits: torch.Tensor | None = None, hit_face: torch.Tensor | None = None):
+        """mode 0: (hits f32[n, K], faces i32[n, K]); mode 1: hits f32[n, K, 2].  ``hits`` may be given
+        (mode 0 leaves entries without a hit as they were, like the reference)."""
+        _require_cuda(rays_o, "rays_o", torch.float32)
+        _require_cuda(rays_d, "rays_d", torch.float32)
+        _require_cuda(visited, "visited", torch.int32)
+        n = rays_d.numel() // 3
+        K = visited.shape[-1]
+        if rays_o.numel() != 3 * n or visited.numel() != n * K:
+            raise ValueError("rays_o / rays_d must be [n, 3] and visited [n, K]")
+        dev = rays_d.device
+        if hits is None:
+            hits = torch.full((n, K, 2) if mode == 1 else (n, K), -1.0, dtype=torch.float32, device=dev)
+        _require_cuda(hits, "hits", torch.float32)
+        if hits.numel() != n * K * (2 if mode == 1 else 1):
+            raise ValueError("hits must hold n*K (mode 0) or n*K*2 (mode 1) values")
+        if mode == 0 and hit_face is None:
+            hit_face = torch.empty((n, K), dtype=torch.int32, device=dev)
+        if hit_face is not None:
+            _require_cuda(hit_face, "hit_face", torch.int32)
+        N.check(N.lib().snisr_mesh_hit_tiles(C.byref(self.desc), _ptr(rays_o), _ptr(rays_d), n, K, _ptr(visited),
+                                             int(mode), _ptr(hits), _ptr(hit_face), _stream()), "mesh_hit_tiles")
+        return hits if mode == 1 else (hits, hit_face)
+
+
+class NativeScene:
+    """A scene uploaded by the library itself (snisr_scene_create: device copies owned by the handle,
+    FP32 and optionally MFMA weight packing), from the host arrays of a ``Scene``."""
+
+    def __init__(self, sc: Scene, mfma_dtype: int = N.SNISR_F32):
+        self._keep = [np.ascontiguousarray(a) for a in (sc.tile_index_map, sc.tile_centers, sc.tile_res,
+                                                        sc.tile_brick_base, sc.brick_map, sc.sigma, sc.feats,
+                                                        sc.mlp_params)]
+        tim, ctr, res, base, bmap, sig, feat, mlp = self._keep
+        d = N.SceneDesc()
+        d.tile_shape[:] = [int(v) for v in sc.tile_shape]
+        d.scene_min[:] = [float(v) for v in sc.scene_min]
+        d.tile_size = sc.tile_size
+        d.n_tiles, d.n_bricks, d.feat_dim = sc.n_tiles, sc.n_bricks, sc.feat_dim
+        d.feat_dtype = N.SNISR_F16 if feat.dtype == np.float16 else N.SNISR_F32
+        d.mlp_in, d.mlp_width, d.mlp_layers = sc.mlp_in, sc.mlp_width, sc.mlp_layers
+        d.mlp_stride = mlp.shape[1]
+        d.brick_map_len = bmap.size
+        for name, a in (("tile_index_map", tim), ("tile_centers", ctr), ("tile_res", res),
+                        ("tile_brick_base", base), ("brick_map", bmap), ("sigma", sig), ("feats", feat),
+                        ("mlp_params", mlp)):
+            setattr(d, name, a.ctypes.data)
+        self.handle = C.c_void_p()
+        N.check(N.lib().snisr_scene_create(C.byref(d), int(mfma_dtype), C.byref(self.handle)), "scene_create")
+        self.host, self.mfma_dtype = sc, mfma_dtype
+
+    @property
+    def desc(self):
+        return N.lib().snisr_scene_get_desc(self.handle).contents
+
+    def close(self):
+        if self.handle:
+            N.lib().snisr_scene_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NativeGraph:
+    """The library's frame driver (snisr_graph_*): camera rays, optional BVH bound and the render of
+    H x W frames captured once in a HIP graph; ``render(cam)`` replays it for a new camera and returns
+    the device outputs as tensors that alias the driver's buffers (overwritten by the next replay)."""
+
+    def __init__(self, scene: NativeScene, cam, mlp_dtype: int = N.SNISR_F16, mesh: "DeviceMesh | None" = None,
+                 sample_headroom: float = 0.05, records_per_ray: int = 0, sample_capacity: int = 0, **march_kw):
+        g = N.GraphDesc()
+        g.H, g.W, g.mlp_dtype = cam.H, cam.W, int(mlp_dtype)
+        g.records_per_ray, g.sample_capacity = int(records_per_ray), int(sample_capacity)
+        if mesh is not None:
+            g.bvh_nodes, g.bvh_tris = mesh.nodes.data_ptr(), mesh.tris.data_ptr()
+            g.bvh_n_nodes, g.bvh_n_tris, g.bvh_depth = mesh.n_nodes, mesh.n_tris, mesh.depth
+        g.sample_headroom = float(sample_headroom)
+        sc = scene.host
+        g.params.max_tiles = march_kw.get("max_tiles", sc.max_tiles)
+        g.params.sample_step = march_kw.get("sample_step", sc.sample_step)
+        g.params.early_stop = march_kw.get("early_stop", sc.early_stop)
+        g.params.trans_th = march_kw.get("trans_th", sc.trans_th)
+        self.scene, self.mesh, self.H, self.W = scene, mesh, cam.H, cam.W
+        self.handle = C.c_void_p()
+        *cam_args, keep = self._camera(cam)  # noqa: F841 (keep: the arrays behind the pointers)
+        N.check(N.lib().snisr_graph_create(scene.handle, C.byref(g), *cam_args, C.byref(self.handle)),
+                "graph_create")
+        ptrs = [C.c_void_p() for _ in range(5)]
+        N.check(N.lib().snisr_graph_outputs(self.handle, *[C.byref(p) for p in ptrs]), "graph_outputs")
+        self.ptrs = [p.value for p in ptrs]
+
+    @staticmethod
+    def _camera(cam):
+        a = [np.ascontiguousarray(v, np.float32) for v in (cam.intrinsic, cam.c2w, cam.origin)]
+        return [x.ctypes.data_as(C.c_void_p) for x in a] + [a]
+
+    def render(self, cam=None):
+        if cam is None:
+            args = [None, None, None]
+        else:
+            *args, keep = self._camera(cam)  # noqa: F841
+        N.check(N.lib().snisr_graph_render(self.handle, *args, _stream()), "graph_render")
+
+    def outputs(self):
+        """rgb [H*W,3], T, n_samples, depth (and t_hit when a mesh is used): device tensors aliasing the
+        driver's buffers (through __cuda_array_interface__), overwritten by the next replay."""
+        N.check(N.lib().snisr_graph_synchronize(self.handle), "graph_synchronize")
+        n = self.H * self.W
+        out = {}
+        for name, p, shape, ts in (("rgb", self.ptrs[0], (n, 3), "<f4"), ("T", self.ptrs[1], (n,), "<f4"),
+                                   ("n_samples", self.ptrs[2], (n,), "<i4"), ("depth", self.ptrs[3], (n,), "<f4"),
+                                   ("t_hit", self.ptrs[4], (n,), "<f4")):
+            if p:
+                out[name] = torch.as_tensor(_DeviceArray(p, shape, ts), device="cuda")
+        return out
+
+    def stats(self):
+        total, over, cpr, cap = C.c_int64(0), C.c_int32(0), C.c_int32(0), C.c_int64(0)
+        N.check(N.lib().snisr_graph_stats(self.handle, C.byref(total), C.byref(over), C.byref(cpr), C.byref(cap)),
+                "graph_stats")
+        return dict(total=total.value, overflow=over.value, records_per_ray=cpr.value, sample_capacity=cap.value)
+
+    def close(self):
+        if self.handle:
+            N.lib().snisr_graph_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _DeviceArray:
+    """A device buffer of the library described by __cuda_array_interface__ (no copy)."""
+
+    def __init__(self, ptr: int, shape, typestr: str):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr, "data": (int(ptr), False),
+                                         "version": 2, "strides": None}

@@ -19,6 +19,7 @@ Differences from the reference, all deliberate:
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 import torch
@@ -187,22 +188,22 @@ _packed_cache: dict = {}
 
 def pack_inference_params(params: torch.Tensor) -> torch.Tensor:
     """The MFMA fragment layout of the reference's per-group parameters (22,556 halves per group),
-    cached per parameter tensor and version."""
+    cached per parameter tensor object and version.  The cache holds a weak reference to the tensor,
+    so a new tensor that reuses a freed tensor's memory (and id) never finds the old packing."""
     _require_cuda(params, "params", torch.float16)
     if params.numel() % REF_MLP_PARAMS:
         raise ValueError(f"params must hold a multiple of {REF_MLP_PARAMS} halves")
     G = params.numel() // REF_MLP_PARAMS
-    key = (params.device, params.data_ptr(), params.numel())
-    hit = _packed_cache.get(key)
-    if hit is not None and hit[0] == params._version:
-        return hit[1]
+    hit = _packed_cache.get(id(params))
+    if hit is not None and hit[0]() is params and hit[1] == params._version:
+        return hit[2]
     nbytes = N.lib().snisr_inference_mlp_packed_size(G)
     packed = torch.empty(max(nbytes, 16), dtype=torch.uint8, device=params.device)
     N.check(N.lib().snisr_pack_inference_mlp(_ptr(params), G, _ptr(packed), _stream()), "pack_inference_mlp")
-    _packed_cache.clear()
-    _packed_cache[key] = (params._version, packed)
+    for k in [k for k, v in _packed_cache.items() if v[0]() is None]:
+        del _packed_cache[k]
+    _packed_cache[id(params)] = (weakref.ref(params), params._version, packed)
     return packed
-
 
 
 def inference_mlp(max_samples_per_ray: int, num_block: int, num_thread: int, inverse_far: float, trans_stop: float,

@@ -56,8 +56,26 @@ class FRender:
         self.far = float(np.linalg.norm(osc.tile_size * np.asarray(osc.tile_shape, np.float64)))
         self.inverse_far = 1.0 / self.far
         self.ksize_boundary = 3
+        self.max_depth = float(np.max(np.asarray(osc.tile_shape, np.float64) * osc.tile_size))
+        self.render_mode = 1
+        self.enable_Unet = True
         self.cnn = None
         self.frame = None
+
+    def set_mode(self, mode_type: int):
+        """Render mode of the reference (rendering/src/__init__.py:199-200, 296-303, 330-332): 1 the
+        full frame, 2 the diffuse layer only, 3 the upsampled reflection layer only, 4 depth
+        (1 / inverse_near / max_depth in all three channels).  A captured frame is recaptured."""
+        if mode_type not in (1, 2, 3, 4):
+            raise ValueError(f"render mode must be 1, 2, 3 or 4, got {mode_type}")
+        self.render_mode = int(mode_type)
+        self._graph = None
+
+    def set_unet(self, flag: bool):
+        """Switches the refinement network on or off (rendering/src/__init__.py:201-202); it runs only
+        once loaded with load_cnn."""
+        self.enable_Unet = bool(flag)
+        self._graph = None
 
     def load_cnn(self, state_dict=None, half: bool = True, backend: str = "hip"):
         """Enables the screen-space refinement network (the reference's CNN / Unet,
@@ -149,11 +167,10 @@ class FRender:
         return mb, query, block_net, block_starts, nblk
 
     def render_graphed(self):
-        """render_device with its two kernel sequences captured in HIP graphs: the diffuse pass and the
-        reflection / composition pass each replay as one graph; the regrouping between them runs
-        eagerly (its radix sort clears device memory with memset operations, which are kept out of
-        graphs), still without a host synchronisation.  The returned frame tensor is the second
-        graph's output buffer, overwritten by the next replay."""
+        """render_device captured once in a HIP graph and replayed: the diffuse pass, the device-side
+        regrouping (its sort and scans clear their counters with their own kernels, no memset) and the
+        reflection / composition pass are one graph.  The returned frame tensor is the graph's output
+        buffer, overwritten by the next replay; set_camera between replays renders the new view."""
         if getattr(self, "_graph", None) is None:
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
@@ -161,22 +178,12 @@ class FRender:
                 self.render_device()
             torch.cuda.current_stream().wait_stream(side)
             torch.cuda.synchronize()
-            with torch.no_grad():
-                ga = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(ga):
-                    a_out = self._diffuse()
-                a_keep = (a_out, self.frame_diffuse, self.large_inverse_near, self.large_netIdx)
-                mb, query, block_net, block_starts, nblk = self._regroup(a_out[2])
-                torch.cuda.synchronize()
-                gc = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(gc):
-                    out = self._reflection_and_compose(a_out[0], a_out[1], mb, block_net, query, block_starts, nblk)
-            self._graph = (ga, gc, a_keep, out)
-        ga, gc, a_keep, out = self._graph
-        ga.replay()
-        with torch.no_grad():
-            self._regroup(a_keep[0][2])
-        gc.replay()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                out = self.render_device()
+            self._graph = (g, out)
+        g, out = self._graph
+        g.replay()
         self.frame = out
         return out
 
@@ -208,6 +215,12 @@ class FRender:
         H, W, s = self.H, self.W, self.reflection_scale
         self.boundary = torch.zeros((H, W, 1), dtype=torch.bool, device=dev)
         FR.compute_group_boundary(self.large_netIdx, self.boundary, self.ksize_boundary, self.num_thread * 2)
+        if self.render_mode == 2:
+            self.frame = self.frame_diffuse
+            return self.frame
+        if self.render_mode == 4:
+            self.frame = (1 / self.large_inverse_near[..., None] / self.max_depth).repeat(1, 1, 3)
+            return self.frame
         sample_far = 1.0 / inverse_near + self.far
         inverse_bound = (1.0 / sample_far - inverse_near) / 63
         transparency = torch.ones((self.DH, self.DW, 1), device=dev, dtype=torch.float32)
@@ -223,10 +236,14 @@ class FRender:
                              transparency, query, new_starts, self.group_centers, samples, rays_ds, self.net_params,
                              sample_results, n_blocks_device=n_blocks_device)
             FR.integrate_points(sample_results, steps, self.num_thread * 2, self.early_stop_R, transparency, spec)
+        self.frame_speculars_lowres = spec
         self.frame_speculars = F.interpolate(spec[None].permute(0, 3, 1, 2), scale_factor=s, mode="bilinear",
                                              align_corners=True)[0].permute(1, 2, 0)
+        if self.render_mode == 3:
+            self.frame = self.frame_speculars
+            return self.frame
         self.frame = torch.clamp(self.frame_diffuse + self.frame_speculars, 0, 1)
-        if self.cnn is not None:
+        if self.cnn is not None and self.enable_Unet:
             from .cnn import HipUnet, refine
             if isinstance(self.cnn, HipUnet):
                 self.frame = self.cnn(self.frame.contiguous(), self.large_inverse_near.contiguous(),

@@ -148,3 +148,120 @@ def pack_reflection_mlp(state_dict) -> np.ndarray:
     flat = np.concatenate(out).astype(np.float16)
     assert flat.size == REF_MLP_PARAMS
     return flat
+
+
+# ---------------------------------------------------------------------------- training checkpoints
+# The training side writes, per tile, its dense dilated grid Voxel-T{tile}-E{epoch}.npy (f32
+# [nv, nv, nv, 4] in x, y, z, channel order, nv = NUM_VOXEL + 2) and cell flags Node-T{tile}-E{epoch}.npy
+# (int [nv, nv, nv], 1 sampled, 0 value only, -1 empty) (training/scripts/tile/tile.py:415-432), and
+# per group the reflection MLP's state dict Coeffi-TShared-E{epoch}.pt and its centre group_center.npy
+# (training/scripts/tile/tileGroup.py:65-72, 531-552).  The exporter below turns them into the
+# renderData.npz octree format: every tile cut into 4^3 blocks of NUM_VOXEL / 4 voxels; a block with a
+# sampled voxel becomes a node holding the (NUM_VOXEL / 4 + 2)^3 window of the tile's dilated grid
+# that starts at the block (its cells plus one cell of overlap on every side); a node cell holding a
+# value (flag >= 0) gets the next index of the node's voxel list and is sampled when its flag is 1.
+
+
+def octree_from_dense(voxels: np.ndarray, nodes: np.ndarray, tile_cells, tile_shape, tile_size: float, scene_min,
+                      groups, sample_step_scale: float = 0.2) -> OctreeScene:
+    """OctreeScene of dense training grids: voxels f32[T, nv, nv, nv, 4], nodes int[T, nv, nv, nv] of the
+    T tiles at dense tile cells ``tile_cells`` (x*Ty*Tz + y*Tz + z) of ``tile_shape``; groups i32[T]."""
+    voxels = np.asarray(voxels, np.float32)
+    nodes = np.asarray(nodes)
+    T, nv = voxels.shape[0], voxels.shape[1]
+    if voxels.shape != (T, nv, nv, nv, 4) or nodes.shape != (T, nv, nv, nv):
+        raise ValueError(f"voxels must be [T, nv, nv, nv, 4] and nodes [T, nv, nv, nv], got {voxels.shape} / "
+                         f"{nodes.shape}")
+    NV = nv - 2
+    nb = 1 << OCTREE_DEPTH
+    if NV % nb:
+        raise ValueError(f"NUM_VOXEL {NV} is not a multiple of {nb}")
+    bs = NV // nb
+    M = bs + 2
+    tile_shape = tuple(int(v) for v in tile_shape)
+    cells = np.asarray(tile_cells, np.int64)
+    if cells.shape != (T,) or cells.min(initial=0) < 0 or cells.max(initial=-1) >= int(np.prod(tile_shape)):
+        raise ValueError("tile_cells must hold one dense tile cell per tile")
+    tim = np.full(int(np.prod(tile_shape)), -1, np.int32)
+    tim[cells] = np.arange(T, dtype=np.int32)
+    smin = np.asarray(scene_min, np.float32).reshape(3)
+    ts = np.float32(tile_size)
+    ix, iy, iz = cells // (tile_shape[1] * tile_shape[2]), (cells // tile_shape[2]) % tile_shape[1], cells % tile_shape[2]
+    centers = (smin + (np.stack([ix, iy, iz], 1).astype(np.float32) + np.float32(0.5)) * ts).astype(np.float32)
+    bmap = np.full(T * nb ** 3, -1, np.int32)
+    n_idx, n_flag, starts, data = [], [], [], []
+    count = 0
+    for t in range(T):
+        f = nodes[t]
+        v = voxels[t]
+        for b in range(nb ** 3):
+            bx, by, bz = b // (nb * nb), (b // nb) % nb, b % nb
+            inner = f[1 + bs * bx:1 + bs * (bx + 1), 1 + bs * by:1 + bs * (by + 1), 1 + bs * bz:1 + bs * (bz + 1)]
+            if not (inner == 1).any():
+                continue
+            win = (slice(bs * bx, bs * bx + M), slice(bs * by, bs * by + M), slice(bs * bz, bs * bz + M))
+            wf = f[win].reshape(-1)
+            has = wf >= 0
+            k = int(has.sum())
+            if k > 32767:
+                raise ValueError("a node holds more voxels than an int16 index reaches")
+            idx = np.full(M ** 3, -1, np.int16)
+            idx[has] = np.arange(k, dtype=np.int16)
+            bmap[t * nb ** 3 + b] = len(starts)
+            n_idx.append(idx)
+            n_flag.append(wf == 1)
+            starts.append(count)
+            data.append(v[win].reshape(-1, 4)[has])
+            count += k
+    data_v = (np.concatenate(data) if data else np.zeros((1, 4), np.float32)).astype(np.float16)
+    if data_v.shape[0] == 0:
+        data_v = np.zeros((1, 4), np.float16)
+    vs = float(np.float64(tile_size) / NV)
+    return OctreeScene(tile_shape=tile_shape, tile_size=float(tile_size), scene_min=smin, tile_index_map=tim,
+                       centers=centers, group_map=np.asarray(groups, np.int32).reshape(T),
+                       block_index_map=bmap,
+                       nodes_index_map=np.concatenate(n_idx) if n_idx else np.full(M ** 3, -1, np.int16),
+                       nodes_sample_flag=np.concatenate(n_flag) if n_flag else np.zeros(M ** 3, bool),
+                       voxels_start=np.asarray(starts, np.int64) if starts else np.zeros(1, np.int64),
+                       data_voxels=data_v, voxel_size=vs, sample_step=float(np.float64(sample_step_scale) * vs),
+                       num_voxel=M)
+
+
+def export_checkpoint(out_path, groups, tile_shape, tile_size: float, scene_min, epoch: int,
+                      sample_step_scale: float = 0.2) -> RenderData:
+    """Writes renderData.npz from training outputs.  ``groups``: one dict per group with ``name``,
+    ``logdir`` (holding Coeffi-TShared-E{epoch}.pt and optionally group_center.npy), ``tile_dir``
+    (holding the Voxel-T / Node-T files) and ``tiles``: a list of (file tile index, dense tile cell).
+    The state dict is read with torch.load(weights_only=True); without group_center.npy the centre is
+    the mean of the group's tile centres (tileGroup.py:65)."""
+    import os
+
+    import torch
+    vox, nod, cells, gidx, params, gcent, names = [], [], [], [], [], [], []
+    smin = np.asarray(scene_min, np.float32).reshape(3)
+    for g, grp in enumerate(groups):
+        tc = []
+        for tile_idx, cell in grp["tiles"]:
+            v = np.load(os.path.join(grp["tile_dir"], f"Voxel-T{tile_idx}-E{epoch}.npy"), allow_pickle=False)
+            f = np.load(os.path.join(grp["tile_dir"], f"Node-T{tile_idx}-E{epoch}.npy"), allow_pickle=False)
+            vox.append(np.asarray(v, np.float32))
+            nod.append(np.asarray(f).astype(np.int16))
+            cells.append(int(cell))
+            gidx.append(g)
+            c = int(cell)
+            ijk = np.array([c // (tile_shape[1] * tile_shape[2]), (c // tile_shape[2]) % tile_shape[1],
+                            c % tile_shape[2]], np.float32)
+            tc.append(smin + (ijk + np.float32(0.5)) * np.float32(tile_size))
+        sd = torch.load(os.path.join(grp["logdir"], f"Coeffi-TShared-E{epoch}.pt"), map_location="cpu",
+                        weights_only=True)
+        params.append(pack_reflection_mlp(sd))
+        cpath = os.path.join(grp["logdir"], "group_center.npy")
+        gcent.append(np.load(cpath, allow_pickle=False).reshape(3) if os.path.exists(cpath) else np.mean(tc, axis=0))
+        names.append(grp.get("name", f"group-{g}"))
+    order = np.argsort(cells, kind="stable")          # tiles in dense cell order
+    osc = octree_from_dense(np.stack(vox)[order], np.stack(nod)[order], np.asarray(cells)[order], tile_shape,
+                            tile_size, scene_min, np.asarray(gidx, np.int32)[order], sample_step_scale)
+    rd = RenderData(osc, np.concatenate(params).astype(np.float16), np.asarray(gcent, np.float32),
+                    np.asarray(names, dtype=str))
+    save_render_data(out_path, osc, rd.net_params, rd.group_centers, rd.group_names)
+    return rd

@@ -564,3 +564,89 @@ def make_octree_scene(seed: int, tile_shape=(4, 4, 1), tile_size=1.0, scene_min=
                        np.concatenate(nodes_flag) if nodes_flag else np.zeros(M ** 3, bool),
                        np.asarray(starts, np.int64) if starts else np.zeros(1, np.int64), data_v, vs,
                        float(np.float32(sample_step_scale) * np.float32(vs)))
+
+
+# ---------------------------------------------------------------------------- dense training grids
+# The training tiles' own format (training/scripts/tile/tile.py:415-432, written per tile as
+# Voxel-T{tile}-E{epoch}.npy and Node-T{tile}-E{epoch}.npy): a dilated grid of NUM_VOXEL + 2 cells per
+# side (one cell of overlap with the neighbours), float4 (r, g, b, sigma) in x, y, z, channel order and
+# an int16 flag per cell: 1 sampled, 0 a value kept for interpolation only, -1 empty (zero value).
+
+@dataclasses.dataclass
+class DenseScene:
+    tile_shape: tuple
+    tile_size: float
+    scene_min: np.ndarray        # f32[3]
+    num_voxel: int               # NUM_VOXEL (voxels per tile side, without the overlap)
+    tile_ids: np.ndarray         # i32[T] dense tile cell of each stored tile (x*Ty*Tz + y*Tz + z)
+    centers: np.ndarray          # f32[T, 3]
+    voxels: np.ndarray           # f32[T, nv, nv, nv, 4], nv = num_voxel + 2
+    nodes: np.ndarray            # i16[T, nv, nv, nv]
+    groups: np.ndarray           # i32[T]
+
+    @property
+    def voxel_size(self) -> float:
+        return float(np.float32(self.tile_size) / np.float32(self.num_voxel))
+
+    @property
+    def tile_index_map(self) -> np.ndarray:
+        m = np.full(int(np.prod(self.tile_shape)), -1, np.int32)
+        m[self.tile_ids] = np.arange(self.tile_ids.size, dtype=np.int32)
+        return m
+
+
+def make_dense_scene(seed: int, tile_shape=(2, 2, 1), tile_size=1.0, scene_min=(0.0, 0.0, 0.0), num_voxel=64,
+                     block=8, block_occupancy=0.25, voxel_occupancy=0.6, n_groups=2, drop_tiles=0) -> DenseScene:
+    """Seeded scene in the training tiles' dense format.  A global voxel grid over the whole scene is
+    occupied by blocks ~ Bernoulli(block_occupancy), voxels of an occupied block ~ Bernoulli(
+    voxel_occupancy); occupied voxels are flagged 1, their 26-neighbours that are not occupied 0 (they
+    keep a value, as the training bakes values wherever the flag is not -1), the rest -1 with zero
+    value; rgb ~ U(0, 1), sigma = exp(N(0, 1)) / voxel_size.  Each tile stores the window of its
+    voxels plus one cell of overlap on every side; overlap cells are flagged 0 where they hold a value
+    (the training tiles keep only their interior flags, tile.py:405-413).  ``drop_tiles`` tiles are left
+    out (empty tiles of a floor plan)."""
+    rs = np.random.SeedSequence(seed).spawn(4)
+    rng_b, rng_v, rng_d, rng_t = (np.random.Generator(np.random.PCG64(r)) for r in rs)
+    tile_shape = tuple(int(v) for v in tile_shape)
+    NV = int(num_voxel)
+    vs = float(np.float32(tile_size) / np.float32(NV))
+    G = np.asarray(tile_shape) * NV
+    nb = G // block
+    occ_b = rng_b.random(tuple(nb)) < block_occupancy
+    occ = np.repeat(np.repeat(np.repeat(occ_b, block, 0), block, 1), block, 2)
+    occ &= rng_v.random(tuple(G)) < voxel_occupancy
+    pad = np.zeros(tuple(G + 2), bool)
+    pad[1:-1, 1:-1, 1:-1] = occ
+    dil = np.zeros_like(pad)
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                dil |= np.roll(np.roll(np.roll(pad, dx, 0), dy, 1), dz, 2)
+    dil[0, :, :] = dil[-1, :, :] = False
+    dil[:, 0, :] = dil[:, -1, :] = False
+    dil[:, :, 0] = dil[:, :, -1] = False
+    flag = np.where(pad, 1, np.where(dil, 0, -1)).astype(np.int16)
+    val = np.zeros(tuple(G + 2) + (4,), np.float32)
+    has = flag >= 0
+    k = int(has.sum())
+    val[has, :3] = rng_d.random((k, 3))
+    val[has, 3] = np.exp(rng_d.standard_normal(k)) / vs
+    nt = int(np.prod(tile_shape))
+    keep = np.sort(rng_t.permutation(nt)[: nt - int(drop_tiles)]).astype(np.int32)
+    smin = np.asarray(scene_min, np.float32)
+    nv = NV + 2
+    vox = np.empty((keep.size, nv, nv, nv, 4), np.float32)
+    nodes = np.empty((keep.size, nv, nv, nv), np.int16)
+    centers = np.empty((keep.size, 3), np.float32)
+    for j, t in enumerate(keep):
+        ix, iy, iz = t // (tile_shape[1] * tile_shape[2]), (t // tile_shape[2]) % tile_shape[1], t % tile_shape[2]
+        sl = tuple(slice(c * NV, c * NV + nv) for c in (ix, iy, iz))   # padded global coordinates
+        vox[j] = val[sl]
+        f = flag[sl].copy()
+        border = np.ones((nv, nv, nv), bool)
+        border[1:-1, 1:-1, 1:-1] = False
+        f[border & (f == 1)] = 0          # overlap cells are never sampled by this tile
+        nodes[j] = f
+        centers[j] = smin + (np.array([ix, iy, iz], np.float32) + np.float32(0.5)) * np.float32(tile_size)
+    groups = (np.arange(keep.size) % max(1, n_groups)).astype(np.int32)
+    return DenseScene(tile_shape, float(tile_size), smin, NV, keep, centers, vox, nodes, groups)

@@ -13,11 +13,20 @@ from .device import DeviceScene, _attach_events, _ptr, _rays_desc, _render_outpu
 
 
 class TrainWorkspace:
-    """Workspace of one forward/backward pair (grown on demand)."""
+    """Workspace of one forward/backward pair (grown on demand).
+
+    The first forward on a ray count checks the sample capacity (one host synchronisation) and grows
+    the workspace to the batch's sample count with 25% headroom; later forwards with the same ray
+    count run without synchronising, so a training step can be captured in a HIP graph.  A forward
+    that outgrows the workspace renders NaN and is reported by ``overflowed()`` (synchronises).
+    ``generation`` counts the forwards: a backward whose forward is no longer the last one on this
+    workspace (its records were overwritten) raises."""
 
     def __init__(self, capacity_per_ray: int = 48):
         self.buf = None
         self.capacity_per_ray = capacity_per_ray
+        self.sized_for = None
+        self.generation = 0
 
     def get(self, ds: DeviceScene, n: int):
         need = N.lib().snisr_train_workspace_size(C.byref(ds.desc), n, self.capacity_per_ray * n)
@@ -27,15 +36,29 @@ class TrainWorkspace:
             self.buf = torch.empty(need, dtype=torch.uint8, device=ds.sigma.device)
         return self.buf, need
 
+    def overflowed(self) -> bool:
+        """True if the last forward had more samples than the workspace holds (synchronises)."""
+        return self.buf is not None and mfma_stats(self.buf)[1] != 0
+
+    def samples_last(self) -> int:
+        """Samples of the last forward (synchronises)."""
+        return mfma_stats(self.buf)[0] if self.buf is not None else 0
+
 
 def train_forward(ds: DeviceScene, origins, dirs, t_hit=None, workspace: TrainWorkspace | None = None,
-                  events=None, check: bool = True, **march_kw):
+                  events=None, check: bool | None = None, **march_kw):
+    """snisr_train_forward.  ``check``: verify the sample capacity after the call and redo it with a
+    grown workspace on overflow (synchronises); None = only the first time this workspace sees this
+    ray count."""
     r = _rays_desc(origins, dirs, t_hit)
     p = ds.march_params(**march_kw)
     out, o = _render_outputs(r.n, dirs.device, True, 0)
     keep = _attach_events(o, events)  # noqa: F841
     ws = workspace or TrainWorkspace()
+    if check is None:
+        check = ws.sized_for != r.n
     buf, need = ws.get(ds, r.n)
+    ws.generation += 1
     N.check(N.lib().snisr_train_forward(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), _ptr(buf), need,
                                         _stream()), "train_forward")
     if check:
@@ -45,6 +68,10 @@ def train_forward(ds: DeviceScene, origins, dirs, t_hit=None, workspace: TrainWo
             buf, need = ws.get(ds, r.n)
             N.check(N.lib().snisr_train_forward(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), _ptr(buf),
                                                 need, _stream()), "train_forward")
+            total, over = mfma_stats(buf)
+            if over:
+                raise RuntimeError("train_forward: sample capacity still exceeded after growing the workspace")
+        ws.sized_for = r.n
         out["total_samples"] = total
     return out, ws
 
@@ -77,6 +104,7 @@ class VoxelRender(torch.autograd.Function):
                       sigma=sigma.detach().contiguous())
         out, ws = train_forward(ds, origins, dirs, t_hit, workspace, **(march_kw or {}))
         ctx.ds, ctx.ws, ctx.march_kw = ds, ws, march_kw or {}
+        ctx.generation = ws.generation
         ctx.save_for_backward(origins, dirs, t_hit if t_hit is not None else torch.empty(0, device=dirs.device))
         ctx.has_t_hit = t_hit is not None
         return out["rgb"]
@@ -84,6 +112,9 @@ class VoxelRender(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_rgb):
         origins, dirs, t_hit = ctx.saved_tensors
+        if ctx.ws.generation != ctx.generation:
+            raise RuntimeError("VoxelRender.backward: the workspace was used by another forward after this one; "
+                               "give each pending forward its own TrainWorkspace")
         dsig, dfeat, dmlp = train_backward(ctx.ds, origins, dirs, grad_rgb, ctx.ws,
                                            t_hit if ctx.has_t_hit else None, **ctx.march_kw)
         return dsig, dfeat, dmlp, None, None, None, None, None, None

@@ -108,3 +108,20 @@ def test_unet_entry_points_validate_before_device_work():
     assert rc != 0 and b"multiples of 4" in L.snisr_last_error()
     nul = (C.c_void_p * 10)()
     assert L.snisr_unet_pack(nul, nul, C.c_void_p(256), None) != 0
+
+
+def test_regroup_refuses_too_few_blocks_before_device_work():
+    """snisr_regroup_pixels checks max_blocks against snisr_regroup_max_blocks(n, P) before any launch
+    (fewer output blocks would merge the last groups' pixels into one block)."""
+    L = N.lib()
+    n, P = 1000, 64
+    mb = L.snisr_regroup_max_blocks(n, P)
+    assert mb == -(-n // P) + n
+    ws = L.snisr_regroup_workspace_size(n)
+    assert ws > 0
+    fake = C.c_void_p(256)
+    rc = L.snisr_regroup_pixels(fake, n, P, mb - 1, fake, fake, fake, fake, fake, ws, None)
+    assert rc == -5 and b"max_blocks" in L.snisr_last_error()
+    rc = L.snisr_regroup_pixels(fake, n, P, mb, fake, fake, fake, fake, fake, ws - 1, None)
+    assert rc == -5 and b"workspace" in L.snisr_last_error()
+    assert L.snisr_sort_by_key_workspace_size(-1) == -1

@@ -152,3 +152,92 @@ def test_frender_device_and_graphed_frames_equal_reference_form(H, W):
             torch.cuda.synchronize()
             assert torch.equal(out, eager[k]), (rep, k)
             _ = torch.randn(1000, device="cuda").sum().item()  # eager work between replays
+
+
+def _psnr(a, b):
+    mse = np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2)
+    return 10 * np.log10(1.0 / max(mse, 1e-30))
+
+
+@pytest.mark.timeout(600)
+def test_frender_720p_frame_psnr_vs_fp64_chain(oracle_mod):
+    """The benched reference-form frame (1280x720, octree room, 8 reflection groups with FP16 MFMA
+    MLPs) against the oracle chain on 600 seeded pixels: the diffuse pass in the reference's FP32
+    arithmetic (within 1e-4), the reflection MLPs and compositing in FP64 without FP16 rounding of the
+    activations, the x2 bilinear upsampling and the clamp.  Frame and low-resolution reflection layer
+    >= 50 dB PSNR; the FP16-quantised oracle chain stays within the reflection pass tolerance
+    (2e-2 max, 1e-3 mean).  (rendering/src/__init__.py:247-334, rayreflection_render.cu:111-301)"""
+    print("frender 720p PSNR vs FP64", flush=True)
+    H, W = 720, 1280
+    osc = S.make_octree_scene(12, tile_shape=(4, 4, 1), block_occupancy=0.3, n_groups=8)
+    params, centers = S.make_reflection_groups(13, 8, (0, 0, 0), (4, 4, 1))
+    cam = S.room_camera(2, (0, 0, 0), (4, 4, 1), H, W)
+    fr = frender.FRender(osc, params, centers, H=H, W=W)
+    fr.set_camera(cam)
+    frame = fr.render().cpu().numpy().reshape(-1, 3)
+    spec_lo = fr.frame_speculars_lowres.cpu().numpy().reshape(-1, 3)
+    diffuse = fr.frame_diffuse.cpu().numpy().reshape(-1, 3)
+    hi = np.sort(np.random.default_rng(31).choice(H * W, 600, replace=False))
+    dirs = oracle_mod.compute_rays(H, W, cam.intrinsic, cam.c2w)
+    grid = oracle_mod.OracleScene(_grid_scene(osc))
+    ref = oracle_mod.frender_subset(osc, grid, params, centers, cam.origin, dirs, H, W, fr.far, hi)
+    assert np.abs(diffuse[hi] - ref["diffuse"]).max() <= 1e-4
+    lo = ref["lo_idx"]
+    assert (ref["net_lo"] >= 0).mean() > 0.3             # most sampled pixels hit a surface with a group
+    p_spec = _psnr(spec_lo[lo], ref["spec_lo"])
+    p_frame = _psnr(frame[hi], ref["frame"])
+    print(f"720p reference-form frame: PSNR vs FP64 chain frame {p_frame:.2f} dB, reflection {p_spec:.2f} dB",
+          flush=True)
+    assert p_frame >= 50.0 and p_spec >= 50.0
+    q16 = oracle_mod.frender_subset(osc, grid, params, centers, cam.origin, dirs, H, W, fr.far, hi, quant16=True)
+    err = np.abs(spec_lo[lo] - q16["spec_lo"])
+    assert err.max() <= 2e-2 and err.mean() <= 1e-3, (err.max(), err.mean())
+
+
+def test_frender_render_modes():
+    """Render modes 2 (diffuse), 3 (reflection), 4 (depth) and 1 (full frame) as the reference's
+    FRender.render returns them (rendering/src/__init__.py:296-303, 330-334), eager and graphed."""
+    H, W = 96, 160
+    osc = S.make_octree_scene(9, tile_shape=(4, 4, 1), block_occupancy=0.3, n_groups=6)
+    params, centers = S.make_reflection_groups(19, 6, (0, 0, 0), (4, 4, 1))
+    cam = S.room_camera(9, (0, 0, 0), (4, 4, 1), H, W, focal=95.0)
+    fr = frender.FRender(osc, params, centers, H=H, W=W)
+    fr.set_camera(cam)
+    full = fr.render().clone()
+    diffuse, spec = fr.frame_diffuse.clone(), fr.frame_speculars.clone()
+    depth = (1 / fr.large_inverse_near[..., None] / 4.0).repeat(1, 1, 3)   # max_depth = max(4, 4, 1) * 1
+    assert torch.equal(full, torch.clamp(diffuse + spec, 0, 1))
+    for mode, want in ((2, diffuse), (3, spec), (4, depth), (1, full)):
+        fr.set_mode(mode)
+        assert torch.equal(fr.render(), want), mode
+        assert torch.equal(fr.render_device(), want), mode
+        assert torch.equal(fr.render_graphed(), want), mode
+    with pytest.raises(ValueError):
+        fr.set_mode(5)
+
+
+def test_frender_renders_an_exported_training_checkpoint(oracle_mod, tmp_path):
+    """Training outputs (dense Voxel-T / Node-T grids and Coeffi state dicts) exported to
+    renderData.npz and rendered by FRender.from_render_data: the diffuse layer equals the octree
+    oracle on the exported scene (within 1e-4; inverse depth and groups bit-exact)."""
+    from snisr import render_data as RD
+    from test_checkpoint_export import _write_checkpoint
+    ds = S.make_dense_scene(4, tile_shape=(3, 2, 1), num_voxel=64, block=8, block_occupancy=0.3, drop_tiles=1)
+    groups = _write_checkpoint(tmp_path, ds)
+    f = tmp_path / "renderData.npz"
+    RD.export_checkpoint(f, groups, ds.tile_shape, ds.tile_size, ds.scene_min, epoch=7)
+    H, W = 96, 160
+    fr = frender.FRender.from_render_data(str(f), H=H, W=W)
+    cam = S.look_at_camera(np.array([-0.6, -0.5, 0.7]), np.array([1.5, 1.0, 0.5]), H, W, 95.0)
+    fr.set_camera(cam)
+    frame = fr.render()
+    assert torch.isfinite(frame).all()
+    rd = RD.load_render_data(str(f))
+    dirs = oracle_mod.compute_rays(H, W, cam.intrinsic, cam.c2w)
+    grid = oracle_mod.OracleScene(_grid_scene(rd.scene))
+    vis = oracle_mod.trace_tiles(grid, cam.origin, dirs, 30)
+    rf, ri, rn, rs = oracle_mod.render_octree(rd.scene, cam.origin, dirs, vis, trans_th=0.8, early_stop=0.01)
+    assert np.abs(fr.frame_diffuse.cpu().numpy().reshape(-1, 3) - rf).max() <= 1e-4
+    assert np.array_equal(fr.large_inverse_near.cpu().numpy().reshape(-1).view(np.uint32), ri.view(np.uint32))
+    assert np.array_equal(fr.large_netIdx.cpu().numpy().reshape(-1), rn)
+    assert (rs > 0).mean() > 0.2

@@ -88,3 +88,31 @@ def test_packet_traversal_matches_per_lane_and_brute_force(oracle_mod):
     t_ref, id_ref = oracle_mod.mesh_hit(small.vertices, small.faces, origins, oracle_mod.normalize(rd))
     assert np.array_equal(ids.cpu().numpy(), id_ref)
     assert np.array_equal(t.cpu().numpy().view(np.uint32), t_ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("nb", [8, 32])
+def test_mesh_hit_tiles_first_and_double_match_oracle(oracle_mod, nb):
+    """Per-tile first and double hits (ray_firstHit_inTile_kernel / ray_doubleHit_inTile_kernel,
+    training/src/cuda_func.cu:130-444) through the host-built face lists against the oracle's walk
+    over every nearby face: t, faces and both double-hit t bit-exact."""
+    mesh = S.room_mesh(11, (0, 0, 0), (2, 2, 1), 5000, size_range=(0.02, 0.15))
+    grid = S.make_scene(0, tile_shape=(2, 2, 1), res=8, occupancy=0.0)
+    tm = D.TileFaceMesh(mesh.vertices, mesh.faces, grid.tile_centers, 1.0, num_block=nb)
+    rng = np.random.default_rng(nb)
+    eye = np.array([0.3, 0.4, 0.5], np.float32)
+    n = 800
+    d = rng.normal(0, 1, (n, 3))
+    d[:50, 1:] = 0.0                                  # axis-parallel rays
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    vis = oracle_mod.trace_tiles(oracle_mod.OracleScene(grid), eye, d, 4)
+    o = np.repeat(eye[None], n, 0)
+    ref_t, ref_f = oracle_mod.mesh_hit_tiles(mesh.vertices, mesh.faces, grid.tile_centers, 1.0, nb, o, d, vis, 0)
+    ref_d = oracle_mod.mesh_hit_tiles(mesh.vertices, mesh.faces, grid.tile_centers, 1.0, nb, o, d, vis, 1)
+    hits = torch.full((n, 4), float("nan"), device=DEV)
+    t, f = tm.hits(to_dev(o), to_dev(d), to_dev(vis), mode=0, hits=hits)
+    dd = tm.hits(to_dev(o), to_dev(d), to_dev(vis), mode=1)
+    t, f, dd = t.cpu().numpy(), f.cpu().numpy(), dd.cpu().numpy()
+    assert np.array_equal(f, ref_f)
+    assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32))    # NaN where no hit, in both
+    assert np.array_equal(dd.view(np.uint32), ref_d.view(np.uint32))
+    assert (f >= 0).sum() > 300 and (dd[..., 1] > 0).sum() > 200

@@ -107,3 +107,27 @@ def test_autograd_function_and_gpu_finite_difference():
         num = float((lp - lm) / (2 * eps))
         an = float(mlp.grad.flatten()[i])
         assert abs(num - an) <= 1e-2 * abs(an) + 1e-4, (i, num, an)
+
+
+def test_workspace_reuse_before_backward_raises_and_steps_do_not_synchronise():
+    """One TrainWorkspace shared by two pending forwards: the first forward's backward would read the
+    second forward's records, so it raises.  After the first (sizing) forward, forwards on the same
+    ray count leave the capacity check to the caller (no host synchronisation per step)."""
+    sc = S.make_scene(5, tile_shape=(2, 1, 1), res=16, occupancy=0.6, feat_dim=16, mlp_width=32, mlp_layers=3,
+                      sample_step=1.0 / 32, early_stop=0.0)
+    origins, dirs, g = case(sc, 2000, 7, [-0.5, -0.5, -0.5], [2.5, 1.5, 1.5], [0.1, 0.1, 0.1], [1.9, 0.9, 0.9])
+    ds = D.DeviceScene(sc)
+    sigma = ds.sigma.clone().requires_grad_(True)
+    feats = ds.feats.clone().requires_grad_(True)
+    mlp = ds.mlp_params.clone().requires_grad_(True)
+    o, d, gt = to_dev(origins), to_dev(dirs), to_dev(g)
+    ws = TR.TrainWorkspace()
+    a = TR.render(sigma, feats, mlp, o, d, ds, workspace=ws)
+    assert ws.sized_for == o.shape[0]
+    b = TR.render(sigma, feats, mlp, o, d, ds, workspace=ws)
+    with pytest.raises(RuntimeError, match="another forward"):
+        (a * gt).sum().backward()
+    (b * gt).sum().backward()          # the last forward's backward is fine
+    assert not ws.overflowed()
+    assert torch.equal(a.detach(), b.detach())
+    assert ws.samples_last() > 0
