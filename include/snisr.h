@@ -364,6 +364,17 @@ int snisr_regroup_pixels(const int16_t* net_idx, int64_t n, int32_t pixels_per_b
                          int32_t* query, int16_t* block_net, int32_t* block_starts, int32_t* n_blocks,
                          void* workspace, int64_t workspace_bytes, void* stream);
 
+/* Allocation-free form of snisr_regroup_pixels.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The same regrouping and outputs, with the workspace in a fixed device array of the
+ * library (about 10 MB per GPU, held until the process ends) instead of a caller workspace; n is at
+ * most snisr_regroup_pixels_static_max() (1,048,576 pixels), larger n returns SNISR_ERR_CAPACITY.
+ * Calls on one GPU share that array: calls that overlap (other streams or threads) race. */
+int64_t snisr_regroup_pixels_static_max(void);
+int snisr_regroup_pixels_static(const int16_t* net_idx, int64_t n, int32_t pixels_per_block, int64_t max_blocks,
+                                int32_t* query, int16_t* block_net, int32_t* block_starts, int32_t* n_blocks,
+                                void* stream);
+
 /* A11 — training (FP32 features and MLP).  snisr_train_forward renders like snisr_render_fp32
  * (identical traversal and samples) and keeps per-sample records in the workspace;
  * snisr_train_backward then computes the gradients of L = sum_i <grad_rgb_i, rgb_i> with respect to

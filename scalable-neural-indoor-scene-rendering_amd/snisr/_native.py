@@ -130,6 +130,9 @@ SIGNATURES = [
     ("snisr_regroup_workspace_size", C.c_int64, [C.c_int64]),
     ("snisr_regroup_pixels", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("snisr_regroup_pixels_static_max", C.c_int64, []),
+    ("snisr_regroup_pixels_static", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p]),
     ("snisr_adam_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_double,
                                   C.c_double, C.c_double, C.c_double, C.c_int64, C.c_void_p]),
     ("snisr_unet_packed_size", C.c_int64, []),
