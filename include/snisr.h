@@ -148,12 +148,19 @@ int snisr_render_fp32(const snisr_scene* scene, const snisr_march_params* params
  * 512 threads per MLP workgroup (one workgroup per CU); sort_grid: workgroups of the record / ray
  * counting sorts (chosen per workload by an autotuner; results do not depend on them).
  * records_per_ray: sample-record slots per ray (0 = 64); every other workspace byte goes to the
- * tile-sorted sample arrays. */
+ * tile-sorted sample arrays.  chunks (0 or 1 = one pass; at most 64; ray_order 0 only): the rays are
+ * split into that many contiguous ranges of whole 8-row packet bands (or 64-ray wavefronts); the
+ * march of range c runs on the caller's stream while a second stream of the library's sorts and
+ * shades range c - 1, and the call joins that stream back before it returns (capturable).  Each
+ * range has its own tile-sorted arrays of the full sample capacity.  ray_order 2: rays
+ * counting-sorted by (first tile, octant, origin cell of a c^3 grid over the scene box, c as large
+ * as 8192 buckets allow); 3: by (first tile, origin cell). */
 typedef struct snisr_tuning {
   int32_t ray_order;
   int32_t mlp_threads;
   int32_t sort_grid;
   int32_t records_per_ray;
+  int32_t chunks;
 } snisr_tuning;
 
 /* Workspace bytes for snisr_render_mfma on n rays with `records_per_ray` record slots per ray
@@ -162,6 +169,11 @@ typedef struct snisr_tuning {
  * for one ray or in total is reported by snisr_render_mfma_stats (overflow; its rgb is NaN). */
 int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t records_per_ray,
                                          int64_t sample_capacity);
+
+/* Workspace bytes for snisr_render_mfma_tuned with this schedule (records_per_ray and chunks are
+ * read; every chunk gets room for `sample_capacity` samples, 0 = 16 per ray). */
+int64_t snisr_render_mfma_workspace_size_tuned(const snisr_scene* scene, int64_t n, const snisr_tuning* tuning,
+                                               int64_t sample_capacity);
 
 /* Same render with the per-tile MLP on MFMA (FP16 or BF16 operands, FP32 accumulation,
  * FP32 transmittance).  Stages: one sigma-only march writing every ray's sample records (voxel

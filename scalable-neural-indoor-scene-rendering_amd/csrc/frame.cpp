@@ -82,7 +82,7 @@ int graph_step(snisr_graph* g) {
   o.n_samples = g->n_samples;
   o.depth = g->depth;
   if (c.mlp_dtype == SNISR_F32) return snisr_render_fp32(&g->scene->desc, &c.params, &r, &o, st);
-  snisr_tuning tn{0, 512, 512, g->records_per_ray};
+  snisr_tuning tn{0, 512, 512, g->records_per_ray, 1};
   return snisr_render_mfma_tuned(&g->scene->desc, &c.params, &r, &o, &tn, g->workspace, g->workspace_bytes, st);
 }
 

@@ -13,6 +13,7 @@
 //   16 values become the next layer's B operand for the two k-chunks (2*ob, 2*ob+1) without any
 //   data movement; the next layer's weight columns are permuted to match (pack_input_index).
 #include <atomic>
+#include <vector>
 
 #include "snisr_common.hpp"
 #include "snisr_device.hpp"
@@ -139,7 +140,6 @@ static inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * 
 // schedule keys: first occupied tile along the ray (T when none) x 8 direction octants, or the tile
 // alone when that would exceed the LDS histogram
 constexpr int kMaxBuckets = 8192;
-__host__ __device__ constexpr int32_t key_buckets(int32_t T) { return 8 * (T + 1) <= kMaxBuckets ? 8 * (T + 1) : T + 1; }
 
 static int64_t layout(int64_t n, int64_t cap, int32_t T, uint8_t* base, Workspace* w, int64_t train_floats = 0,
                       int64_t frag_floats = 0) {
@@ -214,41 +214,66 @@ struct RenderWs {
   float4* rec_f;          // [n * cpr] voxel coordinate + weight; after the MLP: weighted colour
   int32_t* rec_t;         // [n * cpr] at the first record of each tile run: tile | run length << 16
   int32_t* perm;          // [scap] record slots in tile order, 64-aligned buckets, -1 = padding
-  int32_t* skip;          // header: 1 when the sorted samples exceed scap (render skipped, rgb NaN)
+  int32_t* skip;          // 1 when the sorted samples exceed scap (render skipped, rgb NaN)
   int64_t n;
   int64_t scap;           // sorted positions: sample capacity + 64 per tile of bucket padding
   int32_t cpr;
   int32_t n_tiles;
   int32_t n_buckets;
+  int32_t chunks;         // ray chunks with their own tile sort (tile_*, perm and skip per chunk)
 };
+
+// The view of ray chunk c = rays [r0, r0 + nc) of a chunked workspace: its records, counts and view
+// encodings at local ray indices 0 .. nc-1, and its own tile arrays, sorted positions and skip flag,
+// so the tile sort, the MLP and the reduction run on it unchanged.
+static RenderWs chunk_view(const RenderWs& W, int c, int64_t r0, int64_t nc) {
+  RenderWs v = W;
+  const int64_t T = W.n_tiles;
+  v.counts = W.counts + r0;
+  v.enc = W.enc + 16 * r0;
+  v.rec_f = W.rec_f + r0 * W.cpr;
+  v.rec_t = W.rec_t + r0 * W.cpr;
+  v.tile_count = W.tile_count + c * T;
+  v.tile_start = W.tile_start + c * (T + 1);
+  v.tile_cursor = W.tile_cursor + c * T;
+  v.perm = W.perm + c * W.scap;
+  v.skip = W.skip + c;
+  v.n = nc;
+  v.chunks = 1;
+  return v;
+}
 
 constexpr int32_t kDefaultRecordsPerRay = 64;
 constexpr int64_t kDefaultSamplesPerRay = 16;  // default sample capacity per ray of the sorted arrays
 constexpr int64_t kRenderBytesPerSlot = 16 + 4;
 
 constexpr int64_t kSortedBytes = 4;  // bytes per tile-sorted position (perm)
+constexpr int32_t kMaxChunks = 64;   // ray chunks of a pipelined render
 
-static int64_t render_layout(int64_t n, int64_t cpr, int64_t samples, int32_t T, uint8_t* base, RenderWs* w) {
+static int64_t render_layout(int64_t n, int64_t cpr, int64_t samples, int32_t T, uint8_t* base, RenderWs* w,
+                             int32_t chunks = 1) {
   int64_t off = 256;
-  const int32_t nbk = key_buckets(T);
-  const int64_t slots = n * cpr, sorted = samples + 64 * (int64_t)T;
+  const int32_t nbk = kMaxBuckets;   // the largest key of any schedule mode
+  const int64_t k = chunks > 0 ? chunks : 1;
+  const int64_t slots = n * cpr, sorted = samples + 64 * (int64_t)T;   // per chunk
+  const int64_t o_skip = off; off = align_up(off + 4 * k, 256);
   const int64_t o_cnt = off; off = align_up(off + 4 * n, 256);
   const int64_t o_enc = off; off = align_up(off + 32 * n, 256);
   const int64_t o_ord = off; off = align_up(off + 4 * n, 256);
   const int64_t o_key = off; off = align_up(off + 4 * n, 256);
   const int64_t o_bkc = off; off = align_up(off + 4 * (int64_t)nbk, 256);
   const int64_t o_bku = off; off = align_up(off + 8 * (int64_t)nbk, 256);
-  const int64_t o_tc = off; off = align_up(off + 4 * (int64_t)T, 256);
-  const int64_t o_ts = off; off = align_up(off + 8 * ((int64_t)T + 1), 256);
-  const int64_t o_cu = off; off = align_up(off + 8 * (int64_t)T, 256);
+  const int64_t o_tc = off; off = align_up(off + 4 * k * (int64_t)T, 256);
+  const int64_t o_ts = off; off = align_up(off + 8 * k * ((int64_t)T + 1), 256);
+  const int64_t o_cu = off; off = align_up(off + 8 * k * (int64_t)T, 256);
   const int64_t o_rf = off; off = align_up(off + 16 * slots, 256);
   const int64_t o_rt = off; off = align_up(off + 4 * slots, 256);
-  const int64_t o_pm = off; off = align_up(off + 4 * sorted, 256);
+  const int64_t o_pm = off; off = align_up(off + 4 * k * sorted, 256);
   if (w && base) {
     w->total = reinterpret_cast<int64_t*>(base);
     w->overflow = reinterpret_cast<int32_t*>(base + 8);
     w->max_count = reinterpret_cast<int32_t*>(base + 12);
-    w->skip = reinterpret_cast<int32_t*>(base + 16);
+    w->skip = reinterpret_cast<int32_t*>(base + o_skip);
     w->scap = sorted;
     w->counts = reinterpret_cast<int32_t*>(base + o_cnt);
     w->enc = reinterpret_cast<uint16_t*>(base + o_enc);
@@ -266,6 +291,7 @@ static int64_t render_layout(int64_t n, int64_t cpr, int64_t samples, int32_t T,
     w->cpr = (int32_t)cpr;
     w->n_tiles = T;
     w->n_buckets = nbk;
+    w->chunks = (int32_t)k;
   }
   return off;
 }
@@ -491,11 +517,33 @@ __global__ __launch_bounds__(kSortBlock) void k_tile_scatter(Workspace W) {
 
 
 // ------------------------------------------------------------------------- ray schedule
-// Orders the rays by (first occupied tile, direction octant) so that the wavefronts of the march
-// passes walk the same tile in similar directions: a counting sort (LDS histograms, one scan,
-// atomic bucket cursors).  Only the thread -> ray map changes; every ray's records and pixel are
-// the same as without the schedule.
-__global__ __launch_bounds__(256) void k_ray_key(snisr_scene S, snisr_rays R, RenderWs W) {
+// Orders the rays so that the wavefronts of the march walk the same bricks together: a counting sort
+// (LDS histograms, one scan, atomic bucket cursors) by a key of
+//   mode 1: (first occupied tile, direction octant);
+//   mode 2: (first occupied tile, direction octant, origin cell of a c^3 grid over the scene box);
+//   mode 3: (first occupied tile, origin cell of a c^3 grid over the scene box);
+// c is the largest cell count per axis whose buckets fit the LDS histogram.  Only the thread -> ray
+// map changes; every ray's records and pixel are the same as without the schedule.
+struct KeyShape {
+  int32_t mode, cells, n_buckets;
+};
+
+static KeyShape key_shape(int mode, int32_t T) {
+  KeyShape k{mode, 1, T + 1};
+  if (mode == 1) {
+    k.n_buckets = 8 * (T + 1) <= kMaxBuckets ? 8 * (T + 1) : T + 1;
+  } else if (mode >= 2) {
+    const int per = kMaxBuckets / ((T + 1) * (mode == 2 ? 8 : 1));
+    int c = 1;
+    while ((c + 1) * (c + 1) * (c + 1) <= per) ++c;
+    k.cells = c;
+    k.n_buckets = (T + 1) * (mode == 2 ? 8 : 1) * c * c * c;
+    if (mode == 2 && per < 1) k = KeyShape{1, 1, T + 1};   // too many tiles for octants: first tile only
+  }
+  return k;
+}
+
+__global__ __launch_bounds__(256) void k_ray_key(snisr_scene S, snisr_rays R, RenderWs W, KeyShape K) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < W.n_buckets) W.bucket_count[i] = 0;
   if (i >= R.n) return;
@@ -507,7 +555,22 @@ __global__ __launch_bounds__(256) void k_ray_key(snisr_scene S, snisr_rays R, Re
   int t = w.next();
   if (t < 0) t = W.n_tiles;
   const int oct = (d.x < 0.0f ? 1 : 0) + (d.y < 0.0f ? 2 : 0) + (d.z < 0.0f ? 4 : 0);
-  W.ray_key[i] = W.n_buckets == 8 * (W.n_tiles + 1) ? 8 * t + oct : t;
+  int key = t;
+  if (K.mode == 1) {
+    key = W.n_buckets == 8 * (W.n_tiles + 1) ? 8 * t + oct : t;
+  } else if (K.mode >= 2) {
+    const int c = K.cells;
+    int cell = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float ext = (float)S.tile_shape[a] * S.tile_size;
+      int q = (int)((get(o, a) - S.scene_min[a]) / ext * (float)c);
+      q = q < 0 ? 0 : (q >= c ? c - 1 : q);
+      cell = cell * c + q;
+    }
+    key = K.mode == 2 ? (t * 8 + oct) * c * c * c + cell : t * c * c * c + cell;
+  }
+  W.ray_key[i] = key;
 }
 
 __device__ __forceinline__ void ray_range(int64_t n, int64_t& b, int64_t& e) {
@@ -570,9 +633,9 @@ __global__ void k_render_begin(RenderWs W) {
     *W.total = 0;
     *W.overflow = 0;
     *W.max_count = 0;
-    *W.skip = 0;
   }
-  for (int t = threadIdx.x; t < W.n_tiles; t += blockDim.x) W.tile_count[t] = 0;
+  for (int c = threadIdx.x; c < W.chunks; c += blockDim.x) W.skip[c] = 0;
+  for (int t = threadIdx.x; t < W.chunks * W.n_tiles; t += blockDim.x) W.tile_count[t] = 0;
 }
 
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
@@ -589,12 +652,16 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 // One wavefront per 64 rays: march each ray once, writing its records (voxel coordinate + weight,
 // tile) at its own slots, its sample count, T, depth and view encoding; per-wave totals feed the
 // header (sample total, largest count, overflow when a ray has more samples than slots).
+// PACKET: camera rays in 8x8 pixel packets, marched with the nested tile / brick / sample loops;
+// otherwise (arbitrary rays, or a ray schedule) with the flattened loop.  One instantiation per
+// kernel keeps the other variant's code and scalar state out of it.
+template <bool PACKET>
 __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_params P, snisr_rays R,
                                                      snisr_render_out O, RenderWs W, int packet_w, int enc_dtype,
-                                                     const int32_t* __restrict__ order) {
-  const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
+                                                     const int32_t* __restrict__ order, int64_t tid0, int64_t tid1) {
+  const int64_t tid = tid0 + (int64_t)blockIdx.x * 64 + threadIdx.x;
   int ns = 0;
-  if (tid < R.n) {
+  if (tid < tid1) {
     const int64_t i = order ? (int64_t)order[tid] : ray_of_thread(tid, R.n, packet_w);
     const float* og = R.origins + i * R.origin_stride;
     const V3 origin = v3(og[0], og[1], og[2]);
@@ -606,8 +673,7 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
     int k = 0, run_tile = -1, run_start = 0;
     MarchResult res;
     int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
-    march_ray_any(packet_w > 0 && !order, S, P, origin, d, t_hit, vis, bricks, O.brick_cap,
-              [&](int tile, const V3& f, const Corners&, float w, float, float, float) {
+    auto on_sample = [&](int tile, const V3& f, const Corners&, float w, float, float, float) {
                 if (k < cpr) {
                   if (tile != run_tile) {
                     if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((k - run_start) << 16);
@@ -617,8 +683,11 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
                   W.rec_f[slot0 + k] = make_float4(f.x, f.y, f.z, w);
                 }
                 ++k;
-              },
-              res);
+              };
+    if (PACKET)
+      march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, on_sample, res);
+    else
+      march_ray_flat(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, on_sample, res);
     if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((min(k, cpr) - run_start) << 16);
     if (bricks)
       for (int b = res.n_bricks; b < O.brick_cap; ++b) bricks[b] = -1;
@@ -1649,13 +1718,19 @@ extern "C" {
 
 int64_t snisr_render_mfma_workspace_size(const snisr_scene* scene, int64_t n, int64_t records_per_ray,
                                          int64_t sample_capacity) {
-  if (!scene || n < 0 || records_per_ray < 0 || sample_capacity < 0 || scene->mlp_in <= 0) {
+  const snisr_tuning tn{0, 512, kSortGrid, (int32_t)(records_per_ray <= 4096 ? records_per_ray : -1), 1};
+  return snisr_render_mfma_workspace_size_tuned(scene, n, &tn, sample_capacity);
+}
+
+int64_t snisr_render_mfma_workspace_size_tuned(const snisr_scene* scene, int64_t n, const snisr_tuning* tuning,
+                                               int64_t sample_capacity) {
+  if (!scene || !tuning || n < 0 || tuning->records_per_ray < 0 || sample_capacity < 0 || scene->mlp_in <= 0) {
     set_error("render_mfma_workspace_size: bad arguments");
     return -1;
   }
-  const int64_t cpr = records_per_ray ? records_per_ray : kDefaultRecordsPerRay;
+  const int64_t cpr = tuning->records_per_ray ? tuning->records_per_ray : kDefaultRecordsPerRay;
   const int64_t cap = sample_capacity ? sample_capacity : kDefaultSamplesPerRay * n;
-  return render_layout(n, cpr, cap, scene->n_tiles, nullptr, nullptr);
+  return render_layout(n, cpr, cap, scene->n_tiles, nullptr, nullptr, tuning->chunks > 1 ? tuning->chunks : 1);
 }
 
 int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
@@ -1663,16 +1738,61 @@ int snisr_render_mfma(const snisr_scene* scene, const snisr_march_params* params
   return snisr_render_mfma_tuned(scene, params, rays, out, nullptr, workspace, workspace_bytes, stream);
 }
 
+}  // extern "C"
+
+namespace snisr {
+
+// The second stream of a chunked render and its events, per host thread and device: a chunked
+// render forks the stream it is given onto this one and joins it back before returning, so it can
+// be captured into a HIP graph like an unchunked one.
+struct ChunkStreams {
+  hipStream_t aux = nullptr;
+  std::vector<hipEvent_t> ev;
+};
+
+static int chunk_streams(int need_events, ChunkStreams** out) {
+  thread_local ChunkStreams per_dev[64];
+  int dev = 0;
+  SNISR_HIP(hipGetDevice(&dev));
+  SNISR_REQUIRE(dev >= 0 && dev < 64, SNISR_ERR_DEVICE, "render_mfma: device %d beyond 64", dev);
+  ChunkStreams& cs = per_dev[dev];
+  if (!cs.aux) SNISR_HIP(hipStreamCreateWithFlags(&cs.aux, hipStreamNonBlocking));
+  while ((int)cs.ev.size() < need_events) {
+    hipEvent_t e;
+    SNISR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    cs.ev.push_back(e);
+  }
+  *out = &cs;
+  return SNISR_OK;
+}
+
+// chunk c of k over n rays: whole 8-row packet bands when the march runs 8x8 packets (band of
+// 8 * image_width rays), else whole 64-ray wavefronts
+static void chunk_range(int64_t n, int64_t unit, int c, int k, int64_t* r0, int64_t* r1) {
+  const int64_t units = (n + unit - 1) / unit;
+  const int64_t u0 = units * c / k, u1 = units * (c + 1) / k;
+  *r0 = u0 * unit < n ? u0 * unit : n;
+  *r1 = u1 * unit < n ? u1 * unit : n;
+}
+
+}  // namespace snisr
+
+extern "C" {
+
 int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* params, const snisr_rays* rays,
                             const snisr_render_out* out, const snisr_tuning* tuning, void* workspace,
                             int64_t workspace_bytes, void* stream) {
-  const snisr_tuning tn = tuning ? *tuning : snisr_tuning{0, 512, kSortGrid, 0};
+  const snisr_tuning tn = tuning ? *tuning : snisr_tuning{0, 512, kSortGrid, 0, 1};
   SNISR_REQUIRE(tn.records_per_ray >= 0 && tn.records_per_ray <= 4096, SNISR_ERR_ARG,
                 "render_mfma: records_per_ray out of range");
-  SNISR_REQUIRE(tn.ray_order == 0 || tn.ray_order == 1, SNISR_ERR_ARG, "render_mfma: ray_order must be 0 or 1");
+  SNISR_REQUIRE(tn.ray_order >= 0 && tn.ray_order <= 3, SNISR_ERR_ARG, "render_mfma: ray_order must be 0..3");
   SNISR_REQUIRE(tn.mlp_threads == 256 || tn.mlp_threads == 512, SNISR_ERR_ARG,
                 "render_mfma: mlp_threads must be 256 or 512");
   SNISR_REQUIRE(tn.sort_grid >= 1 && tn.sort_grid <= 65536, SNISR_ERR_ARG, "render_mfma: sort_grid out of range");
+  SNISR_REQUIRE(tn.chunks >= 0 && tn.chunks <= kMaxChunks, SNISR_ERR_ARG, "render_mfma: chunks must be 0..%d",
+                kMaxChunks);
+  SNISR_REQUIRE(tn.chunks <= 1 || tn.ray_order == 0, SNISR_ERR_ARG,
+                "render_mfma: chunks > 1 needs ray_order 0 (chunks are contiguous ray ranges)");
   SNISR_REQUIRE(out && out->rgb, SNISR_ERR_ARG, "render_mfma: out.rgb is required");
   if (int e = check_scene(scene, true)) return e;
   if (int e = check_params(params)) return e;
@@ -1684,85 +1804,131 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
   const int64_t n = rays->n;
   if (n == 0) return SNISR_OK;
   SNISR_REQUIRE(scene->n_tiles <= kMaxTiles, SNISR_ERR_SHAPE, "render_mfma: at most %d tiles", kMaxTiles);
-  // records per ray from the schedule (default 64); every remaining byte goes to the sorted arrays
+  const int pw = packet_width(*rays);
+  // ray chunks: contiguous ranges of whole packet bands (or wavefronts), at most one per unit
+  const int64_t unit = pw > 0 ? 8 * (int64_t)pw : 64;
+  const int64_t units = (n + unit - 1) / unit;
+  const int k = (int)(tn.chunks > 1 ? (tn.chunks < units ? tn.chunks : units) : 1);
+  // records per ray from the schedule (default 64); every remaining byte goes to the sorted arrays,
+  // split evenly between the chunks
   int64_t cpr = tn.records_per_ray > 0 ? tn.records_per_ray : kDefaultRecordsPerRay;
   if (n * cpr >= (int64_t(1) << 31)) cpr = ((int64_t(1) << 31) - 1) / n;  // slot ids are 32-bit
   SNISR_REQUIRE(cpr > 0, SNISR_ERR_CAPACITY, "render_mfma: batch too large for 32-bit record slots");
-  const int64_t fixed = render_layout(n, cpr, 0, scene->n_tiles, nullptr, nullptr);
+  const int64_t fixed = render_layout(n, cpr, 0, scene->n_tiles, nullptr, nullptr, k);
   SNISR_REQUIRE(workspace_bytes > fixed, SNISR_ERR_CAPACITY,
                 "render_mfma: workspace too small for %lld records per ray", (long long)cpr);
-  int64_t samples = (workspace_bytes - fixed) / kSortedBytes;
-  while (samples > 0 && render_layout(n, cpr, samples, scene->n_tiles, nullptr, nullptr) > workspace_bytes)
+  int64_t samples = (workspace_bytes - fixed) / (kSortedBytes * k);
+  while (samples > 0 && render_layout(n, cpr, samples, scene->n_tiles, nullptr, nullptr, k) > workspace_bytes)
     --samples;
   SNISR_REQUIRE(samples > 0, SNISR_ERR_CAPACITY, "render_mfma: workspace too small for the sorted samples");
   RenderWs W;
-  render_layout(n, cpr, samples, scene->n_tiles, (uint8_t*)workspace, &W);
+  render_layout(n, cpr, samples, scene->n_tiles, (uint8_t*)workspace, &W, k);
 
   const int F = scene->feat_dim, WD = scene->mlp_width, L = scene->mlp_layers;
   const bool half = scene->feat_dtype == SNISR_F16;
   const int dt = scene->mlp_packed_dtype;
   hipStream_t st = (hipStream_t)stream;
-  const int pw = packet_width(*rays);
+  const dim3 sort_grid((unsigned)tn.sort_grid), sort_block(kSortBlock);
+
+  // the shaded half of the render on one chunk's view: tile sort, MFMA MLP, per-ray reduction
+  auto launch_march = [&](int64_t r0, int64_t r1, const int32_t* order) -> int {
+    const dim3 g((unsigned)((r1 - r0 + 63) / 64));
+    if (pw > 0 && !order)
+      hipLaunchKernelGGL(k_march_render<true>, g, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt, order,
+                         r0, r1);
+    else
+      hipLaunchKernelGGL(k_march_render<false>, g, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt, order,
+                         r0, r1);
+    SNISR_LAUNCHED();
+    return SNISR_OK;
+  };
+  auto launch_mlp = [&](const RenderWs& V, hipStream_t s) -> int {
+    const dim3 grid((unsigned)device_cus()), block((unsigned)tn.mlp_threads);
+    bool launched = false;
+#define SNISR_MFMA_CASE(F_, W_, L_, H_, D_)                                                        \
+    if (!launched && F == F_ && WD == W_ && L == L_ && half == H_ && dt == D_) {                    \
+      auto kern = k_mlp_mfma<F_, W_, L_, H_, D_>;                                                   \
+      const size_t lds = (size_t)tile_bytes(F_ + 15, W_, L_);                                       \
+      static std::atomic<bool> lds_set{false};                                                      \
+      if (!lds_set.load()) {                                                                        \
+        SNISR_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+        lds_set.store(true);                                                                        \
+      }                                                                                             \
+      hipLaunchKernelGGL(kern, grid, block, lds, s, *scene, *rays, V);                              \
+      launched = true;                                                                              \
+    }
+    SNISR_MFMA_CASE(32, 128, 4, true, SNISR_F16)
+    SNISR_MFMA_CASE(32, 128, 4, true, SNISR_BF16)
+    SNISR_MFMA_CASE(16, 64, 3, false, SNISR_F16)
+    SNISR_MFMA_CASE(16, 64, 3, false, SNISR_BF16)
+    SNISR_MFMA_CASE(32, 128, 4, false, SNISR_F16)
+#undef SNISR_MFMA_CASE
+    if (!launched) {
+      set_error("render_mfma: unsupported shape F=%d width=%d layers=%d feat_dtype=%d packed=%d", F, WD, L,
+                scene->feat_dtype, dt);
+      return SNISR_ERR_SHAPE;
+    }
+    SNISR_LAUNCHED();
+    return SNISR_OK;
+  };
+  auto launch_shade = [&](const RenderWs& V, float* rgb, hipStream_t s, bool stages) -> int {
+    hipLaunchKernelGGL(k_rtile_hist, sort_grid, sort_block, 0, s, V);
+    SNISR_LAUNCHED();
+    hipLaunchKernelGGL(k_rtile_scan, dim3(1), dim3(kScanBlock), 0, s, V);
+    SNISR_LAUNCHED();
+    if (stages) SNISR_HIP(stage_event(out, 3, s));
+    hipLaunchKernelGGL(k_rtile_scatter, sort_grid, sort_block, 0, s, V);
+    SNISR_LAUNCHED();
+    if (stages) SNISR_HIP(stage_event(out, 4, s));
+    if (int e = launch_mlp(V, s)) return e;
+    if (stages) SNISR_HIP(stage_event(out, 5, s));
+    hipLaunchKernelGGL(k_render_reduce, dim3((unsigned)((4 * V.n + 255) / 256)), dim3(256), 0, s, V, rgb);
+    SNISR_LAUNCHED();
+    return SNISR_OK;
+  };
 
   SNISR_HIP(stage_event(out, 0, st));
   hipLaunchKernelGGL(k_render_begin, dim3(1), dim3(256), 0, st, W);
   SNISR_LAUNCHED();
   const int32_t* order = nullptr;
-  if (tn.ray_order == 1) {
+  if (tn.ray_order >= 1) {
+    const KeyShape ks = key_shape(tn.ray_order, scene->n_tiles);
+    W.n_buckets = ks.n_buckets;
     const int64_t kn = n > W.n_buckets ? n : W.n_buckets;
-    hipLaunchKernelGGL(k_ray_key, dim3((unsigned)((kn + 255) / 256)), dim3(256), 0, st, *scene, *rays, W);
+    hipLaunchKernelGGL(k_ray_key, dim3((unsigned)((kn + 255) / 256)), dim3(256), 0, st, *scene, *rays, W, ks);
     SNISR_LAUNCHED();
-    hipLaunchKernelGGL(k_key_hist, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W, n);
+    hipLaunchKernelGGL(k_key_hist, sort_grid, sort_block, 0, st, W, n);
     SNISR_LAUNCHED();
     hipLaunchKernelGGL(k_key_scan, dim3(1), dim3(kScanBlock), 0, st, W);
     SNISR_LAUNCHED();
-    hipLaunchKernelGGL(k_key_scatter, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W, n);
+    hipLaunchKernelGGL(k_key_scatter, sort_grid, sort_block, 0, st, W, n);
     SNISR_LAUNCHED();
     order = W.order;
   }
   SNISR_HIP(stage_event(out, 1, st));
-  hipLaunchKernelGGL(k_march_render, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, *scene, *params, *rays, *out,
-                     W, pw, dt, order);
-  SNISR_LAUNCHED();
-  SNISR_HIP(stage_event(out, 2, st));
-  hipLaunchKernelGGL(k_rtile_hist, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W);
-  SNISR_LAUNCHED();
-  hipLaunchKernelGGL(k_rtile_scan, dim3(1), dim3(kScanBlock), 0, st, W);
-  SNISR_LAUNCHED();
-  SNISR_HIP(stage_event(out, 3, st));
-  hipLaunchKernelGGL(k_rtile_scatter, dim3((unsigned)tn.sort_grid), dim3(kSortBlock), 0, st, W);
-  SNISR_LAUNCHED();
-  SNISR_HIP(stage_event(out, 4, st));
-
-  const dim3 grid((unsigned)device_cus()), block((unsigned)tn.mlp_threads);
-  bool launched = false;
-#define SNISR_MFMA_CASE(F_, W_, L_, H_, D_)                                                        \
-  if (!launched && F == F_ && WD == W_ && L == L_ && half == H_ && dt == D_) {                      \
-    auto kern = k_mlp_mfma<F_, W_, L_, H_, D_>;                                                     \
-    const size_t lds = (size_t)tile_bytes(F_ + 15, W_, L_);                                         \
-    static std::atomic<bool> lds_set{false};                                                        \
-    if (!lds_set.load()) {                                                                          \
-      SNISR_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
-      lds_set.store(true);                                                                          \
-    }                                                                                               \
-    hipLaunchKernelGGL(kern, grid, block, lds, st, *scene, *rays, W);                               \
-    launched = true;                                                                                \
+  if (k == 1) {
+    if (int e = launch_march(0, n, order)) return e;
+    SNISR_HIP(stage_event(out, 2, st));
+    if (int e = launch_shade(W, out->rgb, st, true)) return e;
+  } else {
+    // march chunk c on the given stream while the second stream sorts and shades chunk c - 1:
+    // the march is bound by dependent brick-table loads and the MLP by matrix-core issue, so
+    // the two share the CUs
+    ChunkStreams* cs = nullptr;
+    if (int e = chunk_streams(k + 1, &cs)) return e;
+    for (int c = 0; c < k; ++c) {
+      int64_t r0, r1;
+      chunk_range(n, unit, c, k, &r0, &r1);
+      if (int e = launch_march(r0, r1, nullptr)) return e;
+      SNISR_HIP(hipEventRecord(cs->ev[c], st));
+      SNISR_HIP(hipStreamWaitEvent(cs->aux, cs->ev[c], 0));
+      if (int e = launch_shade(chunk_view(W, c, r0, r1 - r0), out->rgb + 3 * r0, cs->aux, false)) return e;
+    }
+    SNISR_HIP(hipEventRecord(cs->ev[k], cs->aux));
+    SNISR_HIP(hipStreamWaitEvent(st, cs->ev[k], 0));
+    // the stages overlap: 2..5 are all taken at the join
+    for (int q = 2; q <= 5; ++q) SNISR_HIP(stage_event(out, q, st));
   }
-  SNISR_MFMA_CASE(32, 128, 4, true, SNISR_F16)
-  SNISR_MFMA_CASE(32, 128, 4, true, SNISR_BF16)
-  SNISR_MFMA_CASE(16, 64, 3, false, SNISR_F16)
-  SNISR_MFMA_CASE(16, 64, 3, false, SNISR_BF16)
-  SNISR_MFMA_CASE(32, 128, 4, false, SNISR_F16)
-#undef SNISR_MFMA_CASE
-  if (!launched) {
-    set_error("render_mfma: unsupported shape F=%d width=%d layers=%d feat_dtype=%d packed=%d", F, WD, L,
-              scene->feat_dtype, dt);
-    return SNISR_ERR_SHAPE;
-  }
-  SNISR_LAUNCHED();
-  SNISR_HIP(stage_event(out, 5, st));
-  hipLaunchKernelGGL(k_render_reduce, dim3((unsigned)((4 * n + 255) / 256)), dim3(256), 0, st, W, out->rgb);
-  SNISR_LAUNCHED();
   SNISR_HIP(stage_event(out, 6, st));
   return SNISR_OK;
 }

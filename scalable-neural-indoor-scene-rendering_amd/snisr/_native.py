@@ -79,7 +79,7 @@ class GraphDesc(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("ray_order", C.c_int32), ("mlp_threads", C.c_int32), ("sort_grid", C.c_int32),
-                ("records_per_ray", C.c_int32)]
+                ("records_per_ray", C.c_int32), ("chunks", C.c_int32)]
 
 
 _lib = None
@@ -95,6 +95,8 @@ SIGNATURES = [
     ("snisr_render_fp32", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
                                     C.POINTER(RenderOut), C.c_void_p]),
     ("snisr_render_mfma_workspace_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int64, C.c_int64, C.c_int64]),
+    ("snisr_render_mfma_workspace_size_tuned", C.c_int64, [C.POINTER(SceneDesc), C.c_int64, C.POINTER(Tuning),
+                                                           C.c_int64]),
     ("snisr_render_mfma", C.c_int, [C.POINTER(SceneDesc), C.POINTER(MarchParams), C.POINTER(RaysDesc),
                                     C.POINTER(RenderOut), C.c_void_p, C.c_int64, C.c_void_p]),
     ("snisr_render_mfma_tuned", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Tuning),
@@ -199,6 +201,8 @@ def lib():
     return L
 
 
+SNISR_OK, SNISR_ERR_ARG, SNISR_ERR_SHAPE, SNISR_ERR_ALIGN, SNISR_ERR_DEVICE, SNISR_ERR_CAPACITY, \
+    SNISR_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
 STATUS_NAMES = {-1: "SNISR_ERR_ARG", -2: "SNISR_ERR_SHAPE", -3: "SNISR_ERR_ALIGN", -4: "SNISR_ERR_DEVICE",
                 -5: "SNISR_ERR_CAPACITY", -6: "SNISR_ERR_UNSUPPORTED"}
 

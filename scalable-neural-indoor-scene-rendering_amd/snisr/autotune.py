@@ -5,8 +5,11 @@ each with HIP events on the current stream, checks that every schedule produced 
 and sample counts (the schedule must never change results), and keeps the fastest.  Results are
 cached in memory and, when ``cache_path`` is given, in a JSON file keyed by the workload signature.
 
-Candidates: ray_order in {0 (index / pixel-packet order), 1 (counting sort by first tile and
-direction octant)} x mlp_threads in {256, 512} x sort_grid in {256, 512, 1024}.
+Candidates: ray_order in {0 (index / pixel-packet order), 1 (counting sort by first occupied tile
+and direction octant), 2 (first tile, octant and origin cell), 3 (first tile and a finer origin
+cell)} x mlp_threads in {256, 512} x sort_grid in {256, 1024} x record slots per ray in {the
+workload's largest count + 25% ("tight"), twice the largest count ("loose")}, and for ray_order 0
+the rays split into 2 or 4 chunks whose march overlaps the previous chunk's sort and MLP.
 """
 from __future__ import annotations
 
@@ -18,8 +21,10 @@ import torch
 
 from . import device as D
 
-CANDIDATES = [dict(ray_order=o, mlp_threads=m, sort_grid=g)
-              for o, m, g in itertools.product((0, 1), (256, 512), (256, 512, 1024))]
+CANDIDATES = [dict(ray_order=o, mlp_threads=m, sort_grid=g, records=r)
+              for o, m, g, r in itertools.product((0, 1, 2, 3), (256, 512), (256, 1024), ("tight", "loose"))] + \
+             [dict(ray_order=0, mlp_threads=m, sort_grid=g, records=r, chunks=k)
+              for k, m, g, r in itertools.product((2, 4), (256, 512), (256, 1024), ("tight", "loose"))]
 _memo: dict = {}
 
 
@@ -44,6 +49,10 @@ def tune_render(ds: D.DeviceScene, origins, dirs, t_hit=None, image_width=0, wor
             return _memo[sig]
     ws = workspace or D.MfmaWorkspace()
     ref = D.render_mfma(ds, origins, dirs, t_hit=t_hit, image_width=image_width, workspace=ws, check=True)
+    loose = D.schedule_records(dict(records="loose"), ref["max_samples_per_ray"])
+    # one workspace holding the largest slot count of the candidates and the whole batch's samples
+    ws.get(ds, n, loose, int(ref["total_samples"] * 1.25) + 64,
+           max(int(c.get("chunks", 1)) for c in (candidates or CANDIDATES)))
     table = {}
     for cand in candidates or CANDIDATES:
         out = D.render_mfma(ds, origins, dirs, t_hit=t_hit, image_width=image_width, workspace=ws, check=False,

@@ -201,17 +201,21 @@ class MfmaWorkspace:
         self.buf = None
         self.cpr = 0
         self.samples = 0
+        self.chunks = 1
+        self.max_count = 0   # largest per-ray sample count seen by a checked render
 
-    def get(self, ds: DeviceScene, n: int, records_per_ray: int = 0, samples: int = 0):
+    def get(self, ds: DeviceScene, n: int, records_per_ray: int = 0, samples: int = 0, chunks: int = 1):
         cpr = max(int(records_per_ray), self.cpr) or 64
         cap = max(int(samples), self.samples) or 16 * max(n, 1)
-        need = N.lib().snisr_render_mfma_workspace_size(C.byref(ds.desc), n, cpr, cap)
+        k = max(int(chunks), self.chunks, 1)
+        tn = N.Tuning(0, 512, 512, cpr, k)
+        need = N.lib().snisr_render_mfma_workspace_size_tuned(C.byref(ds.desc), n, C.byref(tn), cap)
         if need < 0:
             raise RuntimeError(N.lib().snisr_last_error().decode())
         if self.buf is None or self.buf.numel() < need:
             self.buf = None
             self.buf = torch.empty(need, dtype=torch.uint8, device=ds.sigma.device)
-        self.cpr, self.samples = cpr, cap
+        self.cpr, self.samples, self.chunks = cpr, cap, k
         return self.buf, self.buf.numel()
 
 
@@ -229,11 +233,21 @@ def records_for(max_count: int, headroom: float = 1.25) -> int:
     return max(8, -(-int(max_count * headroom + 1) // 8) * 8)
 
 
+def schedule_records(t: dict | None, max_count: int = 0) -> int:
+    """Record slots per ray a schedule asks for: ``records`` "tight" (the largest per-ray count seen
+    + 25%), "loose" (twice it) or a number; 0 when the schedule leaves it to the workspace."""
+    r = (t or {}).get("records")
+    if r in ("tight", "loose"):
+        return records_for(max_count, 1.25 if r == "tight" else 2.0) if max_count else 0
+    return int(r) if r else 0
+
+
 def tuning_struct(t: dict | None, records_per_ray: int = 0):
-    """snisr_tuning from a dict {ray_order, mlp_threads, sort_grid} and the record slots per ray."""
+    """snisr_tuning from a dict {ray_order, mlp_threads, sort_grid, chunks} and the record slots per
+    ray."""
     t = t or {}
     return N.Tuning(int(t.get("ray_order", 0)), int(t.get("mlp_threads", 512)), int(t.get("sort_grid", 512)),
-                    int(records_per_ray))
+                    int(records_per_ray), int(t.get("chunks", 1)))
 
 
 def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_width=0, brick_cap=0,
@@ -256,21 +270,27 @@ def render_mfma(ds: DeviceScene, origins, dirs, t_hit=None, visited=None, image_
         o.n_samples, o.depth = out["n_samples"].data_ptr(), out["depth"].data_ptr()
     keep = _attach_events(o, events)  # noqa: F841
     ws = workspace or MfmaWorkspace()
-    buf, nbytes = ws.get(ds, r.n, capacity, sample_capacity)
-    tn = tuning_struct(tuning, ws.cpr)
+    rec = schedule_records(tuning, ws.max_count)
+    chunks = int((tuning or {}).get("chunks", 1))
+    buf, nbytes = ws.get(ds, r.n, max(int(capacity), rec), sample_capacity, chunks)
+    tn = tuning_struct(tuning, rec or ws.cpr)
     N.check(N.lib().snisr_render_mfma_tuned(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o), C.byref(tn),
                                             _ptr(buf), nbytes, _stream()), "render_mfma")
     if check:
         total, over, mx = mfma_stats(buf, with_max=True)
         if over:
-            buf, nbytes = ws.get(ds, r.n, records_for(mx), int(total * 1.25) + 64)
-            tn = tuning_struct(tuning, ws.cpr)
+            ws.max_count = max(ws.max_count, mx)
+            # the schedule's slots (e.g. twice the largest count) or the largest count + 25%
+            rec = max(schedule_records(tuning, ws.max_count), records_for(mx))
+            buf, nbytes = ws.get(ds, r.n, rec, int(total * 1.25) + 64, chunks)
+            tn = tuning_struct(tuning, rec if (tuning or {}).get("records") else ws.cpr)
             N.check(N.lib().snisr_render_mfma_tuned(C.byref(ds.desc), C.byref(p), C.byref(r), C.byref(o),
                                                     C.byref(tn), _ptr(buf), nbytes, _stream()), "render_mfma")
             total, over, mx = mfma_stats(buf, with_max=True)
             assert not over
         out["total_samples"] = total
         out["max_samples_per_ray"] = mx
+        ws.max_count = max(ws.max_count, mx)
     return out
 
 

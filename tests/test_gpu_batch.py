@@ -59,7 +59,8 @@ def test_autotuner_returns_fastest_valid_schedule(c4, tmp_path):
     assert best2 == best
 
 
-@pytest.mark.parametrize("tuning", [None, dict(ray_order=1, mlp_threads=256, sort_grid=256)])
+@pytest.mark.parametrize("tuning", [None, dict(ray_order=1, mlp_threads=256, sort_grid=256),
+                                    dict(ray_order=0, mlp_threads=256, sort_grid=256, chunks=3)])
 def test_graph_replays_match_eager(c4, tuning):
     sc, ds, o, d = c4
     eager = D.render_mfma(ds, o, d, check=True, tuning=tuning)
@@ -82,3 +83,48 @@ def test_graph_replays_match_eager(c4, tuning):
     torch.cuda.synchronize()
     assert not g.overflowed()
     assert torch.equal(out["rgb"], eager["rgb"][perm])
+
+
+@pytest.mark.parametrize("chunks", [2, 5, 64])
+def test_chunked_pixel_frame_equals_one_pass(chunks):
+    """Chunks of whole 8-row packet bands (a 256x96 image has 12 bands: 5 chunks split them
+    unevenly, 64 is clamped to 12) give the one-pass render bit for bit, and so does a ray count
+    that is not a whole number of 64-ray wavefronts."""
+    sc, cam, _ = S.config_c2(n_triangles=0, H=96, W=256)
+    cam.intrinsic[2] *= 5.0
+    ds = D.DeviceScene(sc, mfma_dtype=N.SNISR_F16)
+    dirs = torch.from_numpy(S.camera_rays_np(cam)).to(DEV)
+    o = torch.from_numpy(cam.origin).to(DEV)
+    ref = D.render_mfma(ds, o, dirs, image_width=cam.W, check=True)
+    out = D.render_mfma(ds, o, dirs, image_width=cam.W, check=True, tuning=dict(chunks=chunks, mlp_threads=256))
+    for k in ("rgb", "n_samples", "T", "depth"):
+        assert torch.equal(out[k], ref[k]), k
+    assert out["total_samples"] == ref["total_samples"]
+    # no packets: 64-ray chunks over 24565 rays (the last wavefront partly filled)
+    flat = dirs.reshape(-1, 3)[:24565].contiguous()
+    ref = D.render_mfma(ds, o, flat, check=True)
+    out = D.render_mfma(ds, o, flat, check=True, tuning=dict(chunks=chunks))
+    for k in ("rgb", "n_samples", "T", "depth"):
+        assert torch.equal(out[k], ref[k]), k
+
+
+def test_chunked_schedule_rejects_ray_sort_and_bad_counts(c4):
+    sc, ds, o, d = c4
+    with pytest.raises(N.SnisrError) as e:
+        D.render_mfma(ds, o, d, check=False, tuning=dict(ray_order=1, chunks=2))
+    assert e.value.status == N.SNISR_ERR_ARG
+    with pytest.raises(N.SnisrError) as e:
+        D.render_mfma(ds, o, d, check=False, tuning=dict(chunks=65))
+    assert e.value.status == N.SNISR_ERR_ARG
+
+
+@pytest.mark.parametrize("records", ["tight", "loose", 96])
+def test_record_schedules_recover_from_a_small_first_workspace(c4, records):
+    """A checked render that overflows its first workspace (8 slots per ray) is repeated with room
+    for the schedule's own slot count, which may exceed the largest count + 25%."""
+    sc, ds, o, d = c4
+    ref = D.render_mfma(ds, o, d, check=True)
+    ws = D.MfmaWorkspace()
+    out = D.render_mfma(ds, o, d, workspace=ws, capacity=8, check=True, tuning=dict(records=records))
+    assert torch.equal(out["rgb"], ref["rgb"]) and torch.equal(out["n_samples"], ref["n_samples"])
+    assert ws.cpr >= D.schedule_records(dict(records=records), ws.max_count)
