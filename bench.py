@@ -229,7 +229,10 @@ def c4_bench(args, dev):
     d = torch.from_numpy(d_np).to(dev)
     n = d.shape[0]
     ws = D.MfmaWorkspace()
-    best, table = autotune.tune_render(ds, o, d, workspace=ws, reps=2)
+    if args.c4_tuning == "auto":
+        best, table = autotune.tune_render(ds, o, d, workspace=ws, reps=2)
+    else:
+        best, table = json.loads(args.c4_tuning), {}
     # eager (default schedule and tuned) and graph replay of the tuned schedule
     def timed(fn, k):
         fn()
@@ -342,6 +345,8 @@ def main():
     ap.add_argument("--fp32-frames", type=int, default=3)
     ap.add_argument("--train-steps", type=int, default=10, help="timed C3 training steps (0 = skip)")
     ap.add_argument("--c4-steps", type=int, default=5, help="timed C4 batches (0 = skip)")
+    ap.add_argument("--c4-tuning", default="auto",
+                    help='C4 launch schedule: "auto" (autotuner) or a JSON dict, e.g. {} for the default')
     ap.add_argument("--c5-steps", type=int, default=3, help="timed C5 steps (0 = skip)")
     ap.add_argument("--ref-frames", type=int, default=10,
                     help="timed frames of the reference-format FRender pipeline (0 = skip)")

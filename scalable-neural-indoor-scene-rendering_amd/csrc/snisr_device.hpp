@@ -14,6 +14,11 @@
 namespace snisr {
 
 constexpr float kSafeDivBig = 100000000.0f;  // safe_divide's 1e8 (cutil_math.h:867-869)
+#ifndef SNISR_TILE_QUORUM
+#define SNISR_TILE_QUORUM 4
+#endif
+// march_ray_flat runs a tile step once 1/kTileQuorum of a wavefront's live lanes wait for it
+constexpr int kTileQuorum = SNISR_TILE_QUORUM;
 
 // Floats per tile of the FP32 "transposed" MLP packing: per layer Wt[in][out4] then b[out4].
 __host__ __device__ constexpr int64_t f32t_stride(int in, int width, int layers) {
@@ -355,7 +360,17 @@ __device__ __forceinline__ void march_ray_flat(const snisr_scene& S, const snisr
     far_ = fminx(fminx(tmax[0], tmax[1]), tmax[2]);
   };
   while (phase != kDone) {
+    // A wavefront of diverged lanes issues every phase that any of its lanes is in, each
+    // iteration.  The tile step (tile walk, slab test, twelve divisions) costs more vector
+    // instructions than the cell and sample steps together and each lane needs it only a few
+    // times, so a lane waiting for it idles until a quorum of the live lanes waits (or nothing
+    // else is left): the wavefront then issues it once for all of them.  Each lane's arithmetic
+    // is unchanged, only the interleaving of lanes.
+    const uint64_t live = __builtin_amdgcn_ballot_w64(true);
+    const uint64_t wait = __builtin_amdgcn_ballot_w64(phase == kTile);
+    const bool tile_turn = wait == live || kTileQuorum * __builtin_popcountll(wait) >= __builtin_popcountll(live);
     if (phase == kTile) {
+      if (!tile_turn) continue;
       if (vi >= P.max_tiles) { phase = kDone; continue; }
       const int t = vis ? vis[vi] : walker.next();
       ++vi;
