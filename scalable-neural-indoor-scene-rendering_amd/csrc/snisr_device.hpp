@@ -42,13 +42,8 @@ __device__ __forceinline__ float fmaxx(float a, float b) { return (a < b) ? b : 
 __device__ __forceinline__ float clampx(float v, float lo, float hi) { return fmaxx(lo, fminx(v, hi)); }
 __device__ __forceinline__ float safe_div(float a, float b) { return b != 0.0f ? a / b : kSafeDivBig; }
 
-// e^x by Cody-Waite reduction (x = k ln2 + r with ln2 split into a 16-bit head and a tail)
-// and the degree-7 Taylor polynomial of e^r evaluated by Horner's rule with FMAs, scaled by
-// 2^k.  Inputs below ln(FLT_MIN) give 0, above ln(FLT_MAX) give +inf.
-__device__ __forceinline__ float exp_cw(float x) {
-  if (!(x == x)) return x;
-  if (x < -87.33654475f) return 0.0f;
-  if (x > 88.72283905f) return __builtin_inff();
+// e^x for x in [ln(FLT_MIN), ln(FLT_MAX)] (the reduction and polynomial of exp_cw)
+__device__ __forceinline__ float exp_cw_in_range(float x) {
   const float k = __builtin_rintf(x * 0x1.715476p+0f);
   float r = __builtin_fmaf(-k, 0x1.62e400p-1f, x);
   r = __builtin_fmaf(-k, 0x1.7f7d1cp-20f, r);
@@ -61,6 +56,17 @@ __device__ __forceinline__ float exp_cw(float x) {
   p = __builtin_fmaf(p, r, 1.0f);
   p = __builtin_fmaf(p, r, 1.0f);
   return __builtin_ldexpf(p, (int)k);
+}
+
+// e^x by Cody-Waite reduction (x = k ln2 + r with ln2 split into a 16-bit head and a tail)
+// and the degree-7 Taylor polynomial of e^r evaluated by Horner's rule with FMAs, scaled by
+// 2^k.  Inputs below ln(FLT_MIN) give 0, above ln(FLT_MAX) give +inf, NaN gives itself.  Branch
+// free: the polynomial runs on the input clamped to that range and the ends are selected after.
+__device__ __forceinline__ float exp_cw(float x) {
+  const float lo = -87.33654475f, hi = 88.72283905f;
+  const float xc = x < lo ? lo : (x > hi ? hi : x);
+  const float e = exp_cw_in_range(xc);
+  return !(x == x) ? x : (x < lo ? 0.0f : (x > hi ? __builtin_inff() : e));
 }
 
 __device__ __forceinline__ float sigmoid_cw(float z) { return 1.0f / (1.0f + exp_cw(-z)); }
@@ -165,7 +171,9 @@ struct Corners {
 
 // Branch-free: all eight brick-table loads are issued together (a corner outside the tile reads
 // the tile's first entry, always present since n >= 8, and is then discarded), so one memory
-// round trip resolves the sample instead of eight dependent ones.
+// round trip resolves the sample instead of eight dependent ones.  The brick index and the
+// in-brick offset of a corner are sums of per-axis terms, formed once per axis in 32-bit integer
+// arithmetic; the indices are non-negative, so the loads use zero-extended offsets.
 __device__ __forceinline__ void find_corners(const int32_t* __restrict__ bmap, int n, V3 f, Corners& c) {
   const float fx = __builtin_floorf(f.x), fy = __builtin_floorf(f.y), fz = __builtin_floorf(f.z);
   const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
@@ -173,18 +181,34 @@ __device__ __forceinline__ void find_corners(const int32_t* __restrict__ bmap, i
   c.wy = f.y - fy;
   c.wz = f.z - fz;
   const int nb = n >> 3;
-  bool in[8];
+  const int i0[3] = {ix, iy, iz};
+  const int scale[3] = {nb * nb, nb, 1};
+  const int lscale[3] = {64, 8, 1};
+  bool in[3][2];
+  uint32_t bt[3][2], lt[3][2];  // brick-index and in-brick-offset terms per axis and corner side
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int x = i0[a] + s;
+      in[a][s] = (unsigned)x < (unsigned)n;
+      bt[a][s] = (uint32_t)((x >> 3) * scale[a]);
+      lt[a][s] = (uint32_t)((x & 7) * lscale[a]);
+    }
+  bool inside[8];
   int32_t b[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const int x = ix + (k & 1), y = iy + ((k >> 1) & 1), z = iz + (k >> 2);
-    in[k] = (unsigned)x < (unsigned)n && (unsigned)y < (unsigned)n && (unsigned)z < (unsigned)n;
-    b[k] = bmap[in[k] ? ((x >> 3) * nb + (y >> 3)) * nb + (z >> 3) : 0];
+    const int sx = k & 1, sy = (k >> 1) & 1, sz = k >> 2;
+    inside[k] = in[0][sx] && in[1][sy] && in[2][sz];
+    const uint32_t idx = inside[k] ? bt[0][sx] + bt[1][sy] + bt[2][sz] : 0u;
+    b[k] = bmap[idx];
   }
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const int x = ix + (k & 1), y = iy + ((k >> 1) & 1), z = iz + (k >> 2);
-    c.slot[k] = in[k] && b[k] >= 0 ? b[k] * 512 + (((x & 7) * 8 + (y & 7)) * 8 + (z & 7)) : -1;
+    const int sx = k & 1, sy = (k >> 1) & 1, sz = k >> 2;
+    const int32_t local = (int32_t)(lt[0][sx] + lt[1][sy] + lt[2][sz]);
+    c.slot[k] = inside[k] && b[k] >= 0 ? b[k] * 512 + local : -1;
   }
 }
 
@@ -213,7 +237,7 @@ __device__ __forceinline__ float tri_sigma(const float* __restrict__ sigma, cons
   float v[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const float s = sigma[c.slot[k] >= 0 ? c.slot[k] : 0];
+    const float s = sigma[c.slot[k] >= 0 ? (uint32_t)c.slot[k] : 0u];
     v[k] = c.slot[k] >= 0 ? s : 0.0f;
   }
   return tri_combine(v, c.wx, c.wy, c.wz);
