@@ -26,7 +26,7 @@ MFMA_PEAK_F32_TFLOPS = 157.3    # MI355X dense FP32 MFMA
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_mlp_mfma launch on this workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (factors from a 256 MiB device copy), see profiles/c2_render_kernels.md
-MLP_HBM_BYTES_PER_LAUNCH = 217.0e6 + 129.5e6
+MLP_HBM_BYTES_PER_LAUNCH = 217.9e6 + 129.5e6
 
 _T0 = time.perf_counter()
 
