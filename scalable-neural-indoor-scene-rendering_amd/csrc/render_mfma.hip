@@ -659,6 +659,11 @@ template <bool PACKET>
 __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_params P, snisr_rays R,
                                                      snisr_render_out O, RenderWs W, int packet_w, int enc_dtype,
                                                      const int32_t* __restrict__ order, int64_t tid0, int64_t tid1) {
+  // Records are staged per lane in LDS and written 8 at a time (128 bytes, one cache line when the
+  // slot count is a multiple of 8): lines written piecemeal by many rays in flight would leave L2
+  // partly written, several times over.
+  __shared__ float4 rbuf[8][64];
+  const int lane = threadIdx.x;
   const int64_t tid = tid0 + (int64_t)blockIdx.x * 64 + threadIdx.x;
   int ns = 0;
   if (tid < tid1) {
@@ -673,6 +678,13 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
     int k = 0, run_tile = -1, run_start = 0;
     MarchResult res;
     int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
+    // this lane's staged records 0..cnt-1 go to slots base.. of its ray
+    auto flush = [&](int base, int cnt) {
+      float4* dst = W.rec_f + slot0 + base;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (j < cnt) dst[j] = rbuf[j][lane];
+    };
     auto on_sample = [&](int tile, const V3& f, const Corners&, float w, float, float, float) {
                 if (k < cpr) {
                   if (tile != run_tile) {
@@ -680,7 +692,9 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
                     run_tile = tile;
                     run_start = k;
                   }
-                  W.rec_f[slot0 + k] = make_float4(f.x, f.y, f.z, w);
+                  const int j = k & 7;
+                  rbuf[j][lane] = make_float4(f.x, f.y, f.z, w);
+                  if (j == 7) flush(k - 7, 8);
                 }
                 ++k;
               };
@@ -688,7 +702,9 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
       march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, on_sample, res);
     else
       march_ray_flat(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, on_sample, res);
-    if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((min(k, cpr) - run_start) << 16);
+    const int nrec = min(k, cpr);
+    if (nrec & 7) flush(nrec & ~7, nrec & 7);
+    if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((nrec - run_start) << 16);
     if (bricks)
       for (int b = res.n_bricks; b < O.brick_cap; ++b) bricks[b] = -1;
     W.counts[i] = res.n_samples;
