@@ -189,3 +189,36 @@ def test_inference_mlp_blocks_with_every_pixel_stopped(oracle_mod):
     check_close(got, ref, live)
     dead = (T.reshape(-1)[:, None] < fr.trans_stop).repeat(spr, 1).reshape(-1)
     assert np.all(got.reshape(-1, 4)[dead] == SENTINEL)
+
+
+def test_packed_weights_follow_the_parameter_tensor():
+    """Two same-sized parameter tensors used one after the other, the first freed before the second
+    is made (so the caching allocator hands the second the first's memory): the second frame is
+    rendered with its own weights, equal to a render after the packing cache is cleared."""
+    H, W, spr = 40, 70, 5
+    fr = S.reflection_frame(seed=21, H=H, W=W, n_groups=3)
+    DH, DW = fr.net_idx.shape
+    rng = np.random.default_rng(22)
+    qpi, bnet, bps = S.reflection_blocks(fr.net_idx)
+    T = np.ones(DH * DW, np.float32)
+    samples = (fr.origin + rng.uniform(-4, 4, (DH * DW * spr, 3))).astype(np.float32)
+    rays = to_dev(fr.rays_d)[::2, ::2]
+
+    def render(params):
+        res = torch.full((DH, DW, spr, 4), SENTINEL, device=DEV)
+        FR.inference_mlp(spr, len(bnet), 256, 1.0 / fr.far, fr.trans_stop, to_dev(bnet), to_dev(T), to_dev(qpi),
+                         to_dev(bps), to_dev(fr.centers), to_dev(samples), rays, params, res)
+        torch.cuda.synchronize()
+        return res
+
+    a = to_dev(fr.params)
+    first = render(a)
+    addr = a.data_ptr()
+    del a
+    b = to_dev(fr.params) * torch.tensor(-0.5, dtype=torch.float16, device=DEV)   # different weights
+    second = render(b)
+    FR._packed_cache.clear()
+    fresh = render(b)
+    assert torch.equal(second, fresh)
+    assert not torch.equal(second, first)
+    print("second tensor reused the first's memory:", b.data_ptr() == addr)
