@@ -215,6 +215,7 @@ struct RenderWs {
   int32_t* rec_t;         // [n * cpr] at the first record of each tile run: tile | run length << 16
   int32_t* perm;          // [scap] record slots in tile order, 64-aligned buckets, -1 = padding
   int32_t* skip;          // 1 when the sorted samples exceed scap (render skipped, rgb NaN)
+  unsigned long long* cursor;  // per chunk: rays handed out by the queue-fed march
   int64_t n;
   int64_t scap;           // sorted positions: sample capacity + 64 per tile of bucket padding
   int32_t cpr;
@@ -238,6 +239,7 @@ static RenderWs chunk_view(const RenderWs& W, int c, int64_t r0, int64_t nc) {
   v.tile_cursor = W.tile_cursor + c * T;
   v.perm = W.perm + c * W.scap;
   v.skip = W.skip + c;
+  v.cursor = W.cursor + c;
   v.n = nc;
   v.chunks = 1;
   return v;
@@ -257,6 +259,7 @@ static int64_t render_layout(int64_t n, int64_t cpr, int64_t samples, int32_t T,
   const int64_t k = chunks > 0 ? chunks : 1;
   const int64_t slots = n * cpr, sorted = samples + 64 * (int64_t)T;   // per chunk
   const int64_t o_skip = off; off = align_up(off + 4 * k, 256);
+  const int64_t o_qc = off; off = align_up(off + 8 * k, 256);
   const int64_t o_cnt = off; off = align_up(off + 4 * n, 256);
   const int64_t o_enc = off; off = align_up(off + 32 * n, 256);
   const int64_t o_ord = off; off = align_up(off + 4 * n, 256);
@@ -274,6 +277,7 @@ static int64_t render_layout(int64_t n, int64_t cpr, int64_t samples, int32_t T,
     w->overflow = reinterpret_cast<int32_t*>(base + 8);
     w->max_count = reinterpret_cast<int32_t*>(base + 12);
     w->skip = reinterpret_cast<int32_t*>(base + o_skip);
+    w->cursor = reinterpret_cast<unsigned long long*>(base + o_qc);
     w->scap = sorted;
     w->counts = reinterpret_cast<int32_t*>(base + o_cnt);
     w->enc = reinterpret_cast<uint16_t*>(base + o_enc);
@@ -634,7 +638,10 @@ __global__ void k_render_begin(RenderWs W) {
     *W.overflow = 0;
     *W.max_count = 0;
   }
-  for (int c = threadIdx.x; c < W.chunks; c += blockDim.x) W.skip[c] = 0;
+  for (int c = threadIdx.x; c < W.chunks; c += blockDim.x) {
+    W.skip[c] = 0;
+    W.cursor[c] = 0;
+  }
   for (int t = threadIdx.x; t < W.chunks * W.n_tiles; t += blockDim.x) W.tile_count[t] = 0;
 }
 
@@ -652,10 +659,8 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 // One wavefront per 64 rays: march each ray once, writing its records (voxel coordinate + weight,
 // tile) at its own slots, its sample count, T, depth and view encoding; per-wave totals feed the
 // header (sample total, largest count, overflow when a ray has more samples than slots).
-// PACKET: camera rays in 8x8 pixel packets, marched with the nested tile / brick / sample loops;
-// otherwise (arbitrary rays, or a ray schedule) with the flattened loop.  One instantiation per
-// kernel keeps the other variant's code and scalar state out of it.
-template <bool PACKET>
+// Camera rays in 8x8 pixel packets (one packet per wavefront), marched with the nested tile / brick /
+// sample loops; arbitrary rays go through k_march_queue.
 __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_params P, snisr_rays R,
                                                      snisr_render_out O, RenderWs W, int packet_w, int enc_dtype,
                                                      const int32_t* __restrict__ order, int64_t tid0, int64_t tid1) {
@@ -698,10 +703,7 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
                 }
                 ++k;
               };
-    if (PACKET)
-      march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, on_sample, res);
-    else
-      march_ray_flat(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, on_sample, res);
+    march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, on_sample, res);
     const int nrec = min(k, cpr);
     if (nrec & 7) flush(nrec & ~7, nrec & 7);
     if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((nrec - run_start) << 16);
@@ -728,6 +730,120 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
     if (sum) atomicAdd((unsigned long long*)W.total, (unsigned long long)sum);
     if (mx) atomicMax(W.max_count, mx);
     if (mx > W.cpr) *W.overflow = 1;
+  }
+}
+
+// 15-value view encoding of every ray, in the MLP's operand type (the packet march writes it itself)
+__global__ __launch_bounds__(256) void k_ray_enc(snisr_rays R, RenderWs W, int enc_dtype, int64_t r0, int64_t r1) {
+  const int64_t i = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= r1) return;
+  const V3 d = normalize_dir(v3(R.dirs[3 * i], R.dirs[3 * i + 1], R.dirs[3 * i + 2]));
+  float enc[15];
+  view_encoding(d, enc);
+  uint16_t e16[16];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) e16[q] = to_bits(enc[q], enc_dtype);
+  e16[15] = 0;
+  uint4* dst = reinterpret_cast<uint4*>(W.enc + i * 16);
+  dst[0] = *reinterpret_cast<const uint4*>(e16);
+  dst[1] = *reinterpret_cast<const uint4*>(e16 + 8);
+}
+
+// The march of arbitrary rays (no pixel packets, or a ray schedule) as a queue: a wavefront's lanes
+// take rays (thread indices tid0 .. tid1 - 1, ray = order[tid] or tid) from a per-chunk counter and,
+// when a quarter of the lanes have finished theirs, take new ones together, so that a wavefront is
+// not held by its longest ray while most lanes idle.  Each ray is marched exactly as by
+// march_ray_flat (same steps, same records), only the lanes that run it change.  The grid is sized
+// to the device's resident wavefronts; every wavefront ends once the counter passes tid1 and its
+// lanes' rays are done.
+constexpr int kRefillQuorum = 4;
+__global__ __launch_bounds__(64) void k_march_queue(snisr_scene S, snisr_march_params P, snisr_rays R,
+                                                    snisr_render_out O, RenderWs W, const int32_t* __restrict__ order,
+                                                    int64_t tid0, int64_t tid1) {
+  __shared__ float4 rbuf[8][64];   // records staged per lane, written 8 at a time (see k_march_render)
+  const int lane = threadIdx.x;
+  const int cpr = W.cpr;
+  FlatMarch m;
+  m.phase = FlatMarch::kDone;
+  bool has = false, more = true;
+  int64_t i = 0, slot0 = 0;
+  int kr = 0, run_tile = -1, run_start = 0;
+  int32_t* bricks = nullptr;
+  int64_t sum = 0;
+  int mx = 0;
+  auto flush = [&](int base, int cnt) {
+    float4* dst = W.rec_f + slot0 + base;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j < cnt) dst[j] = rbuf[j][lane];
+  };
+  auto on_sample = [&](int tile, const V3& f, const Corners&, float w, float, float, float) {
+    if (kr < cpr) {
+      if (tile != run_tile) {
+        if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((kr - run_start) << 16);
+        run_tile = tile;
+        run_start = kr;
+      }
+      const int j = kr & 7;
+      rbuf[j][lane] = make_float4(f.x, f.y, f.z, w);
+      if (j == 7) flush(kr - 7, 8);
+    }
+    ++kr;
+  };
+  while (true) {
+    if (has && m.phase == FlatMarch::kDone) {  // finish this lane's ray
+      const int nrec = min(kr, cpr);
+      if (nrec & 7) flush(nrec & ~7, nrec & 7);
+      if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((nrec - run_start) << 16);
+      if (bricks)
+        for (int b = m.nbr; b < O.brick_cap; ++b) bricks[b] = -1;
+      W.counts[i] = m.nsamp;
+      if (O.T) O.T[i] = m.T;
+      if (O.n_samples) O.n_samples[i] = m.nsamp;
+      if (O.depth) O.depth[i] = m.depth;
+      sum += m.nsamp;
+      mx = m.nsamp > mx ? m.nsamp : mx;
+      has = false;
+    }
+    const uint64_t idle = __builtin_amdgcn_ballot_w64(!has && more);
+    const uint64_t busy = __builtin_amdgcn_ballot_w64(has);
+    if (idle && (busy == 0 || kRefillQuorum * __builtin_popcountll(idle) >= 64)) {
+      const int first = (int)__builtin_ctzll(idle);
+      unsigned long long base = 0;
+      if (lane == first) base = atomicAdd(W.cursor, (unsigned long long)__builtin_popcountll(idle));
+      base = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(base >> 32), first) << 32) |
+             (uint32_t)__shfl((int)(uint32_t)base, first);
+      if (!has && more) {
+        const int64_t t = tid0 + (int64_t)base + __builtin_popcountll(idle & ((1ull << lane) - 1));
+        if (t < tid1) {
+          i = order ? (int64_t)order[t] : t;
+          const float* og = R.origins + i * R.origin_stride;
+          const V3 d = normalize_dir(v3(R.dirs[3 * i], R.dirs[3 * i + 1], R.dirs[3 * i + 2]));
+          m.init(S, P, v3(og[0], og[1], og[2]), d, R.t_hit ? R.t_hit[i] : __builtin_inff(),
+                 R.visited ? R.visited + i * P.max_tiles : nullptr);
+          slot0 = i * (int64_t)cpr;
+          kr = 0;
+          run_tile = -1;
+          run_start = 0;
+          bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
+          has = true;
+        } else {
+          more = false;
+        }
+      }
+    }
+    if (__builtin_amdgcn_ballot_w64(has) == 0) break;
+    if (has) {
+      const bool turn = tile_quorum(m.phase == FlatMarch::kTile);
+      m.step(S, P, turn, bricks, O.brick_cap, on_sample);
+    }
+  }
+  const int64_t wsum = wave_sum_i64(sum);
+  const int wmx = wave_max_i32(mx);
+  if (lane == 0) {
+    if (wsum) atomicAdd((unsigned long long*)W.total, (unsigned long long)wsum);
+    if (wmx) atomicMax(W.max_count, wmx);
+    if (wmx > W.cpr) *W.overflow = 1;
   }
 }
 
@@ -1726,6 +1842,19 @@ int device_cus() {
   return v;
 }
 
+// resident wavefronts per CU of the queue-fed march (its grid covers the device once), per device
+static int march_queue_waves_per_cu() {
+  static std::atomic<int> per[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 8;
+  int v = per[dev].load(std::memory_order_relaxed);
+  if (v <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_march_queue, 64, 0) != hipSuccess || v <= 0) v = 8;
+    per[dev].store(v, std::memory_order_relaxed);
+  }
+  return v;
+}
+
 }  // namespace snisr
 
 using namespace snisr;
@@ -1847,14 +1976,21 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
   const dim3 sort_grid((unsigned)tn.sort_grid), sort_block(kSortBlock);
 
   // the shaded half of the render on one chunk's view: tile sort, MFMA MLP, per-ray reduction
-  auto launch_march = [&](int64_t r0, int64_t r1, const int32_t* order) -> int {
-    const dim3 g((unsigned)((r1 - r0 + 63) / 64));
-    if (pw > 0 && !order)
-      hipLaunchKernelGGL(k_march_render<true>, g, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt, order,
-                         r0, r1);
-    else
-      hipLaunchKernelGGL(k_march_render<false>, g, dim3(64), 0, st, *scene, *params, *rays, *out, W, pw, dt, order,
-                         r0, r1);
+  auto launch_march = [&](int64_t r0, int64_t r1, const int32_t* order, int c) -> int {
+    const int64_t units = (r1 - r0 + 63) / 64;
+    if (pw > 0 && !order) {
+      hipLaunchKernelGGL(k_march_render, dim3((unsigned)units), dim3(64), 0, st, *scene, *params, *rays, *out,
+                         W, pw, dt, order, r0, r1);
+      SNISR_LAUNCHED();
+      return SNISR_OK;
+    }
+    hipLaunchKernelGGL(k_ray_enc, dim3((unsigned)((r1 - r0 + 255) / 256)), dim3(256), 0, st, *rays, W, dt, r0, r1);
+    SNISR_LAUNCHED();
+    RenderWs Wq = W;
+    Wq.cursor = W.cursor + c;
+    const int64_t resident = (int64_t)device_cus() * march_queue_waves_per_cu();
+    hipLaunchKernelGGL(k_march_queue, dim3((unsigned)(units < resident ? units : resident)), dim3(64), 0, st,
+                       *scene, *params, *rays, *out, Wq, order, r0, r1);
     SNISR_LAUNCHED();
     return SNISR_OK;
   };
@@ -1923,7 +2059,7 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
   }
   SNISR_HIP(stage_event(out, 1, st));
   if (k == 1) {
-    if (int e = launch_march(0, n, order)) return e;
+    if (int e = launch_march(0, n, order, 0)) return e;
     SNISR_HIP(stage_event(out, 2, st));
     if (int e = launch_shade(W, out->rgb, st, true)) return e;
   } else {
@@ -1935,7 +2071,7 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
     for (int c = 0; c < k; ++c) {
       int64_t r0, r1;
       chunk_range(n, unit, c, k, &r0, &r1);
-      if (int e = launch_march(r0, r1, nullptr)) return e;
+      if (int e = launch_march(r0, r1, nullptr, c)) return e;
       SNISR_HIP(hipEventRecord(cs->ev[c], st));
       SNISR_HIP(hipStreamWaitEvent(cs->aux, cs->ev[c], 0));
       if (int e = launch_shade(chunk_view(W, c, r0, r1 - r0), out->rgb + 3 * r0, cs->aux, false)) return e;

@@ -17,7 +17,7 @@ constexpr float kSafeDivBig = 100000000.0f;  // safe_divide's 1e8 (cutil_math.h:
 #ifndef SNISR_TILE_QUORUM
 #define SNISR_TILE_QUORUM 4
 #endif
-// march_ray_flat runs a tile step once 1/kTileQuorum of a wavefront's live lanes wait for it
+// the flat march runs a tile step once 1/kTileQuorum of a wavefront's live lanes wait for it
 constexpr int kTileQuorum = SNISR_TILE_QUORUM;
 
 // Floats per tile of the FP32 "transposed" MLP packing: per layer Wt[in][out4] then b[out4].
@@ -348,32 +348,50 @@ __device__ __forceinline__ void march_ray(const snisr_scene& S, const snisr_marc
 }
 
 // march_ray with its three nested loops (tiles, bricks of a tile, samples of a brick) flattened into
-// one loop whose every iteration does one unit of work: set up the next tile, test / step one brick
-// cell, or take one sample.  Lanes of a wavefront that are in different phases then share
-// iterations instead of waiting for each other's inner loops.  The arithmetic and the order of the
-// tiles, cells, samples and callbacks are those of march_ray, so the results are identical.
-template <class OnSample>
-__device__ __forceinline__ void march_ray_flat(const snisr_scene& S, const snisr_march_params& P, V3 origin, V3 d,
-                                               float t_hit, const int32_t* __restrict__ vis, int32_t* bricks,
-                                               int brick_cap, OnSample&& on_sample, MarchResult& res) {
-  TileWalker walker;
-  if (!vis) walker.init(S, origin, d, P.max_tiles);
-  float T = 1.0f;
-  int nsamp = 0, nbr = 0;
-  float depth = -1.0f;
-  bool first_hit = false;
-  const float ts = S.tile_size;
-  const float step = P.sample_step;
-  const float hs = ts / 2.0f;
-  int vi = 0, tile = -1, n = 0, nb = 0, k = 0;
-  float tl = 0.0f, inv_vs = 0.0f, near_ = 0.0f, far_ = 0.0f;
-  V3 o = v3(0.0f, 0.0f, 0.0f);
-  const int32_t* __restrict__ bmap = nullptr;
-  int cur[3] = {0, 0, 0}, stp[3] = {1, 1, 1};
-  float tmax[3] = {0.0f, 0.0f, 0.0f}, tdel[3] = {0.0f, 0.0f, 0.0f};
+// a resumable state machine whose every step does one unit of work: set up the next tile, test /
+// step one brick cell, or take one sample.  Lanes of a wavefront that are in different phases then
+// share iterations instead of waiting for each other's inner loops, and a lane whose ray ended can
+// take a new ray while the others continue.  The arithmetic and the order of the tiles, cells,
+// samples and callbacks are those of march_ray, so the results are identical.
+struct FlatMarch {
   enum { kTile, kCell, kSample, kDone };
-  int phase = kTile;
-  auto dda_step = [&]() {
+  TileWalker walker;
+  V3 origin, d, o;
+  float t_hit, T, depth, tl, inv_vs, near_, far_;
+  const int32_t* vis;
+  const int32_t* bmap;
+  int nsamp, nbr, vi, tile, n, nb, k, phase;
+  bool first_hit;
+  int cur[3], stp[3];
+  float tmax[3], tdel[3];
+
+  __device__ __forceinline__ void init(const snisr_scene& S, const snisr_march_params& P, V3 origin_, V3 d_,
+                                       float t_hit_, const int32_t* __restrict__ vis_) {
+    origin = origin_;
+    d = d_;
+    t_hit = t_hit_;
+    vis = vis_;
+    if (!vis) walker.init(S, origin, d, P.max_tiles);
+    T = 1.0f;
+    nsamp = 0;
+    nbr = 0;
+    depth = -1.0f;
+    first_hit = false;
+    vi = 0;
+    tile = -1;
+    n = nb = k = 0;
+    tl = inv_vs = near_ = far_ = 0.0f;
+    o = v3(0.0f, 0.0f, 0.0f);
+    bmap = nullptr;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      cur[a] = 0;
+      stp[a] = 1;
+      tmax[a] = tdel[a] = 0.0f;
+    }
+    phase = kTile;
+  }
+  __device__ __forceinline__ void dda_step() {
     int a;
     if (tmax[0] < tmax[1]) a = (tmax[0] < tmax[2]) ? 0 : 2;
     else a = (tmax[1] < tmax[2]) ? 1 : 2;
@@ -382,27 +400,24 @@ __device__ __forceinline__ void march_ray_flat(const snisr_scene& S, const snisr
     else { cur[2] += stp[2]; tmax[2] += tdel[2]; }
     near_ = far_;
     far_ = fminx(fminx(tmax[0], tmax[1]), tmax[2]);
-  };
-  while (phase != kDone) {
-    // A wavefront of diverged lanes issues every phase that any of its lanes is in, each
-    // iteration.  The tile step (tile walk, slab test, twelve divisions) costs more vector
-    // instructions than the cell and sample steps together and each lane needs it only a few
-    // times, so a lane waiting for it idles until a quorum of the live lanes waits (or nothing
-    // else is left): the wavefront then issues it once for all of them.  Each lane's arithmetic
-    // is unchanged, only the interleaving of lanes.
-    const uint64_t live = __builtin_amdgcn_ballot_w64(true);
-    const uint64_t wait = __builtin_amdgcn_ballot_w64(phase == kTile);
-    const bool tile_turn = wait == live || kTileQuorum * __builtin_popcountll(wait) >= __builtin_popcountll(live);
+  }
+  // One unit of work.  A lane waiting for a tile step does nothing unless `tile_turn` (see
+  // march_ray_flat).
+  template <class OnSample>
+  __device__ __forceinline__ void step(const snisr_scene& S, const snisr_march_params& P, bool tile_turn,
+                                       int32_t* bricks, int brick_cap, OnSample&& on_sample) {
+    const float ts = S.tile_size;
+    const float hs = ts / 2.0f;
     if (phase == kTile) {
-      if (!tile_turn) continue;
-      if (vi >= P.max_tiles) { phase = kDone; continue; }
+      if (!tile_turn) return;
+      if (vi >= P.max_tiles) { phase = kDone; return; }
       const int t = vis ? vis[vi] : walker.next();
       ++vi;
-      if (t == -1 || T < P.early_stop) { phase = kDone; continue; }
+      if (t == -1 || T < P.early_stop) { phase = kDone; return; }
       const V3 tc = v3(S.tile_centers[3 * t], S.tile_centers[3 * t + 1], S.tile_centers[3 * t + 2]);
       float th;
       ray_aabb(origin, d, tc, v3(hs, hs, hs), tl, th);
-      if (tl == -1.0f) continue;
+      if (tl == -1.0f) return;
       tile = t;
       const V3 corner = v3(tc.x - hs, tc.y - hs, tc.z - hs);
       const float hi = ts - 0.00001f;
@@ -429,7 +444,7 @@ __device__ __forceinline__ void march_ray_flat(const snisr_scene& S, const snisr
     } else if (phase == kCell) {
       if (cur[0] < 0 || cur[1] < 0 || cur[2] < 0 || cur[0] >= nb || cur[1] >= nb || cur[2] >= nb) {
         phase = kTile;
-        continue;
+        return;
       }
       const int b = bmap[(cur[0] * nb + cur[1]) * nb + cur[2]];
       if (b != -1) {
@@ -440,15 +455,16 @@ __device__ __forceinline__ void march_ray_flat(const snisr_scene& S, const snisr
       } else {
         dda_step();
       }
-    } else {
+    } else if (phase == kSample) {
+      const float step = P.sample_step;
       const float z = __builtin_fmaf((float)k, step, near_);
       if (!(z < far_)) {
         dda_step();
         phase = kCell;
-        continue;
+        return;
       }
       const float tg = tl + z;
-      if (tg >= t_hit) { phase = kDone; continue; }
+      if (tg >= t_hit) { phase = kDone; return; }
       const V3 p = v3(o.x + z * d.x, o.y + z * d.y, o.z + z * d.z);
       const V3 f = v3(p.x * inv_vs - 0.5f, p.y * inv_vs - 0.5f, p.z * inv_vs - 0.5f);
       Corners c;
@@ -467,10 +483,39 @@ __device__ __forceinline__ void march_ray_flat(const snisr_scene& S, const snisr
       if (T < P.early_stop) phase = kDone;
     }
   }
-  res.T = T;
-  res.n_samples = nsamp;
-  res.depth = depth;
-  res.n_bricks = nbr;
+  __device__ __forceinline__ void result(MarchResult& res) const {
+    res.T = T;
+    res.n_samples = nsamp;
+    res.depth = depth;
+    res.n_bricks = nbr;
+  }
+};
+
+// Whether the lanes waiting for a tile step run it in this iteration: a wavefront of diverged lanes
+// issues every phase that any of its lanes is in, each iteration.  The tile step (tile walk, slab
+// test, twelve divisions) costs more vector instructions than the cell and sample steps together
+// and each lane needs it only a few times, so a lane waiting for it idles until a quorum of the live
+// lanes waits (or nothing else is left): the wavefront then issues it once for all of them.  Each
+// lane's arithmetic is unchanged, only the interleaving of lanes.  Called by every live lane.
+__device__ __forceinline__ bool tile_quorum(bool waiting) {
+  const uint64_t live = __builtin_amdgcn_ballot_w64(true);
+  const uint64_t wait = __builtin_amdgcn_ballot_w64(waiting);
+  return wait == live || kTileQuorum * __builtin_popcountll(wait) >= __builtin_popcountll(live);
+}
+
+template <class OnSample>
+__device__ __forceinline__ void march_ray_flat(const snisr_scene& S, const snisr_march_params& P, V3 origin, V3 d,
+                                               float t_hit, const int32_t* __restrict__ vis, int32_t* bricks,
+                                               int brick_cap, OnSample&& on_sample, MarchResult& res) {
+  FlatMarch m;
+  m.init(S, P, origin, d, t_hit, vis);
+  // the ballots at the head of the single loop also keep the compiler from turning the phase
+  // machine back into nested loops
+  while (m.phase != FlatMarch::kDone) {
+    const bool turn = tile_quorum(m.phase == FlatMarch::kTile);
+    m.step(S, P, turn, bricks, brick_cap, on_sample);
+  }
+  m.result(res);
 }
 
 // Camera packets (8x8 pixels per wavefront) walk nearly the same cells in the same order, so the
