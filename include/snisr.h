@@ -406,6 +406,31 @@ int snisr_train_backward(const snisr_scene* scene, const snisr_march_params* par
                          const float* grad_rgb, float* dsigma, float* dfeats, float* dmlp, void* workspace,
                          int64_t workspace_bytes, void* const* stage_events, int32_t n_stage_events, void* stream);
 
+/* A5 training form — utils.volume_rendering (training/scripts/utils.py:823-837), forward and
+ * hand-written adjoint.  Per ray r over its n samples (arrays [n_rays, n], rgb [n_rays, n, 3]):
+ * alpha_i = 1 - exp(-sigma_i dists_i), trans_i = init_T_r prod_{j<i} (1 - alpha_j + 1e-10) (exclusive
+ * cumprod), out_r = sum_i alpha_i trans_i rgb_i.  init_T NULL = 1 (the in-tile pass); the background
+ * pass chained behind the in-tile pass takes the in-tile trans[:, n-1] (tile.py:703-704).  Outputs
+ * out [n_rays, 3], trans and alpha [n_rays, n] (all three, as the reference returns them).  The
+ * backward takes the forward's trans and alpha and dL/dout [n_rays, 3] plus optional dL/dtrans and
+ * dL/dalpha (NULL = 0) and writes dL/drgb, dL/dsigma and, when g_init is not NULL, dL/dinit_T. */
+int snisr_volume_render_forward(const float* rgb, const float* sigma, const float* dists, const float* init_T,
+                                int64_t n_rays, int32_t n, float* out, float* trans, float* alpha, void* stream);
+int snisr_volume_render_backward(const float* rgb, const float* dists, const float* trans, const float* alpha,
+                                 int64_t n_rays, int32_t n, const float* g_out, const float* g_trans,
+                                 const float* g_alpha, float* g_rgb, float* g_sigma, float* g_init, void* stream);
+
+/* A7 training form — the specular head of Tile.inference_specular (training/scripts/tile/tile.py:
+ * 198-223) with eval_sh (training/scripts/sh.py:55-111): spec[s][c] = sigmoid(sum_b Y_b(d)
+ * coeffs[s][3 b + c]) for the n samples s (rows of coeff_stride floats, the MLP output whose first
+ * 3 (deg+1)^2 entries are [basis][rgb]), d the normalised direction dirs[s / samples_per_ray], real
+ * SH of degree deg <= 2.  The backward takes the forward's spec and dL/dspec and writes dL/dcoeffs
+ * into entries 0..26 of each row of g_coeffs (zero above the degree); other entries are untouched. */
+int snisr_sh_sigmoid_forward(const float* coeffs, int32_t coeff_stride, const float* dirs, int64_t n,
+                             int32_t samples_per_ray, int32_t deg, float* spec, void* stream);
+int snisr_sh_sigmoid_backward(const float* spec, const float* dirs, int64_t n, int32_t samples_per_ray, int32_t deg,
+                              const float* g_spec, float* g_coeffs, int32_t coeff_stride, void* stream);
+
 /* Training samplers of the reference's CUDA_EXT (training/src/Sample.cpp:97-158, kernels in
  * training/src/Sample_kernel.cu): per ray of one tile (centre cx, cy, cz; edge tile_size), the sample
  * depths of its training batch, written in place into z_vals f32[n, n_samples] (and dists for

@@ -498,3 +498,48 @@ def mesh_hit_tiles(vertices, faces, centers, tile_size, num_block, rays_o, rays_
                                  C.c_int(num_block), _p(o), _p(d), C.c_longlong(n), C.c_int(K), _p(vis), _p(hits),
                                  _p(face))
     return hits if mode == 1 else (hits, face)
+
+
+def volume_render(rgb, sigma, dists, init_T=None, g_out=None, g_trans=None, g_alpha=None):
+    """FP64 restatement of utils.volume_rendering (training/scripts/utils.py:823-837) and its adjoint.
+    rgb [B, N, 3], sigma and dists [B, N]; returns dict(out, trans, alpha) and, with g_out, the
+    gradients g_rgb, g_sigma (and g_init when init_T is given)."""
+    rgb = np.ascontiguousarray(rgb, np.float64)
+    B, n = rgb.shape[0], rgb.shape[1]
+    sig = np.ascontiguousarray(sigma, np.float64).reshape(B, n)
+    dst = np.ascontiguousarray(dists, np.float64).reshape(B, n)
+    it = None if init_T is None else np.ascontiguousarray(init_T, np.float64).reshape(B)
+    out, trans, alpha = np.empty((B, 3)), np.empty((B, n)), np.empty((B, n))
+    go = None if g_out is None else np.ascontiguousarray(g_out, np.float64).reshape(B, 3)
+    gt = None if g_trans is None else np.ascontiguousarray(g_trans, np.float64).reshape(B, n)
+    ga = None if g_alpha is None else np.ascontiguousarray(g_alpha, np.float64).reshape(B, n)
+    g_rgb, g_sig = np.zeros((B, n, 3)), np.zeros((B, n))
+    g_init = np.zeros(B) if it is not None else None
+    lib().orc_volume_render_f64(_p(rgb), _p(sig), _p(dst), _p(it) if it is not None else None, C.c_longlong(B),
+                                C.c_int(n), _p(go) if go is not None else None, _p(gt) if gt is not None else None,
+                                _p(ga) if ga is not None else None, _p(out), _p(trans), _p(alpha), _p(g_rgb),
+                                _p(g_sig), _p(g_init) if g_init is not None else None)
+    res = dict(out=out, trans=trans, alpha=alpha)
+    if go is not None:
+        res.update(g_rgb=g_rgb, g_sigma=g_sig)
+        if g_init is not None:
+            res["g_init"] = g_init
+    return res
+
+
+def sh_sigmoid(coeffs, dirs, deg, g_spec=None):
+    """FP64 restatement of sigmoid(eval_sh(deg, ...)) (tile.py:212-219, sh.py:55-111) for coefficient
+    rows [n, K] ([basis][rgb] first) at the normalised directions of the rows' rays (dirs [n_rays, 3],
+    n / n_rays rows per ray); with g_spec also dL/dcoeffs."""
+    q = np.ascontiguousarray(coeffs, np.float64)
+    K = q.shape[-1]
+    q2 = q.reshape(-1, K)
+    n = q2.shape[0]
+    d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    spr = n // d.shape[0]
+    spec = np.empty((n, 3))
+    gq = np.zeros((n, K))
+    gs = None if g_spec is None else np.ascontiguousarray(g_spec, np.float64).reshape(n, 3)
+    lib().orc_sh_sigmoid_f64(_p(q2), C.c_int(K), _p(d), C.c_longlong(n), C.c_int(spr), C.c_int(deg),
+                             _p(gs) if gs is not None else None, _p(spec), _p(gq))
+    return (spec, gq) if gs is not None else spec

@@ -1632,3 +1632,71 @@ void orc_mesh_hit_tiles_f32(int mode, const float* V, const int* Fc, int nf, con
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ A5 / A7 training forms
+// utils.volume_rendering (training/scripts/utils.py:823-837) in FP64 and its adjoint, per ray over
+// n samples: alpha = 1 - exp(-sigma d), trans = init * exclusive cumprod(1 - alpha + 1e-10),
+// out = sum alpha trans rgb.  The adjoint is the reverse recursion of the cumprod (the gradient of
+// L = <g_out, out> + <g_trans, trans> + <g_alpha, alpha>).
+extern "C" {
+
+void orc_volume_render_f64(const double* rgb, const double* sigma, const double* dists, const double* init_T,
+                           long long n_rays, int n, const double* g_out, const double* g_trans,
+                           const double* g_alpha, double* out, double* trans, double* alpha, double* g_rgb,
+                           double* g_sigma, double* g_init) {
+  for (long long r = 0; r < n_rays; ++r) {
+    double T = init_T ? init_T[r] : 1.0;
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < n; ++i) {
+      const long long k = r * n + i;
+      const double a = 1.0 - std::exp(-(sigma[k] * dists[k]));
+      for (int ch = 0; ch < 3; ++ch) c[ch] += a * T * rgb[3 * k + ch];
+      trans[k] = T;
+      alpha[k] = a;
+      T = T * (1.0 - a + 1e-10);
+    }
+    for (int ch = 0; ch < 3; ++ch) out[3 * r + ch] = c[ch];
+    if (!g_out) continue;
+    double G = 0.0;  // dL/dT_{i+1}
+    for (int i = n - 1; i >= 0; --i) {
+      const long long k = r * n + i;
+      const double Ti = trans[k], a = alpha[k], e = std::exp(-(sigma[k] * dists[k]));
+      double gc = 0.0;
+      for (int ch = 0; ch < 3; ++ch) {
+        gc += g_out[3 * r + ch] * rgb[3 * k + ch];
+        g_rgb[3 * k + ch] = g_out[3 * r + ch] * a * Ti;
+      }
+      const double d_alpha = Ti * gc + (g_alpha ? g_alpha[k] : 0.0);
+      const double d_e = -d_alpha + Ti * G;
+      g_sigma[k] = -dists[k] * e * d_e;
+      G = a * gc + (g_trans ? g_trans[k] : 0.0) + (e + 1e-10) * G;
+    }
+    if (g_init) g_init[r] = G;
+  }
+}
+
+// specular head of Tile.inference_specular (training/scripts/tile/tile.py:212-219, sh.py:55-111):
+// spec = sigmoid(sum_b Y_b(d) coeffs[3 b + c]) at the normalised direction of each sample's ray, and
+// the adjoint dL/dcoeffs[3 b + c] = g_c spec_c (1 - spec_c) Y_b (degree <= 2)
+void orc_sh_sigmoid_f64(const double* coeffs, int stride, const double* dirs, long long n, int spr, int deg,
+                        const double* g_spec, double* spec, double* g_coeffs) {
+  for (long long s = 0; s < n; ++s) {
+    const double* d = dirs + 3 * (s / spr);
+    const double len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double u[3] = {d[0] / len, d[1] / len, d[2] / len};
+    double b[9];
+    orc::sh2_basis(u, b);
+    const int nb = (deg + 1) * (deg + 1);
+    for (int j = nb; j < 9; ++j) b[j] = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      double z = 0.0;
+      for (int j = 0; j < nb; ++j) z += b[j] * coeffs[s * stride + 3 * j + c];
+      const double v = 1.0 / (1.0 + std::exp(-z));
+      spec[3 * s + c] = v;
+      if (g_spec)
+        for (int j = 0; j < 9; ++j) g_coeffs[s * stride + 3 * j + c] = g_spec[3 * s + c] * v * (1.0 - v) * b[j];
+    }
+  }
+}
+
+}  // extern "C"

@@ -175,6 +175,15 @@ SIGNATURES = [
                                     C.POINTER(C.c_int64)]),
     ("snisr_graph_synchronize", C.c_int, [C.c_void_p]),
     ("snisr_graph_destroy", C.c_int, [C.c_void_p]),
+    ("snisr_volume_render_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("snisr_volume_render_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p]),
+    ("snisr_sh_sigmoid_forward", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                                           C.c_void_p, C.c_void_p]),
+    ("snisr_sh_sigmoid_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
+                                            C.c_void_p, C.c_int32, C.c_void_p]),
     ("snisr_pack_mlp_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int32]),
     ("snisr_pack_mlp", C.c_int, [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p]),
 ]

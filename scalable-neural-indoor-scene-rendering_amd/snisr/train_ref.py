@@ -1,0 +1,111 @@
+"""The reference's training-form compositing and specular head as autograd Functions over the HIP
+library (forward and hand-written adjoint kernels, include/snisr.h):
+
+- ``volume_rendering(rgb, sigma, dists, init_T=None)`` is utils.volume_rendering
+  (training/scripts/utils.py:823-837): returns (out [B, 3], transparency [B, N, 1], alpha [B, N, 1])
+  with transparency the exclusive cumprod of (1 - alpha + 1e-10) started at init_T.  Passing the
+  in-tile pass's ``transparency[:, -1:]`` as init_T chains the background pass behind it
+  (training/scripts/tile/tile.py:703-704); the gradient flows into both passes.
+- ``sh_sigmoid(coeffs, viewdirs, deg)`` is the specular head of Tile.inference_specular
+  (training/scripts/tile/tile.py:212-219): sigmoid(eval_sh(deg, coeffs as [..., 3, (deg+1)^2],
+  normalised viewdirs)) for coefficients laid out [basis][rgb] as the MLP emits them; ``viewdirs``
+  is one direction per ray, shared by the ray's samples.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _native as N
+from .device import _ptr, _stream
+
+
+def _f32(t: torch.Tensor, name: str) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float32:
+        raise ValueError(f"{name} must be a float32 CUDA (HIP) tensor")
+    return t.contiguous()
+
+
+class _VolumeRendering(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rgb, sigma, dists, init_T):
+        B, n = sigma.shape[0], sigma.shape[1]
+        rgb_c, sig_c, dst_c = _f32(rgb, "rgb"), _f32(sigma, "sigma"), _f32(dists, "dists")
+        if rgb_c.numel() != 3 * B * n or dst_c.numel() != B * n or sig_c.numel() != B * n:
+            raise ValueError("rgb [B, N, 3], sigma and dists [B, N(, 1)] must agree")
+        it = None if init_T is None else _f32(init_T, "init_T")
+        if it is not None and it.numel() != B:
+            raise ValueError("init_T must hold one value per ray")
+        dev = rgb.device
+        out = torch.empty((B, 3), dtype=torch.float32, device=dev)
+        trans = torch.empty((B, n, 1), dtype=torch.float32, device=dev)
+        alpha = torch.empty((B, n, 1), dtype=torch.float32, device=dev)
+        N.check(N.lib().snisr_volume_render_forward(_ptr(rgb_c), _ptr(sig_c), _ptr(dst_c), _ptr(it), B, n, _ptr(out),
+                                                    _ptr(trans), _ptr(alpha), _stream()), "volume_render_forward")
+        ctx.save_for_backward(rgb_c, dst_c, trans, alpha)
+        ctx.init_shape = None if init_T is None else init_T.shape
+        ctx.sigma_shape = sigma.shape
+        return out, trans, alpha
+
+    @staticmethod
+    def backward(ctx, g_out, g_trans, g_alpha):
+        rgb, dists, trans, alpha = ctx.saved_tensors
+        B, n = trans.shape[0], trans.shape[1]
+        dev = rgb.device
+        g_out = torch.zeros((B, 3), dtype=torch.float32, device=dev) if g_out is None else g_out.contiguous()
+        gt = None if g_trans is None else g_trans.contiguous()
+        ga = None if g_alpha is None else g_alpha.contiguous()
+        g_rgb = torch.empty_like(rgb)
+        g_sig = torch.empty((B, n), dtype=torch.float32, device=dev)
+        g_init = torch.empty(B, dtype=torch.float32, device=dev) if ctx.init_shape is not None else None
+        N.check(N.lib().snisr_volume_render_backward(_ptr(rgb), _ptr(dists), _ptr(trans), _ptr(alpha), B, n,
+                                                     _ptr(g_out), _ptr(gt), _ptr(ga), _ptr(g_rgb), _ptr(g_sig),
+                                                     _ptr(g_init), _stream()), "volume_render_backward")
+        g_init_out = None if g_init is None else g_init.reshape(ctx.init_shape)
+        return g_rgb, g_sig.reshape(ctx.sigma_shape), None, g_init_out
+
+
+def volume_rendering(rgb: torch.Tensor, sigma: torch.Tensor, dists: torch.Tensor, init_T=None):
+    """utils.volume_rendering on the HIP kernels: rgb [B, N, 3], sigma and dists [B, N, 1] (or
+    [B, N]); init_T a float (1 by default) or a [B, 1] tensor.  Returns (out, transparency, alpha)."""
+    if init_T is not None and not isinstance(init_T, torch.Tensor):
+        if float(init_T) == 1.0:
+            init_T = None
+        else:
+            init_T = torch.full((sigma.shape[0], 1), float(init_T), dtype=torch.float32, device=sigma.device)
+    return _VolumeRendering.apply(rgb, sigma, dists, init_T)
+
+
+class _SHSigmoid(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, coeffs, dirs, deg, spr):
+        q = _f32(coeffs, "coeffs")
+        d = _f32(dirs, "viewdirs")
+        n = q.numel() // q.shape[-1]
+        stride = q.shape[-1]
+        if d.numel() * spr != 3 * n:
+            raise ValueError("viewdirs must hold one direction per ray of samples_per_ray samples")
+        spec = torch.empty((*q.shape[:-1], 3), dtype=torch.float32, device=q.device)
+        N.check(N.lib().snisr_sh_sigmoid_forward(_ptr(q), stride, _ptr(d), n, spr, deg, _ptr(spec), _stream()),
+                "sh_sigmoid_forward")
+        ctx.save_for_backward(spec, d)
+        ctx.meta = (n, spr, deg, stride, q.shape)
+        return spec
+
+    @staticmethod
+    def backward(ctx, g_spec):
+        spec, d = ctx.saved_tensors
+        n, spr, deg, stride, shape = ctx.meta
+        g = torch.zeros(shape, dtype=torch.float32, device=spec.device)
+        N.check(N.lib().snisr_sh_sigmoid_backward(_ptr(spec), _ptr(d), n, spr, deg, _ptr(g_spec.contiguous()), _ptr(g),
+                                                  stride, _stream()), "sh_sigmoid_backward")
+        return g, None, None, None
+
+
+def sh_sigmoid(coeffs: torch.Tensor, viewdirs: torch.Tensor, deg: int) -> torch.Tensor:
+    """sigmoid(eval_sh(deg, ...)) of the MLP output rows ``coeffs`` [B, S, K] (K >= 27: [basis][rgb]
+    then e.g. sigma) at the rays' directions ``viewdirs`` [B, 3] (normalised inside).  Returns [B, S, 3];
+    the gradient reaches coeffs[..., :27]."""
+    if coeffs.shape[-1] < 27:
+        raise ValueError("coeffs rows need 27 entries ([basis][rgb] of degree 2)")
+    spr = coeffs.numel() // coeffs.shape[-1] // max(viewdirs.numel() // 3, 1)
+    return _SHSigmoid.apply(coeffs, viewdirs, int(deg), int(spr))

@@ -1,0 +1,100 @@
+"""Pins of the oracle's training-form compositing (utils.volume_rendering) and specular head
+(sigmoid of degree-2 SH) and of their adjoints: closed forms, torch autograd in FP64 as an
+independent gradient, and central finite differences."""
+import numpy as np
+import pytest
+import torch
+
+
+def _ref_volume_rendering(rgb, sigma, dists, init_T):
+    """torch FP64 evaluation of the composite (exclusive cumprod of 1 - alpha + 1e-10)."""
+    alpha = 1.0 - torch.exp(-sigma * dists)
+    B = rgb.shape[0]
+    lead = init_T.reshape(B, 1)
+    T = torch.cumprod(torch.cat([lead, 1.0 - alpha + 1e-10], 1), 1)[:, :-1]
+    return (alpha[..., None] * T[..., None] * rgb).sum(1), T, alpha
+
+
+def test_constant_sigma_closed_form(oracle_mod):
+    B, n = 4, 9
+    rng = np.random.default_rng(1)
+    sig = np.repeat(rng.uniform(0.5, 4.0, (B, 1)), n, 1)
+    dst = np.full((B, n), 0.07)
+    init = rng.uniform(0.3, 1.0, B)
+    col = rng.uniform(0, 1, (B, 3))
+    rgb = np.repeat(col[:, None, :], n, 1)
+    r = oracle_mod.volume_render(rgb, sig, dst, init_T=init)
+    e = np.exp(-sig[:, 0] * 0.07)
+    T = init[:, None] * (e[:, None] + 1e-10) ** np.arange(n)[None, :]
+    np.testing.assert_allclose(r["trans"], T, rtol=1e-12)
+    np.testing.assert_allclose(r["alpha"], np.repeat((1 - e)[:, None], n, 1), rtol=1e-12)
+    np.testing.assert_allclose(r["out"], col * ((1 - e) * T.sum(1))[:, None], rtol=1e-12)
+
+
+@pytest.mark.parametrize("with_init", [False, True])
+def test_volume_render_adjoint_vs_torch_autograd(oracle_mod, with_init):
+    B, n = 5, 12
+    rng = np.random.default_rng(2 + with_init)
+    rgb, sig, dst = rng.uniform(0, 1, (B, n, 3)), rng.uniform(0, 6, (B, n)), rng.uniform(0, 0.4, (B, n))
+    dst[:, -1] = 1e10      # the reference's last interval
+    init = rng.uniform(0.1, 1.0, B) if with_init else np.ones(B)
+    g_out, g_tr, g_al = rng.normal(size=(B, 3)), rng.normal(size=(B, n)), rng.normal(size=(B, n))
+    r = oracle_mod.volume_render(rgb, sig, dst, init_T=init if with_init else None, g_out=g_out, g_trans=g_tr,
+                                 g_alpha=g_al)
+    t = [torch.tensor(a, dtype=torch.float64, requires_grad=True) for a in (rgb, sig, init)]
+    out, T, alpha = _ref_volume_rendering(t[0], t[1], torch.tensor(dst), t[2])
+    L = (out * torch.tensor(g_out)).sum() + (T * torch.tensor(g_tr)).sum() + (alpha * torch.tensor(g_al)).sum()
+    L.backward()
+    np.testing.assert_allclose(r["out"], out.detach().numpy(), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(r["g_rgb"], t[0].grad.numpy(), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(r["g_sigma"], t[1].grad.numpy(), rtol=1e-9, atol=1e-12)
+    if with_init:
+        np.testing.assert_allclose(r["g_init"], t[2].grad.numpy(), rtol=1e-9, atol=1e-12)
+
+
+def test_volume_render_adjoint_vs_finite_differences(oracle_mod):
+    B, n = 2, 6
+    rng = np.random.default_rng(4)
+    rgb, sig, dst = rng.uniform(0, 1, (B, n, 3)), rng.uniform(0.1, 3, (B, n)), rng.uniform(0.05, 0.3, (B, n))
+    init = rng.uniform(0.2, 1.0, B)
+    g_out = rng.normal(size=(B, 3))
+    r = oracle_mod.volume_render(rgb, sig, dst, init_T=init, g_out=g_out)
+    L = lambda rg, sg, it: float((oracle_mod.volume_render(rg, sg, dst, init_T=it)["out"] * g_out).sum())
+    eps = 1e-6
+    for (b, i) in [(0, 0), (0, 3), (1, 5)]:
+        sp, sm = sig.copy(), sig.copy()
+        sp[b, i] += eps
+        sm[b, i] -= eps
+        fd = (L(rgb, sp, init) - L(rgb, sm, init)) / (2 * eps)
+        assert abs(fd - r["g_sigma"][b, i]) <= 1e-7 * max(1.0, abs(fd))
+    for b in range(B):
+        ip, im = init.copy(), init.copy()
+        ip[b] += eps
+        im[b] -= eps
+        fd = (L(rgb, sig, ip) - L(rgb, sig, im)) / (2 * eps)
+        assert abs(fd - r["g_init"][b]) <= 1e-7 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2])
+def test_sh_sigmoid_forward_and_adjoint(oracle_mod, deg):
+    rng = np.random.default_rng(5 + deg)
+    n_rays, spr, K = 3, 4, 28
+    q = rng.normal(size=(n_rays * spr, K))
+    dirs = rng.normal(size=(n_rays, 3))
+    g = rng.normal(size=(n_rays * spr, 3))
+    spec, gq = oracle_mod.sh_sigmoid(q, dirs, deg, g_spec=g)
+    u = dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
+    Y = oracle_mod.sh2_basis(np.repeat(u, spr, 0))      # pinned against scipy's complex SH
+    nb = (deg + 1) ** 2
+    z = np.einsum("sb,sbc->sc", Y[:, :nb], q[:, :27].reshape(-1, 9, 3)[:, :nb])
+    np.testing.assert_allclose(spec, 1 / (1 + np.exp(-z)), rtol=1e-12)
+    assert not gq[:, 27].any() and not gq[:, 3 * nb:27].any()
+    eps = 1e-6
+    for (s, j) in [(0, 0), (5, 3 * nb - 1), (11, 4)]:
+        if j >= 3 * nb:
+            continue
+        qp, qm = q.copy(), q.copy()
+        qp[s, j] += eps
+        qm[s, j] -= eps
+        fd = ((oracle_mod.sh_sigmoid(qp, dirs, deg) - oracle_mod.sh_sigmoid(qm, dirs, deg)) * g).sum() / (2 * eps)
+        assert abs(fd - gq[s, j]) <= 1e-7 * max(1.0, abs(fd))
