@@ -431,6 +431,22 @@ int snisr_sh_sigmoid_forward(const float* coeffs, int32_t coeff_stride, const fl
 int snisr_sh_sigmoid_backward(const float* spec, const float* dirs, int64_t n, int32_t samples_per_ray, int32_t deg,
                               const float* g_spec, float* g_coeffs, int32_t coeff_stride, void* stream);
 
+/* A7/A11 training form — networks.MLP (training/scripts/networks.py:4-41), FP32 forward and
+ * hand-written adjoint: Positional_Encoding with L = 10 frequencies of a 3-d point ([x, sin(2^0 x),
+ * cos(2^0 x), ..., cos(2^9 x)], 63 values), then `depth` Linear layers of width 64 with LeakyReLU(slope)
+ * between them (slope 0 = ReLU) and n_out outputs: the surface MLP (depth 8, n_out 4, ReLU) and the
+ * reflection MLP (depth 6, n_out 28, LeakyReLU 0.01) of the reference's configuration
+ * (training/config/cfg.yaml:38-46).  params: snisr_pemlp_param_count floats in nn.Linear order (per
+ * layer W[out][in], then b[out]).  points f32[n, 3], out f32[n, n_out].  The forward stores the layer
+ * inputs for the backward in `workspace` (snisr_pemlp_workspace_size bytes; NULL for inference);
+ * the backward takes dL/dout and the forward's workspace and writes dL/dparams (overwritten). */
+int64_t snisr_pemlp_param_count(int32_t L, int32_t depth, int32_t n_out);
+int64_t snisr_pemlp_workspace_size(int64_t n, int32_t L, int32_t depth, int32_t n_out);
+int snisr_pemlp_forward(const float* params, const float* points, int64_t n, int32_t L, int32_t depth, int32_t n_out,
+                        float slope, float* out, void* workspace, int64_t workspace_bytes, void* stream);
+int snisr_pemlp_backward(const float* params, int64_t n, int32_t L, int32_t depth, int32_t n_out, float slope,
+                         const float* g_out, float* g_params, void* workspace, int64_t workspace_bytes, void* stream);
+
 /* Training samplers of the reference's CUDA_EXT (training/src/Sample.cpp:97-158, kernels in
  * training/src/Sample_kernel.cu): per ray of one tile (centre cx, cy, cz; edge tile_size), the sample
  * depths of its training batch, written in place into z_vals f32[n, n_samples] (and dists for

@@ -543,3 +543,16 @@ def sh_sigmoid(coeffs, dirs, deg, g_spec=None):
     lib().orc_sh_sigmoid_f64(_p(q2), C.c_int(K), _p(d), C.c_longlong(n), C.c_int(spr), C.c_int(deg),
                              _p(gs) if gs is not None else None, _p(spec), _p(gq))
     return (spec, gq) if gs is not None else spec
+
+
+def pemlp(params, points, depth, n_out, slope, g_out=None):
+    """FP64 networks.MLP (L = 10, width 64) forward; with g_out also the parameter gradient."""
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    out = np.empty((n, n_out))
+    go = None if g_out is None else np.ascontiguousarray(g_out, np.float64).reshape(n, n_out)
+    gp = np.zeros_like(p) if go is not None else None
+    lib().orc_pemlp_f64(_p(p), _p(x), C.c_longlong(n), C.c_int(depth), C.c_int(n_out), C.c_double(slope),
+                        _p(go) if go is not None else None, _p(out), _p(gp) if gp is not None else None)
+    return (out, gp) if go is not None else out

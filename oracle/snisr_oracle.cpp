@@ -1700,3 +1700,65 @@ void orc_sh_sigmoid_f64(const double* coeffs, int stride, const double* dirs, lo
 }
 
 }  // extern "C"
+
+// networks.MLP (training/scripts/networks.py:4-41) in FP64 with its adjoint: Positional_Encoding
+// with L = 10 ([x, sin(2^0 x), cos(2^0 x), ..., cos(2^9 x)]), depth Linear layers of width 64 with
+// LeakyReLU(slope) between them and n_out outputs; params in nn.Linear order (W[out][in], b[out]).
+// With g_out, g_params receives the gradient of <g_out, out> (accumulated over the samples).
+extern "C" {
+
+void orc_pemlp_f64(const double* params, const double* pts, long long n, int depth, int n_out, double slope,
+                   const double* g_out, double* out, double* g_params) {
+  const int W = 64, L = 10, in0 = 3 + 6 * L;
+  std::vector<long long> off(depth + 1, 0);
+  for (int l = 0; l < depth; ++l) {
+    const int ni = l == 0 ? in0 : W, no = l == depth - 1 ? n_out : W;
+    off[l + 1] = off[l] + (long long)no * ni + no;
+  }
+  if (g_params) std::fill(g_params, g_params + off[depth], 0.0);
+  std::vector<std::vector<double>> act(depth + 1);
+  for (long long s = 0; s < n; ++s) {
+    std::vector<double>& x0 = act[0];
+    x0.assign(in0, 0.0);
+    for (int j = 0; j < 3; ++j) x0[j] = pts[3 * s + j];
+    for (int f = 0; f < L; ++f)
+      for (int j = 0; j < 3; ++j) {
+        const double a = std::ldexp(pts[3 * s + j], f);
+        x0[3 + 6 * f + j] = std::sin(a);
+        x0[6 + 6 * f + j] = std::cos(a);
+      }
+    for (int l = 0; l < depth; ++l) {
+      const int ni = l == 0 ? in0 : W, no = l == depth - 1 ? n_out : W;
+      const double* Wl = params + off[l];
+      std::vector<double>& y = act[l + 1];
+      y.assign(no, 0.0);
+      for (int o = 0; o < no; ++o) {
+        double acc = Wl[(long long)no * ni + o];
+        for (int i = 0; i < ni; ++i) acc += Wl[(long long)o * ni + i] * act[l][i];
+        y[o] = (l == depth - 1) ? acc : (acc > 0.0 ? acc : slope * acc);
+      }
+    }
+    for (int o = 0; o < n_out; ++o) out[s * n_out + o] = act[depth][o];
+    if (!g_out) continue;
+    std::vector<double> d(g_out + s * n_out, g_out + (s + 1) * n_out);
+    for (int l = depth - 1; l >= 0; --l) {
+      const int ni = l == 0 ? in0 : W, no = l == depth - 1 ? n_out : W;
+      const double* Wl = params + off[l];
+      double* Gl = g_params + off[l];
+      for (int o = 0; o < no; ++o) {
+        for (int i = 0; i < ni; ++i) Gl[(long long)o * ni + i] += d[o] * act[l][i];
+        Gl[(long long)no * ni + o] += d[o];
+      }
+      if (l == 0) break;
+      std::vector<double> dp(ni, 0.0);
+      for (int i = 0; i < ni; ++i) {
+        double acc = 0.0;
+        for (int o = 0; o < no; ++o) acc += Wl[(long long)o * ni + i] * d[o];
+        dp[i] = acc * (act[l][i] > 0.0 ? 1.0 : slope);   // act[l] = leaky(z): same sign as z
+      }
+      d.swap(dp);
+    }
+  }
+}
+
+}  // extern "C"

@@ -184,6 +184,12 @@ SIGNATURES = [
                                            C.c_void_p, C.c_void_p]),
     ("snisr_sh_sigmoid_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
                                             C.c_void_p, C.c_int32, C.c_void_p]),
+    ("snisr_pemlp_param_count", C.c_int64, [C.c_int32, C.c_int32, C.c_int32]),
+    ("snisr_pemlp_workspace_size", C.c_int64, [C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
+    ("snisr_pemlp_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                      C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("snisr_pemlp_backward", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     ("snisr_pack_mlp_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int32]),
     ("snisr_pack_mlp", C.c_int, [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p]),
 ]

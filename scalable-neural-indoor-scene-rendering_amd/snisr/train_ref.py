@@ -6,6 +6,9 @@ library (forward and hand-written adjoint kernels, include/snisr.h):
   with transparency the exclusive cumprod of (1 - alpha + 1e-10) started at init_T.  Passing the
   in-tile pass's ``transparency[:, -1:]`` as init_T chains the background pass behind it
   (training/scripts/tile/tile.py:703-704); the gradient flows into both passes.
+- ``PEMLP`` is networks.MLP (training/scripts/networks.py:4-41) with L = 10, width 64: the surface
+  MLP (depth 8, 4 outputs, ReLU) and the reflection MLP (depth 6, 28 outputs, LeakyReLU 0.01), its
+  parameters one flat tensor in nn.Linear order, forward and backward on the HIP kernels.
 - ``sh_sigmoid(coeffs, viewdirs, deg)`` is the specular head of Tile.inference_specular
   (training/scripts/tile/tile.py:212-219): sigmoid(eval_sh(deg, coeffs as [..., 3, (deg+1)^2],
   normalised viewdirs)) for coefficients laid out [basis][rgb] as the MLP emits them; ``viewdirs``
@@ -109,3 +112,62 @@ def sh_sigmoid(coeffs: torch.Tensor, viewdirs: torch.Tensor, deg: int) -> torch.
         raise ValueError("coeffs rows need 27 entries ([basis][rgb] of degree 2)")
     spr = coeffs.numel() // coeffs.shape[-1] // max(viewdirs.numel() // 3, 1)
     return _SHSigmoid.apply(coeffs, viewdirs, int(deg), int(spr))
+
+
+class _PEMLP(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, params, points, depth, n_out, slope):
+        p = _f32(params, "params")
+        x = _f32(points, "points")
+        n = x.numel() // 3
+        L = N.lib()
+        if p.numel() != L.snisr_pemlp_param_count(10, depth, n_out):
+            raise ValueError("params do not match the MLP shape")
+        out = torch.empty((*points.shape[:-1], n_out), dtype=torch.float32, device=x.device)
+        ws = None
+        if ctx.needs_input_grad[0]:               # keep the layer rows for the backward
+            ws = torch.empty(max(L.snisr_pemlp_workspace_size(n, 10, depth, n_out), 16), dtype=torch.uint8,
+                             device=x.device)
+        N.check(L.snisr_pemlp_forward(_ptr(p), _ptr(x), n, 10, depth, n_out, float(slope), _ptr(out), _ptr(ws),
+                                      0 if ws is None else ws.numel(), _stream()), "pemlp_forward")
+        ctx.save_for_backward(p)
+        ctx.ws, ctx.meta = ws, (n, depth, n_out, float(slope))
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        (p,) = ctx.saved_tensors
+        n, depth, n_out, slope = ctx.meta
+        if ctx.ws is None:
+            raise RuntimeError("PEMLP backward needs a forward run with gradients enabled")
+        g = torch.empty_like(p)
+        N.check(N.lib().snisr_pemlp_backward(_ptr(p), n, 10, depth, n_out, slope, _ptr(g_out.contiguous()), _ptr(g),
+                                             _ptr(ctx.ws), ctx.ws.numel(), _stream()), "pemlp_backward")
+        ctx.ws = None
+        return g, None, None, None, None
+
+
+class PEMLP(torch.nn.Module):
+    """networks.MLP(hiden_depth=depth, hiden_width=64, in_channel=3, L=10, num_out=n_out, activation)
+    with activation LeakyReLU(slope) (slope 0: ReLU) on the HIP kernels.  ``params`` is the flat
+    parameter vector in nn.Linear order; ``load_linear_state`` takes the reference module's state dict
+    (keys mlp.mlp.{2k}.weight / .bias)."""
+
+    def __init__(self, depth: int, n_out: int, slope: float, device="cuda"):
+        super().__init__()
+        self.depth, self.n_out, self.slope = int(depth), int(n_out), float(slope)
+        count = N.lib().snisr_pemlp_param_count(10, self.depth, self.n_out)
+        if count < 0:
+            raise ValueError(N.lib().snisr_last_error().decode())
+        self.params = torch.nn.Parameter(torch.zeros(count, dtype=torch.float32, device=device))
+
+    def load_linear_state(self, state: dict):
+        parts = []
+        for k in range(self.depth):
+            parts += [state[f"mlp.mlp.{2 * k}.weight"].reshape(-1), state[f"mlp.mlp.{2 * k}.bias"].reshape(-1)]
+        with torch.no_grad():
+            self.params.copy_(torch.cat([t.to(self.params.device, torch.float32) for t in parts]))
+        return self
+
+    def forward(self, points: torch.Tensor) -> torch.Tensor:
+        return _PEMLP.apply(self.params, points, self.depth, self.n_out, self.slope)

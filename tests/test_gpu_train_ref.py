@@ -1,7 +1,8 @@
 """GPU parity of the training-form compositing (utils.volume_rendering, cumprod of 1 - alpha + 1e-10
 with an initial transparency) and of the specular head (sigmoid of degree-2 SH), forward and
-adjoint, through their autograd Functions, against the FP64 oracle; and one specular-branch
-training step of the reference's form (PE -> MLP -> SH + sigmoid, relu sigma -> composite ->
+adjoint, through their autograd Functions, against the FP64 oracle; networks.MLP (positional
+encoding + Linear/LeakyReLU, the PEMLP module) forward and parameter gradient against the oracle and
+against torch's own layers; and one specular-branch training step of the reference's form (PE -> MLP -> SH + sigmoid, relu sigma -> composite ->
 clamp(diffuse + specular) -> MSE) whose parameter gradients agree with an FP64 evaluation of the
 same chain within 1e-3 relative."""
 import numpy as np
@@ -101,11 +102,12 @@ def _mlp(dtype, device, seed=3):
     return torch.nn.Sequential(*layers).to(device=device, dtype=dtype)
 
 
-def test_specular_branch_step_gradients(oracle_mod):
-    """One step of the reference's specular branch (tile.py:745-760 with train_FLAG 1): the MLP runs in
-    torch, the SH head and the composite on the HIP adjoints; the MLP's parameter gradients equal
-    those of the same chain evaluated in FP64 (composite and SH by their formulas, Y from the oracle)
-    within 1e-3 relative."""
+@pytest.mark.parametrize("mlp", ["hip", "torch"])
+def test_specular_branch_step_gradients(oracle_mod, mlp):
+    """One step of the reference's specular branch (tile.py:745-760 with train_FLAG 1): the MLP (the
+    HIP PEMLP, or torch layers), the SH head and the composite on the HIP adjoints; the MLP's parameter
+    gradients equal those of the same chain evaluated in FP64 (composite and SH by their formulas, Y
+    from the oracle) within 1e-3 relative."""
     rng = np.random.default_rng(13)
     B, S = 512, 64
     o = rng.uniform(-1, 1, (B, 3))
@@ -117,7 +119,12 @@ def test_specular_branch_step_gradients(oracle_mod):
     mask = (rng.uniform(0, 1, (B, 1)) > 0.1).astype(np.float64)
 
     m32 = _mlp(torch.float32, DEV)
-    raw = m32(_pe(dev(pts)))
+    if mlp == "hip":
+        state = {f"mlp.mlp.{k}": v for k, v in m32.state_dict().items()}
+        pe = TR.PEMLP(6, 28, 0.01, device=DEV).load_linear_state(state)
+        raw = pe(dev(pts))
+    else:
+        raw = m32(_pe(dev(pts)))
     spec = TR.sh_sigmoid(raw, dev(d), 2)
     sigma = torch.relu(raw[..., 27:])
     rgb_vd, _, _ = TR.volume_rendering(spec, sigma, dev(dists))
@@ -138,5 +145,91 @@ def test_specular_branch_step_gradients(oracle_mod):
     torch.mean((pred64 - torch.tensor(gt)) ** 2).backward()
 
     np.testing.assert_allclose(pred.detach().cpu().numpy(), pred64.detach().numpy(), atol=1e-4)
+    if mlp == "hip":
+        g64 = torch.cat([q.grad.reshape(-1) for q in m64.parameters()]).numpy()
+        assert rel(pe.params.grad.cpu().numpy(), g64) < 1e-3
+        return
     for p32, p64 in zip(m32.parameters(), m64.parameters()):
         assert rel(p32.grad.cpu().numpy(), p64.grad.numpy()) < 1e-3
+
+
+def _pemlp_params(rng, depth, n_out):
+    parts = []
+    for k in range(depth):
+        ni, no = (63 if k == 0 else 64), (n_out if k == depth - 1 else 64)
+        parts += [rng.normal(0, 1.2 / np.sqrt(ni), no * ni), rng.normal(0, 0.1, no)]
+    return np.concatenate(parts)
+
+
+def _min_abs_preactivation(p, pts, depth, slope):
+    """Per sample, the smallest |pre-activation| over the hidden layers (numpy FP64)."""
+    x = np.concatenate([pts] + [f(2.0 ** i * pts) for i in range(10) for f in (np.sin, np.cos)], 1)
+    off, m = 0, np.full(len(pts), np.inf)
+    for k in range(depth - 1):
+        ni = 63 if k == 0 else 64
+        z = x @ p[off:off + 64 * ni].reshape(64, ni).T + p[off + 64 * ni:off + 64 * ni + 64]
+        off += 64 * ni + 64
+        m = np.minimum(m, np.abs(z).min(1))
+        x = np.where(z > 0, z, slope * z)
+    return m
+
+
+@pytest.mark.parametrize("depth,n_out,slope,n", [(6, 28, 0.01, 70001), (8, 4, 0.0, 4099), (8, 4, 0.0, 1),
+                                                 (2, 28, 0.2, 300)])
+def test_pemlp_matches_oracle(oracle_mod, depth, n_out, slope, n):
+    """networks.MLP (L = 10, width 64) forward and parameter gradient on the HIP kernels against the
+    FP64 oracle: outputs within 2e-5 absolute + 1e-4 relative, the gradient within 2e-5 relative in
+    norm.  Samples with a hidden pre-activation within 1e-4 of zero get no output gradient: there an
+    FP32 evaluation (the reference's torch one included) may take the other branch of the
+    activation's derivative, a discrete difference that is not a rounding error."""
+    rng = np.random.default_rng(depth * 1000 + n)
+    p = _pemlp_params(rng, depth, n_out)
+    pts = rng.uniform(-1, 1, (n, 3))
+    g_out = rng.normal(size=(n, n_out)) / np.sqrt(n)
+    g_out[_min_abs_preactivation(p.astype(np.float32).astype(np.float64), pts.astype(np.float32), depth, slope)
+          < 1e-4] = 0.0
+    m = TR.PEMLP(depth, n_out, slope, device=DEV)
+    with torch.no_grad():
+        m.params.copy_(dev(p))
+    x = dev(pts)
+    out = m(x)
+    out.backward(dev(g_out))
+    p32 = m.params.detach().double().cpu().numpy()            # the FP32-rounded weights the kernels used
+    ro, rg = oracle_mod.pemlp(p32, pts.astype(np.float32), depth, n_out, slope, g_out=g_out.astype(np.float32))
+    np.testing.assert_allclose(out.detach().cpu().numpy(), ro, rtol=1e-4, atol=2e-5)
+    assert rel(m.params.grad.cpu().numpy(), rg) < 2e-5
+    with torch.no_grad():
+        np.testing.assert_allclose(m(x).cpu().numpy(), out.detach().cpu().numpy(), rtol=0, atol=0)
+
+
+def test_pemlp_loads_the_reference_module_state():
+    """load_linear_state takes networks.MLP's state dict (mlp.mlp.{2k}.weight / bias) and reproduces
+    the torch module's forward and parameter gradients."""
+    torch.manual_seed(9)
+
+    class Ref(torch.nn.Module):                 # networks.MLP's module tree: MLP.mlp (GeneralMLP).mlp
+        def __init__(self):
+            super().__init__()
+            self.mlp = torch.nn.Module()
+            self.mlp.mlp = _mlp(torch.float32, DEV, seed=9)
+
+        def forward(self, x):
+            return self.mlp.mlp(_pe(x))
+
+    ref = Ref()
+    m = TR.PEMLP(6, 28, 0.01, device=DEV).load_linear_state(ref.state_dict())
+    x = torch.rand((2000, 3), device=DEV) * 2 - 1
+    g = torch.randn((2000, 28), device=DEV)
+    a, b = m(x), ref(x)
+    np.testing.assert_allclose(a.detach().cpu().numpy(), b.detach().cpu().numpy(), rtol=1e-4, atol=1e-4)
+    a.backward(g)
+    b.backward(g)
+    gref = torch.cat([t.grad.reshape(-1) for t in ref.parameters()]).cpu().numpy()
+    assert rel(m.params.grad.cpu().numpy(), gref.astype(np.float64)) < 1e-4
+
+
+def test_pemlp_rejects_bad_shapes():
+    with pytest.raises(ValueError):
+        TR.PEMLP(9, 4, 0.0, device=DEV)
+    with pytest.raises(ValueError):
+        TR.PEMLP(6, 27, 0.0, device=DEV)

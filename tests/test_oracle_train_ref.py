@@ -98,3 +98,84 @@ def test_sh_sigmoid_forward_and_adjoint(oracle_mod, deg):
         qm[s, j] -= eps
         fd = ((oracle_mod.sh_sigmoid(qp, dirs, deg) - oracle_mod.sh_sigmoid(qm, dirs, deg)) * g).sum() / (2 * eps)
         assert abs(fd - gq[s, j]) <= 1e-7 * max(1.0, abs(fd))
+
+
+def _torch_pemlp(params, pts, depth, n_out, slope):
+    """networks.MLP in FP64 from torch's own layers (nn.Linear, LeakyReLU / ReLU, sin / cos), its
+    weights taken from the flat vector in nn.Linear order; returns (out, the Linear layers)."""
+    layers, off = [], 0
+    for k in range(depth):
+        ni, no = (63 if k == 0 else 64), (n_out if k == depth - 1 else 64)
+        lin = torch.nn.Linear(ni, no).double()
+        with torch.no_grad():
+            lin.weight.copy_(torch.tensor(params[off:off + no * ni]).reshape(no, ni))
+            lin.bias.copy_(torch.tensor(params[off + no * ni:off + no * ni + no]))
+        off += no * ni + no
+        layers.append(lin)
+    assert off == params.size
+    x = torch.tensor(pts)
+    h = torch.cat([x] + [f(2.0 ** i * x) for i in range(10) for f in (torch.sin, torch.cos)], -1)
+    act = torch.nn.ReLU() if slope == 0 else torch.nn.LeakyReLU(slope)
+    for k, lin in enumerate(layers):
+        h = lin(h)
+        if k < depth - 1:
+            h = act(h)
+    return h, layers
+
+
+def _pemlp_params(rng, depth, n_out):
+    parts = []
+    for k in range(depth):
+        ni, no = (63 if k == 0 else 64), (n_out if k == depth - 1 else 64)
+        parts += [rng.normal(0, 1.2 / np.sqrt(ni), no * ni), rng.normal(0, 0.1, no)]
+    return np.concatenate(parts)
+
+
+@pytest.mark.parametrize("depth,n_out,slope", [(6, 28, 0.01), (8, 4, 0.0), (2, 4, 0.2)])
+def test_pemlp_vs_torch_layers_and_autograd(oracle_mod, depth, n_out, slope):
+    rng = np.random.default_rng(depth * 31 + n_out)
+    n = 37
+    p = _pemlp_params(rng, depth, n_out)
+    pts = rng.uniform(-1, 1, (n, 3))
+    g_out = rng.normal(size=(n, n_out))
+    out, gp = oracle_mod.pemlp(p, pts, depth, n_out, slope, g_out=g_out)
+    ref, layers = _torch_pemlp(p, pts, depth, n_out, slope)
+    (ref * torch.tensor(g_out)).sum().backward()
+    gref = np.concatenate([t.grad.numpy().reshape(-1) for lin in layers for t in (lin.weight, lin.bias)])
+    np.testing.assert_allclose(out, ref.detach().numpy(), rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(oracle_mod.pemlp(p, pts, depth, n_out, slope), out, rtol=0, atol=0)
+    np.testing.assert_allclose(gp, gref, rtol=1e-9, atol=1e-11)
+
+
+def test_pemlp_gradient_vs_finite_differences(oracle_mod):
+    rng = np.random.default_rng(77)
+    depth, n_out, slope = 3, 4, 0.05
+    p = _pemlp_params(rng, depth, n_out)
+    pts = rng.uniform(-1, 1, (5, 3))
+    g_out = rng.normal(size=(5, n_out))
+    _, gp = oracle_mod.pemlp(p, pts, depth, n_out, slope, g_out=g_out)
+    h = 1e-6
+    for k in rng.choice(p.size, 24, replace=False):
+        e = np.zeros_like(p)
+        e[k] = h
+        fd = ((oracle_mod.pemlp(p + e, pts, depth, n_out, slope) - oracle_mod.pemlp(p - e, pts, depth, n_out, slope))
+              * g_out).sum() / (2 * h)
+        assert abs(fd - gp[k]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_pemlp_linear_closed_form(oracle_mod):
+    """slope 1 makes the network affine: out = W_eff PE(x) + b_eff with W_eff the product of the
+    layers' matrices."""
+    rng = np.random.default_rng(5)
+    depth, n_out = 4, 28
+    p = _pemlp_params(rng, depth, n_out)
+    pts = rng.uniform(-1, 1, (9, 3))
+    W_eff, b_eff, off = np.eye(63), np.zeros(63), 0
+    for k in range(depth):
+        ni, no = (63 if k == 0 else 64), (n_out if k == depth - 1 else 64)
+        Wk, bk = p[off:off + no * ni].reshape(no, ni), p[off + no * ni:off + no * ni + no]
+        W_eff, b_eff = Wk @ W_eff, Wk @ b_eff + bk
+        off += no * ni + no
+    f = np.concatenate([pts] + [fn(2.0 ** i * pts) for i in range(10) for fn in (np.sin, np.cos)], 1)
+    np.testing.assert_allclose(oracle_mod.pemlp(p, pts, depth, n_out, 1.0), f @ W_eff.T + b_eff, rtol=1e-10,
+                               atol=1e-11)
