@@ -447,6 +447,40 @@ int snisr_pemlp_forward(const float* params, const float* points, int64_t n, int
 int snisr_pemlp_backward(const float* params, int64_t n, int32_t L, int32_t depth, int32_t n_out, float slope,
                          const float* g_out, float* g_params, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* A11 training form — the diffuse branch around the surface MLP (Tile.compute_voxels,
+ * Tile.inference_diffuse and Tile.query_voxel, training/scripts/tile/tile.py:175-236, 283-295).
+ * The voxel grid of a tile is f32[D][D][D][4] channel-last ([z][y][x][r, g, b, sigma]; D = the
+ * dilated node count NUM_VOXEL + 2), the memory order of the reference's permuted voxels view.
+ *  - flag_cells: the row-major indices of the cells whose node flag (i16[D^3]) is not -1, in order,
+ *    into cells i32[D^3 capacity] and their number into *count (device memory); workspace of
+ *    snisr_flag_cells_workspace_size bytes;
+ *  - bake_points: MLP inputs of those cells, ((index + 0.5) voxel_size + min_corner - center) / scale
+ *    per axis (regular_sample, tile.py:91-95, then norm_diffuse_point, tileGroup.py:74), f32[n, 3];
+ *  - normalize_points: (x - center) / scale of world points (ray samples), f32[n, 3];
+ *  - diffuse_head_forward: raw f32[n, 4] MLP outputs -> (sigmoid(r0..r2), relu(r3)); with cells, row k
+ *    goes to row cells[k] of out f32[n_rows, 4] after the n_rows rows are zeroed (the bake), without
+ *    cells out has n_rows = n rows.  The backward reads dL/dout at the same rows and writes dL/draw;
+ *  - voxel_query_forward: F.grid_sample (trilinear, align_corners, zero padding) of the grid at world
+ *    points f32[n, 3] through the reference's coordinate chain ((x - min_corner) / voxel_size - 0.5,
+ *    then / (D - 1) * 2 - 1, then the sampler's ((g + 1) / 2) (D - 1)); rgb f32[n, 3], sigma f32[n].
+ *    The backward zeroes g_voxels f32[D^3, 4] and accumulates dL/dgrid (either gradient may be NULL). */
+int64_t snisr_flag_cells_workspace_size(int64_t n_cells);
+int snisr_flag_cells(const int16_t* flags, int64_t n_cells, int32_t* cells, int32_t* count, void* workspace,
+                     int64_t workspace_bytes, void* stream);
+int snisr_bake_points(const int32_t* cells, int64_t n, int32_t D, float mc_x, float mc_y, float mc_z, float voxel_size,
+                      float center_x, float center_y, float center_z, float scale, float* points, void* stream);
+int snisr_normalize_points(const float* x, int64_t n, float center_x, float center_y, float center_z, float scale,
+                           float* out, void* stream);
+int snisr_diffuse_head_forward(const float* raw, int64_t n, const int32_t* cells, int64_t n_rows, float* out,
+                               void* stream);
+int snisr_diffuse_head_backward(const float* raw, int64_t n, const int32_t* cells, const float* g_out, float* g_raw,
+                                void* stream);
+int snisr_voxel_query_forward(const float* voxels, int32_t D, const float* points, int64_t n, float mc_x, float mc_y,
+                              float mc_z, float voxel_size, float* rgb, float* sigma, void* stream);
+int snisr_voxel_query_backward(int32_t D, const float* points, int64_t n, float mc_x, float mc_y, float mc_z,
+                               float voxel_size, const float* g_rgb, const float* g_sigma, float* g_voxels,
+                               void* stream);
+
 /* Training samplers of the reference's CUDA_EXT (training/src/Sample.cpp:97-158, kernels in
  * training/src/Sample_kernel.cu): per ray of one tile (centre cx, cy, cz; edge tile_size), the sample
  * depths of its training batch, written in place into z_vals f32[n, n_samples] (and dists for

@@ -556,3 +556,51 @@ def pemlp(params, points, depth, n_out, slope, g_out=None):
     lib().orc_pemlp_f64(_p(p), _p(x), C.c_longlong(n), C.c_int(depth), C.c_int(n_out), C.c_double(slope),
                         _p(go) if go is not None else None, _p(out), _p(gp) if gp is not None else None)
     return (out, gp) if go is not None else out
+
+
+def flag_cells(flags):
+    """Row-major indices of the cells whose node flag is not -1 (Tile.compute_voxels' torch.where)."""
+    return np.flatnonzero(np.asarray(flags).reshape(-1) != -1).astype(np.int32)
+
+
+def bake_points(cells, D, min_corner, voxel_size, center, scale):
+    """FP32 MLP inputs of the cells: ((index + 0.5) voxel_size + min_corner - center) / scale per axis,
+    in the reference's operation order (tile.py:91-95, tileGroup.py:74)."""
+    c = np.asarray(cells, np.int64)
+    idx = np.stack([c % D, (c // D) % D, c // (D * D)], 1).astype(np.float32)
+    f = np.float32
+    p = (idx + f(0.5)) * f(voxel_size) + np.asarray(min_corner, np.float32)
+    return ((p - np.asarray(center, np.float32)) / f(scale)).astype(np.float32)
+
+
+def diffuse_head(raw, g_out=None):
+    """FP64 (sigmoid(r0..r2), relu(r3)) of MLP rows; with g_out also dL/draw."""
+    r = np.asarray(raw, np.float64).reshape(-1, 4)
+    s = 1.0 / (1.0 + np.exp(-r[:, :3]))
+    out = np.concatenate([s, np.maximum(r[:, 3:], 0.0)], 1)
+    if g_out is None:
+        return out
+    g = np.asarray(g_out, np.float64).reshape(-1, 4)
+    return out, np.concatenate([g[:, :3] * s * (1 - s), np.where(r[:, 3:] > 0, g[:, 3:], 0.0)], 1)
+
+
+def voxel_query(vox, points, min_corner, voxel_size, g_rgb=None, g_sigma=None):
+    """FP64 Tile.query_voxel of a channel-last grid vox[D][D][D][4] at world points; returns
+    dict(rgb, sigma[, g_vox])."""
+    v = np.ascontiguousarray(vox, np.float64)
+    D = v.shape[0]
+    x = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    mc = np.ascontiguousarray(min_corner, np.float64)
+    rgb, sig = np.empty((n, 3)), np.empty(n)
+    want = g_rgb is not None or g_sigma is not None
+    gr = None if g_rgb is None else np.ascontiguousarray(g_rgb, np.float64).reshape(n, 3)
+    gs = None if g_sigma is None else np.ascontiguousarray(g_sigma, np.float64).reshape(n)
+    gv = np.empty_like(v) if want else None
+    lib().orc_voxel_query_f64(_p(v), C.c_int(D), _p(x), C.c_longlong(n), _p(mc), C.c_double(voxel_size),
+                              _p(gr) if gr is not None else None, _p(gs) if gs is not None else None, _p(rgb), _p(sig),
+                              _p(gv) if gv is not None else None)
+    r = dict(rgb=rgb, sigma=sig)
+    if want:
+        r["g_vox"] = gv
+    return r

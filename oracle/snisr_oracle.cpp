@@ -1762,3 +1762,41 @@ void orc_pemlp_f64(const double* params, const double* pts, long long n, int dep
 }
 
 }  // extern "C"
+
+// Tile.query_voxel (training/scripts/tile/tile.py:225-236) in FP64: F.grid_sample (trilinear,
+// align_corners=True, zero padding) of a channel-last grid vox[D][D][D][4] at world points, through
+// the reference's coordinate chain.  With g_rgb / g_sigma, g_vox (zeroed here) receives the gradient.
+extern "C" {
+
+void orc_voxel_query_f64(const double* vox, int D, const double* pts, long long n, const double* mc, double vs,
+                         const double* g_rgb, const double* g_sigma, double* rgb, double* sigma, double* g_vox) {
+  const double dm1 = D - 1;
+  if (g_vox) std::fill(g_vox, g_vox + 4LL * D * D * D, 0.0);
+  for (long long s = 0; s < n; ++s) {
+    double u[3];
+    long long c0[3];
+    for (int a = 0; a < 3; ++a) {
+      const double idx = (pts[3 * s + a] - mc[a]) / vs - 0.5;
+      const double g = idx / dm1 * 2.0 - 1.0;
+      u[a] = (g + 1.0) / 2.0 * dm1;
+      c0[a] = (long long)std::floor(u[a]);
+    }
+    double acc[4] = {0, 0, 0, 0};
+    const double gg[4] = {g_rgb ? g_rgb[3 * s] : 0.0, g_rgb ? g_rgb[3 * s + 1] : 0.0, g_rgb ? g_rgb[3 * s + 2] : 0.0,
+                          g_sigma ? g_sigma[s] : 0.0};
+    for (int j = 0; j < 8; ++j) {
+      const long long x = c0[0] + (j & 1), y = c0[1] + ((j >> 1) & 1), z = c0[2] + (j >> 2);
+      if (x < 0 || x >= D || y < 0 || y >= D || z < 0 || z >= D) continue;
+      const double w = (1.0 - std::fabs(u[0] - x)) * (1.0 - std::fabs(u[1] - y)) * (1.0 - std::fabs(u[2] - z));
+      const long long cell = (z * D + y) * D + x;
+      for (int c = 0; c < 4; ++c) {
+        acc[c] += w * vox[4 * cell + c];
+        if (g_vox) g_vox[4 * cell + c] += w * gg[c];
+      }
+    }
+    for (int c = 0; c < 3; ++c) rgb[3 * s + c] = acc[c];
+    sigma[s] = acc[3];
+  }
+}
+
+}  // extern "C"

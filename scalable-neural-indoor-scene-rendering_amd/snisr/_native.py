@@ -190,6 +190,16 @@ SIGNATURES = [
                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     ("snisr_pemlp_backward", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("snisr_flag_cells_workspace_size", C.c_int64, [C.c_int64]),
+    ("snisr_flag_cells", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("snisr_bake_points", C.c_int, [C.c_void_p, C.c_int64, C.c_int32] + [C.c_float] * 8 + [C.c_void_p, C.c_void_p]),
+    ("snisr_normalize_points", C.c_int, [C.c_void_p, C.c_int64] + [C.c_float] * 4 + [C.c_void_p, C.c_void_p]),
+    ("snisr_diffuse_head_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    ("snisr_diffuse_head_backward", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("snisr_voxel_query_forward", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64] + [C.c_float] * 4
+     + [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("snisr_voxel_query_backward", C.c_int, [C.c_int32, C.c_void_p, C.c_int64] + [C.c_float] * 4
+     + [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("snisr_pack_mlp_size", C.c_int64, [C.POINTER(SceneDesc), C.c_int32]),
     ("snisr_pack_mlp", C.c_int, [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p]),
 ]

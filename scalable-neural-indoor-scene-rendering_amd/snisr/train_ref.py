@@ -9,6 +9,13 @@ library (forward and hand-written adjoint kernels, include/snisr.h):
 - ``PEMLP`` is networks.MLP (training/scripts/networks.py:4-41) with L = 10, width 64: the surface
   MLP (depth 8, 4 outputs, ReLU) and the reflection MLP (depth 6, 28 outputs, LeakyReLU 0.01), its
   parameters one flat tensor in nn.Linear order, forward and backward on the HIP kernels.
+- ``flag_cells``, ``bake_voxels``, ``inference_diffuse`` and ``query_voxel`` are the diffuse branch
+  around the surface MLP: Tile.compute_voxels (training/scripts/tile/tile.py:283-295, the MLP at the
+  occupied cells' centres, sigmoid / ReLU head, scattered into the dilated grid), Tile.inference_diffuse
+  (tile.py:175-194, background samples) and Tile.query_voxel (tile.py:225-236, trilinear grid_sample
+  of the baked grid at the in-tile samples), each with its adjoint, so the gradient of a diffuse
+  training step reaches the MLP through the bake as in the reference.  The grid is [D, D, D, 4]
+  channel-last, the memory of the reference's permuted [1, 4, D, D, D] voxels.
 - ``sh_sigmoid(coeffs, viewdirs, deg)`` is the specular head of Tile.inference_specular
   (training/scripts/tile/tile.py:212-219): sigmoid(eval_sh(deg, coeffs as [..., 3, (deg+1)^2],
   normalised viewdirs)) for coefficients laid out [basis][rgb] as the MLP emits them; ``viewdirs``
@@ -171,3 +178,107 @@ class PEMLP(torch.nn.Module):
 
     def forward(self, points: torch.Tensor) -> torch.Tensor:
         return _PEMLP.apply(self.params, points, self.depth, self.n_out, self.slope)
+
+
+def _vec3(v, name):
+    t = [float(a) for a in (v.tolist() if hasattr(v, "tolist") else v)]
+    if len(t) != 3:
+        raise ValueError(f"{name} must have three components")
+    return t
+
+
+def flag_cells(nodes_flag: torch.Tensor) -> torch.Tensor:
+    """Row-major int32 indices of the cells of a tile's dilated node grid (int16 [D, D, D]) whose flag
+    is not -1, in order (the torch.where of Tile.compute_voxels).  Reads the count back once; cache
+    the result while the flags do not change."""
+    if not nodes_flag.is_cuda or nodes_flag.dtype != torch.int16:
+        raise ValueError("nodes_flag must be an int16 CUDA (HIP) tensor")
+    f = nodes_flag.contiguous()
+    n = f.numel()
+    L = N.lib()
+    cells = torch.empty(max(n, 1), dtype=torch.int32, device=f.device)
+    count = torch.zeros(1, dtype=torch.int32, device=f.device)
+    ws = torch.empty(max(L.snisr_flag_cells_workspace_size(n), 16), dtype=torch.uint8, device=f.device)
+    N.check(L.snisr_flag_cells(_ptr(f), n, _ptr(cells), _ptr(count), _ptr(ws), ws.numel(), _stream()), "flag_cells")
+    return cells[:int(count.item())]
+
+
+class _DiffuseHead(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, raw, cells, n_rows):
+        r = _f32(raw, "raw").reshape(-1, 4)
+        out = torch.empty((n_rows, 4), dtype=torch.float32, device=r.device)
+        N.check(N.lib().snisr_diffuse_head_forward(_ptr(r), r.shape[0], _ptr(cells), n_rows, _ptr(out), _stream()),
+                "diffuse_head_forward")
+        ctx.save_for_backward(r, cells)
+        ctx.raw_shape = raw.shape
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        r, cells = ctx.saved_tensors
+        g = torch.empty_like(r)
+        N.check(N.lib().snisr_diffuse_head_backward(_ptr(r), r.shape[0], _ptr(cells), _ptr(g_out.contiguous()), _ptr(g),
+                                                    _stream()), "diffuse_head_backward")
+        return g.reshape(ctx.raw_shape), None, None
+
+
+def bake_voxels(mlp: "PEMLP", cells: torch.Tensor, D: int, min_corner, voxel_size: float, center, scale: float):
+    """Tile.compute_voxels: the surface MLP at the centres of ``cells`` (from flag_cells) of the
+    D^3 dilated grid whose first cell starts at ``min_corner``, inputs normalised as
+    (x - center) / scale, through the sigmoid / ReLU head; returns the [D, D, D, 4] grid (zero at the
+    other cells), differentiable with respect to ``mlp.params``."""
+    if cells.dtype != torch.int32 or not cells.is_cuda:
+        raise ValueError("cells must be an int32 CUDA (HIP) tensor")
+    n = cells.numel()
+    mc, c = _vec3(min_corner, "min_corner"), _vec3(center, "center")
+    pts = torch.empty((n, 3), dtype=torch.float32, device=cells.device)
+    N.check(N.lib().snisr_bake_points(_ptr(cells), n, int(D), *mc, float(voxel_size), *c, float(scale), _ptr(pts),
+                                      _stream()), "bake_points")
+    return _DiffuseHead.apply(mlp(pts), cells.contiguous(), int(D) ** 3).reshape(D, D, D, 4)
+
+
+def inference_diffuse(mlp: "PEMLP", x: torch.Tensor, center, scale: float):
+    """Tile.inference_diffuse: (sigmoid rgb [..., 3], relu sigma [..., 1]) of the surface MLP at world
+    points x [..., 3] normalised as (x - center) / scale."""
+    xc = _f32(x, "x")
+    n = xc.numel() // 3
+    c = _vec3(center, "center")
+    pts = torch.empty((n, 3), dtype=torch.float32, device=xc.device)
+    N.check(N.lib().snisr_normalize_points(_ptr(xc), n, *c, float(scale), _ptr(pts), _stream()), "normalize_points")
+    out = _DiffuseHead.apply(mlp(pts), None, n).reshape(*x.shape[:-1], 4)
+    return out[..., :3], out[..., 3:]
+
+
+class _VoxelQuery(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, voxels, points, mc, vs):
+        v, p = _f32(voxels, "voxels"), _f32(points, "points")
+        D = v.shape[0]
+        if v.shape != (D, D, D, 4):
+            raise ValueError("voxels must be [D, D, D, 4]")
+        n = p.numel() // 3
+        rgb = torch.empty((*points.shape[:-1], 3), dtype=torch.float32, device=p.device)
+        sigma = torch.empty((*points.shape[:-1], 1), dtype=torch.float32, device=p.device)
+        N.check(N.lib().snisr_voxel_query_forward(_ptr(v), D, _ptr(p), n, *mc, vs, _ptr(rgb), _ptr(sigma), _stream()),
+                "voxel_query_forward")
+        ctx.save_for_backward(p)
+        ctx.meta = (D, mc, vs)
+        return rgb, sigma
+
+    @staticmethod
+    def backward(ctx, g_rgb, g_sigma):
+        (p,) = ctx.saved_tensors
+        D, mc, vs = ctx.meta
+        g = torch.empty((D, D, D, 4), dtype=torch.float32, device=p.device)
+        gr = None if g_rgb is None else g_rgb.contiguous()
+        gs = None if g_sigma is None else g_sigma.contiguous()
+        N.check(N.lib().snisr_voxel_query_backward(D, _ptr(p), p.numel() // 3, *mc, vs, _ptr(gr), _ptr(gs), _ptr(g),
+                                                   _stream()), "voxel_query_backward")
+        return g, None, None, None
+
+
+def query_voxel(voxels: torch.Tensor, samples: torch.Tensor, min_corner, voxel_size: float):
+    """Tile.query_voxel: trilinear samples of the [D, D, D, 4] grid at world points [..., 3] (zero
+    outside); returns (rgb [..., 3], sigma [..., 1]), differentiable with respect to the grid."""
+    return _VoxelQuery.apply(voxels, samples, tuple(_vec3(min_corner, "min_corner")), float(voxel_size))

@@ -233,3 +233,126 @@ def test_pemlp_rejects_bad_shapes():
         TR.PEMLP(9, 4, 0.0, device=DEV)
     with pytest.raises(ValueError):
         TR.PEMLP(6, 27, 0.0, device=DEV)
+
+
+def _flags(rng, D, p):
+    return np.where(rng.uniform(0, 1, (D, D, D)) < p, 1, -1).astype(np.int16)
+
+
+@pytest.mark.parametrize("D,p", [(18, 0.4), (130, 0.3), (3, 0.0), (5, 1.0)])
+def test_flag_cells_and_bake_points(oracle_mod, D, p):
+    """Ordered compaction of the occupied cells (more than 1024 workgroups at D = 130) and their MLP
+    inputs, bit-exact against the oracle (itself equal to the reference's torch expressions)."""
+    rng = np.random.default_rng(D)
+    flags = _flags(rng, D, p)
+    cells = TR.flag_cells(torch.tensor(flags, device=DEV))
+    ref = oracle_mod.flag_cells(flags)
+    np.testing.assert_array_equal(cells.cpu().numpy(), ref)
+    if ref.size == 0:
+        return
+    mc, vs, c, s = np.array([0.37, -1.25, 2.5], np.float32), 0.0625, np.array([0.5, -0.75, 2.0], np.float32), 3.7
+    pts = torch.empty((ref.size, 3), dtype=torch.float32, device=DEV)
+    from snisr import _native as NN
+    from snisr.device import _ptr
+    NN.check(NN.lib().snisr_bake_points(_ptr(cells), ref.size, D, *mc.tolist(), vs, *c.tolist(), s, _ptr(pts), None),
+             "bake_points")
+    np.testing.assert_array_equal(pts.cpu().numpy(), oracle_mod.bake_points(ref, D, mc, vs, c, s))
+
+
+def test_diffuse_head_and_voxel_query_match_oracle(oracle_mod):
+    rng = np.random.default_rng(12)
+    raw = rng.normal(size=(5000, 4)) * 3
+    g = rng.normal(size=(5000, 4))
+    t = dev(raw).requires_grad_(True)
+    out = TR._DiffuseHead.apply(t, None, 5000)
+    out.backward(dev(g))
+    ro, rg = oracle_mod.diffuse_head(raw.astype(np.float32), g.astype(np.float32))
+    np.testing.assert_allclose(out.detach().cpu().numpy(), ro, rtol=1e-6, atol=1e-7)
+    # s (1 - s) in FP32 (torch's sigmoid backward form) cancels for |raw| > 8: absolute bound
+    np.testing.assert_allclose(t.grad.cpu().numpy(), rg, rtol=1e-5, atol=1e-6)
+
+    D, vs = 11, 0.07
+    mc = np.array([-0.3, 0.2, 1.1], np.float32)
+    vox = rng.uniform(0, 1, (D, D, D, 4))
+    pts = mc + rng.uniform(-0.6, D + 0.6, (20000, 3)) * vs
+    g_rgb, g_sig = rng.normal(size=(20000, 3)), rng.normal(size=(20000, 1))
+    tv = dev(vox).requires_grad_(True)
+    rgb, sig = TR.query_voxel(tv, dev(pts), mc, vs)
+    ((rgb * dev(g_rgb)).sum() + (sig * dev(g_sig)).sum()).backward()
+    r = oracle_mod.voxel_query(vox.astype(np.float32), pts.astype(np.float32), mc, vs, g_rgb=g_rgb.astype(np.float32),
+                               g_sigma=g_sig.astype(np.float32))
+    np.testing.assert_allclose(rgb.detach().cpu().numpy(), r["rgb"], atol=2e-5)
+    np.testing.assert_allclose(sig.detach().cpu().numpy()[:, 0], r["sigma"], atol=2e-5)
+    assert rel(tv.grad.cpu().numpy(), r["g_vox"]) < 1e-5
+
+
+def test_diffuse_training_step_through_the_bake():
+    """One diffuse training step of the reference's form (tile.py:686-710, train_FLAG 0): the surface
+    MLP baked into the dilated grid (compute_voxels), the in-tile samples read from it (query_voxel),
+    the background samples through the MLP (inference_diffuse), both composited with the background
+    chained behind the tile's leftover transparency, MSE.  Everything runs on the HIP kernels; the
+    loss and the MLP's parameter gradients equal those of the same step in FP64 torch (nn.Linear,
+    F.grid_sample, the compositing formula) within 1e-3 relative."""
+    rng = np.random.default_rng(31)
+    NV, vs = 16, 0.05
+    D, tile_size = NV + 2, NV * vs
+    tc = np.array([0.2, -0.1, 0.4])
+    mc = (tc - tile_size / 2 - vs).astype(np.float32)
+    gc, gs = np.array([0.0, 0.0, 0.3], np.float32), 2.0
+    flags = _flags(rng, D, 0.4)
+    B, S, NB = 512, 32, 8
+    o = tc + rng.uniform(-0.4, 0.4, (B, 3)) * tile_size
+    d = rng.normal(size=(B, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    z = np.concatenate([np.sort(rng.uniform(0, 0.8, (B, S)), 1), np.sort(rng.uniform(0.9, 2.0, (B, NB)), 1)], 1)
+    dists = np.concatenate([z[:, 1:] - z[:, :-1], np.full((B, 1), 1e10)], 1)
+    pts = (o[:, None] + z[..., None] * d[:, None]).astype(np.float32)
+    gt = rng.uniform(0, 1, (B, 3))
+    bg_mask = (rng.uniform(0, 1, (B, 1)) < 0.8).astype(np.float64)
+    p = _pemlp_params(rng, 8, 4)
+
+    m = TR.PEMLP(8, 4, 0.0, device=DEV)
+    with torch.no_grad():
+        m.params.copy_(dev(p))
+    cells = TR.flag_cells(torch.tensor(flags, device=DEV))
+    vox = TR.bake_voxels(m, cells, D, mc, vs, gc, gs)
+    rgb, sigma = TR.query_voxel(vox, dev(pts[:, :S]), mc, vs)
+    bg_rgb, bg_sigma = TR.inference_diffuse(m, dev(pts[:, S:]), gc, gs)
+    d_in, trans, _ = TR.volume_rendering(rgb, sigma, dev(dists[:, :S, None]))
+    d_bg, _, _ = TR.volume_rendering(bg_rgb, bg_sigma, dev(dists[:, S:, None]), trans[:, -1:])
+    loss = torch.mean((d_in + d_bg * dev(bg_mask) - dev(gt)) ** 2)
+    loss.backward()
+
+    seq = torch.nn.Sequential(*sum([[torch.nn.Linear(63 if k == 0 else 64, 4 if k == 7 else 64).double()]
+                                    + ([torch.nn.ReLU()] if k < 7 else []) for k in range(8)], []))
+    off = 0
+    with torch.no_grad():
+        for lin in seq[::2]:
+            no, ni = lin.weight.shape
+            lin.weight.copy_(torch.tensor(p.astype(np.float32)[off:off + no * ni].astype(np.float64)).reshape(no, ni))
+            lin.bias.copy_(torch.tensor(p.astype(np.float32)[off + no * ni:off + no * ni + no].astype(np.float64)))
+            off += no * ni + no
+    Zs, Ys, Xs = torch.meshgrid(torch.arange(D), torch.arange(D), torch.arange(D), indexing="ij")
+    regular = (torch.stack([Xs, Ys, Zs], -1).double() + 0.5) * vs + torch.tensor(mc, dtype=torch.float64)
+    ne = torch.where(torch.tensor(flags) != -1)
+    raw = seq(_pe((regular[ne] - torch.tensor(gc, dtype=torch.float64)) / gs))
+    voxels = torch.zeros((D, D, D, 4), dtype=torch.float64)
+    voxels[ne] = torch.cat([torch.sigmoid(raw[:, :3]), torch.relu(raw[:, 3:])], 1)
+    grids = (((torch.tensor(pts[:, :S], dtype=torch.float64) - torch.tensor(mc, dtype=torch.float64)) / vs - 0.5)
+             / (D - 1) * 2 - 1)[None, None]
+    rgba = torch.nn.functional.grid_sample(voxels[None].permute(0, 4, 1, 2, 3), grids, padding_mode="zeros",
+                                           align_corners=True)[0, :, 0].permute(1, 2, 0)
+    raw_bg = seq(_pe((torch.tensor(pts[:, S:], dtype=torch.float64) - torch.tensor(gc, dtype=torch.float64)) / gs))
+
+    def comp(c, s, dd, init):
+        a = 1 - torch.exp(-s * torch.tensor(dd))
+        T = torch.cumprod(torch.cat([init, 1 - a + 1e-10], 1), 1)[:, :-1]
+        return (a[..., None] * T[..., None] * c).sum(1), T
+
+    c_in, T_in = comp(rgba[..., :3], rgba[..., 3], dists[:, :S], torch.ones(B, 1, dtype=torch.float64))
+    c_bg, _ = comp(torch.sigmoid(raw_bg[..., :3]), torch.relu(raw_bg[..., 3]), dists[:, S:], T_in[:, -1:])
+    loss64 = torch.mean((c_in + c_bg * torch.tensor(bg_mask) - torch.tensor(gt)) ** 2)
+    loss64.backward()
+    g64 = torch.cat([q.grad.reshape(-1) for q in seq.parameters()]).numpy()
+    assert abs(loss.item() - loss64.item()) <= 1e-5 * loss64.item()
+    assert rel(m.params.grad.cpu().numpy(), g64) < 1e-3

@@ -179,3 +179,79 @@ def test_pemlp_linear_closed_form(oracle_mod):
     f = np.concatenate([pts] + [fn(2.0 ** i * pts) for i in range(10) for fn in (np.sin, np.cos)], 1)
     np.testing.assert_allclose(oracle_mod.pemlp(p, pts, depth, n_out, 1.0), f @ W_eff.T + b_eff, rtol=1e-10,
                                atol=1e-11)
+
+
+def _torch_query_voxel(vox_cl, pts, min_corner, voxel_size):
+    """Tile.query_voxel's torch form (tile.py:225-236) in FP64 on the channel-last grid's permuted view."""
+    D = vox_cl.shape[0]
+    voxels = vox_cl[None].permute(0, 4, 1, 2, 3)
+    indices = (pts - min_corner) / voxel_size - 0.5
+    grids = (indices / (D - 1) * 2 - 1)[None, None, None]
+    rgba = torch.nn.functional.grid_sample(voxels, grids, padding_mode="zeros", align_corners=True)
+    rgba = rgba[0, :, 0, 0].permute(1, 0)
+    return rgba[:, :3], rgba[:, 3]
+
+
+def test_voxel_query_vs_grid_sample_and_autograd(oracle_mod):
+    rng = np.random.default_rng(21)
+    D, vs = 7, 0.13
+    mc = np.array([-0.3, 0.2, 1.1])
+    vox = rng.uniform(0, 1, (D, D, D, 4))
+    # inside, near the faces and outside (zero padding), plus exact cell centres
+    pts = mc + rng.uniform(-0.5, D + 0.5, (300, 3)) * vs
+    pts[:20] = mc + (rng.integers(0, D, (20, 3)) + 0.5) * vs
+    g_rgb, g_sig = rng.normal(size=(300, 3)), rng.normal(size=300)
+    r = oracle_mod.voxel_query(vox, pts, mc, vs, g_rgb=g_rgb, g_sigma=g_sig)
+    tv = torch.tensor(vox, requires_grad=True)
+    rgb, sig = _torch_query_voxel(tv, torch.tensor(pts), torch.tensor(mc), vs)
+    ((rgb * torch.tensor(g_rgb)).sum() + (sig * torch.tensor(g_sig)).sum()).backward()
+    np.testing.assert_allclose(r["rgb"], rgb.detach().numpy(), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(r["sigma"], sig.detach().numpy(), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(r["g_vox"], tv.grad.numpy(), rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(r["rgb"][:20], vox[tuple(((pts[:20] - mc) / vs - 0.5).round().astype(int)[:, ::-1].T)][:, :3],
+                               rtol=1e-12)
+
+
+def test_voxel_query_reproduces_a_linear_field(oracle_mod):
+    """Trilinear interpolation is exact on an affine field: grid values a . centre + b give a . x + b
+    at any point between the outermost cell centres."""
+    D, vs = 6, 0.25
+    mc = np.array([1.0, -2.0, 0.5])
+    a, b = np.array([[0.3, -1.2, 2.0], [1.0, 0.0, 0.0], [0.0, 0.5, 0.5], [-0.7, 0.1, 0.9]]), np.array([0.1, 2.0, -1.0, 0.0])
+    zz, yy, xx = np.meshgrid(np.arange(D), np.arange(D), np.arange(D), indexing="ij")
+    centres = mc + (np.stack([xx, yy, zz], -1) + 0.5) * vs
+    vox = centres @ a.T + b
+    rng = np.random.default_rng(3)
+    pts = mc + (0.5 + rng.uniform(0, D - 1, (200, 3))) * vs
+    r = oracle_mod.voxel_query(vox, pts, mc, vs)
+    exp = pts @ a.T + b
+    np.testing.assert_allclose(r["rgb"], exp[:, :3], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(r["sigma"], exp[:, 3], rtol=1e-12, atol=1e-12)
+
+
+def test_bake_points_and_cells_follow_the_reference_expressions(oracle_mod):
+    """Tile.compute_voxels' inputs evaluated the reference's way in torch FP32 (create_meshgrid,
+    (index + 0.5) * voxel_size + min_corner, torch.where(nodes_flag != -1), norm_diffuse_point)."""
+    rng = np.random.default_rng(8)
+    D, vs = 9, 0.0625
+    mc = np.array([0.37, -1.25, 2.5], np.float32)
+    center, scale = np.array([0.5, -0.75, 2.0], np.float32), 3.7
+    flags = np.where(rng.uniform(0, 1, (D, D, D)) < 0.3, 1, -1).astype(np.int16)
+    Zs, Ys, Xs = torch.meshgrid(torch.arange(D), torch.arange(D), torch.arange(D), indexing="ij")
+    regular = (torch.stack([Xs, Ys, Zs], -1).float() + 0.5) * vs + torch.tensor(mc)
+    ref = (regular[torch.where(torch.tensor(flags) != -1)] - torch.tensor(center)) / scale
+    cells = oracle_mod.flag_cells(flags)
+    assert cells.dtype == np.int32 and np.all(np.diff(cells) > 0)
+    pts = oracle_mod.bake_points(cells, D, mc, vs, center, scale)
+    np.testing.assert_array_equal(pts, ref.numpy())
+
+
+def test_diffuse_head_adjoint_vs_autograd(oracle_mod):
+    rng = np.random.default_rng(4)
+    raw, g = rng.normal(size=(50, 4)) * 3, rng.normal(size=(50, 4))
+    out, graw = oracle_mod.diffuse_head(raw, g)
+    t = torch.tensor(raw, requires_grad=True)
+    o = torch.cat([torch.sigmoid(t[:, :3]), torch.relu(t[:, 3:])], 1)
+    (o * torch.tensor(g)).sum().backward()
+    np.testing.assert_allclose(out, o.detach().numpy(), rtol=1e-14)
+    np.testing.assert_allclose(graw, t.grad.numpy(), rtol=1e-13, atol=1e-15)
