@@ -238,15 +238,21 @@ def bake_voxels(mlp: "PEMLP", cells: torch.Tensor, D: int, min_corner, voxel_siz
     return _DiffuseHead.apply(mlp(pts), cells.contiguous(), int(D) ** 3).reshape(D, D, D, 4)
 
 
+def normalize_points(x: torch.Tensor, center, scale: float) -> torch.Tensor:
+    """(x - center) / scale of world points [..., 3]: norm_diffuse_point with the group centre and
+    size, norm_specular_point with the group centre and cfg.far (training/scripts/tile/tileGroup.py:74-75)."""
+    xc = _f32(x, "x")
+    out = torch.empty_like(xc)
+    N.check(N.lib().snisr_normalize_points(_ptr(xc), xc.numel() // 3, *_vec3(center, "center"), float(scale),
+                                           _ptr(out), _stream()), "normalize_points")
+    return out
+
+
 def inference_diffuse(mlp: "PEMLP", x: torch.Tensor, center, scale: float):
     """Tile.inference_diffuse: (sigmoid rgb [..., 3], relu sigma [..., 1]) of the surface MLP at world
     points x [..., 3] normalised as (x - center) / scale."""
-    xc = _f32(x, "x")
-    n = xc.numel() // 3
-    c = _vec3(center, "center")
-    pts = torch.empty((n, 3), dtype=torch.float32, device=xc.device)
-    N.check(N.lib().snisr_normalize_points(_ptr(xc), n, *c, float(scale), _ptr(pts), _stream()), "normalize_points")
-    out = _DiffuseHead.apply(mlp(pts), None, n).reshape(*x.shape[:-1], 4)
+    n = x.numel() // 3
+    out = _DiffuseHead.apply(mlp(normalize_points(x, center, scale).reshape(n, 3)), None, n).reshape(*x.shape[:-1], 4)
     return out[..., :3], out[..., 3:]
 
 
