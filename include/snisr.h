@@ -437,15 +437,18 @@ int snisr_sh_sigmoid_backward(const float* spec, const float* dirs, int64_t n, i
  * between them (slope 0 = ReLU) and n_out outputs: the surface MLP (depth 8, n_out 4, ReLU) and the
  * reflection MLP (depth 6, n_out 28, LeakyReLU 0.01) of the reference's configuration
  * (training/config/cfg.yaml:38-46).  params: snisr_pemlp_param_count floats in nn.Linear order (per
- * layer W[out][in], then b[out]).  points f32[n, 3], out f32[n, n_out].  The forward stores the layer
- * inputs for the backward in `workspace` (snisr_pemlp_workspace_size bytes; NULL for inference);
- * the backward takes dL/dout and the forward's workspace and writes dL/dparams (overwritten). */
+ * layer W[out][in], then b[out]); slope >= 0.  points f32[n, 3], out f32[n, n_out] (16-byte aligned).
+ * Both calls pack the weights into `workspace` first: the forward needs
+ * snisr_pemlp_workspace_size(0, ...) bytes, the backward snisr_pemlp_workspace_size(n, ...) (the
+ * weights plus each layer's input and delta rows).  The backward recomputes the forward from the same
+ * points, takes dL/dout and writes dL/dparams (overwritten). */
 int64_t snisr_pemlp_param_count(int32_t L, int32_t depth, int32_t n_out);
 int64_t snisr_pemlp_workspace_size(int64_t n, int32_t L, int32_t depth, int32_t n_out);
 int snisr_pemlp_forward(const float* params, const float* points, int64_t n, int32_t L, int32_t depth, int32_t n_out,
                         float slope, float* out, void* workspace, int64_t workspace_bytes, void* stream);
-int snisr_pemlp_backward(const float* params, int64_t n, int32_t L, int32_t depth, int32_t n_out, float slope,
-                         const float* g_out, float* g_params, void* workspace, int64_t workspace_bytes, void* stream);
+int snisr_pemlp_backward(const float* params, const float* points, int64_t n, int32_t L, int32_t depth, int32_t n_out,
+                         float slope, const float* g_out, float* g_params, void* workspace, int64_t workspace_bytes,
+                         void* stream);
 
 /* A11 training form — the diffuse branch around the surface MLP (Tile.compute_voxels,
  * Tile.inference_diffuse and Tile.query_voxel, training/scripts/tile/tile.py:175-236, 283-295).

@@ -7,13 +7,17 @@
 // outputs, LeakyReLU 0.01) of the reference's training configuration (training/config/cfg.yaml:38-46),
 // used for the compute_voxels bake, the background samples and the specular branch.
 //
-// One thread per sample with the 64 activations in registers; the weights, the same for every lane,
-// are scalar loads.  The forward stores each layer's input row for the backward, which stores each
-// layer's output-delta row; the weight gradients are the row products dW = sum delta (x) input,
-// reduced on FP32 MFMA.
+// FP32 MFMA (v_mfma_f32_32x32x2_f32) with the training chain's fragment layout: a wavefront takes 32
+// samples as the 32 columns, computes their positional encoding cooperatively (30 sin/cos pairs per
+// sample spread over the lanes, through LDS), and every layer's output (the D layout) is the next
+// layer's B operand, so activations never leave registers.  The weights are packed once per call as
+// forward and transposed fragments.  The forward writes only the outputs.  The backward recomputes
+// the forward from the points, keeps the activation signs as bit masks, writes each layer's input
+// row and output-delta row, and reduces the weight gradients dW = sum delta (x) input on FP32 MFMA.
 #include "snisr_common.hpp"
 #include "snisr_device.hpp"
 #include "train_dw.hpp"
+#include "train_mma.hpp"
 
 namespace snisr {
 namespace {
@@ -47,103 +51,150 @@ __host__ __device__ inline int64_t pe_d_off(int l, int L, int depth, int n_out, 
   return s * n;
 }
 
-__device__ __forceinline__ void store_row(float* dst, const float (&v)[kPW]) {
-#pragma unroll
-  for (int q = 0; q < kPW; q += 4) *reinterpret_cast<float4*>(dst + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+constexpr int kPIn = 63;                 // encoding width (3 + 6 L)
+constexpr int kPNs0 = tr_ns0(kPIn);      // input steps of layer 0 (rows 0..63, row 63 zero)
+constexpr int kPeLds = 65;               // LDS row pitch of the encoding (conflict-free column reads)
+
+// float offsets of the packed fragments (one block of tr_stride(63, 64, depth, n_out) floats)
+__host__ __device__ inline int64_t pe_frag_floats(int depth, int n_out) { return tr_stride(kPIn, kPW, depth, n_out); }
+
+__device__ __forceinline__ float leaky_mask(uint64_t m, int r, float v, float slope) {
+  return ((m >> r) & 1) ? v : slope * v;
 }
 
-// y[o] = b[o] + sum_i W[o][i] x[i] for o < NO (the first `nout` real), W row-major with `nin`
-// columns.  Every lane of a wavefront reads the same weights, so they are scalar loads (the
-// weights never occupy vector registers or LDS).
-template <int NO>
-__device__ __forceinline__ void pe_layer(const float* __restrict__ W, int nin, int nout, const float (&x)[kPW],
-                                         float (&y)[kPW]) {
+// One wavefront = 32 samples.  Forward (BWD false): out[j][0..NOUT).  Backward: recomputes the
+// forward, stores each layer's input row (pitch 64) and output-delta row (pitch 64, the last layer
+// NOUT) for the weight-gradient reduction.
+template <int NOUT, bool BWD>
+__global__ __launch_bounds__(256, 2) void k_pemlp_mma(const float* __restrict__ P, const float* __restrict__ pts,
+                                                      int64_t n, int depth, float slope, float* __restrict__ out,
+                                                      const float* __restrict__ g_out, float* __restrict__ rows) {
+  static_assert(NOUT % 4 == 0 && NOUT <= 32, "output rows in whole 4-row groups of one block");
+  constexpr int NSBO = NOUT <= 8 ? 4 : 16;  // backward steps of the output layer
+  __shared__ float pe_lds[4 * 32 * kPeLds];
+  const int lane = threadIdx.x & 63, col = lane & 31, h = lane >> 5, wv = threadIdx.x >> 6;
+  const int64_t j0 = ((int64_t)blockIdx.x * 4 + wv) * 32;
+  if (j0 >= n) return;  // whole wavefront past the end (only wavefront-level LDS sharing below)
+  const int64_t j = j0 + col;
+  const bool valid = j < n;
+  float* E = pe_lds + wv * 32 * kPeLds;
+  // ---- positional encoding of the wavefront's 32 samples: [x, sin(2^f x), cos(2^f x)] (f < 10)
+  for (int q = lane; q < 32 * 30; q += 64) {
+    const int smp = q / 30, t = q - 30 * smp, f = t / 3, c = t - 3 * f;
+    const float v = j0 + smp < n ? pts[3 * (j0 + smp) + c] : 0.0f;
+    float sn, cs;
+    sincosf((float)(1 << f) * v, &sn, &cs);
+    E[smp * kPeLds + 3 + 6 * f + c] = sn;
+    E[smp * kPeLds + 6 + 6 * f + c] = cs;
+  }
+  for (int q = lane; q < 32 * 4; q += 64) {
+    const int smp = q >> 2, c = q & 3;
+    E[smp * kPeLds + (c < 3 ? c : kPIn)] = (c < 3 && j0 + smp < n) ? pts[3 * (j0 + smp) + c] : 0.0f;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  float x0[kPNs0];
 #pragma unroll
-  for (int o = 0; o < NO; ++o) {
-    float acc = 0.0f;
-    if (o < nout) {
-      const float* w = W + (int64_t)o * nin;
-      acc = W[(int64_t)nout * nin + o];
+  for (int s = 0; s < kPNs0; ++s) x0[s] = E[col * kPeLds + tr_row(s, h)];
+  const int64_t in_rows = (int64_t)n * kPW;  // floats of one layer's input rows
+  if (BWD && valid) {
 #pragma unroll
-      for (int i = 0; i < kPW; ++i)
-        if (i < nin) acc = __builtin_fmaf(w[i], x[i], acc);
+    for (int g = 0; g < kPNs0 / 4; ++g) store4(rows + j * kPW + 32 * (g / 4) + 8 * (g % 4) + 4 * h, x0 + 4 * g);
+  }
+  // ---- forward
+  float a[32];
+  uint64_t mask[8];
+  mma_blocks<kPNs0, 2>(
+      P + tr_fwd_off(kPIn, kPW, depth, 0), x0, lane,
+      [&](int ob) { return bias_block(P + tr_bias_off(kPIn, kPW, depth, 0), ob, h); },
+      [&](int ob, const f32x16& acc) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) a[ob * 16 + r] = acc[r] > 0.0f ? acc[r] : slope * acc[r];
+      });
+  uint64_t mk_last = 0;  // mask of the output layer's input
+#pragma unroll
+  for (int l = 1; l < 8; ++l) {
+    mask[l] = 0;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) mask[l] |= (uint64_t)(a[r] > 0.0f) << r;
+    if (BWD && valid) {
+      float* Al = rows + l * in_rows + j * kPW;
+#pragma unroll
+      for (int g = 0; g < 8; ++g) store4(Al + 32 * (g / 4) + 8 * (g % 4) + 4 * h, a + 4 * g);
     }
-    y[o] = acc;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_pemlp_fwd(const float* __restrict__ params, const float* __restrict__ pts,
-                                                   int64_t n, int L, int depth, int n_out, float slope,
-                                                   float* __restrict__ out, float* __restrict__ rows) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  float x[kPW];
-  const float p[3] = {pts[3 * s], pts[3 * s + 1], pts[3 * s + 2]};
-#pragma unroll
-  for (int i = 0; i < kPW; ++i) x[i] = 0.0f;
-  x[0] = p[0];
-  x[1] = p[1];
-  x[2] = p[2];
-#pragma unroll
-  for (int f = 0; f < kPL; ++f) {
-    const float sc = (float)(1 << f);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) sincosf(sc * p[j], &x[3 + 6 * f + j], &x[6 + 6 * f + j]);
-  }
-  if (rows) store_row(rows + pe_x_off(0, L, n) + s * kPW, x);   // pe_xw(0, 10) = 64
-  for (int l = 0; l < depth - 1; ++l) {
-    float y[kPW];
-    pe_layer<kPW>(params + pe_param_off(l, L, depth, n_out), pe_in_l(l, L), kPW, x, y);
-#pragma unroll
-    for (int o = 0; o < kPW; ++o) x[o] = y[o] > 0.0f ? y[o] : slope * y[o];
-    if (rows) store_row(rows + pe_x_off(l + 1, L, n) + s * kPW, x);
-  }
-  float y[kPW];
-  pe_layer<kPOut>(params + pe_param_off(depth - 1, L, depth, n_out), kPW, n_out, x, y);
-#pragma unroll
-  for (int o = 0; o < kPOut; ++o)
-    if (o < n_out) out[s * n_out + o] = y[o];
-}
-
-__global__ __launch_bounds__(256) void k_pemlp_bwd(const float* __restrict__ params, int64_t n, int L, int depth,
-                                                   int n_out, float slope, const float* __restrict__ g_out,
-                                                   float* __restrict__ rows) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  float d[kPW];
-#pragma unroll
-  for (int o = 0; o < kPW; ++o) d[o] = (o < kPOut && o < n_out) ? g_out[s * n_out + (o < n_out ? o : 0)] : 0.0f;
-  {
-    const int dw = pe_dw(depth - 1, depth, n_out);
-    float* dst = rows + pe_d_off(depth - 1, L, depth, n_out, n) + s * dw;
-#pragma unroll
-    for (int o = 0; o < kPOut; o += 4)
-      if (o < dw) *reinterpret_cast<float4*>(dst + o) = make_float4(d[o], d[o + 1], d[o + 2], d[o + 3]);
-  }
-  for (int l = depth - 1; l >= 1; --l) {
-    const int nout = pe_out_l(l, depth, n_out);
-    const float* W = params + pe_param_off(l, L, depth, n_out);  // W[o][i], 64 inputs
-    float acc[kPW];
-#pragma unroll
-    for (int i = 0; i < kPW; ++i) acc[i] = 0.0f;
-#pragma unroll
-    for (int o = 0; o < kPW; ++o)
-      if (o < nout) {
-        const float* w = W + (int64_t)o * kPW;
-#pragma unroll
-        for (int i = 0; i < kPW; ++i) acc[i] = __builtin_fmaf(w[i], d[o], acc[i]);
-      }
-    // the layer's input is the previous layer's activation: its sign is the pre-activation's
-    const float4* xr = reinterpret_cast<const float4*>(rows + pe_x_off(l, L, n) + s * kPW);
-#pragma unroll
-    for (int q = 0; q < kPW / 4; ++q) {
-      const float4 x4 = xr[q];
-      d[4 * q] = acc[4 * q] * (x4.x > 0.0f ? 1.0f : slope);
-      d[4 * q + 1] = acc[4 * q + 1] * (x4.y > 0.0f ? 1.0f : slope);
-      d[4 * q + 2] = acc[4 * q + 2] * (x4.z > 0.0f ? 1.0f : slope);
-      d[4 * q + 3] = acc[4 * q + 3] * (x4.w > 0.0f ? 1.0f : slope);
+    if (l == depth - 1) {
+      mk_last = mask[l];
+      break;
     }
-    store_row(rows + pe_d_off(l - 1, L, depth, n_out, n) + s * kPW, d);
+    float b[32];
+    mma_blocks<32, 2>(
+        P + tr_fwd_off(kPIn, kPW, depth, l), a, lane,
+        [&](int ob) { return bias_block(P + tr_bias_off(kPIn, kPW, depth, l), ob, h); },
+        [&](int ob, const f32x16& acc) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) b[ob * 16 + r] = acc[r] > 0.0f ? acc[r] : slope * acc[r];
+        });
+#pragma unroll
+    for (int r = 0; r < 32; ++r) a[r] = b[r];
+  }
+  // ---- output layer: rows 8 (r / 4) + 4 h + r % 4 of column col
+  if (!BWD) {
+    f32x16 z = bias_block(P + tr_bias_off(kPIn, kPW, depth, depth - 1), 0, h);
+    z = mma_f32_chain<32>(P + tr_fwd_off(kPIn, kPW, depth, depth - 1), a, z, lane);
+    if (valid) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+        if (8 * g + 4 * h < NOUT) {
+          const float v[4] = {z[4 * g], z[4 * g + 1], z[4 * g + 2], z[4 * g + 3]};
+          store4(out + j * NOUT + 8 * g + 4 * h, v);
+        }
+    }
+    return;
+  }
+  // ---- backward: output delta (rows tr_row(s, h), s < NSBO), then d_{l-1} = (W_l^T d_l) * leaky'
+  const int64_t d_base = (int64_t)depth * in_rows;  // delta rows follow the input rows
+  float dout[NSBO];
+#pragma unroll
+  for (int g = 0; g < NSBO / 4; ++g) {
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (valid && 8 * g + 4 * h < NOUT) v = *reinterpret_cast<const float4*>(g_out + j * NOUT + 8 * g + 4 * h);
+    dout[4 * g] = v.x;
+    dout[4 * g + 1] = v.y;
+    dout[4 * g + 2] = v.z;
+    dout[4 * g + 3] = v.w;
+  }
+  if (valid) {
+    float* Dl = rows + d_base + (int64_t)(depth - 1) * in_rows + j * NOUT;
+#pragma unroll
+    for (int g = 0; g < NSBO / 4; ++g)
+      if (8 * g + 4 * h < NOUT) store4(Dl + 8 * g + 4 * h, dout + 4 * g);
+  }
+  float d[32];
+  mma_blocks<NSBO, 2>(
+      P + tr_bwd_off(kPIn, kPW, depth, depth - 1, NOUT), dout, lane, [&](int) { return f32x16{}; },
+      [&](int ib, const f32x16& acc) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) d[ib * 16 + r] = leaky_mask(mk_last, ib * 16 + r, acc[r], slope);
+      });
+#pragma unroll
+  for (int l = 6; l >= 0; --l) {
+    if (l > depth - 2) continue;
+    if (valid) {  // d = delta of layer l's outputs
+      float* Dl = rows + d_base + (int64_t)l * in_rows + j * kPW;
+#pragma unroll
+      for (int g = 0; g < 8; ++g) store4(Dl + 32 * (g / 4) + 8 * (g % 4) + 4 * h, d + 4 * g);
+    }
+    if (l == 0) break;
+    float e[32];
+    mma_blocks<32, 2>(
+        P + tr_bwd_off(kPIn, kPW, depth, l, NOUT), d, lane, [&](int) { return f32x16{}; },
+        [&](int ib, const f32x16& acc) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) e[ib * 16 + r] = leaky_mask(mask[l], ib * 16 + r, acc[r], slope);
+        });
+#pragma unroll
+    for (int r = 0; r < 32; ++r) d[r] = e[r];
   }
 }
 
@@ -200,46 +251,76 @@ int64_t snisr_pemlp_param_count(int32_t L, int32_t depth, int32_t n_out) {
 
 int64_t snisr_pemlp_workspace_size(int64_t n, int32_t L, int32_t depth, int32_t n_out) {
   if (n < 0 || pe_check(L, depth, n_out)) return -1;
-  return 4 * pe_d_off(depth, L, depth, n_out, n) + 256;
+  return 4 * (pe_frag_floats(depth, n_out) + pe_d_off(depth, L, depth, n_out, n)) + 256;
 }
+
+namespace {
+int pe_pack(const float* params, int depth, int n_out, float* frag, hipStream_t st) {
+  const int64_t tot = pe_frag_floats(depth, n_out);
+  hipLaunchKernelGGL(k_pack_train_frag, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, params,
+                     pe_param_off(depth, kPL, depth, n_out), frag, kPIn, kPW, depth, 1, n_out);
+  SNISR_LAUNCHED();
+  return SNISR_OK;
+}
+}  // namespace
 
 int snisr_pemlp_forward(const float* params, const float* points, int64_t n, int32_t L, int32_t depth, int32_t n_out,
                         float slope, float* out, void* workspace, int64_t workspace_bytes, void* stream) {
   if (int e = pe_check(L, depth, n_out)) return e;
-  SNISR_REQUIRE(n >= 0, SNISR_ERR_ARG, "pemlp_forward: negative size");
+  SNISR_REQUIRE(n >= 0 && slope >= 0.0f, SNISR_ERR_ARG, "pemlp_forward: negative size or slope");
   if (n == 0) return SNISR_OK;
-  SNISR_REQUIRE(params && points && out, SNISR_ERR_ARG, "pemlp_forward: NULL pointer");
-  SNISR_REQUIRE(!workspace || workspace_bytes >= snisr_pemlp_workspace_size(n, L, depth, n_out), SNISR_ERR_CAPACITY,
-                "pemlp_forward: workspace too small");
-  SNISR_REQUIRE(!workspace || aligned(workspace, 16), SNISR_ERR_ALIGN, "pemlp_forward: workspace alignment");
-  hipLaunchKernelGGL(k_pemlp_fwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, params,
-                     points, n, (int)L, (int)depth, (int)n_out, slope, out, (float*)workspace);
+  SNISR_REQUIRE(params && points && out && workspace, SNISR_ERR_ARG, "pemlp_forward: NULL pointer");
+  SNISR_REQUIRE(workspace_bytes >= snisr_pemlp_workspace_size(0, L, depth, n_out), SNISR_ERR_CAPACITY,
+                "pemlp_forward: workspace too small (snisr_pemlp_workspace_size(0, ...) bytes)");
+  SNISR_REQUIRE(aligned(workspace, 16) && aligned(out, 16), SNISR_ERR_ALIGN, "pemlp_forward: alignment");
+  hipStream_t st = (hipStream_t)stream;
+  float* frag = (float*)workspace;
+  if (int e = pe_pack(params, depth, n_out, frag, st)) return e;
+  const dim3 grid((unsigned)((n + 127) / 128));
+  if (n_out == 28)
+    hipLaunchKernelGGL((k_pemlp_mma<28, false>), grid, dim3(256), 0, st, frag, points, n, (int)depth, slope, out,
+                       nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((k_pemlp_mma<4, false>), grid, dim3(256), 0, st, frag, points, n, (int)depth, slope, out,
+                       nullptr, nullptr);
   SNISR_LAUNCHED();
   return SNISR_OK;
 }
 
-int snisr_pemlp_backward(const float* params, int64_t n, int32_t L, int32_t depth, int32_t n_out, float slope,
-                         const float* g_out, float* g_params, void* workspace, int64_t workspace_bytes, void* stream) {
+int snisr_pemlp_backward(const float* params, const float* points, int64_t n, int32_t L, int32_t depth, int32_t n_out,
+                         float slope, const float* g_out, float* g_params, void* workspace, int64_t workspace_bytes,
+                         void* stream) {
   if (int e = pe_check(L, depth, n_out)) return e;
-  SNISR_REQUIRE(n >= 0, SNISR_ERR_ARG, "pemlp_backward: negative size");
-  SNISR_REQUIRE(params && g_params && workspace && (n == 0 || g_out), SNISR_ERR_ARG, "pemlp_backward: NULL pointer");
-  SNISR_REQUIRE(workspace_bytes >= snisr_pemlp_workspace_size(n, L, depth, n_out), SNISR_ERR_CAPACITY,
-                "pemlp_backward: workspace too small (the forward's rows)");
+  SNISR_REQUIRE(n >= 0 && slope >= 0.0f, SNISR_ERR_ARG, "pemlp_backward: negative size or slope");
+  SNISR_REQUIRE(params && g_params && (n == 0 || (points && g_out && workspace)), SNISR_ERR_ARG,
+                "pemlp_backward: NULL pointer");
+  SNISR_REQUIRE(n == 0 || workspace_bytes >= snisr_pemlp_workspace_size(n, L, depth, n_out), SNISR_ERR_CAPACITY,
+                "pemlp_backward: workspace too small (snisr_pemlp_workspace_size(n, ...) bytes)");
+  SNISR_REQUIRE(n == 0 || (aligned(workspace, 16) && aligned(g_out, 16)), SNISR_ERR_ALIGN,
+                "pemlp_backward: alignment");
   hipStream_t st = (hipStream_t)stream;
   const int64_t np = pe_param_off(depth, L, depth, n_out);
   hipLaunchKernelGGL(k_pe_zero, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, g_params, np);
   SNISR_LAUNCHED();
   if (n == 0) return SNISR_OK;
-  hipLaunchKernelGGL(k_pemlp_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, params, n, (int)L,
-                     (int)depth, (int)n_out, slope, g_out, (float*)workspace);
+  float* frag = (float*)workspace;
+  float* rows = frag + pe_frag_floats(depth, n_out);
+  if (int e = pe_pack(params, depth, n_out, frag, st)) return e;
+  const dim3 grid((unsigned)((n + 127) / 128));
+  if (n_out == 28)
+    hipLaunchKernelGGL((k_pemlp_mma<28, true>), grid, dim3(256), 0, st, (const float*)frag, points, n, (int)depth,
+                       slope, nullptr, g_out, rows);
+  else
+    hipLaunchKernelGGL((k_pemlp_mma<4, true>), grid, dim3(256), 0, st, (const float*)frag, points, n, (int)depth,
+                       slope, nullptr, g_out, rows);
   SNISR_LAUNCHED();
   int64_t splits = (n + 8191) / 8192;
   splits = splits < 1 ? 1 : (splits > 4 * (int64_t)device_cus() ? 4 * (int64_t)device_cus() : splits);
   const dim3 g((unsigned)splits, (unsigned)depth);
   if (n_out == 28)
-    hipLaunchKernelGGL(k_pemlp_dw<28>, g, dim3(256), 0, st, (const float*)workspace, n, (int)L, (int)depth, g_params);
+    hipLaunchKernelGGL(k_pemlp_dw<28>, g, dim3(256), 0, st, (const float*)rows, n, (int)L, (int)depth, g_params);
   else
-    hipLaunchKernelGGL(k_pemlp_dw<4>, g, dim3(256), 0, st, (const float*)workspace, n, (int)L, (int)depth, g_params);
+    hipLaunchKernelGGL(k_pemlp_dw<4>, g, dim3(256), 0, st, (const float*)rows, n, (int)L, (int)depth, g_params);
   SNISR_LAUNCHED();
   return SNISR_OK;
 }

@@ -188,8 +188,8 @@ SIGNATURES = [
     ("snisr_pemlp_workspace_size", C.c_int64, [C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
     ("snisr_pemlp_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_float,
                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
-    ("snisr_pemlp_backward", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_void_p,
-                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("snisr_pemlp_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     ("snisr_flag_cells_workspace_size", C.c_int64, [C.c_int64]),
     ("snisr_flag_cells", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     ("snisr_bake_points", C.c_int, [C.c_void_p, C.c_int64, C.c_int32] + [C.c_float] * 8 + [C.c_void_p, C.c_void_p]),

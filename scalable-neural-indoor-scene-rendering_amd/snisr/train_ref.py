@@ -131,26 +131,22 @@ class _PEMLP(torch.autograd.Function):
         if p.numel() != L.snisr_pemlp_param_count(10, depth, n_out):
             raise ValueError("params do not match the MLP shape")
         out = torch.empty((*points.shape[:-1], n_out), dtype=torch.float32, device=x.device)
-        ws = None
-        if ctx.needs_input_grad[0]:               # keep the layer rows for the backward
-            ws = torch.empty(max(L.snisr_pemlp_workspace_size(n, 10, depth, n_out), 16), dtype=torch.uint8,
-                             device=x.device)
+        ws = torch.empty(L.snisr_pemlp_workspace_size(0, 10, depth, n_out), dtype=torch.uint8, device=x.device)
         N.check(L.snisr_pemlp_forward(_ptr(p), _ptr(x), n, 10, depth, n_out, float(slope), _ptr(out), _ptr(ws),
-                                      0 if ws is None else ws.numel(), _stream()), "pemlp_forward")
-        ctx.save_for_backward(p)
-        ctx.ws, ctx.meta = ws, (n, depth, n_out, float(slope))
+                                      ws.numel(), _stream()), "pemlp_forward")
+        ctx.save_for_backward(p, x)
+        ctx.meta = (n, depth, n_out, float(slope))
         return out
 
     @staticmethod
     def backward(ctx, g_out):
-        (p,) = ctx.saved_tensors
+        p, x = ctx.saved_tensors
         n, depth, n_out, slope = ctx.meta
-        if ctx.ws is None:
-            raise RuntimeError("PEMLP backward needs a forward run with gradients enabled")
+        L = N.lib()
+        ws = torch.empty(L.snisr_pemlp_workspace_size(n, 10, depth, n_out), dtype=torch.uint8, device=p.device)
         g = torch.empty_like(p)
-        N.check(N.lib().snisr_pemlp_backward(_ptr(p), n, 10, depth, n_out, slope, _ptr(g_out.contiguous()), _ptr(g),
-                                             _ptr(ctx.ws), ctx.ws.numel(), _stream()), "pemlp_backward")
-        ctx.ws = None
+        N.check(L.snisr_pemlp_backward(_ptr(p), _ptr(x), n, 10, depth, n_out, slope, _ptr(g_out.contiguous()), _ptr(g),
+                                       _ptr(ws), ws.numel(), _stream()), "pemlp_backward")
         return g, None, None, None, None
 
 
