@@ -314,8 +314,13 @@ int snisr_pemlp_backward(const float* params, const float* points, int64_t n, in
     hipLaunchKernelGGL((k_pemlp_mma<4, true>), grid, dim3(256), 0, st, (const float*)frag, points, n, (int)depth,
                        slope, nullptr, g_out, rows);
   SNISR_LAUNCHED();
-  int64_t splits = (n + 8191) / 8192;
-  splits = splits < 1 ? 1 : (splits > 4 * (int64_t)device_cus() ? 4 * (int64_t)device_cus() : splits);
+  // sample splits of the weight-gradient reduction: about four workgroups per CU over all layers,
+  // at least 256 samples and at most 8192 per split
+  const int64_t cus = device_cus();
+  int64_t splits = (4 * cus + depth - 1) / depth;
+  const int64_t lo = (n + 8191) / 8192, hi = (n + 255) / 256;
+  splits = splits < lo ? lo : (splits > hi ? hi : splits);
+  splits = splits < 1 ? 1 : splits;
   const dim3 g((unsigned)splits, (unsigned)depth);
   if (n_out == 28)
     hipLaunchKernelGGL(k_pemlp_dw<28>, g, dim3(256), 0, st, (const float*)rows, n, (int)L, (int)depth, g_params);

@@ -6,7 +6,8 @@
 //   - the specular head of Tile.inference_specular (training/scripts/tile/tile.py:198-223): the MLP
 //     output's 27 coefficients read as [basis][rgb], evaluated in real SH of degree <= 2 at the
 //     normalised ray direction (training/scripts/sh.py:55-111) and passed through a sigmoid.
-// One thread per ray (compositing) or per sample (SH).  FP32 throughout; the exponential is exp_cw.
+// One wavefront per ray (compositing, lane scans) or one thread per sample (SH).  FP32 throughout; the
+// exponential is exp_cw.
 #include "snisr_common.hpp"
 #include "snisr_device.hpp"
 
@@ -15,29 +16,58 @@ namespace {
 
 constexpr float kTransEps = 1e-10f;  // the reference's 1 - alpha + 1e-10
 
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// One wavefront per ray, one lane per sample (64-sample chunks with the transparency carried):
+// T_i = init_T * exclusive product of f_j = 1 - alpha_j + 1e-10 (a lane scan), out = sum alpha T rgb.
 __global__ __launch_bounds__(256) void k_volume_render_fwd(const float* __restrict__ rgb, const float* __restrict__ sigma,
                                                            const float* __restrict__ dists,
                                                            const float* __restrict__ init_T, int64_t n_rays, int n,
                                                            float* __restrict__ out, float* __restrict__ trans,
                                                            float* __restrict__ alpha) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= n_rays) return;
-  float T = init_T ? init_T[r] : 1.0f;
+  const int lane = threadIdx.x & 63;
+  float T0 = init_T ? init_T[r] : 1.0f;
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-  for (int i = 0; i < n; ++i) {
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + lane;
     const int64_t k = r * n + i;
-    const float a = 1.0f - exp_cw(-(sigma[k] * dists[k]));
-    const float w = a * T;
-    c0 = c0 + w * rgb[3 * k];
-    c1 = c1 + w * rgb[3 * k + 1];
-    c2 = c2 + w * rgb[3 * k + 2];
-    trans[k] = T;
-    alpha[k] = a;
-    T = T * (1.0f - a + kTransEps);
+    float a = 0.0f, f = 1.0f, x0 = 0.0f, x1 = 0.0f, x2 = 0.0f;
+    if (i < n) {
+      a = 1.0f - exp_cw(-(sigma[k] * dists[k]));
+      f = 1.0f - a + kTransEps;
+      x0 = rgb[3 * k];
+      x1 = rgb[3 * k + 1];
+      x2 = rgb[3 * k + 2];
+    }
+    float p = f;  // inclusive product over lanes 0..lane
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const float t = __shfl_up(p, o);
+      if (lane >= o) p = p * t;
+    }
+    float ex = __shfl_up(p, 1);
+    if (lane == 0) ex = 1.0f;
+    const float T = T0 * ex, w = a * T;
+    if (i < n) {
+      trans[k] = T;
+      alpha[k] = a;
+    }
+    c0 = c0 + wave_sum(w * x0);
+    c1 = c1 + wave_sum(w * x1);
+    c2 = c2 + wave_sum(w * x2);
+    T0 = T0 * __shfl(p, 63);
   }
-  out[3 * r] = c0;
-  out[3 * r + 1] = c1;
-  out[3 * r + 2] = c2;
+  if (lane == 0) {
+    out[3 * r] = c0;
+    out[3 * r + 1] = c1;
+    out[3 * r + 2] = c2;
+  }
 }
 
 // Reverse pass over the forward's saved transparency and alpha (T_i, alpha_i).  With
@@ -45,7 +75,9 @@ __global__ __launch_bounds__(256) void k_volume_render_fwd(const float* __restri
 // dL/dT_i (G_n = 0):
 //   dL/dc_i = g w_i;  dL/dalpha_i = T_i (g . c_i) + ga_i;  G_i = alpha_i (g . c_i) + gt_i + a_i G_{i+1};
 //   dL/de_i = -dL/dalpha_i + T_i G_{i+1};  dL/dsigma_i = -d_i e_i dL/de_i;  dL/dinit_T = G_0.
-// e_i is recomputed as 1 - alpha_i, the value the forward used.
+// e_i is recomputed as 1 - alpha_i, the value the forward used.  One wavefront per ray: the affine
+// recurrence G_i = A_i + a_i G_{i+1} is a suffix scan of the maps x -> A_i + a_i x over the lanes,
+// 64-sample chunks from the last, with G carried between chunks.
 __global__ __launch_bounds__(256) void k_volume_render_bwd(const float* __restrict__ rgb, const float* __restrict__ dists,
                                                            const float* __restrict__ trans,
                                                            const float* __restrict__ alpha, int64_t n_rays, int n,
@@ -54,25 +86,48 @@ __global__ __launch_bounds__(256) void k_volume_render_bwd(const float* __restri
                                                            const float* __restrict__ g_alpha,
                                                            float* __restrict__ g_rgb, float* __restrict__ g_sigma,
                                                            float* __restrict__ g_init) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= n_rays) return;
+  const int lane = threadIdx.x & 63;
   const float g0 = g_out[3 * r], g1 = g_out[3 * r + 1], g2 = g_out[3 * r + 2];
-  float G = 0.0f;  // dL/dT_{i+1}
-  for (int i = n - 1; i >= 0; --i) {
+  float carry = 0.0f;  // G at the first sample after the chunk
+  for (int i0 = ((n - 1) / 64) * 64; i0 >= 0; i0 -= 64) {
+    const int i = i0 + lane;
     const int64_t k = r * n + i;
-    const float T = trans[k], a = alpha[k];
+    const bool on = i < n;
+    float T = 0.0f, a = 0.0f, gc = 0.0f;
+    if (on) {
+      T = trans[k];
+      a = alpha[k];
+      const float x0 = rgb[3 * k], x1 = rgb[3 * k + 1], x2 = rgb[3 * k + 2];
+      gc = g0 * x0 + g1 * x1 + g2 * x2;
+      const float w = a * T;
+      g_rgb[3 * k] = g0 * w;
+      g_rgb[3 * k + 1] = g1 * w;
+      g_rgb[3 * k + 2] = g2 * w;
+    }
     const float e = 1.0f - a;
-    const float gc = g0 * rgb[3 * k] + g1 * rgb[3 * k + 1] + g2 * rgb[3 * k + 2];
-    const float w = a * T;
-    g_rgb[3 * k] = g0 * w;
-    g_rgb[3 * k + 1] = g1 * w;
-    g_rgb[3 * k + 2] = g2 * w;
-    const float d_alpha = T * gc + (g_alpha ? g_alpha[k] : 0.0f);
-    const float d_e = -d_alpha + T * G;
-    g_sigma[k] = -dists[k] * e * d_e;
-    G = a * gc + (g_trans ? g_trans[k] : 0.0f) + (e + kTransEps) * G;
+    float pf = on ? e + kTransEps : 1.0f;                               // identity past the end
+    float pa = on ? a * gc + (g_trans ? g_trans[k] : 0.0f) : 0.0f;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {  // (pf, pa) := map_lane o ... o map_{lane + 2^s - 1}
+      const float tf = __shfl_down(pf, o), ta = __shfl_down(pa, o);
+      if (lane + o < 64) {
+        pa = pa + pf * ta;
+        pf = pf * tf;
+      }
+    }
+    const float G = pa + pf * carry;     // G_i
+    float Gn = __shfl_down(G, 1);        // G_{i+1}
+    if (lane == 63) Gn = carry;
+    if (on) {
+      const float d_alpha = T * gc + (g_alpha ? g_alpha[k] : 0.0f);
+      const float d_e = -d_alpha + T * Gn;
+      g_sigma[k] = -dists[k] * e * d_e;
+    }
+    carry = __shfl(G, 0);
   }
-  if (g_init) g_init[r] = G;
+  if (g_init && lane == 0) g_init[r] = n > 0 ? carry : 0.0f;
 }
 
 // real SH of degree <= 2 at unit direction (x, y, z), the reference's sign convention; unused
@@ -116,7 +171,9 @@ __global__ __launch_bounds__(256) void k_sh_sigmoid_fwd(const float* __restrict_
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float acc = 0.0f;
-    for (int j = 0; j < nb; ++j) acc = acc + b[j] * q[3 * j + c];
+#pragma unroll
+    for (int j = 0; j < 9; ++j)  // fully unrolled: b stays in registers
+      if (j < nb) acc = acc + b[j] * q[3 * j + c];
     spec[3 * s + c] = sigmoid_cw(acc);
   }
 }
@@ -157,7 +214,7 @@ int snisr_volume_render_forward(const float* rgb, const float* sigma, const floa
   if (n_rays == 0) return SNISR_OK;
   SNISR_REQUIRE(out && (n == 0 || (rgb && sigma && dists && trans && alpha)), SNISR_ERR_ARG,
                 "volume_render_forward: NULL pointer");
-  hipLaunchKernelGGL(k_volume_render_fwd, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(k_volume_render_fwd, dim3((unsigned)((n_rays + 3) / 4)), dim3(256), 0, (hipStream_t)stream,
                      rgb, sigma, dists, init_T, n_rays, (int)n, out, trans, alpha);
   SNISR_LAUNCHED();
   return SNISR_OK;
@@ -170,7 +227,7 @@ int snisr_volume_render_backward(const float* rgb, const float* dists, const flo
   if (n_rays == 0) return SNISR_OK;
   SNISR_REQUIRE(g_out && (n == 0 || (rgb && dists && trans && alpha && g_rgb && g_sigma)), SNISR_ERR_ARG,
                 "volume_render_backward: NULL pointer");
-  hipLaunchKernelGGL(k_volume_render_bwd, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(k_volume_render_bwd, dim3((unsigned)((n_rays + 3) / 4)), dim3(256), 0, (hipStream_t)stream,
                      rgb, dists, trans, alpha, n_rays, (int)n, g_out, g_trans, g_alpha, g_rgb, g_sigma, g_init);
   SNISR_LAUNCHED();
   return SNISR_OK;
