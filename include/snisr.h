@@ -462,7 +462,8 @@ int snisr_pemlp_backward(const float* params, const float* points, int64_t n, in
  *  - normalize_points: (x - center) / scale of world points (ray samples), f32[n, 3];
  *  - diffuse_head_forward: raw f32[n, 4] MLP outputs -> (sigmoid(r0..r2), relu(r3)); with cells, row k
  *    goes to row cells[k] of out f32[n_rows, 4] after the n_rows rows are zeroed (the bake), without
- *    cells out has n_rows = n rows.  The backward reads dL/dout at the same rows and writes dL/draw;
+ *    cells out has n_rows = n rows; a cell index outside 0..n_rows-1 is skipped.  The backward reads
+ *    dL/dout at the same rows (zero for a skipped index) and writes dL/draw;
  *  - voxel_query_forward: F.grid_sample (trilinear, align_corners, zero padding) of the grid at world
  *    points f32[n, 3] through the reference's coordinate chain ((x - min_corner) / voxel_size - 0.5,
  *    then / (D - 1) * 2 - 1, then the sampler's ((g + 1) / 2) (D - 1)); rgb f32[n, 3], sigma f32[n].
@@ -476,8 +477,8 @@ int snisr_normalize_points(const float* x, int64_t n, float center_x, float cent
                            float* out, void* stream);
 int snisr_diffuse_head_forward(const float* raw, int64_t n, const int32_t* cells, int64_t n_rows, float* out,
                                void* stream);
-int snisr_diffuse_head_backward(const float* raw, int64_t n, const int32_t* cells, const float* g_out, float* g_raw,
-                                void* stream);
+int snisr_diffuse_head_backward(const float* raw, int64_t n, const int32_t* cells, int64_t n_rows, const float* g_out,
+                                float* g_raw, void* stream);
 int snisr_voxel_query_forward(const float* voxels, int32_t D, const float* points, int64_t n, float mc_x, float mc_y,
                               float mc_z, float voxel_size, float* rgb, float* sigma, void* stream);
 int snisr_voxel_query_backward(int32_t D, const float* points, int64_t n, float mc_x, float mc_y, float mc_z,

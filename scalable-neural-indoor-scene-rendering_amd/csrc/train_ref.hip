@@ -159,8 +159,8 @@ __device__ __forceinline__ V3 unit_dir(const float* d) {
 
 // spec[s][c] = sigmoid(sum_b Y_b(dir) coeffs[s][b * 3 + c]), dir = dirs of ray s / spr
 __global__ __launch_bounds__(256) void k_sh_sigmoid_fwd(const float* __restrict__ coeffs, int coeff_stride,
-                                                        const float* __restrict__ dirs, int64_t n, int spr, int deg,
-                                                        float* __restrict__ spec) {
+                                                        bool vec_rows, const float* __restrict__ dirs, int64_t n,
+                                                        int spr, int deg, float* __restrict__ spec) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const V3 d = unit_dir(dirs + 3 * (s / spr));
@@ -168,12 +168,28 @@ __global__ __launch_bounds__(256) void k_sh_sigmoid_fwd(const float* __restrict_
   sh_basis(deg, d.x, d.y, d.z, b);
   const int nb = (deg + 1) * (deg + 1);
   const float* q = coeffs + s * coeff_stride;
+  // the 27 coefficients of the row: 16-byte loads when rows are 16-byte aligned (a row per lane,
+  // so scalar loads would touch a cache line per lane 27 times)
+  float v[28];
+  if (vec_rows) {
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+      const float4 x = reinterpret_cast<const float4*>(q)[t];
+      v[4 * t] = x.x;
+      v[4 * t + 1] = x.y;
+      v[4 * t + 2] = x.z;
+      v[4 * t + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < 27; ++t) v[t] = q[t];
+  }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float acc = 0.0f;
 #pragma unroll
-    for (int j = 0; j < 9; ++j)  // fully unrolled: b stays in registers
-      if (j < nb) acc = acc + b[j] * q[3 * j + c];
+    for (int j = 0; j < 9; ++j)  // fully unrolled: b and v stay in registers
+      if (j < nb) acc = acc + b[j] * v[3 * j + c];
     spec[3 * s + c] = sigmoid_cw(acc);
   }
 }
@@ -241,8 +257,9 @@ int snisr_sh_sigmoid_forward(const float* coeffs, int32_t coeff_stride, const fl
                 "sh_sigmoid_forward: coeff_stride below 3 (deg + 1)^2");
   if (n == 0) return SNISR_OK;
   SNISR_REQUIRE(coeffs && dirs && spec, SNISR_ERR_ARG, "sh_sigmoid_forward: NULL pointer");
+  const bool vec_rows = coeff_stride >= 28 && coeff_stride % 4 == 0 && aligned(coeffs, 16);
   hipLaunchKernelGGL(k_sh_sigmoid_fwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, coeffs,
-                     (int)coeff_stride, dirs, n, (int)samples_per_ray, (int)deg, spec);
+                     (int)coeff_stride, vec_rows, dirs, n, (int)samples_per_ray, (int)deg, spec);
   SNISR_LAUNCHED();
   return SNISR_OK;
 }

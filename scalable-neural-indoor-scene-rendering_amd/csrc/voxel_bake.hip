@@ -103,23 +103,30 @@ __global__ __launch_bounds__(256) void k_normalize_points(const float* __restric
 
 __device__ __forceinline__ float sigmoid_f(float v) { return 1.0f / (1.0f + exp_cw(-v)); }
 
-// row k of raw [n][4] -> (sigmoid r0..r2, relu r3) at row cells[k] (or k)
+// row k of raw [n][4] -> (sigmoid r0..r2, relu r3) at row cells[k] (or k); a cell index outside
+// 0..n_rows-1 writes nothing
 __global__ __launch_bounds__(256) void k_head_fwd(const float* __restrict__ raw, int64_t n,
-                                                  const int32_t* __restrict__ cells, float* __restrict__ out) {
+                                                  const int32_t* __restrict__ cells, int64_t n_rows,
+                                                  float* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const float4 r = reinterpret_cast<const float4*>(raw)[k];
   const int64_t row = cells ? (int64_t)cells[k] : k;
+  if (row < 0 || row >= n_rows) return;
   reinterpret_cast<float4*>(out)[row] = make_float4(sigmoid_f(r.x), sigmoid_f(r.y), sigmoid_f(r.z), r.w > 0.0f ? r.w : 0.0f);
 }
 
+// dL/draw of row k from dL/dout at row cells[k] (or k); a cell index outside 0..n_rows-1 reads a
+// zero gradient
 __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ raw, int64_t n,
-                                                  const int32_t* __restrict__ cells, const float* __restrict__ g_out,
-                                                  float* __restrict__ g_raw) {
+                                                  const int32_t* __restrict__ cells, int64_t n_rows,
+                                                  const float* __restrict__ g_out, float* __restrict__ g_raw) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const float4 r = reinterpret_cast<const float4*>(raw)[k];
-  const float4 g = reinterpret_cast<const float4*>(g_out)[cells ? (int64_t)cells[k] : k];
+  const int64_t row = cells ? (int64_t)cells[k] : k;
+  const float4 g = (row >= 0 && row < n_rows) ? reinterpret_cast<const float4*>(g_out)[row]
+                                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float s0 = sigmoid_f(r.x), s1 = sigmoid_f(r.y), s2 = sigmoid_f(r.z);
   reinterpret_cast<float4*>(g_raw)[k] =
       make_float4(g.x * (s0 * (1.0f - s0)), g.y * (s1 * (1.0f - s1)), g.z * (s2 * (1.0f - s2)), r.w > 0.0f ? g.w : 0.0f);
@@ -280,19 +287,21 @@ int snisr_diffuse_head_forward(const float* raw, int64_t n, const int32_t* cells
     SNISR_LAUNCHED();
   }
   if (n == 0) return SNISR_OK;
-  hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(n)), dim3(256), 0, st, raw, n, cells, out);
+  hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(n)), dim3(256), 0, st, raw, n, cells, n_rows, out);
   SNISR_LAUNCHED();
   return SNISR_OK;
 }
 
-int snisr_diffuse_head_backward(const float* raw, int64_t n, const int32_t* cells, const float* g_out, float* g_raw,
-                                void* stream) {
-  SNISR_REQUIRE(n >= 0, SNISR_ERR_SHAPE, "diffuse_head_backward: negative size");
+int snisr_diffuse_head_backward(const float* raw, int64_t n, const int32_t* cells, int64_t n_rows, const float* g_out,
+                                float* g_raw, void* stream) {
+  SNISR_REQUIRE(n >= 0 && n_rows >= 0 && (cells || n_rows == n), SNISR_ERR_SHAPE,
+                "diffuse_head_backward: without cells dL/dout has one row per input row");
   if (n == 0) return SNISR_OK;
   SNISR_REQUIRE(raw && g_out && g_raw, SNISR_ERR_ARG, "diffuse_head_backward: NULL pointer");
   SNISR_REQUIRE(aligned(raw, 16) && aligned(g_out, 16) && aligned(g_raw, 16), SNISR_ERR_ALIGN,
                 "diffuse_head_backward: rows must be 16-byte aligned");
-  hipLaunchKernelGGL(k_head_bwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, raw, n, cells, g_out, g_raw);
+  hipLaunchKernelGGL(k_head_bwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, raw, n, cells, n_rows, g_out,
+                     g_raw);
   SNISR_LAUNCHED();
   return SNISR_OK;
 }

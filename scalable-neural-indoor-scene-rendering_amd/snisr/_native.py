@@ -195,7 +195,8 @@ SIGNATURES = [
     ("snisr_bake_points", C.c_int, [C.c_void_p, C.c_int64, C.c_int32] + [C.c_float] * 8 + [C.c_void_p, C.c_void_p]),
     ("snisr_normalize_points", C.c_int, [C.c_void_p, C.c_int64] + [C.c_float] * 4 + [C.c_void_p, C.c_void_p]),
     ("snisr_diffuse_head_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
-    ("snisr_diffuse_head_backward", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("snisr_diffuse_head_backward", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                              C.c_void_p]),
     ("snisr_voxel_query_forward", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64] + [C.c_float] * 4
      + [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("snisr_voxel_query_backward", C.c_int, [C.c_int32, C.c_void_p, C.c_int64] + [C.c_float] * 4

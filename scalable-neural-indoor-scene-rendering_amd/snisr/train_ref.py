@@ -207,15 +207,15 @@ class _DiffuseHead(torch.autograd.Function):
         N.check(N.lib().snisr_diffuse_head_forward(_ptr(r), r.shape[0], _ptr(cells), n_rows, _ptr(out), _stream()),
                 "diffuse_head_forward")
         ctx.save_for_backward(r, cells)
-        ctx.raw_shape = raw.shape
+        ctx.raw_shape, ctx.n_rows = raw.shape, n_rows
         return out
 
     @staticmethod
     def backward(ctx, g_out):
         r, cells = ctx.saved_tensors
         g = torch.empty_like(r)
-        N.check(N.lib().snisr_diffuse_head_backward(_ptr(r), r.shape[0], _ptr(cells), _ptr(g_out.contiguous()), _ptr(g),
-                                                    _stream()), "diffuse_head_backward")
+        N.check(N.lib().snisr_diffuse_head_backward(_ptr(r), r.shape[0], _ptr(cells), ctx.n_rows, _ptr(g_out.contiguous()),
+                                                    _ptr(g), _stream()), "diffuse_head_backward")
         return g.reshape(ctx.raw_shape), None, None
 
 

@@ -356,3 +356,23 @@ def test_diffuse_training_step_through_the_bake():
     g64 = torch.cat([q.grad.reshape(-1) for q in seq.parameters()]).numpy()
     assert abs(loss.item() - loss64.item()) <= 1e-5 * loss64.item()
     assert rel(m.params.grad.cpu().numpy(), g64) < 1e-3
+
+
+def test_diffuse_head_scatter_skips_out_of_range_cells(oracle_mod):
+    """The bake's scatter writes row cells[k] of a zeroed grid; indices outside the grid are skipped
+    and read a zero gradient back."""
+    rng = np.random.default_rng(3)
+    raw = rng.normal(size=(6, 4))
+    cells = torch.tensor([0, 7, -1, 3, 8, 5], dtype=torch.int32, device=DEV)   # -1 and 8 lie outside 8 rows
+    t = dev(raw).requires_grad_(True)
+    out = TR._DiffuseHead.apply(t, cells, 8)
+    g = rng.normal(size=(8, 4))
+    out.backward(dev(g))
+    ro, _ = oracle_mod.diffuse_head(raw.astype(np.float32), np.zeros((6, 4)))
+    exp = np.zeros((8, 4))
+    for k, c in enumerate([0, 7, -1, 3, 8, 5]):
+        if 0 <= c < 8:
+            exp[c] = ro[k]
+    np.testing.assert_allclose(out.detach().cpu().numpy(), exp, rtol=1e-6, atol=1e-7)
+    gr = t.grad.cpu().numpy()
+    assert np.all(gr[2] == 0) and np.all(gr[4] == 0) and np.any(gr[0] != 0)
