@@ -54,9 +54,11 @@ def cpu_baseline(sc, cam, dirs_np, t_hit_np, frac, seed=0):
     t0 = time.perf_counter()
     O.render(osc, cam.origin, dirs_np, t_hit=t_hit_np, ray_index=idx)
     dt = time.perf_counter() - t0
-    return dict(value=round(len(idx) / dt / 1e6, 4), unit="Mrays/s", cores=O.num_threads(), kind="port",
+    cores = O.num_threads()
+    return dict(value=round(len(idx) / dt / 1e6, 4), unit="Mrays/s", cores=cores, kind="port",
                 sample=f"{len(idx)} seeded rays ({frac:.0%} of the 1280x720 frame), FP32 oracle march+MLP+"
-                       f"compositing with the GPU mesh t_hit, {dt:.2f} s wall")
+                       f"compositing with the GPU mesh t_hit, {dt:.2f} s wall on {cores} threads "
+                       f"({dt * cores:.0f} core-seconds)")
 
 
 def psnr_vs_fp64(sc, cam, dirs_np, t_hit_np, rgb_gpu, n_rays=2000, seed=1):
@@ -421,7 +423,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and PSNR legs")
-    ap.add_argument("--cpu-frac", type=float, default=0.05)
+    ap.add_argument("--cpu-frac", type=float, default=0.3,
+                    help="share of the frame's rays the CPU baseline renders (about 25 core-seconds at 0.3)")
     ap.add_argument("--fp32-frames", type=int, default=3)
     ap.add_argument("--train-steps", type=int, default=10, help="timed C3 training steps (0 = skip)")
     ap.add_argument("--ref-train-steps", type=int, default=5,
