@@ -42,12 +42,16 @@ def Sample_sparse(rays_start, rays_dir, node_flags, cx, cy, cz, tile_size, num_v
 
 
 def Sample_reflection(rays_start, rays_dir, node_flags, voxels, cx, cy, cz, tile_size, far, num_voxel, T_th, z_vals):
-    """training/src/Sample.cpp:47-76 (kernel Sample_kernel.cu:16-179)."""
+    """training/src/Sample.cpp:47-76 (kernel Sample_kernel.cu:16-179).  The densities are read as a
+    [num_voxel]^3 x 4 grid (x-major), as the kernel's OVERLAP interpolation indexes them
+    (interpolation.h:120-134).  The reference's training step passes its dilated (num_voxel + 2)^3
+    grid here (tile.py:743-744 through utils.py:786-793), which that indexing reads with num_voxel
+    strides; a larger grid is therefore accepted and read the same way."""
     _check_rays(rays_start, rays_dir, z_vals)
     fl = _flags(node_flags, int(num_voxel))
     _require_cuda(voxels, "voxels", torch.float32)
-    if voxels.dim() != 4 or voxels.shape[-1] != 4 or voxels.numel() != 4 * int(num_voxel) ** 3:
-        raise ValueError("voxels must be [num_voxel, num_voxel, num_voxel, 4] float32")
+    if voxels.dim() != 4 or voxels.shape[-1] != 4 or voxels.numel() < 4 * int(num_voxel) ** 3:
+        raise ValueError("voxels must be a 4-d float32 [..., 4] grid of at least num_voxel^3 cells")
     N.check(N.lib().snisr_sample_reflection(_ptr(rays_start), _ptr(rays_dir), rays_start.shape[0], _ptr(fl),
                                             _ptr(voxels), float(cx), float(cy), float(cz), float(tile_size),
                                             float(far), int(num_voxel), float(T_th), z_vals.shape[1], _ptr(z_vals),
