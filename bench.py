@@ -131,82 +131,64 @@ def train_bench(args, dev):
 
 
 def ref_train_bench(args, dev):
-    """The reference's tile training step in its own form (training/scripts/tile/tile.py:631-820) at
-    its configuration (training/config/cfg.yaml: NUM_VOXEL 64, VOXEL_SIZE 0.02, one 91 x 91 patch,
-    NSAMPLES 64, BGSAMPLE 32, RSAMPLES 64, surface MLP 8 x 64 ReLU, reflection MLP 6 x 64 LeakyReLU,
-    SH degree 2), synthetic: 15% of the dilated 66^3 nodes occupied, rays through the tile.
-    Diffuse step (train_FLAG 0): bake of the surface MLP into the grid, grid samples, background
-    samples through the MLP, chained compositing, MSE, adjoints, Adam.  Specular step (train_FLAG 1):
-    reflection MLP + SH + sigmoid on the reflection samples, compositing, MSE, adjoints, Adam."""
+    """The reference's tile training step, Tile.train_batch (training/scripts/tile/tile.py:631-785),
+    at its configuration (training/config/cfg.yaml: NUM_VOXEL 64, VOXEL_SIZE 0.02, one 91 x 91 patch,
+    NSAMPLES 64, BGSAMPLE 32 over BGSAMPLE_RANGE 0.5, RSAMPLES 64, surface MLP 8 x 64 ReLU, reflection
+    MLP 6 x 64 LeakyReLU, SH degree 2), on synthetic data: 15% of the dilated 66^3 nodes occupied,
+    rays from around the tile through it with a background depth behind it.  One step = the
+    samplers, the forward of train_FLAG 0 (diffuse), 1 (specular) or 2 (both), a masked MSE, the
+    adjoints and Adam on both MLPs."""
     import numpy as np
     import torch
+    from snisr import tile_train as TT
     from snisr import train as TRN
     from snisr import train_ref as TR
 
     rng = np.random.default_rng(5)
     NV, vs, P, S, NB, RS = 64, 0.02, 91, 64, 32, 64
-    D, tile_size, B = NV + 2, NV * vs, P * P
+    D, tile_size, n = NV + 2, NV * vs, P * P
     tc = np.zeros(3)
-    mc = (tc - tile_size / 2 - vs).astype(np.float32)
-    gc, gs = np.zeros(3, np.float32), 4.0
     flags = torch.tensor(np.where(rng.uniform(0, 1, (D, D, D)) < 0.15, 1, -1).astype(np.int16), device=dev)
-    o = tc + rng.uniform(-0.5, 0.5, (B, 3)) * tile_size
-    d = rng.normal(size=(B, 3))
+    tile = TT.TileState(nodes_flag=flags, num_voxel=NV, voxel_size=vs, tile_center=tuple(tc), group_center=(0.0, 0.0, 0.0),
+                        group_size=4.0, far=8.0)
+    o = tc + rng.normal(size=(n, 3)) / np.sqrt(3) * 1.6 * tile_size
+    tgt = tc + rng.uniform(-0.45, 0.45, (n, 3)) * tile_size
+    d = tgt - o
     d /= np.linalg.norm(d, axis=1, keepdims=True)
-    z = np.concatenate([np.sort(rng.uniform(0, tile_size, (B, S)), 1),
-                        np.sort(rng.uniform(tile_size, tile_size + 0.5, (B, NB)), 1)], 1)
-    dists = np.concatenate([z[:, 1:] - z[:, :-1], np.full((B, 1), 1e10)], 1).astype(np.float32)
-    pts = (o[:, None] + z[..., None] * d[:, None]).astype(np.float32)
-    zr = np.sort(rng.uniform(0.1, 6.0, (B, RS)), 1)
-    rpts = (o[:, None] + zr[..., None] * d[:, None]).astype(np.float32)
-    rd = np.concatenate([zr[:, 1:] - zr[:, :-1], np.full((B, 1), 1e10)], 1).astype(np.float32)
-    f = lambda a: torch.tensor(a, device=dev)  # noqa: E731
-    t_in, t_bg, d_in, d_bg = f(pts[:, :S]), f(pts[:, S:]), f(dists[:, :S, None]), f(dists[:, S:, None])
-    t_r, d_r, dirs = f(rpts), f(rd[..., None]), f(d.astype(np.float32))
-    gt, bg_mask = f(rng.uniform(0, 1, (B, 3)).astype(np.float32)), f((rng.uniform(0, 1, (B, 1)) < 0.8).astype(np.float32))
+    bgd = np.linalg.norm(tgt - o, axis=1, keepdims=True) + tile_size * rng.uniform(0.6, 1.5, (n, 1))
+    batch = np.concatenate([o, d, bgd, rng.uniform(0, 1, (n, 6)), np.ones((n, 1))], 1).reshape(1, P, P, 14)
+    batch = torch.tensor(batch.astype(np.float32), device=dev)
     surf, refl = TR.PEMLP(8, 4, 0.0, device=dev), TR.PEMLP(6, 28, 0.01, device=dev)
     torch.manual_seed(0)
     with torch.no_grad():
         for m in (surf, refl):
             m.params.normal_(0.0, 0.1)
-    cells = TR.flag_cells(flags)
-    opt_d, opt_s = TRN.Adam([surf.params], lr=1e-3), TRN.Adam([refl.params], lr=1e-4)
+    opt = TRN.Adam([surf.params, refl.params], lr=1e-4)
 
-    def diffuse_step():
-        opt_d.zero_grad(set_to_none=True)
-        vox = TR.bake_voxels(surf, cells, D, mc, vs, gc, gs)
-        rgb, sigma = TR.query_voxel(vox, t_in, mc, vs)
-        bg_rgb, bg_sigma = TR.inference_diffuse(surf, t_bg, gc, gs)
-        c_in, trans, _ = TR.volume_rendering(rgb, sigma, d_in)
-        c_bg, _, _ = TR.volume_rendering(bg_rgb, bg_sigma, d_bg, trans[:, -1:])
-        loss = torch.mean((c_in + c_bg * bg_mask - gt) ** 2)
+    def step(flag):
+        opt.zero_grad(set_to_none=True)
+        out = TT.train_batch_forward(tile, surf, refl, batch, flag, S, NB, RS, 0.5, P)
+        m = out["mask"]
+        loss = (((out["pred"] - out["gtcolor"]) ** 2) * m).sum() / m.sum()
         loss.backward()
-        opt_d.step()
-
-    def specular_step():
-        opt_s.zero_grad(set_to_none=True)
-        raw = refl(TR.normalize_points(t_r, gc, 8.0))      # norm_specular_point, far = 8
-        spec = TR.sh_sigmoid(raw, dirs, 2)
-        c, _, _ = TR.volume_rendering(spec, torch.relu(raw[..., 27:]), d_r)
-        loss = torch.mean((torch.clamp(c, 0, 1) - gt) ** 2)
-        loss.backward()
-        opt_s.step()
+        opt.step()
 
     res = {}
-    for name, fn in (("diffuse", diffuse_step), ("specular", specular_step)):
+    for flag in (0, 1, 2):
         for _ in range(2):
-            fn()
+            step(flag)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.ref_train_steps):
-            fn()
+            step(flag)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / args.ref_train_steps
-        res[f"ref_train_{name}_ms_per_step"] = round(ms, 3)
-        res[f"ref_train_{name}_mrays_s"] = round(B / (ms * 1e-3) / 1e6, 3)
-    res["ref_train_config"] = (f"reference tile step at cfg.yaml sizes: {B} rays (one 91x91 patch), {S} tile + {NB} "
-                               f"background samples (diffuse), {RS} reflection samples (specular), "
-                               f"{int(cells.numel())} of {D ** 3} dilated cells baked per diffuse step")
+        res[f"ref_train_flag{flag}_ms_per_step"] = round(ms, 3)
+        res[f"ref_train_flag{flag}_mrays_s"] = round(n / (ms * 1e-3) / 1e6, 3)
+    res["ref_train_config"] = (f"Tile.train_batch at cfg.yaml sizes: {n} rays (one 91x91 patch), sparse sampling "
+                               f"{S} + background {NB} + reflection {RS} samples per ray, {int(tile.cells().numel())} "
+                               f"of {D ** 3} dilated cells baked per diffuse step; flag 0 diffuse, 1 specular, 2 both; "
+                               "samplers, forward, masked MSE, adjoints, Adam")
     return res
 
 

@@ -94,10 +94,52 @@ __device__ __forceinline__ void smp_uniform(float* z, float near, float far, int
 
 __device__ __forceinline__ V3 ray_of(const float* p, int64_t r) { return v3(p[3 * r], p[3 * r + 1], p[3 * r + 2]); }
 
-__global__ __launch_bounds__(256) void k_sample_sparse(const float* __restrict__ rays_o, const float* __restrict__ rays_d,
-                                                       int64_t n, const int16_t* __restrict__ flags, SmpTile T,
-                                                       int nsamp, float* __restrict__ z_vals,
-                                                       float* __restrict__ dists) {
+// The sparse sampler's occupancy test (flag == 1) from a bit per voxel in LDS when the tile has at
+// most kSmpBitsMax voxels (NUM_VOXEL <= 80), else from the int16 flags.  Each DDA step waits for its
+// flag before it branches, so the walk is a chain of dependent loads: an LDS read in place of an
+// L2 read shortens every link.
+constexpr int kSmpBitsMax = 64 * 1024 * 8;
+struct SmpFlags {
+  const int16_t* g;
+  const uint32_t* bits;  // LDS bitmap or nullptr
+  __device__ __forceinline__ bool on(int v) const {
+    return bits ? ((bits[v >> 5] >> (v & 31)) & 1u) != 0u : g[v] == 1;
+  }
+};
+
+__global__ __launch_bounds__(64) void k_sample_sparse(const float* __restrict__ rays_o, const float* __restrict__ rays_d,
+                                                      int64_t n, const int16_t* __restrict__ flags, SmpTile T,
+                                                      int nsamp, float* __restrict__ z_vals,
+                                                      float* __restrict__ dists) {
+  __shared__ uint32_t fbits[kSmpBitsMax / 32];
+  const int64_t nvox = (int64_t)T.nv * T.nv * T.nv;
+  const bool lds = nvox <= kSmpBitsMax;
+  if (lds) {  // word w = flags 32 w .. 32 w + 31, read as four 16-byte loads when aligned
+    const int64_t nw = (nvox + 31) / 32;
+    const bool vec = (reinterpret_cast<uintptr_t>(flags) & 15) == 0 && (nvox & 31) == 0;
+    for (int64_t w = threadIdx.x; w < nw; w += blockDim.x) {
+      uint32_t m = 0;
+      if (vec) {
+        const uint4* q = reinterpret_cast<const uint4*>(flags + 32 * w);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint4 x = q[u];
+          const uint32_t h[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            m |= (uint32_t)((int16_t)(h[t] & 0xffffu) == 1) << (8 * u + 2 * t);
+            m |= (uint32_t)((int16_t)(h[t] >> 16) == 1) << (8 * u + 2 * t + 1);
+          }
+        }
+      } else {
+        for (int t = 0; t < 32; ++t)
+          if (32 * w + t < nvox && flags[32 * w + t] == 1) m |= 1u << t;
+      }
+      fbits[w] = m;
+    }
+    __syncthreads();
+  }
+  const SmpFlags F{flags, lds ? fbits : nullptr};
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const V3 o = ray_of(rays_o, r), d = ray_of(rays_d, r);
@@ -111,7 +153,7 @@ __global__ __launch_bounds__(256) void k_sample_sparse(const float* __restrict__
   const SmpDda D0 = D;
   float total = 0.0f;
   while (true) {
-    if (flags[(D.cur[0] * T.nv + D.cur[1]) * T.nv + D.cur[2]] == 1) {
+    if (F.on((D.cur[0] * T.nv + D.cur[1]) * T.nv + D.cur[2])) {
       float lo, hi;
       smp_voxel_bound(T, o, d, D.cur, lo, hi);
       total = total + (hi - lo);
@@ -124,7 +166,7 @@ __global__ __launch_bounds__(256) void k_sample_sparse(const float* __restrict__
   bool acc = false, first = false;
   const int bsmp = 4, ins = nsamp - 2 * bsmp;
   while (true) {
-    if (flags[(D.cur[0] * T.nv + D.cur[1]) * T.nv + D.cur[2]] == 1) {
+    if (F.on((D.cur[0] * T.nv + D.cur[1]) * T.nv + D.cur[2])) {
       float lo, hi;
       smp_voxel_bound(T, o, d, D.cur, lo, hi);
       if (lo != -1.0f && hi != -1.0f) {
@@ -292,7 +334,8 @@ int snisr_sample_sparse(const float* rays_o, const float* rays_d, int64_t n, con
   SNISR_REQUIRE(rays_o && rays_d && node_flags && z_vals && dists, SNISR_ERR_ARG, "sample_sparse: NULL pointer");
   SNISR_REQUIRE(num_voxel > 0 && tile_size > 0.0f, SNISR_ERR_ARG, "sample_sparse: num_voxel and tile_size must be positive");
   SNISR_REQUIRE(n < (int64_t(1) << 40), SNISR_ERR_SHAPE, "sample_sparse: too many rays");
-  hipLaunchKernelGGL(k_sample_sparse, ray_grid(n), dim3(256), 0, (hipStream_t)stream, rays_o, rays_d, n, node_flags,
+  hipLaunchKernelGGL(k_sample_sparse, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, (hipStream_t)stream, rays_o, rays_d,
+                     n, node_flags,
                      smp_tile(cx, cy, cz, tile_size, num_voxel), n_samples, z_vals, dists);
   SNISR_LAUNCHED();
   return SNISR_OK;

@@ -191,24 +191,65 @@ __global__ __launch_bounds__(256) void k_query_fwd(const float* __restrict__ vox
   sigma[k] = acc.w;
 }
 
+// The adjoint adds w_j * g into the 8 corners of every sample.  Consecutive samples (lanes) of a ray
+// often share a corner cell: for each corner slot the wavefront sums the contributions over runs of
+// lanes with the same cell (a segmented scan over the lanes) and only a run's last lane issues the
+// four float atomics, which removes most of the contention on the cells of densely sampled voxels.
 __global__ __launch_bounds__(256) void k_query_bwd(int D, const float* __restrict__ pts, int64_t n, float3 mc, float vs,
                                                    const float* __restrict__ g_rgb, const float* __restrict__ g_sigma,
                                                    float* __restrict__ g_vox) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const float g[4] = {g_rgb ? g_rgb[3 * k] : 0.0f, g_rgb ? g_rgb[3 * k + 1] : 0.0f, g_rgb ? g_rgb[3 * k + 2] : 0.0f,
-                      g_sigma ? g_sigma[k] : 0.0f};
-  if (g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f && g[3] == 0.0f) return;
-  const float p[3] = {pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]};
-  const QueryCorners q = query_corners(p, mc, vs, D);
+  const int lane = threadIdx.x & 63;
+  if (k - lane >= n) return;  // the whole wavefront is past the end
+  const bool valid = k < n;
+  float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  QueryCorners q;
+  if (valid) {
+    if (g_rgb) {
+      g[0] = g_rgb[3 * k];
+      g[1] = g_rgb[3 * k + 1];
+      g[2] = g_rgb[3 * k + 2];
+    }
+    if (g_sigma) g[3] = g_sigma[k];
+    const float p[3] = {pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]};
+    q = query_corners(p, mc, vs, D);
+  } else {
 #pragma unroll
-  for (int j = 0; j < 8; ++j)
-    if (q.cell[j] >= 0 && q.w[j] != 0.0f) {
-      float* dst = g_vox + 4 * q.cell[j];
+    for (int j = 0; j < 8; ++j) {
+      q.cell[j] = -1;
+      q.w[j] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int cell = (int)q.cell[j];  // < D^3 < 2^31 (checked on the host)
+    float v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = cell >= 0 ? g[c] * q.w[j] : 0.0f;
+    const int up = __shfl_up(cell, 1), dn = __shfl_down(cell, 1);
+    int f = (lane == 0 || up != cell) ? 1 : 0;  // run head
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {  // inclusive sums within runs
+      float t[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) t[c] = __shfl_up(v[c], o);
+      const int tf = __shfl_up(f, o);
+      if (lane >= o) {
+        if (!f) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) v[c] = v[c] + t[c];
+        }
+        f |= tf;
+      }
+    }
+    const bool tail = lane == 63 || dn != cell;
+    if (tail && cell >= 0) {
+      float* dst = g_vox + 4 * (int64_t)cell;
 #pragma unroll
       for (int c = 0; c < 4; ++c)
-        if (g[c] != 0.0f) atomicAdd(dst + c, g[c] * q.w[j]);
+        if (v[c] != 0.0f) atomicAdd(dst + c, v[c]);
     }
+  }
 }
 
 __global__ void k_vox_zero(float4* __restrict__ p, int64_t n) {
@@ -321,7 +362,8 @@ int snisr_voxel_query_forward(const float* voxels, int32_t D, const float* point
 int snisr_voxel_query_backward(int32_t D, const float* points, int64_t n, float mc_x, float mc_y, float mc_z,
                                float voxel_size, const float* g_rgb, const float* g_sigma, float* g_voxels,
                                void* stream) {
-  SNISR_REQUIRE(n >= 0 && D >= 2, SNISR_ERR_SHAPE, "voxel_query_backward: bad sizes");
+  SNISR_REQUIRE(n >= 0 && D >= 2 && (int64_t)D * D * D < ((int64_t)1 << 31), SNISR_ERR_SHAPE,
+                "voxel_query_backward: bad sizes (D^3 must stay below 2^31)");
   SNISR_REQUIRE(g_voxels, SNISR_ERR_ARG, "voxel_query_backward: NULL gradient grid");
   hipStream_t st = (hipStream_t)stream;
   const int64_t cells = (int64_t)D * D * D;

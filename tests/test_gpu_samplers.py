@@ -31,11 +31,15 @@ def _rays(seed, n, ctr, ts):
     return o, d
 
 
-@pytest.mark.parametrize("nv,N", [(16, 48), (32, 64), (12, 5)])
+@pytest.mark.parametrize("nv,N", [(16, 48), (32, 64), (12, 5), (7, 40), (64, 64), (81, 64)])
 def test_sample_sparse_bit_exact(oracle_mod, nv, N):
+    """nv 64 is the reference's NUM_VOXEL; 7 (343 voxels) builds the kernel's LDS occupancy bits with
+    scalar loads, 81 (more voxels than the LDS bitmap holds) reads the int16 flags directly.  Flags
+    other than 1 (0, -1, 2) are unoccupied."""
     ctr, ts = np.array([1.5, 0.5, 2.5], np.float32), np.float32(1.25)
     o, d = _rays(nv, 4000, ctr, ts)
-    flags = (np.random.default_rng(nv + 1).random((nv, nv, nv)) < 0.12).astype(np.int16)
+    rng = np.random.default_rng(nv + 1)
+    flags = np.where(rng.random((nv, nv, nv)) < 0.12, 1, rng.choice([-1, 0, 2], (nv, nv, nv))).astype(np.int16)
     z = np.full((4000, N), -7.0, np.float32)
     ds = np.full((4000, N), -7.0, np.float32)
     zg, dg = to_dev(z), to_dev(ds)
