@@ -195,9 +195,10 @@ static int64_t default_capacity(int64_t n) { return 48 * n; }
 
 // Render workspace: one pass of the march writes every ray's records at fixed slots, ray i owning
 // slots [i * cpr, (i + 1) * cpr) (cpr = record capacity per ray); no count pass, no scan.  A ray's
-// records come in runs of one tile each (the ray walks its tiles in order); the first record of a
-// run carries the tile and the run length, so the tile counting sort works per run, not per
-// record, and writes each run's slots contiguously into the tile's bucket of perm.  The MLP writes
+// records come in runs of one tile each (the ray walks its tiles in order); the ray's run list
+// (tile, length per run, in order) sits in the first slots of its rec_t region, so the tile
+// counting sort works per run, not per record, reads a ray's runs with independent loads, and
+// writes each run's slots contiguously into the tile's bucket of perm.  The MLP writes
 // each record's weighted colour back into its slot of rec_f and the reduction sums every ray's
 // slots in sample order.
 struct RenderWs {
@@ -214,7 +215,7 @@ struct RenderWs {
   int64_t* tile_start;    // [T+1] 64-aligned bucket starts
   int64_t* tile_cursor;   // [T]
   float4* rec_f;          // [n * cpr] voxel coordinate + weight; after the MLP: weighted colour
-  int32_t* rec_t;         // [n * cpr] at the first record of each tile run: tile | run length << 16
+  int32_t* rec_t;         // [n * cpr] run r of a ray (per-tile runs in sample order) at its slot r: tile | length << 16
   int32_t* perm;          // [scap] record slots in tile order, 64-aligned buckets, -1 = padding
   int32_t* skip;          // 1 when the sorted samples exceed scap (render skipped, rgb NaN)
   unsigned long long* cursor;  // per chunk: rays handed out by the queue-fed march
@@ -682,7 +683,7 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
     const int32_t* vis = R.visited ? R.visited + i * P.max_tiles : nullptr;
     const int cpr = W.cpr;
     const int64_t slot0 = i * (int64_t)cpr;
-    int k = 0, run_tile = -1, run_start = 0;
+    int k = 0, run_tile = -1, run_start = 0, nrun = 0;
     MarchResult res;
     int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
     // this lane's staged records 0..cnt-1 go to slots base.. of its ray
@@ -695,7 +696,7 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
     auto on_sample = [&](int tile, const V3& f, const Corners&, float w, float, float, float) {
                 if (k < cpr) {
                   if (tile != run_tile) {
-                    if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((k - run_start) << 16);
+                    if (run_tile >= 0) W.rec_t[slot0 + nrun++] = run_tile | ((k - run_start) << 16);
                     run_tile = tile;
                     run_start = k;
                   }
@@ -708,7 +709,7 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
     march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, on_sample, res);
     const int nrec = min(k, cpr);
     if (nrec & 7) flush(nrec & ~7, nrec & 7);
-    if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((nrec - run_start) << 16);
+    if (run_tile >= 0) W.rec_t[slot0 + nrun] = run_tile | ((nrec - run_start) << 16);
     if (bricks)
       for (int b = res.n_bricks; b < O.brick_cap; ++b) bricks[b] = -1;
     W.counts[i] = res.n_samples;
@@ -769,7 +770,7 @@ __global__ __launch_bounds__(64) void k_march_queue(snisr_scene S, snisr_march_p
   m.phase = FlatMarch::kDone;
   bool has = false, more = true;
   int64_t i = 0, slot0 = 0;
-  int kr = 0, run_tile = -1, run_start = 0;
+  int kr = 0, run_tile = -1, run_start = 0, nrun = 0;
   int32_t* bricks = nullptr;
   int64_t sum = 0;
   int mx = 0;
@@ -782,7 +783,7 @@ __global__ __launch_bounds__(64) void k_march_queue(snisr_scene S, snisr_march_p
   auto on_sample = [&](int tile, const V3& f, const Corners&, float w, float, float, float) {
     if (kr < cpr) {
       if (tile != run_tile) {
-        if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((kr - run_start) << 16);
+        if (run_tile >= 0) W.rec_t[slot0 + nrun++] = run_tile | ((kr - run_start) << 16);
         run_tile = tile;
         run_start = kr;
       }
@@ -796,7 +797,7 @@ __global__ __launch_bounds__(64) void k_march_queue(snisr_scene S, snisr_march_p
     if (has && m.phase == FlatMarch::kDone) {  // finish this lane's ray
       const int nrec = min(kr, cpr);
       if (nrec & 7) flush(nrec & ~7, nrec & 7);
-      if (run_tile >= 0) W.rec_t[slot0 + run_start] = run_tile | ((nrec - run_start) << 16);
+      if (run_tile >= 0) W.rec_t[slot0 + nrun] = run_tile | ((nrec - run_start) << 16);
       if (bricks)
         for (int b = m.nbr; b < O.brick_cap; ++b) bricks[b] = -1;
       W.counts[i] = m.nsamp;
@@ -827,6 +828,7 @@ __global__ __launch_bounds__(64) void k_march_queue(snisr_scene S, snisr_march_p
           kr = 0;
           run_tile = -1;
           run_start = 0;
+          nrun = 0;
           bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
           has = true;
         } else {
@@ -856,18 +858,29 @@ __device__ __forceinline__ void wg_ray_range(int64_t n, int64_t& b, int64_t& e) 
   e = b + per < n ? b + per : n;
 }
 
+// Calls f(tile, first record, length) for the runs of ray i, in order.  The runs sit in the ray's
+// first slots, so run r + 1 is loaded before run r is used (the loads do not wait on each other).
+template <class F>
+__device__ __forceinline__ void for_each_run(const RenderWs& W, int64_t i, F f) {
+  const int cpr = W.cpr;
+  const int c = min(W.counts[i], cpr);
+  if (c <= 0) return;
+  const int32_t* rt = W.rec_t + i * (int64_t)cpr;
+  int v = rt[0];
+  for (int k = 0, r = 0; k < c; ++r) {
+    const int vn = r + 1 < cpr ? rt[r + 1] : 0;
+    const int len = v >> 16;
+    if (len <= 0) break;  // unreachable for runs written by the march; keeps the loop finite
+    f(v & 0xffff, k, len);
+    k += len;
+    v = vn;
+  }
+}
+
 // per-tile record counts of this workgroup's rays into LDS, one atomic per tile run
 __device__ __forceinline__ void rtile_count_block(const RenderWs& W, int* hist, int64_t b, int64_t e) {
-  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-    const int c = min(W.counts[i], W.cpr);
-    const int32_t* rt = W.rec_t + i * (int64_t)W.cpr;
-    for (int k = 0; k < c;) {
-      const int v = rt[k], len = v >> 16;
-      if (len <= 0) break;  // unreachable for runs written by the march; keeps the loop finite
-      atomicAdd(&hist[v & 0xffff], len);
-      k += len;
-    }
-  }
+  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x)
+    for_each_run(W, i, [&](int t, int, int len) { atomicAdd(&hist[t], len); });
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_rtile_hist(RenderWs W) {
@@ -913,32 +926,84 @@ __global__ __launch_bounds__(kScanBlock) void k_rtile_scan(RenderWs W) {
   }
 }
 
+// Scatter of the records into the tile buckets, in chunks of kSortBlock rays (one per thread).  Each
+// chunk reserves its records' positions per tile with one global atomic per tile, stages the
+// record indices in LDS grouped by tile, and then writes each tile's range as consecutive words:
+// written straight from the runs, a wavefront's 4-byte stores would land in 64 places at once.  A
+// chunk whose records exceed the staging buffer (or more than kSortBlock tiles) writes directly.
+constexpr int kScatterLds = 60 * 1024;  // dynamic LDS of k_rtile_scatter (with the static part, under 64 KB)
 __global__ __launch_bounds__(kSortBlock) void k_rtile_scatter(RenderWs W) {
-  __shared__ int hist[kMaxTiles];
-  __shared__ int64_t base[kMaxTiles];
+  extern __shared__ __align__(16) unsigned char sc_lds[];
+  __shared__ int wsum[kSortBlock / 64];
+  __shared__ int chunk_total;
   if (*W.skip) return;
   const int T = W.n_tiles;
-  for (int t = threadIdx.x; t < T; t += blockDim.x) hist[t] = 0;
-  __syncthreads();
+  int64_t* base = reinterpret_cast<int64_t*>(sc_lds);                 // [T]
+  int* hist = reinterpret_cast<int*>(sc_lds + 8 * (size_t)T);         // [T]  counts, then cursors
+  int* lofs = hist + T;                                                // [T + 1] local offsets
+  int32_t* stage = lofs + T + 1;                                       // the rest
+  const int n_stage = (int)((kScatterLds - 16 * (size_t)T - 4) / 4);
+  const bool can_stage = T <= kSortBlock;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int64_t b, e;
   wg_ray_range(W.n, b, e);
-  rtile_count_block(W, hist, b, e);
-  __syncthreads();
-  for (int t = threadIdx.x; t < T; t += blockDim.x) {
-    base[t] = hist[t] ? atomicAdd((unsigned long long*)&W.tile_cursor[t], (unsigned long long)hist[t]) : 0;
-    hist[t] = 0;
-  }
-  __syncthreads();
-  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-    const int c = min(W.counts[i], W.cpr);
-    const int64_t slot0 = i * (int64_t)W.cpr;
-    for (int k = 0; k < c;) {
-      const int v = W.rec_t[slot0 + k], t = v & 0xffff, len = v >> 16;
-      if (len <= 0) break;  // unreachable for runs written by the march; keeps the loop finite
-      int32_t* dst = W.perm + base[t] + atomicAdd(&hist[t], len);
-      for (int j = 0; j < len; ++j) dst[j] = (int32_t)(slot0 + k + j);
-      k += len;
+  for (int64_t c0 = b; c0 < e; c0 += kSortBlock) {
+    const int64_t i = c0 + threadIdx.x;
+    const bool live = i < e;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) hist[t] = 0;
+    __syncthreads();
+    if (live) for_each_run(W, i, [&](int t, int, int len) { atomicAdd(&hist[t], len); });
+    __syncthreads();
+    if (can_stage) {  // exclusive offsets of the tiles inside the chunk (one tile per thread)
+      const int v = threadIdx.x < T ? hist[threadIdx.x] : 0;
+      const int incl = (int)wave_incl_scan((int64_t)v);
+      if (lane == 63) wsum[wid] = incl;
+      __syncthreads();
+      int before = 0, all = 0;
+#pragma unroll
+      for (int k = 0; k < kSortBlock / 64; ++k) {
+        before += k < wid ? wsum[k] : 0;
+        all += wsum[k];
+      }
+      if (threadIdx.x < T) lofs[threadIdx.x] = before + incl - v;
+      if (threadIdx.x == 0) {
+        lofs[T] = all;
+        chunk_total = all;
+      }
     }
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+      base[t] = hist[t] ? atomicAdd((unsigned long long*)&W.tile_cursor[t], (unsigned long long)hist[t]) : 0;
+      hist[t] = 0;
+    }
+    __syncthreads();
+    const int total = can_stage ? chunk_total : 0;
+    const bool staged = can_stage && total <= n_stage;
+    if (live) {
+      const int64_t slot0 = i * (int64_t)W.cpr;
+      for_each_run(W, i, [&](int t, int k, int len) {
+        const int p = atomicAdd(&hist[t], len);
+        if (staged) {
+          int32_t* dst = stage + lofs[t] + p;
+          for (int j = 0; j < len; ++j) dst[j] = (int32_t)(slot0 + k + j);
+        } else {
+          int32_t* dst = W.perm + base[t] + p;
+          for (int j = 0; j < len; ++j) dst[j] = (int32_t)(slot0 + k + j);
+        }
+      });
+    }
+    __syncthreads();
+    if (staged) {
+      for (int q = threadIdx.x; q < total; q += blockDim.x) {
+        int lo = 0, hi = T - 1;  // the tile whose range holds q: the last tile starting at or before q
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (lofs[mid] <= q) lo = mid;
+          else hi = mid - 1;
+        }
+        W.perm[base[lo] + (q - lofs[lo])] = stage[q];
+      }
+    }
+    __syncthreads();  // hist, lofs and stage are reused by the next chunk
   }
 }
 
@@ -1778,7 +1843,7 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
     hipLaunchKernelGGL(k_rtile_scan, dim3(1), dim3(kScanBlock), 0, s, V);
     SNISR_LAUNCHED();
     if (stages) SNISR_HIP(stage_event(out, 3, s));
-    hipLaunchKernelGGL(k_rtile_scatter, sort_grid, sort_block, 0, s, V);
+    hipLaunchKernelGGL(k_rtile_scatter, sort_grid, sort_block, kScatterLds, s, V);
     SNISR_LAUNCHED();
     if (stages) SNISR_HIP(stage_event(out, 4, s));
     if (int e = launch_mlp(V, s)) return e;
