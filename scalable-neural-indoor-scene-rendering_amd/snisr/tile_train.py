@@ -75,7 +75,7 @@ class TileState:
     group_size: float
     far: float
     _cells: torch.Tensor | None = field(default=None, repr=False)
-    _flags_version: int = field(default=-1, repr=False)
+    _flags_seen: tuple | None = field(default=None, repr=False)
     voxels: torch.Tensor | None = field(default=None, repr=False)
 
     @property
@@ -93,10 +93,13 @@ class TileState:
         return tuple((c - self.tile_size / 2.0 - self.voxel_size).tolist())
 
     def cells(self) -> torch.Tensor:
-        """Occupied cells of the dilated grid, recomputed when the flags tensor changed."""
-        v = self.nodes_flag._version
-        if self._cells is None or v != self._flags_version:
-            self._cells, self._flags_version = TR.flag_cells(self.nodes_flag), v
+        """Occupied cells of the dilated grid, recomputed when the flags changed: another tensor
+        assigned (pruning replaces it, tile.py:379) or the same one written in place."""
+        f = self.nodes_flag
+        seen = self._flags_seen
+        if self._cells is None or seen is None or seen[0] is not f or seen[1] != f._version:
+            self._cells = TR.flag_cells(f)
+            self._flags_seen = (f, f._version)
         return self._cells
 
 

@@ -164,5 +164,9 @@ def test_tile_state_recomputes_cells_when_flags_change():
     tile = TT.TileState(nodes_flag=flags, num_voxel=4, voxel_size=0.1, tile_center=(0, 0, 0), group_center=(0, 0, 0),
                         group_size=1.0, far=2.0)
     assert tile.cells().cpu().tolist() == [1 * 36 + 2 * 6 + 3]
-    flags[4, 0, 1] = 0                                    # pruning-style in-place update
+    flags[4, 0, 1] = 0                                    # written in place
     assert tile.cells().cpu().tolist() == [1 * 36 + 2 * 6 + 3, 4 * 36 + 0 * 6 + 1]
+    fresh = torch.full((6, 6, 6), -1, dtype=torch.int16, device=DEV)
+    fresh[5, 5, 5] = 1
+    tile.nodes_flag = fresh                               # replaced, as the reference's pruning does
+    assert tile.cells().cpu().tolist() == [215]
