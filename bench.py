@@ -523,6 +523,21 @@ def main():
     del ga, gb, gc
     extra["gemm_fp16_8192_tflops"] = round(gemm_tflops, 1)
     extra["mlp_vs_gemm_fp16"] = round(achieved / gemm_tflops, 3)
+    # a STREAM-style device copy (1 GiB read + 1 GiB written) beside the published 8 TB/s: the HBM
+    # rate this part sustains
+    src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 4 / ((time.perf_counter() - a) / 10) / 1e9
+    del src, dst
+    extra["hbm_copy_gbs"] = round(copy_gbs, 1)
+    extra["march_vs_copy_rate"] = round(march_gbs / copy_gbs, 3)
     # plain FP32 path (one thread per ray) on the same frame, for the MFMA/FP32 speed ratio
     if args.fp32_frames > 0:
         D.render_fp32(ds, origin, rays_d, t_hit=t_hit, image_width=W)
