@@ -1380,9 +1380,9 @@ __global__ __launch_bounds__(256, 2) void k_train_dw(Workspace W, int64_t mlp_st
   const int64_t je = jb + per < j0 + cnt ? jb + per : j0 + cnt;
   float* G = dmlp + (int64_t)tile * mlp_stride + param_w_off(IN, WD, L, l);
   constexpr int IP0 = in_pad_of(IN, WD, 0);
-  if (l == 0) dw_layer<IP0, WD, IN, WD>(A, D, jb, je, G, Ds, As);
-  else if (l == L - 1) dw_layer<WD, 4, WD, 3>(A, D, jb, je, G, Ds, As);
-  else dw_layer<WD, WD, WD, WD>(A, D, jb, je, G, Ds, As);
+  if (l == 0) dw_layer<IP0, WD, IN, WD, true>(A, D, jb, je, G, Ds, As);
+  else if (l == L - 1) dw_layer<WD, 4, WD, 3, true>(A, D, jb, je, G, Ds, As);
+  else dw_layer<WD, WD, WD, WD, true>(A, D, jb, je, G, Ds, As);
 }
 
 // The FP32 MFMA training MLP helpers (fragment layout, packing, product chains) are in train_mma.hpp.
@@ -1471,9 +1471,11 @@ __global__ __launch_bounds__(256, 2) void k_train_mma(snisr_scene S, snisr_rays 
   const float* P = W.frag + (int64_t)tile * STRIDE;
   float* act = W.act;
   if (BWD && valid) {
-    float* A0 = act + NJ * act_a_off(IN, WD, L, 0) + j * in_pad_of(IN, WD, 0);
+    // rows in the lane layout of the weight-gradient reduction: group g of this wavefront's 32
+    // records is one contiguous 1 KB store
+    float* A0 = act + NJ * act_a_off(IN, WD, L, 0) + (j - col) * in_pad_of(IN, WD, 0) + 4 * lane;
 #pragma unroll
-    for (int g = 0; g < NS0 / 4; ++g) store4(A0 + 32 * (g / 4) + 8 * (g % 4) + 4 * h, x0 + 4 * g);
+    for (int g = 0; g < NS0 / 4; ++g) store4(A0 + 256 * g, x0 + 4 * g);
   }
   // ---- forward: hidden layers (ReLU), masks of every hidden input for the backward
   float a[NB * 16];
@@ -1490,9 +1492,9 @@ __global__ __launch_bounds__(256, 2) void k_train_mma(snisr_scene S, snisr_rays 
 #pragma unroll
     for (int r = 0; r < NB * 16; ++r) mask[l] |= (uint64_t)(a[r] > 0.0f) << (r & 63);
     if (BWD && valid) {
-      float* Al = act + NJ * act_a_off(IN, WD, L, l) + j * WD;
+      float* Al = act + NJ * act_a_off(IN, WD, L, l) + (j - col) * WD + 4 * lane;
 #pragma unroll
-      for (int g = 0; g < NB * 4; ++g) store4(Al + 32 * (g / 4) + 8 * (g % 4) + 4 * h, a + 4 * g);
+      for (int g = 0; g < NB * 4; ++g) store4(Al + 256 * g, a + 4 * g);
     }
     if (l == L - 1) break;
     float b[NB * 16];
@@ -1537,9 +1539,9 @@ __global__ __launch_bounds__(256, 2) void k_train_mma(snisr_scene S, snisr_rays 
 #pragma unroll
   for (int l = L - 2; l >= 0; --l) {
     if (valid) {
-      float* Dl = act + NJ * act_d_off(IN, WD, L, l) + j * WD;
+      float* Dl = act + NJ * act_d_off(IN, WD, L, l) + (j - col) * WD + 4 * lane;
 #pragma unroll
-      for (int g = 0; g < NB * 4; ++g) store4(Dl + 32 * (g / 4) + 8 * (g % 4) + 4 * h, d + 4 * g);
+      for (int g = 0; g < NB * 4; ++g) store4(Dl + 256 * g, d + 4 * g);
     }
     if (l == 0) break;
     float e[NB * 16];

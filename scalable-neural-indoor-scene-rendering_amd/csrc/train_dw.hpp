@@ -13,10 +13,21 @@ namespace snisr {
 // is D rows 2s + h at columns ob*32 + c, the B operand A rows 2s + h at columns ib*32 + c.  Rows are
 // padded to 160 floats so that the two lane halves read disjoint banks.  The next tile's rows are
 // loaded into registers while the current tile's products run.
+//
+// Row layouts: with LANE false the rows are per record ([record][feature], pitch IP / OP); with LANE
+// true each 32-record block (blocks start at multiples of 32) is stored in the order the training
+// kernel's lanes hold it, feature group g (256 floats) = lane l's four features
+// 32 (g / 4) + 8 (g % 4) + 4 (l / 32) .. + 3 of record l % 32, so that the kernel writes every
+// group with one 1 KB wavefront store; a pitch of 4 is the same in both layouts.
 constexpr int kDwRows = 32, kDwPitch = 160;
-template <int IP, int OP, int INL, int OUTL>  // padded row widths, real in / out sizes
+// float offset of (record, feature f, a multiple of 4) inside a 32-record block in the lane layout
+__device__ __forceinline__ int lane_off(int rec, int f) {
+  return 256 * (4 * (f >> 5) + ((f >> 3) & 3)) + 4 * (rec + 32 * ((f >> 2) & 1));
+}
+template <int IP, int OP, int INL, int OUTL, bool LANE = false>  // padded row widths, real in / out sizes
 __device__ __forceinline__ void dw_layer(const float* __restrict__ A, const float* __restrict__ D, int64_t jb,
                                          int64_t je, float* __restrict__ G, float* Ds, float* As) {
+  static_assert(!LANE || ((IP % 8 == 0 || IP == 4) && (OP % 8 == 0 || OP == 4)), "lane layout pitches");
   constexpr int IP4 = IP / 4, OP4 = OP / 4;
   constexpr int ND = (kDwRows * OP4 + 255) / 256, NA = (kDwRows * IP4 + 255) / 256;
   constexpr int NIB = (INL + 31) / 32, NOB = (OUTL + 31) / 32, NPAIR = NIB * NOB;
@@ -28,18 +39,23 @@ __device__ __forceinline__ void dw_layer(const float* __restrict__ A, const floa
   for (int q = 0; q < 4; ++q) acc[q] = f32x16{};
   float accb = 0.0f;
   float4 rd[ND], ra[NA];
+  // thread q moves float4 (record q / P4, features 4 (q % P4) ..) of the 32-record tile, so that the
+  // LDS stores below are conflict-free; in the lane layout its global load then reads 16 bytes of
+  // a line that the workgroup's other loads of the tile complete
   auto fetch = [&](int64_t r0) {
 #pragma unroll
     for (int u = 0; u < ND; ++u) {
       const int q = threadIdx.x + 256 * u, rr = q / OP4, cc = 4 * (q % OP4);
       rd[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (q < kDwRows * OP4 && r0 + rr < je) rd[u] = *reinterpret_cast<const float4*>(D + (r0 + rr) * OP + cc);
+      if (q < kDwRows * OP4 && r0 + rr < je)
+        rd[u] = *reinterpret_cast<const float4*>(D + r0 * OP + (LANE ? lane_off(rr, cc) : rr * OP + cc));
     }
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
       const int q = threadIdx.x + 256 * u, rr = q / IP4, cc = 4 * (q % IP4);
       ra[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (q < kDwRows * IP4 && r0 + rr < je) ra[u] = *reinterpret_cast<const float4*>(A + (r0 + rr) * IP + cc);
+      if (q < kDwRows * IP4 && r0 + rr < je)
+        ra[u] = *reinterpret_cast<const float4*>(A + r0 * IP + (LANE ? lane_off(rr, cc) : rr * IP + cc));
     }
   };
   if (jb < je) fetch(jb);
