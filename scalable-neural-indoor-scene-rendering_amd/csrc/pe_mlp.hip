@@ -31,7 +31,10 @@ __host__ __device__ inline int pe_out_l(int l, int depth, int n_out) { return l 
 __host__ __device__ inline int pe_in_l(int l, int L) { return l == 0 ? pe_in(L) : kPW; }
 // padded row widths of the stored rows (x: layer inputs, d: layer output deltas)
 __host__ __device__ inline int pe_xw(int l, int L) { return l == 0 ? ((pe_in(L) + 3) & ~3) : kPW; }
-__host__ __device__ inline int pe_dw(int l, int depth, int n_out) { return l == depth - 1 ? ((n_out + 3) & ~3) : kPW; }
+// the last layer's delta rows: 4 floats for 4 outputs, 32 for 28 (the lane layout's pitches)
+__host__ __device__ inline int pe_dw(int l, int depth, int n_out) { return l == depth - 1 ? (n_out <= 4 ? 4 : 32) : kPW; }
+// rows stored per layer: the sample count rounded up to whole 32-sample blocks of the lane layout
+__host__ __device__ inline int64_t pe_rows(int64_t n) { return (n + 31) & ~(int64_t)31; }
 // float offset of layer l in the parameter vector
 __host__ __device__ inline int64_t pe_param_off(int l, int L, int depth, int n_out) {
   int64_t s = 0;
@@ -97,10 +100,13 @@ __global__ __launch_bounds__(256, 2) void k_pemlp_mma(const float* __restrict__ 
   float x0[kPNs0];
 #pragma unroll
   for (int s = 0; s < kPNs0; ++s) x0[s] = E[col * kPeLds + tr_row(s, h)];
-  const int64_t in_rows = (int64_t)n * kPW;  // floats of one layer's input rows
+  // rows in the lane layout of the weight-gradient reduction (32-sample blocks, one 1 KB store per
+  // feature group and wavefront); the regions hold whole blocks
+  const int64_t in_rows = pe_rows(n) * kPW;  // floats of one layer's input rows
+  const int64_t blk = j0;                    // first sample of this wavefront's block
   if (BWD && valid) {
 #pragma unroll
-    for (int g = 0; g < kPNs0 / 4; ++g) store4(rows + j * kPW + 32 * (g / 4) + 8 * (g % 4) + 4 * h, x0 + 4 * g);
+    for (int g = 0; g < kPNs0 / 4; ++g) store4(rows + blk * kPW + 256 * g + 4 * lane, x0 + 4 * g);
   }
   // ---- forward
   float a[32];
@@ -119,9 +125,9 @@ __global__ __launch_bounds__(256, 2) void k_pemlp_mma(const float* __restrict__ 
 #pragma unroll
     for (int r = 0; r < 32; ++r) mask[l] |= (uint64_t)(a[r] > 0.0f) << r;
     if (BWD && valid) {
-      float* Al = rows + l * in_rows + j * kPW;
+      float* Al = rows + l * in_rows + blk * kPW + 4 * lane;
 #pragma unroll
-      for (int g = 0; g < 8; ++g) store4(Al + 32 * (g / 4) + 8 * (g % 4) + 4 * h, a + 4 * g);
+      for (int g = 0; g < 8; ++g) store4(Al + 256 * g, a + 4 * g);
     }
     if (l == depth - 1) {
       mk_last = mask[l];
@@ -165,10 +171,11 @@ __global__ __launch_bounds__(256, 2) void k_pemlp_mma(const float* __restrict__ 
     dout[4 * g + 3] = v.w;
   }
   if (valid) {
-    float* Dl = rows + d_base + (int64_t)(depth - 1) * in_rows + j * NOUT;
+    constexpr int NOP = NOUT <= 4 ? 4 : 32;  // delta pitch of the output layer
+    float* Dl = rows + d_base + (int64_t)(depth - 1) * in_rows + blk * NOP;
 #pragma unroll
-    for (int g = 0; g < NSBO / 4; ++g)
-      if (8 * g + 4 * h < NOUT) store4(Dl + 8 * g + 4 * h, dout + 4 * g);
+    for (int g = 0; g < NSBO / 4; ++g)  // rows NOUT .. NOP - 1 carry zero deltas
+      if (8 * g + 4 * h < NOP) store4(Dl + lane_off(col, 8 * g + 4 * h), dout + 4 * g);
   }
   float d[32];
   mma_blocks<NSBO, 2>(
@@ -181,9 +188,9 @@ __global__ __launch_bounds__(256, 2) void k_pemlp_mma(const float* __restrict__ 
   for (int l = 6; l >= 0; --l) {
     if (l > depth - 2) continue;
     if (valid) {  // d = delta of layer l's outputs
-      float* Dl = rows + d_base + (int64_t)l * in_rows + j * kPW;
+      float* Dl = rows + d_base + (int64_t)l * in_rows + blk * kPW + 4 * lane;
 #pragma unroll
-      for (int g = 0; g < 8; ++g) store4(Dl + 32 * (g / 4) + 8 * (g % 4) + 4 * h, d + 4 * g);
+      for (int g = 0; g < 8; ++g) store4(Dl + 256 * g, d + 4 * g);
     }
     if (l == 0) break;
     float e[32];
@@ -205,8 +212,9 @@ __device__ void pe_dw_layer(const float* rows, int64_t n, int L, int depth, int 
   const int64_t per = ((n + gridDim.x - 1) / gridDim.x + kDwRows - 1) / kDwRows * kDwRows;
   const int64_t jb = (int64_t)blockIdx.x * per;
   const int64_t je = jb + per < n ? jb + per : n;
-  dw_layer<IP, OP, INL, OUTL>(rows + pe_x_off(l, L, n), rows + pe_d_off(l, L, depth, n_out, n), jb, je,
-                              g_params + pe_param_off(l, L, depth, n_out), Ds, As);
+  const int64_t nr = pe_rows(n);
+  dw_layer<IP, OP, INL, OUTL, true>(rows + pe_x_off(l, L, nr), rows + pe_d_off(l, L, depth, n_out, nr), jb, je,
+                                    g_params + pe_param_off(l, L, depth, n_out), Ds, As);
 }
 
 template <int NOUT>
@@ -214,7 +222,7 @@ __global__ __launch_bounds__(256, 2) void k_pemlp_dw(const float* __restrict__ r
                                                      float* __restrict__ g_params) {
   __shared__ __align__(16) float Ds[kDwRows * kDwPitch];
   __shared__ __align__(16) float As[kDwRows * kDwPitch];
-  constexpr int NOP = (NOUT + 3) & ~3;
+  constexpr int NOP = NOUT <= 4 ? 4 : 32;
   const int l = blockIdx.y;
   if (l == 0) {
     pe_dw_layer<64, kPW, 63, kPW>(rows, n, L, depth, NOUT, l, g_params, Ds, As);
@@ -251,7 +259,7 @@ int64_t snisr_pemlp_param_count(int32_t L, int32_t depth, int32_t n_out) {
 
 int64_t snisr_pemlp_workspace_size(int64_t n, int32_t L, int32_t depth, int32_t n_out) {
   if (n < 0 || pe_check(L, depth, n_out)) return -1;
-  return 4 * (pe_frag_floats(depth, n_out) + pe_d_off(depth, L, depth, n_out, n)) + 256;
+  return 4 * (pe_frag_floats(depth, n_out) + pe_d_off(depth, L, depth, n_out, pe_rows(n))) + 256;
 }
 
 namespace {
