@@ -23,7 +23,7 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("B,n,with_init", [(1000, 64, False), (333, 7, True), (5, 1, True)])
+@pytest.mark.parametrize("B,n,with_init", [(1000, 64, False), (333, 7, True), (5, 1, True), (77, 150, True), (40, 129, False)])
 def test_volume_rendering_matches_oracle(oracle_mod, B, n, with_init):
     rng = np.random.default_rng(B + n)
     rgb, sig = rng.uniform(0, 1, (B, n, 3)), rng.uniform(0, 8, (B, n, 1))
