@@ -195,10 +195,11 @@ static int64_t default_capacity(int64_t n) { return 48 * n; }
 
 // Render workspace: one pass of the march writes every ray's records at fixed slots, ray i owning
 // slots [i * cpr, (i + 1) * cpr) (cpr = record capacity per ray); no count pass, no scan.  A ray's
-// records come in runs of one tile each (the ray walks its tiles in order); the ray's run list
-// (tile, length per run, in order) sits in the first slots of its rec_t region, so the tile
-// counting sort works per run, not per record, reads a ray's runs with independent loads, and
-// writes each run's slots contiguously into the tile's bucket of perm.  The MLP writes
+// records come in runs of one tile each (the ray walks its tiles in order); the ray's first four
+// runs (tile, length, in order) sit in one 16-byte entry of runs4, any further run r at slot r of
+// its rec_t region, so the tile counting sort works per run, not per record, reads the runs of
+// consecutive rays with coalesced loads, and writes each run's slots contiguously into the tile's
+// bucket of perm.  The MLP writes
 // each record's weighted colour back into its slot of rec_f and the reduction sums every ray's
 // slots in sample order.
 struct RenderWs {
@@ -215,7 +216,8 @@ struct RenderWs {
   int64_t* tile_start;    // [T+1] 64-aligned bucket starts
   int64_t* tile_cursor;   // [T]
   float4* rec_f;          // [n * cpr] voxel coordinate + weight; after the MLP: weighted colour
-  int32_t* rec_t;         // [n * cpr] run r of a ray (per-tile runs in sample order) at its slot r: tile | length << 16
+  int32_t* rec_t;         // [n * cpr] run r >= 4 of a ray (per-tile runs in sample order) at its slot r
+  int4* runs4;            // [n] the ray's runs 0..3: tile | length << 16 each
   int32_t* perm;          // [scap] record slots in tile order, 64-aligned buckets, -1 = padding
   int32_t* skip;          // 1 when the sorted samples exceed scap (render skipped, rgb NaN)
   unsigned long long* cursor;  // per chunk: rays handed out by the queue-fed march
@@ -237,6 +239,7 @@ static RenderWs chunk_view(const RenderWs& W, int c, int64_t r0, int64_t nc) {
   v.enc = W.enc + 16 * r0;
   v.rec_f = W.rec_f + r0 * W.cpr;
   v.rec_t = W.rec_t + r0 * W.cpr;
+  v.runs4 = W.runs4 + r0;
   v.tile_count = W.tile_count + c * T;
   v.tile_start = W.tile_start + c * (T + 1);
   v.tile_cursor = W.tile_cursor + c * T;
@@ -274,6 +277,7 @@ static int64_t render_layout(int64_t n, int64_t cpr, int64_t samples, int32_t T,
   const int64_t o_cu = off; off = align_up(off + 8 * k * (int64_t)T, 256);
   const int64_t o_rf = off; off = align_up(off + 16 * slots, 256);
   const int64_t o_rt = off; off = align_up(off + 4 * slots, 256);
+  const int64_t o_r4 = off; off = align_up(off + 16 * n, 256);
   const int64_t o_pm = off; off = align_up(off + 4 * k * sorted, 256);
   if (w && base) {
     w->total = reinterpret_cast<int64_t*>(base);
@@ -293,6 +297,7 @@ static int64_t render_layout(int64_t n, int64_t cpr, int64_t samples, int32_t T,
     w->tile_cursor = reinterpret_cast<int64_t*>(base + o_cu);
     w->rec_f = reinterpret_cast<float4*>(base + o_rf);
     w->rec_t = reinterpret_cast<int32_t*>(base + o_rt);
+    w->runs4 = reinterpret_cast<int4*>(base + o_r4);
     w->perm = reinterpret_cast<int32_t*>(base + o_pm);
     w->n = n;
     w->cpr = (int32_t)cpr;
@@ -659,6 +664,26 @@ __device__ __forceinline__ int wave_max_i32(int v) {
   return v;
 }
 
+// The runs of one ray as a march emits them (tile | length << 16, in sample order): runs 0..3 stay
+// in registers and go out as one 16-byte store at the ray's end, later ones straight to the ray's
+// rec_t slots.
+struct RunList {
+  int v0, v1, v2, v3, n;
+  __device__ __forceinline__ void reset() {
+    v0 = v1 = v2 = v3 = 0;
+    n = 0;
+  }
+  __device__ __forceinline__ void add(int v, int32_t* rt) {
+    if (n == 0) v0 = v;
+    else if (n == 1) v1 = v;
+    else if (n == 2) v2 = v;
+    else if (n == 3) v3 = v;
+    else rt[n] = v;
+    ++n;
+  }
+  __device__ __forceinline__ void store(int4* dst) const { *dst = make_int4(v0, v1, v2, v3); }
+};
+
 // One wavefront per 64 rays: march each ray once, writing its records (voxel coordinate + weight,
 // tile) at its own slots, its sample count, T, depth and view encoding; per-wave totals feed the
 // header (sample total, largest count, overflow when a ray has more samples than slots).
@@ -683,7 +708,9 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
     const int32_t* vis = R.visited ? R.visited + i * P.max_tiles : nullptr;
     const int cpr = W.cpr;
     const int64_t slot0 = i * (int64_t)cpr;
-    int k = 0, run_tile = -1, run_start = 0, nrun = 0;
+    int k = 0, run_tile = -1, run_start = 0;
+    RunList rl;
+    rl.reset();
     MarchResult res;
     int32_t* bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
     // this lane's staged records 0..cnt-1 go to slots base.. of its ray
@@ -696,7 +723,7 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
     auto on_sample = [&](int tile, const V3& f, const Corners&, float w, float, float, float) {
                 if (k < cpr) {
                   if (tile != run_tile) {
-                    if (run_tile >= 0) W.rec_t[slot0 + nrun++] = run_tile | ((k - run_start) << 16);
+                    if (run_tile >= 0) rl.add(run_tile | ((k - run_start) << 16), W.rec_t + slot0);
                     run_tile = tile;
                     run_start = k;
                   }
@@ -709,7 +736,8 @@ __global__ __launch_bounds__(64) void k_march_render(snisr_scene S, snisr_march_
     march_ray(S, P, origin, d, t_hit, vis, bricks, O.brick_cap, on_sample, res);
     const int nrec = min(k, cpr);
     if (nrec & 7) flush(nrec & ~7, nrec & 7);
-    if (run_tile >= 0) W.rec_t[slot0 + nrun] = run_tile | ((nrec - run_start) << 16);
+    if (run_tile >= 0) rl.add(run_tile | ((nrec - run_start) << 16), W.rec_t + slot0);
+    rl.store(W.runs4 + i);
     if (bricks)
       for (int b = res.n_bricks; b < O.brick_cap; ++b) bricks[b] = -1;
     W.counts[i] = res.n_samples;
@@ -770,7 +798,9 @@ __global__ __launch_bounds__(64) void k_march_queue(snisr_scene S, snisr_march_p
   m.phase = FlatMarch::kDone;
   bool has = false, more = true;
   int64_t i = 0, slot0 = 0;
-  int kr = 0, run_tile = -1, run_start = 0, nrun = 0;
+  int kr = 0, run_tile = -1, run_start = 0;
+  RunList rl;
+  rl.reset();
   int32_t* bricks = nullptr;
   int64_t sum = 0;
   int mx = 0;
@@ -783,7 +813,7 @@ __global__ __launch_bounds__(64) void k_march_queue(snisr_scene S, snisr_march_p
   auto on_sample = [&](int tile, const V3& f, const Corners&, float w, float, float, float) {
     if (kr < cpr) {
       if (tile != run_tile) {
-        if (run_tile >= 0) W.rec_t[slot0 + nrun++] = run_tile | ((kr - run_start) << 16);
+        if (run_tile >= 0) rl.add(run_tile | ((kr - run_start) << 16), W.rec_t + slot0);
         run_tile = tile;
         run_start = kr;
       }
@@ -797,7 +827,8 @@ __global__ __launch_bounds__(64) void k_march_queue(snisr_scene S, snisr_march_p
     if (has && m.phase == FlatMarch::kDone) {  // finish this lane's ray
       const int nrec = min(kr, cpr);
       if (nrec & 7) flush(nrec & ~7, nrec & 7);
-      if (run_tile >= 0) W.rec_t[slot0 + nrun] = run_tile | ((nrec - run_start) << 16);
+      if (run_tile >= 0) rl.add(run_tile | ((nrec - run_start) << 16), W.rec_t + slot0);
+      rl.store(W.runs4 + i);
       if (bricks)
         for (int b = m.nbr; b < O.brick_cap; ++b) bricks[b] = -1;
       W.counts[i] = m.nsamp;
@@ -828,7 +859,7 @@ __global__ __launch_bounds__(64) void k_march_queue(snisr_scene S, snisr_march_p
           kr = 0;
           run_tile = -1;
           run_start = 0;
-          nrun = 0;
+          rl.reset();
           bricks = O.bricks ? O.bricks + i * O.brick_cap : nullptr;
           has = true;
         } else {
@@ -858,23 +889,26 @@ __device__ __forceinline__ void wg_ray_range(int64_t n, int64_t& b, int64_t& e) 
   e = b + per < n ? b + per : n;
 }
 
-// Calls f(tile, first record, length) for the runs of ray i, in order.  The runs sit in the ray's
-// first slots, so run r + 1 is loaded before run r is used (the loads do not wait on each other).
+// Calls f(tile, first record, length) for the runs of ray i, in order: runs 0..3 from the ray's
+// runs4 entry (one 16-byte load, coalesced over consecutive rays), later ones from its rec_t slots.
 template <class F>
 __device__ __forceinline__ void for_each_run(const RenderWs& W, int64_t i, F f) {
   const int cpr = W.cpr;
   const int c = min(W.counts[i], cpr);
   if (c <= 0) return;
-  const int32_t* rt = W.rec_t + i * (int64_t)cpr;
-  int v = rt[0];
-  for (int k = 0, r = 0; k < c; ++r) {
-    const int vn = r + 1 < cpr ? rt[r + 1] : 0;
+  const int4 q = W.runs4[i];
+  int k = 0;
+  auto run = [&](int v) {  // false once the runs cover the ray's records
     const int len = v >> 16;
-    if (len <= 0) break;  // unreachable for runs written by the march; keeps the loop finite
+    if (len <= 0) return false;  // unreachable for runs written by the march; keeps the loop finite
     f(v & 0xffff, k, len);
     k += len;
-    v = vn;
-  }
+    return k < c;
+  };
+  if (!run(q.x) || !run(q.y) || !run(q.z) || !run(q.w)) return;
+  const int32_t* rt = W.rec_t + i * (int64_t)cpr;
+  for (int r = 4; r < cpr; ++r)
+    if (!run(rt[r])) return;
 }
 
 // per-tile record counts of this workgroup's rays into LDS, one atomic per tile run
