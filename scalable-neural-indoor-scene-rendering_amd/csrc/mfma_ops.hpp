@@ -117,8 +117,9 @@ __device__ __forceinline__ void relu_cvt<Mfma<SNISR_F16>>(const f32x16& acc, f16
   hi = __builtin_elementwise_max(Mfma<SNISR_F16>::cvt8(v + 8), z);
 }
 
-// BF16 operands: convert, then clear every packed value whose sign bit is set (a packed arithmetic
-// shift by 15 gives all ones for those halves).  max(x, 0) commutes with rounding, and a value that
+// BF16 operands: convert, then a packed 16-bit signed max with 0: a BF16 value whose sign bit is
+// set is a negative 16-bit integer and becomes +0, every other value is unchanged, so one vector
+// instruction applies the ReLU to two values.  max(x, 0) commutes with rounding, and a value that
 // rounds to -0 becomes +0 either way.  A float max would first canonicalize its operand (IEEE mode):
 // two vector instructions per value instead of one per two.
 template <>
@@ -130,9 +131,8 @@ __device__ __forceinline__ void relu_cvt<Mfma<SNISR_BF16>>(const f32x16& acc, bf
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const bf16x2 b = __builtin_convertvector((f32x2){acc[2 * q], acc[2 * q + 1]}, bf16x2);
-    const uint32_t u = __builtin_bit_cast(uint32_t, b);
-    const s16x2 neg = __builtin_bit_cast(s16x2, u) >> (s16x2){15, 15};
-    w[q] = u & ~__builtin_bit_cast(uint32_t, neg);
+    const s16x2 r = __builtin_elementwise_max(__builtin_bit_cast(s16x2, b), (s16x2){0, 0});
+    w[q] = __builtin_bit_cast(uint32_t, r);
   }
   const uint4 l4 = make_uint4(w[0], w[1], w[2], w[3]), h4 = make_uint4(w[4], w[5], w[6], w[7]);
   lo = __builtin_bit_cast(bf16x8, l4);
