@@ -966,6 +966,13 @@ __global__ __launch_bounds__(kScanBlock) void k_rtile_scan(RenderWs W) {
 // written straight from the runs, a wavefront's 4-byte stores would land in 64 places at once.  A
 // chunk whose records exceed the staging buffer (or more than kSortBlock tiles) writes directly.
 constexpr int kScatterLds = 60 * 1024;  // dynamic LDS of k_rtile_scatter (with the static part, under 64 KB)
+// dst[j] = v0 + j for j < len: single words up to a 16-byte boundary, then four words per store
+__device__ __forceinline__ void write_run(int32_t* dst, int32_t v0, int len) {
+  int j = 0;
+  for (; j < len && (reinterpret_cast<uintptr_t>(dst + j) & 15); ++j) dst[j] = v0 + j;
+  for (; j + 4 <= len; j += 4) *reinterpret_cast<int4*>(dst + j) = make_int4(v0 + j, v0 + j + 1, v0 + j + 2, v0 + j + 3);
+  for (; j < len; ++j) dst[j] = v0 + j;
+}
 __global__ __launch_bounds__(kSortBlock) void k_rtile_scatter(RenderWs W) {
   extern __shared__ __align__(16) unsigned char sc_lds[];
   __shared__ int wsum[kSortBlock / 64];
@@ -1016,13 +1023,9 @@ __global__ __launch_bounds__(kSortBlock) void k_rtile_scatter(RenderWs W) {
       const int64_t slot0 = i * (int64_t)W.cpr;
       for_each_run(W, i, [&](int t, int k, int len) {
         const int p = atomicAdd(&hist[t], len);
-        if (staged) {
-          int32_t* dst = stage + lofs[t] + p;
-          for (int j = 0; j < len; ++j) dst[j] = (int32_t)(slot0 + k + j);
-        } else {
-          int32_t* dst = W.perm + base[t] + p;
-          for (int j = 0; j < len; ++j) dst[j] = (int32_t)(slot0 + k + j);
-        }
+        const int32_t v0 = (int32_t)(slot0 + k);
+        if (staged) write_run(stage + lofs[t] + p, v0, len);
+        else write_run(W.perm + base[t] + p, v0, len);
       });
     }
     __syncthreads();
