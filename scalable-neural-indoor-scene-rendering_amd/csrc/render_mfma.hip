@@ -1030,12 +1030,19 @@ __global__ __launch_bounds__(kSortBlock) void k_rtile_scatter(RenderWs W) {
     }
     __syncthreads();
     if (staged) {
+      // the tile whose range holds q is the last tile starting at or before q; q grows along a
+      // thread's loop, so the search starts at the previous q's tile: a few steps forward, then a
+      // binary search over the rest when the tile is farther
+      int lo = 0;
       for (int q = threadIdx.x; q < total; q += blockDim.x) {
-        int lo = 0, hi = T - 1;  // the tile whose range holds q: the last tile starting at or before q
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (lofs[mid] <= q) lo = mid;
-          else hi = mid - 1;
+        for (int step = 0; step < 4 && lo + 1 < T && lofs[lo + 1] <= q; ++step) ++lo;
+        if (lo + 1 < T && lofs[lo + 1] <= q) {
+          int hi = T - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (lofs[mid] <= q) lo = mid;
+            else hi = mid - 1;
+          }
         }
         W.perm[base[lo] + (q - lofs[lo])] = stage[q];
       }
