@@ -97,14 +97,18 @@ __global__ void k_pack_ref_layer(const uint16_t* __restrict__ src, uint8_t* __re
   }
 }
 
-// the 32 layer-0 inputs of lane half h for one sample, in k-slot order
+// the 32 layer-0 inputs of lane half h for one sample, in k-slot order.  The hardware sin / cos
+// take revolutions: rev = the argument times 1 / (2 pi), or times 32 / (2 pi) for half 1 (the
+// power-of-two factor is exact, so this is the same product as __sinf(32 p) forms).
 __device__ __forceinline__ void pe_half(const float (&p)[3], int h, float (&v)[32]) {
+  constexpr float kRev = 0.15915494309189535f;  // 1 / (2 pi)
+  const float rev_scale = h ? 32.0f * kRev : kRev;
   float sn[5][3], cs[5][3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    const float a = h ? 32.0f * p[r] : p[r];
-    sn[0][r] = __sinf(a);
-    cs[0][r] = __cosf(a);
+    const float rev = p[r] * rev_scale;
+    sn[0][r] = __builtin_amdgcn_sinf(rev);
+    cs[0][r] = __builtin_amdgcn_cosf(rev);
 #pragma unroll
     for (int k = 1; k < 5; ++k) {
       const float t = sn[k - 1][r] + sn[k - 1][r];
@@ -138,7 +142,7 @@ __device__ __forceinline__ void sh2_basis(float x, float y, float z, float (&b)[
 // 3 (5h + i/3) + i%3 (zero rows in slots 12..14 of half 1), w[j] the basis value of 5h + j
 __device__ __forceinline__ void sh_partial(const f32x16& acc, const float (&w)[5], float (&pr)[3]) {
 #pragma unroll
-  for (int i = 0; i < 15; ++i) pr[i % 3] += w[i / 3] * acc[i];
+  for (int i = 0; i < 15; ++i) pr[i % 3] = __builtin_fmaf(w[i / 3], acc[i], pr[i % 3]);
 }
 
 // The inputs of one 64-column chunk as lane (h, s) needs them: the positions of columns s and
