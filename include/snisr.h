@@ -205,7 +205,7 @@ int snisr_render_mfma_stats(const void* workspace, int64_t* total, int32_t* over
  *
  * snisr_bvh_build_bounds: host bytes to allocate for snisr_bvh_build's two outputs.
  * snisr_bvh_build (host, synchronous): vertices f32[V,3], faces i32[F,3] -> nodes (64 B each) and
- * leaf-ordered triangles (48 B each); n_nodes and the tree depth are returned.  Upload both arrays
+ * leaf-ordered triangles (64 B each); n_nodes and the tree depth are returned.  Upload both arrays
  * to device memory unchanged for snisr_mesh_first_hit, which must be given the same n_nodes, n_tris
  * and depth.
  * snisr_mesh_first_hit: per ray (directions normalised as in the marcher) the smallest t > 0 over

@@ -6,7 +6,9 @@
 //   node  = 64 bytes: child 0 box (lo.xyz, hi.xyz), child 1 box, child 0 code, child 1 code;
 //           code >= 0: internal node index; code < 0: leaf ~(first << 3 | count), count <= 7
 //           (count 0 = empty child); node 0 is the root.
-//   tri   = 48 bytes: v0.xyz + original id (as int bits), v1.xyz + 0, v2.xyz + 0, in leaf order.
+//   tri   = 64 bytes: v0.xyz + original id (as int bits), e1 = v1 - v0 + 0, e2 = v2 - v0 + 0,
+//           n = (v1 - v0) x (v2 - v1) + 0, in leaf order (the ray-independent terms of the triangle
+//           test, in FP32 with the device's operation order).
 // Boxes are widened outward by a relative 1e-6 of the scene extent (plus one float ulp) so the
 // traversal's float slab test never rejects a box whose triangles the exact test would hit.
 #include <algorithm>
@@ -168,7 +170,7 @@ int snisr_bvh_build_bounds(int64_t n_faces, int64_t* node_bytes, int64_t* tri_by
   SNISR_REQUIRE(n_faces >= 0 && n_faces < (int64_t(1) << 28) && node_bytes && tri_bytes, SNISR_ERR_ARG,
                 "bvh_build_bounds: bad arguments");
   *node_bytes = 64 * (n_faces + 2);
-  *tri_bytes = 48 * (n_faces + 1);
+  *tri_bytes = 64 * (n_faces + 1);
   return SNISR_OK;
 }
 
@@ -176,7 +178,7 @@ int snisr_bvh_build(const float* vertices, int64_t n_vertices, const int32_t* fa
                     int64_t node_bytes, void* tris_out, int64_t tri_bytes, int32_t* n_nodes, int32_t* depth) {
   SNISR_REQUIRE(vertices && faces && nodes_out && tris_out && n_nodes, SNISR_ERR_ARG, "bvh_build: NULL pointer");
   SNISR_REQUIRE(n_faces >= 0 && n_faces < (int64_t(1) << 28), SNISR_ERR_ARG, "bvh_build: too many faces");
-  SNISR_REQUIRE(node_bytes >= 64 * (n_faces + 2) && tri_bytes >= 48 * n_faces, SNISR_ERR_CAPACITY,
+  SNISR_REQUIRE(node_bytes >= 64 * (n_faces + 2) && tri_bytes >= 64 * n_faces, SNISR_ERR_CAPACITY,
                 "bvh_build: output buffers too small");
   Builder B;
   B.prims.resize(n_faces);
@@ -225,14 +227,30 @@ int snisr_bvh_build(const float* vertices, int64_t n_vertices, const int32_t* fa
   float* T = reinterpret_cast<float*>(tris_out);
   for (int64_t i = 0; i < n_faces; ++i) {
     const Prim& p = B.prims[i];
-    for (int k = 0; k < 3; ++k) {
-      const float* v = vertices + 3 * (int64_t)faces[3 * (int64_t)p.id + k];
-      T[12 * i + 4 * k + 0] = v[0];
-      T[12 * i + 4 * k + 1] = v[1];
-      T[12 * i + 4 * k + 2] = v[2];
-      int32_t tag = k == 0 ? p.id : 0;
-      std::memcpy(&T[12 * i + 4 * k + 3], &tag, 4);
+    const float* v[3];
+    for (int k = 0; k < 3; ++k) v[k] = vertices + 3 * (int64_t)faces[3 * (int64_t)p.id + k];
+    float e1[3], e2[3], f[3], n[3];
+    for (int a = 0; a < 3; ++a) {
+      e1[a] = v[1][a] - v[0][a];
+      e2[a] = v[2][a] - v[0][a];
+      f[a] = v[2][a] - v[1][a];
     }
+    // cross((v1 - v0), (v2 - v1)) component by component, each a difference of two products
+    n[0] = e1[1] * f[2] - e1[2] * f[1];
+    n[1] = e1[2] * f[0] - e1[0] * f[2];
+    n[2] = e1[0] * f[1] - e1[1] * f[0];
+    float* t = T + 16 * i;
+    for (int a = 0; a < 3; ++a) {
+      t[a] = v[0][a];
+      t[4 + a] = e1[a];
+      t[8 + a] = e2[a];
+      t[12 + a] = n[a];
+    }
+    const int32_t tag = p.id, zero = 0;
+    std::memcpy(&t[3], &tag, 4);
+    std::memcpy(&t[7], &zero, 4);
+    std::memcpy(&t[11], &zero, 4);
+    std::memcpy(&t[15], &zero, 4);
   }
   *n_nodes = (int32_t)B.nodes.size();
   if (depth) *depth = B.max_depth;

@@ -6,7 +6,8 @@
 // first, which does not change its sign).  A hit needs t > 0; the nearest t wins, ties go to the
 // smaller triangle id, so the result does not depend on the traversal order and equals a brute
 // force over all triangles.  Each 64-byte node holds both child boxes, so one node fetch (four
-// 16-byte loads of adjacent addresses) tests two children.
+// 16-byte loads of adjacent addresses) tests two children.  A triangle is 64 bytes: its first vertex
+// and id, its two edges from that vertex and the edge-cross normal that decides the vertex flip.
 #include "snisr_common.hpp"
 #include "snisr_device.hpp"
 
@@ -20,12 +21,12 @@ __device__ __forceinline__ V3 cross3(V3 a, V3 b) {
   return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
-// t of the hit, or -1e8 on a miss
-__device__ __forceinline__ float ray_triangle(V3 o, V3 d, V3 a, V3 b, V3 c) {
-  const float sim = dot3(d, cross3(sub3(b, a), sub3(c, b)));
-  V3 v1 = b, v2 = c;
-  if (sim > 0.0f) { v1 = c; v2 = b; }
-  const V3 v0v1 = sub3(v1, a), v0v2 = sub3(v2, a), v0o = sub3(o, a);
+// t of the hit, or -1e8 on a miss, from the triangle's first vertex a, its edges e1 = b - a and
+// e2 = c - a and n = (b - a) x (c - b), the terms that do not depend on the ray (the BVH stores them
+// per triangle): the same operations as from the three vertices, so the same t.
+__device__ __forceinline__ float ray_triangle_pre(V3 o, V3 d, V3 a, V3 e1, V3 e2, V3 n) {
+  const bool flip = dot3(d, n) > 0.0f;
+  const V3 v0v1 = flip ? e2 : e1, v0v2 = flip ? e1 : e2, v0o = sub3(o, a);
   const V3 pv = cross3(d, v0v2);
   float det = dot3(v0v1, pv);
   if (det < 0.00000001f) return -100000000.0f;
@@ -37,6 +38,16 @@ __device__ __forceinline__ float ray_triangle(V3 o, V3 d, V3 a, V3 b, V3 c) {
   if (v < 0.0f || v > 1.0f) return -100000000.0f;
   if ((u + v) < 0.0f || (u + v) > 1.0f) return -100000000.0f;
   return dot3(v0v2, qv) * det;
+}
+__device__ __forceinline__ float ray_triangle(V3 o, V3 d, V3 a, V3 b, V3 c) {
+  return ray_triangle_pre(o, d, a, sub3(b, a), sub3(c, a), cross3(sub3(b, a), sub3(c, b)));
+}
+
+// triangle j of the BVH's leaf-ordered array (four float4: a + id, e1, e2, n)
+__device__ __forceinline__ float bvh_triangle(const float4* __restrict__ tris, int j, V3 o, V3 d, int& id) {
+  const float4 t0 = tris[4 * j], t1 = tris[4 * j + 1], t2 = tris[4 * j + 2], t3 = tris[4 * j + 3];
+  id = __float_as_int(t0.w);
+  return ray_triangle_pre(o, d, v3(t0.x, t0.y, t0.z), v3(t1.x, t1.y, t1.z), v3(t2.x, t2.y, t2.z), v3(t3.x, t3.y, t3.z));
 }
 
 // slab test against a (widened) box: entry/exit t; a direction component of 0 uses +-1e30
@@ -89,9 +100,8 @@ __global__ __launch_bounds__(64) void k_mesh_first_hit(const float4* __restrict_
         const int code = ~c[k];
         const int first = code >> 3, cnt = code & 7;
         for (int j = 0; j < cnt; ++j) {
-          const float4 t0 = tris[3 * (first + j)], t1 = tris[3 * (first + j) + 1], t2 = tris[3 * (first + j) + 2];
-          const int id = __float_as_int(t0.w);
-          const float t = ray_triangle(o, d, v3(t0.x, t0.y, t0.z), v3(t1.x, t1.y, t1.z), v3(t2.x, t2.y, t2.z));
+          int id;
+          const float t = bvh_triangle(tris, first + j, o, d, id);
           if (t > 0.0f && (t < best || (t == best && id < best_id))) {
             best = t;
             best_id = id;
@@ -171,9 +181,8 @@ __global__ __launch_bounds__(64) void k_mesh_first_hit_packet(const float4* __re
         const int code = ~c[k];
         const int first = code >> 3, cnt = code & 7;
         for (int j = 0; j < cnt; ++j) {
-          const float4 t0 = tris[3 * (first + j)], t1 = tris[3 * (first + j) + 1], t2 = tris[3 * (first + j) + 2];
-          const int id = __float_as_int(t0.w);
-          const float t = ray_triangle(o, d, v3(t0.x, t0.y, t0.z), v3(t1.x, t1.y, t1.z), v3(t2.x, t2.y, t2.z));
+          int id;
+          const float t = bvh_triangle(tris, first + j, o, d, id);
           if (live && t > 0.0f && (t < best || (t == best && id < best_id))) {
             best = t;
             best_id = id;

@@ -326,7 +326,7 @@ class DeviceMesh:
         nb, tb = C.c_int64(0), C.c_int64(0)
         N.check(L.snisr_bvh_build_bounds(F.shape[0], C.byref(nb), C.byref(tb)), "bvh_build_bounds")
         nodes = np.empty(nb.value // 4, np.float32)
-        tris = np.empty(max(tb.value, 48) // 4, np.float32)
+        tris = np.empty(max(tb.value, 64) // 4, np.float32)
         nn, depth = C.c_int32(0), C.c_int32(0)
         N.check(L.snisr_bvh_build(V.ctypes.data_as(C.c_void_p), V.shape[0], F.ctypes.data_as(C.c_void_p), F.shape[0],
                                   nodes.ctypes.data_as(C.c_void_p), nb.value, tris.ctypes.data_as(C.c_void_p),

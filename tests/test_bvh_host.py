@@ -13,13 +13,13 @@ def build(vertices, faces):
     nb, tb = C.c_int64(0), C.c_int64(0)
     assert L.snisr_bvh_build_bounds(faces.shape[0], C.byref(nb), C.byref(tb)) == 0
     nodes = np.empty(nb.value // 4, np.float32)
-    tris = np.empty(max(tb.value, 48) // 4, np.float32)
+    tris = np.empty(max(tb.value, 64) // 4, np.float32)
     nn, depth = C.c_int32(0), C.c_int32(0)
     rc = L.snisr_bvh_build(vertices.ctypes.data_as(C.c_void_p), vertices.shape[0], faces.ctypes.data_as(C.c_void_p),
                            faces.shape[0], nodes.ctypes.data_as(C.c_void_p), nb.value,
                            tris.ctypes.data_as(C.c_void_p), tris.nbytes, C.byref(nn), C.byref(depth))
     assert rc == 0, L.snisr_last_error()
-    return nodes[: 16 * nn.value].reshape(-1, 16), tris.reshape(-1, 12)[: faces.shape[0]], depth.value
+    return nodes[: 16 * nn.value].reshape(-1, 16), tris.reshape(-1, 16)[: faces.shape[0]], depth.value
 
 
 def leaves(nodes):
@@ -33,9 +33,15 @@ def test_bvh_layout_covers_every_triangle_once():
     assert depth < 64
     ids = tris[:, 3].copy().view(np.int32)
     assert sorted(ids.tolist()) == list(range(m.faces.shape[0]))
-    # leaf-ordered triangles carry their original vertices
-    v = m.vertices[m.faces[ids]]
-    assert np.array_equal(tris[:, [0, 1, 2]], v[:, 0]) and np.array_equal(tris[:, [4, 5, 6]], v[:, 1])
+    # leaf-ordered triangles carry their first vertex, the edges from it and the edge-cross normal
+    # (v1 - v0) x (v2 - v1), each an FP32 difference of FP32 products as the device test forms it
+    v = m.vertices[m.faces[ids]].astype(np.float32)
+    e1, e2, f = v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], v[:, 2] - v[:, 1]
+    n = np.stack([e1[:, 1] * f[:, 2] - e1[:, 2] * f[:, 1], e1[:, 2] * f[:, 0] - e1[:, 0] * f[:, 2],
+                  e1[:, 0] * f[:, 1] - e1[:, 1] * f[:, 0]], 1)
+    assert np.array_equal(tris[:, 0:3], v[:, 0]) and np.array_equal(tris[:, 4:7], e1)
+    assert np.array_equal(tris[:, 8:11], e2) and np.array_equal(tris[:, 12:15], n)
+    assert not np.any(tris[:, [7, 11, 15]].copy().view(np.int32))
     # walk the tree from the root: every triangle index is reached exactly once, child boxes contain them
     seen = np.zeros(m.faces.shape[0], np.int32)
     codes = leaves(nodes)
@@ -58,7 +64,7 @@ def test_bvh_layout_covers_every_triangle_once():
                 assert cnt <= 4
                 seen[first:first + cnt] += 1
                 for j in range(first, first + cnt):
-                    pts = tris[j].reshape(3, 4)[:, :3]
+                    pts = m.vertices[m.faces[int(ids[j])]]
                     assert np.all(pts >= box[:3]) and np.all(pts <= box[3:])
     assert np.all(seen == 1)
 
