@@ -50,11 +50,14 @@ __device__ __forceinline__ float bvh_triangle(const float4* __restrict__ tris, i
   return ray_triangle_pre(o, d, v3(t0.x, t0.y, t0.z), v3(t1.x, t1.y, t1.z), v3(t2.x, t2.y, t2.z), v3(t3.x, t3.y, t3.z));
 }
 
-// slab test against a (widened) box: entry/exit t; a direction component of 0 uses +-1e30
-__device__ __forceinline__ void slab(const float* b, V3 o, V3 inv, float& tn, float& tf) {
-  const float x0 = (b[0] - o.x) * inv.x, x1 = (b[3] - o.x) * inv.x;
-  const float y0 = (b[1] - o.y) * inv.y, y1 = (b[4] - o.y) * inv.y;
-  const float z0 = (b[2] - o.z) * inv.z, z1 = (b[5] - o.z) * inv.z;
+// slab test against a (widened) box: entry/exit t; a direction component of 0 uses +-1e30.  Each
+// plane's t is one FMA, plane * inv - o * inv with o * inv formed once per ray: it differs from
+// (plane - o) * inv by a few ulps of the products, far inside the boxes' widening (a relative 1e-6 of
+// the scene extent), so no box whose triangles the exact test would hit is rejected.
+__device__ __forceinline__ void slab(const float* b, V3 oi, V3 inv, float& tn, float& tf) {
+  const float x0 = __builtin_fmaf(b[0], inv.x, -oi.x), x1 = __builtin_fmaf(b[3], inv.x, -oi.x);
+  const float y0 = __builtin_fmaf(b[1], inv.y, -oi.y), y1 = __builtin_fmaf(b[4], inv.y, -oi.y);
+  const float z0 = __builtin_fmaf(b[2], inv.z, -oi.z), z1 = __builtin_fmaf(b[5], inv.z, -oi.z);
   tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
   tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
 }
@@ -73,6 +76,7 @@ __global__ __launch_bounds__(64) void k_mesh_first_hit(const float4* __restrict_
   const V3 inv = v3(d.x != 0.0f ? 1.0f / d.x : __builtin_copysignf(1e30f, d.x),
                     d.y != 0.0f ? 1.0f / d.y : __builtin_copysignf(1e30f, d.y),
                     d.z != 0.0f ? 1.0f / d.z : __builtin_copysignf(1e30f, d.z));
+  const V3 oi = v3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
   float best = __builtin_inff();
   int best_id = -1;
   int sp = 0;
@@ -83,8 +87,8 @@ __global__ __launch_bounds__(64) void k_mesh_first_hit(const float4* __restrict_
     const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
     const int c[2] = {__float_as_int(q3.x), __float_as_int(q3.y)};
     float tn[2], tf[2];
-    slab(b0, o, inv, tn[0], tf[0]);
-    slab(b1, o, inv, tn[1], tf[1]);
+    slab(b0, oi, inv, tn[0], tf[0]);
+    slab(b1, oi, inv, tn[1], tf[1]);
     bool hit[2];
     int inner[2], ni = 0;
     float inner_t[2];
@@ -152,6 +156,7 @@ __global__ __launch_bounds__(64) void k_mesh_first_hit_packet(const float4* __re
   const V3 inv = v3(d.x != 0.0f ? 1.0f / d.x : __builtin_copysignf(1e30f, d.x),
                     d.y != 0.0f ? 1.0f / d.y : __builtin_copysignf(1e30f, d.y),
                     d.z != 0.0f ? 1.0f / d.z : __builtin_copysignf(1e30f, d.z));
+  const V3 oi = v3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
   float best = live ? __builtin_inff() : -1.0f;  // a dead lane never wants a node
   int best_id = -1;
   int sp = 0;
@@ -162,8 +167,8 @@ __global__ __launch_bounds__(64) void k_mesh_first_hit_packet(const float4* __re
     const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
     const int c[2] = {__float_as_int(q3.x), __float_as_int(q3.y)};
     float tn[2], tf[2];
-    slab(b0, o, inv, tn[0], tf[0]);
-    slab(b1, o, inv, tn[1], tf[1]);
+    slab(b0, oi, inv, tn[0], tf[0]);
+    slab(b1, oi, inv, tn[1], tf[1]);
     bool any[2];
     bool mine[2];
 #pragma unroll
