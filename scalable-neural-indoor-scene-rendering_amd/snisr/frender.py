@@ -190,8 +190,10 @@ class FRender:
     def _diffuse(self):
         dev = self.device
         H, W, s = self.H, self.W, self.reflection_scale
-        rays_d = torch.full((H, W, 3), -1, device=dev, dtype=torch.float32)
-        visited = torch.full((H, W, self.max_tracing_tile), -1, device=dev, dtype=torch.int32)
+        # compute_rays writes every direction and tracing_tiles every one of the K entries per pixel
+        # (-1 padded), so these two need no fill (the reference's render() fills them with -1)
+        rays_d = torch.empty((H, W, 3), device=dev, dtype=torch.float32)
+        visited = torch.empty((H, W, self.max_tracing_tile), device=dev, dtype=torch.int32)
         self.frame_diffuse = torch.full((H, W, 3), -1, device=dev, dtype=torch.float32)
         inverse_near = torch.full((H, W), 1e-8, device=dev, dtype=torch.float32)
         netIdxs = torch.full((H, W), -1, device=dev, dtype=torch.int16)
@@ -226,10 +228,14 @@ class FRender:
         transparency = torch.ones((self.DH, self.DW, 1), device=dev, dtype=torch.float32)
         spec = torch.zeros((self.DH, self.DW, 3), device=dev, dtype=torch.float32)
         rays_ds = rays_d[::s, ::s]
+        # inverse_z_sampling writes every sample point and step, so those buffers are allocated once
+        # without a fill; inference_mlp writes only the samples it evaluates, so the results are
+        # cleared before every round of 16 samples
+        sample_results = torch.empty((self.DH, self.DW, 16, 4), device=dev, dtype=torch.float32)
+        samples = torch.empty((self.DH, self.DW, 16, 3), device=dev, dtype=torch.float32)
+        steps = torch.empty((self.DH, self.DW, 16), device=dev, dtype=torch.float32)
         for step in range(0, 64, 16):
-            sample_results = torch.zeros((self.DH, self.DW, 16, 4), device=dev, dtype=torch.float32)
-            samples = torch.full((self.DH, self.DW, 16, 3), -1, device=dev, dtype=torch.float32)
-            steps = torch.zeros((self.DH, self.DW, 16), device=dev, dtype=torch.float32)
+            sample_results.zero_()
             FR.inverse_z_sampling(self.origin, rays_ds, inverse_near, inverse_bound, self.num_thread, step, 16,
                                   samples, steps)
             FR.inference_mlp(16, n_blocks, self.num_thread, self.inverse_far, self.early_stop_R, new_net,
