@@ -1946,10 +1946,12 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
 }
 
 
-// record splits per (tile, layer) of the weight-gradient reduction: about four workgroups per CU
+// record splits per (tile, layer) of the weight-gradient reduction: about sixteen workgroups per CU
+// (two are resident at a time), so that the layers' unequal costs even out over many short
+// workgroups (C3: 4 per CU 3.63 ms, 8: 3.05, 16: 2.95, 32: 2.94, 64: 3.01)
 static int dw_splits(int64_t tile_layers) {
-  const int64_t s = 4 * (int64_t)device_cus() / (tile_layers > 0 ? tile_layers : 1);
-  return (int)(s < 1 ? 1 : (s > 64 ? 64 : s));
+  const int64_t s = 16 * (int64_t)device_cus() / (tile_layers > 0 ? tile_layers : 1);
+  return (int)(s < 1 ? 1 : (s > 128 ? 128 : s));
 }
 
 static int64_t train_frag_floats(const snisr_scene* s) {
