@@ -1855,7 +1855,14 @@ int snisr_render_mfma_tuned(const snisr_scene* scene, const snisr_march_params* 
     return SNISR_OK;
   };
   auto launch_mlp = [&](const RenderWs& V, hipStream_t s) -> int {
-    const dim3 grid((unsigned)device_cus()), block((unsigned)tn.mlp_threads);
+    // one workgroup per CU (its tile weights fill the LDS), and for large batches several per CU in
+    // turn, each over a shorter span of chunks, so that the spans' unequal costs even out: about one
+    // workgroup per 1024 sorted chunks per CU, at most 8 (C2: 1; C5: 0.2 ms and C4: 0.4 ms faster
+    // with 4-8 than with 1, C2 0.5-3% slower with more than 1)
+    const int64_t cus = device_cus();
+    int64_t per_cu = V.scap / (64 * 1024 * cus);
+    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
+    const dim3 grid((unsigned)(cus * per_cu)), block((unsigned)tn.mlp_threads);
     bool launched = false;
 #define SNISR_MFMA_CASE(F_, W_, L_, H_, D_)                                                        \
     if (!launched && F == F_ && WD == W_ && L == L_ && half == H_ && dt == D_) {                    \
