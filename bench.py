@@ -185,10 +185,100 @@ def ref_train_bench(args, dev):
         ms = (time.perf_counter() - t0) * 1e3 / args.ref_train_steps
         res[f"ref_train_flag{flag}_ms_per_step"] = round(ms, 3)
         res[f"ref_train_flag{flag}_mrays_s"] = round(n / (ms * 1e-3) / 1e6, 3)
+    # the four training samplers alone (utils.py:786-820) on the same batch; the reflection sampler
+    # reads the grid the last diffuse step baked
+    rays_o = batch[..., :3].reshape(-1, 3).contiguous()
+    rays_d = batch[..., 3:6].reshape(-1, 3).contiguous()
+    bgdepth = batch[..., 6:7].reshape(-1, 1).contiguous()
+    inner = tile.nodes_flag[1:-1, 1:-1, 1:-1]
+    samplers = {
+        "sparse": lambda: TT.sparse_sampling(rays_o, rays_d, inner, NV, tile.tile_center, tile.tile_size, S),
+        "background": lambda: TT.sampling_background(rays_o, rays_d, bgdepth, tile.tile_center, tile.tile_size, NB,
+                                                     0.5),
+        "uniform": lambda: TT.uniform_sampling(rays_o, rays_d, tile.tile_center, tile.tile_size, S),
+        "reflection": lambda: TT.reflection_sampling(rays_o, rays_d, inner, tile.voxels, NV, tile.tile_center,
+                                                     tile.tile_size, RS, tile.far, 1.0),
+    }
+    for name, fn in samplers.items():
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            fn()
+        torch.cuda.synchronize()
+        res[f"sampler_{name}_ms"] = round((time.perf_counter() - t0) * 1e3 / 20, 4)
     res["ref_train_config"] = (f"Tile.train_batch at cfg.yaml sizes: {n} rays (one 91x91 patch), sparse sampling "
                                f"{S} + background {NB} + reflection {RS} samples per ray, {int(tile.cells().numel())} "
                                f"of {D ** 3} dilated cells baked per diffuse step; flag 0 diffuse, 1 specular, 2 both; "
                                "samplers, forward, masked MSE, adjoints, Adam")
+    return res
+
+
+def operators_bench(args, dev):
+    """The remaining operators at frame size: the library's own C++ frame driver (snisr_scene_* /
+    snisr_graph_*: camera rays, BVH bound and MFMA render captured in one HIP graph) on the C2 frame,
+    and the reference's dense FP32 marcher (render_diffuse_kernel, training/src/cuda_func.cu:
+    1687-1802) on a 1280x720 view of a dense-format scene (4x4x1 tiles of 64^3 voxels, 25% of 8^3
+    blocks occupied at 60%), its visited tiles from tracing_tiles."""
+    import numpy as np
+    import torch
+    from snisr import _native as N
+    from snisr import dense
+    from snisr import device as D
+    from snisr import fastrendering as FR
+    from snisr import scene as S
+
+    res = {}
+
+    def timed(fn, k):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / k
+
+    sc, cam, mesh = S.config_c2()
+    ns = D.NativeScene(sc, N.SNISR_F16)
+    dm = D.DeviceMesh(mesh.vertices, mesh.faces)
+    ng = D.NativeGraph(ns, cam, N.SNISR_F16, mesh=dm)
+    ms = timed(ng.render, args.extra_steps)
+    assert ng.stats()["overflow"] == 0
+    res["native_graph_c2_ms_per_frame"] = round(ms, 3)
+    res["native_graph_c2_mrays_s"] = round(cam.H * cam.W / ms / 1e3, 2)
+    del ng, ns
+
+    ds = S.make_dense_scene(3, tile_shape=(4, 4, 1), num_voxel=64, block=8, block_occupancy=0.25)
+    T, H, W, K = ds.tile_ids.size, 720, 1280, 8
+    eye = np.asarray(ds.scene_min, np.float64) + np.array([0.6, 0.7, 0.5]) * np.asarray(ds.tile_shape) * ds.tile_size
+    look = np.asarray(ds.scene_min, np.float64) + np.array([2.5, 2.2, 0.4]) * ds.tile_size
+    dcam = S.look_at_camera(eye, look, H, W, 900.0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    d_np = S.camera_rays_np(dcam)
+    dirs = t((d_np / np.linalg.norm(d_np, axis=1, keepdims=True)).astype(np.float32))
+    n = dirs.shape[0]
+    origin = t(dcam.origin.astype(np.float32))
+    vis = torch.empty((n, K), dtype=torch.int32, device=dev)
+    FR.tracing_tiles(dirs.reshape(H, W, 3), t(ds.tile_index_map), origin, ds.tile_shape, ds.scene_min,
+                     np.asarray(ds.tile_shape, np.float32) * np.float32(ds.tile_size), ds.tile_size, 512,
+                     vis.reshape(H, W, K))
+    o = origin.expand(n, 3).contiguous()
+    s2g = torch.arange(T, dtype=torch.int32, device=dev)
+    vox, nodes, cen = t(ds.voxels), t(ds.nodes), t(ds.centers)
+    dif = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    dep = torch.empty(n, dtype=torch.float32, device=dev)
+    val = torch.empty(n, dtype=torch.int32, device=dev)
+    step = 0.2 * ds.voxel_size
+    ms = timed(lambda: dense.render_diffuse(o, dirs, vis, s2g, ds.tile_size, ds.num_voxel + 2, ds.voxel_size, step,
+                                            cen, vox, nodes, dif, dep, val), args.extra_steps)
+    res["dense_fp32_ms_per_frame"] = round(ms, 3)
+    res["dense_fp32_mrays_s"] = round(n / ms / 1e3, 2)
+    res["dense_fp32_valid_frac"] = round(float(val.float().mean()), 4)
+    res["operators_config"] = ("native_graph: C2 frame through snisr_graph_render (one HIP graph: rays, BVH bound, "
+                               "MFMA render); dense_fp32: render_diffuse on a 1280x720 view of a 4x4x1-tile dense "
+                               "scene, 64^3 voxels per tile, step 0.2 voxel, K=8 visited tiles")
     return res
 
 
@@ -412,6 +502,8 @@ def main():
     ap.add_argument("--ref-train-steps", type=int, default=5,
                     help="timed steps of the reference-form tile training legs (0 = skip)")
     ap.add_argument("--c4-steps", type=int, default=5, help="timed C4 batches (0 = skip)")
+    ap.add_argument("--extra-steps", type=int, default=10,
+                    help="timed frames of the native frame driver and the dense FP32 marcher (0 = skip)")
     ap.add_argument("--c4-tuning", default="auto",
                     help='C4 launch schedule: "auto" (autotuner) or a JSON dict, e.g. {} for the default')
     ap.add_argument("--c5-steps", type=int, default=3, help="timed C5 steps (0 = skip)")
@@ -562,6 +654,9 @@ def main():
     if args.ref_frames > 0:
         extra.update(ref_pipeline_bench(args, dev))
         progress("reference pipeline done")
+    if args.extra_steps > 0:
+        extra.update(operators_bench(args, dev))
+        progress("native frame driver and dense FP32 marcher done")
     if args.c4_steps > 0:
         extra.update(c4_bench(args, dev))
         progress("C4 done")
