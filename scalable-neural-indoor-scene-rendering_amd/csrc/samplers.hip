@@ -99,6 +99,7 @@ __device__ __forceinline__ V3 ray_of(const float* p, int64_t r) { return v3(p[3 
 // flag before it branches, so the walk is a chain of dependent loads: an LDS read in place of an
 // L2 read shortens every link.
 constexpr int kSmpBitsMax = 64 * 1024 * 8;
+constexpr int kSmpBoundCache = 32;  // voxel bounds kept per lane between the sparse sampler's passes
 struct SmpFlags {
   const int16_t* g;
   const uint32_t* bits;  // LDS bitmap or nullptr
@@ -112,6 +113,9 @@ __global__ __launch_bounds__(64) void k_sample_sparse(const float* __restrict__ 
                                                       int nsamp, float* __restrict__ z_vals,
                                                       float* __restrict__ dists) {
   __shared__ uint32_t fbits[kSmpBitsMax / 32];
+  // the voxel bounds of a lane's first kSmpBoundCache occupied voxels, from the first pass, so that the
+  // second pass does not evaluate the same ray-box slabs (and their divisions) again
+  __shared__ float2 bcache[kSmpBoundCache * 64];
   const int64_t nvox = (int64_t)T.nv * T.nv * T.nv;
   const bool lds = nvox <= kSmpBitsMax;
   if (lds) {  // word w = flags 32 w .. 32 w + 31, read as four 16-byte loads when aligned
@@ -152,23 +156,35 @@ __global__ __launch_bounds__(64) void k_sample_sparse(const float* __restrict__ 
   smp_dda_init(T, o, d, near, D);
   const SmpDda D0 = D;
   float total = 0.0f;
+  int m = 0;  // occupied voxels so far
+  float2* cache = bcache + threadIdx.x;
   while (true) {
     if (F.on((D.cur[0] * T.nv + D.cur[1]) * T.nv + D.cur[2])) {
       float lo, hi;
       smp_voxel_bound(T, o, d, D.cur, lo, hi);
       total = total + (hi - lo);
+      if (m < kSmpBoundCache) cache[64 * m] = make_float2(lo, hi);
+      ++m;
     }
     if (!smp_dda_step(T, D)) break;
   }
   if (total == 0.0f) return;
   D = D0;
+  m = 0;
   int count = 0;
   bool acc = false, first = false;
   const int bsmp = 4, ins = nsamp - 2 * bsmp;
   while (true) {
     if (F.on((D.cur[0] * T.nv + D.cur[1]) * T.nv + D.cur[2])) {
       float lo, hi;
-      smp_voxel_bound(T, o, d, D.cur, lo, hi);
+      if (m < kSmpBoundCache) {
+        const float2 b = cache[64 * m];
+        lo = b.x;
+        hi = b.y;
+      } else {
+        smp_voxel_bound(T, o, d, D.cur, lo, hi);
+      }
+      ++m;
       if (lo != -1.0f && hi != -1.0f) {
         if (acc) {
           far = hi;
